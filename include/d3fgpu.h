@@ -1,0 +1,214 @@
+/*
+ * d3fgpu.h -- C ABI of the MI355X physics backend for forceSimulationGPU.
+ *
+ * One context = one simulation resident on one GPU (positions, velocities, pins, the
+ * force passes and their device buffers).  The JavaScript host (an N-API addon) and the
+ * Python bindings call only these functions; no C++ or torch type crosses this line.
+ *
+ * The entry points replace the engine object of the reference, src/gpu/engine.js
+ * (createEngine, engine.js:73-428), and the plugin contract behind it: a force's
+ * gpu() descriptor {type, version, ...} (docs/differences.md:71-78) becomes an
+ * fg_force_desc, and passes[type].create/update/encode (passes/index.js:8-14) happen
+ * inside fg_set_forces and fg_enqueue.
+ *
+ * Conventions
+ *   - Every function returns FG_OK (0) or a negative FG_ERR_* status; the message of the
+ *     last failure on a context is fg_last_error(ctx).
+ *   - Host arrays are owned by the caller and are copied before the call returns
+ *     (as queue.writeBuffer copies, passes/util.js:13-20).
+ *   - Node vectors on the host are interleaved doubles, 3 per node (x, y, z), whatever
+ *     the number of dimensions; unused axes are ignored on input and 0 on output.
+ *   - Calls on one context come from one thread.  Contexts are independent.
+ */
+#ifndef D3FGPU_H
+#define D3FGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FG_ABI_VERSION 1
+
+/* status codes */
+#define FG_OK 0
+#define FG_ERR_INVALID (-1)     /* bad argument (null pointer, out of range, wrong order) */
+#define FG_ERR_HIP (-2)         /* a HIP runtime call failed (device lost, launch failure) */
+#define FG_ERR_NOMEM (-3)       /* device or pinned host memory exhausted */
+#define FG_ERR_UNSUPPORTED (-4) /* force type or size this backend does not run */
+#define FG_ERR_STATE (-5)       /* call not valid in the context's current state */
+#define FG_ERR_NODEVICE (-6)    /* no usable GPU */
+
+/* Force types: the `type` strings of the reference descriptors (passes/index.js:8-14). */
+enum fg_force_type {
+  FG_FORCE_CENTER = 1,   /* src/center.js:49-51   -> passes/center.js   */
+  FG_FORCE_COLLIDE = 2,  /* src/collide.js:134-136 -> passes/collide.js, collideGrid.js */
+  FG_FORCE_LINK = 3,     /* src/link.js:127-129   -> passes/link.js     */
+  FG_FORCE_MANYBODY = 4, /* src/manyBody.js:142-144 -> passes/manyBody.js, barnesHut.js */
+  FG_FORCE_POSITION = 5, /* src/x.js:43-45 (y.js, z.js) -> passes/position.js */
+  FG_FORCE_RADIAL = 6    /* src/radial.js:69-71   -> passes/radial.js   */
+};
+
+/*
+ * One force, as its gpu() descriptor describes it.  Array members are read only when
+ * (id, version) differs from what the context last saw for that id; scalars are
+ * refreshed on every fg_set_forces call (engine.js:357 refreshes them every step).
+ */
+typedef struct fg_force_desc {
+  int32_t type;          /* enum fg_force_type */
+  int32_t iterations;    /* link, collide: passes per tick (>= 0) */
+  uint64_t id;           /* identity of the force object: a retained id keeps its pass */
+  uint64_t version;      /* bumped by the force whenever its per-node/per-link arrays change */
+  double x, y, z;        /* center, radial: the centre */
+  double strength;       /* center, collide: the scalar strength */
+  double theta2;         /* manyBody: theta^2 (<= 1e-6 selects the exact all-pairs sum) */
+  double distance_min2;  /* manyBody */
+  double distance_max2;  /* manyBody (may be +inf) */
+  int32_t axis;          /* position: 0 = forceX, 1 = forceY, 2 = forceZ */
+  uint32_t count;        /* link: number of links m */
+  const uint32_t* links; /* link: 2*m node indices, (source, target) per link */
+  const double* strengths; /* manyBody[n], link[m], radial[n], position[n] */
+  const double* radii;     /* collide[n]; radial: radiuses[n] */
+  const double* distances; /* link[m] */
+  const double* bias;      /* link[m] */
+  const double* targets;   /* position[n] */
+} fg_force_desc;
+
+typedef struct fg_ctx fg_ctx;
+
+/* Device-resident state for rendering without readback (engine.js:418-420). */
+typedef struct fg_buffer_info {
+  void* positions;   /* device pointer: float4 per node, xyz used (AoS mirror) */
+  void* velocities;  /* device pointer: float4 per node, xyz used */
+  uint32_t count;
+  uint32_t stride;   /* 16 */
+  uint64_t epoch;    /* changes when the pointers are replaced (set_nodes, destroy) */
+} fg_buffer_info;
+
+/* Counters of the last step that ran a many-body tree (debug and roofline). */
+typedef struct fg_stats {
+  uint64_t steps;            /* steps run since set_nodes */
+  uint64_t interactions;     /* body-source evaluations in the last walk (0 if not counted) */
+  uint64_t opened;           /* wave-level node openings in the last walk */
+  uint32_t long_runs;        /* runs of > 64 equal high halves sorted separately */
+  uint32_t sort_bits;        /* bits per radix digit used by the tree sort */
+  uint32_t walk_errors;      /* walks that hit their stack or iteration bound (must be 0) */
+  uint32_t pad;
+} fg_stats;
+
+/* ---- library ------------------------------------------------------------------- */
+
+/* FG_ABI_VERSION of the loaded library. */
+int fg_abi_version(void);
+
+/* Number of usable GPUs (0 when none). Probe for isWebGPUAvailable (device.js:6-18). */
+int fg_device_count(int* count);
+
+/* ---- context lifetime (createEngine / destroy, engine.js:73, 422-427) ------------ */
+
+int fg_create(fg_ctx** out, int device);
+int fg_destroy(fg_ctx* ctx);
+const char* fg_last_error(const fg_ctx* ctx);
+
+/* Whether `n` nodes in `ndim` dimensions fit this device (engine.fits, engine.js:334-338). */
+int fg_fits(fg_ctx* ctx, uint32_t n, int32_t ndim, int32_t* fits);
+
+/* ---- nodes (setNodes / uploadAll / uploadPositions / uploadPinned, engine.js:115-169, 267-310) */
+
+/*
+ * Replace the node set.  pos: 3n (required), vel: 3n (NULL = zero), fix: 3n pinned
+ * coordinates (NULL = none), fixmask: n bytes, bit a = axis a pinned (NULL = none).
+ * Forces must be set again afterwards.  n = 0 frees the buffers.
+ */
+int fg_set_nodes(fg_ctx* ctx, uint32_t n, int32_t ndim, const double* pos, const double* vel,
+                 const double* fix, const uint8_t* fixmask);
+/* Overwrite positions [first, first + count) (user edits between steps). */
+int fg_update_positions(fg_ctx* ctx, uint32_t first, uint32_t count, const double* pos);
+/* Overwrite velocities [first, first + count). */
+int fg_update_velocities(fg_ctx* ctx, uint32_t first, uint32_t count, const double* vel);
+/* Overwrite pins [first, first + count): fix 3*count, fixmask count. */
+int fg_update_pins(fg_ctx* ctx, uint32_t first, uint32_t count, const double* fix,
+                   const uint8_t* fixmask);
+
+/* ---- forces (setForces + pass.update, engine.js:314-330, 357) -------------------- */
+
+/* The forces, in application order.  Passes of ids no longer listed are released. */
+int fg_set_forces(fg_ctx* ctx, uint32_t k, const fg_force_desc* descs);
+
+/* Seed of the coincident-node jiggle (engine.setSeed, engine.js:341-343). */
+int fg_set_seed(fg_ctx* ctx, uint32_t seed);
+
+/* ---- stepping (enqueue, engine.js:352-391) -------------------------------------- */
+
+/*
+ * Queue `steps` ticks: tick s applies every force with alpha = alphas[s], then the
+ * integrator with velocity multiplier `decay` (1 - velocityDecay).  Returns at once.
+ */
+int fg_enqueue(fg_ctx* ctx, uint32_t steps, const double* alphas, double decay);
+
+/* Apply every force once with `alpha` and no integrator (forces-only probe: the
+ * velocity increment of one tick is read back with fg_readback). */
+int fg_apply_forces(fg_ctx* ctx, double alpha);
+
+/* True while 3 or more queued submissions are unfinished (busy, engine.js:394-396). */
+int fg_busy(fg_ctx* ctx, int32_t* busy);
+
+/* Block until every queued step has finished (idle, engine.js:402-404). */
+int fg_sync(fg_ctx* ctx);
+
+/* ---- readback (readback / land, engine.js:198-239) ------------------------------ */
+
+/* Wait for queued work and copy the state: pos, vel 3n doubles each (either may be NULL). */
+int fg_readback(fg_ctx* ctx, double* pos, double* vel);
+
+/*
+ * Asynchronous readback: start copies the state into pinned staging and returns a
+ * ticket; wait blocks (callable from a worker thread) until those copies are done;
+ * fetch converts them into the caller's arrays.  One readback is in flight at a time.
+ * fetch fails with FG_ERR_STATE if the nodes were replaced since start (epoch).
+ */
+int fg_readback_start(fg_ctx* ctx, uint64_t* ticket);
+int fg_readback_wait(fg_ctx* ctx, uint64_t ticket);
+int fg_readback_fetch(fg_ctx* ctx, uint64_t ticket, double* pos, double* vel);
+
+/* Device pointers of the state (gpuBuffers, engine.js:418-420).  Keeps an AoS float4
+ * mirror up to date after every queued step from the first call on. */
+int fg_buffers(fg_ctx* ctx, fg_buffer_info* out);
+
+int fg_get_stats(fg_ctx* ctx, fg_stats* out);
+
+/* ---- kernels on their own (the reference tests drive sort.js the same way,
+ *      test/gpu-sort-test.js:9-38) ----------------------------------------------- */
+
+/*
+ * Stable LSD radix sort of (key, value) u32 pairs on bits [first_bit, first_bit + bits)
+ * of the key, ascending, ties in input order (createSort, sort.js:223-273).
+ * Host arrays of n entries; runs on ctx's device.
+ */
+int fg_radix_sort_pairs(fg_ctx* ctx, uint32_t n, const uint32_t* keys, const uint32_t* values,
+                        uint32_t first_bit, uint32_t bits, uint32_t* keys_out,
+                        uint32_t* values_out);
+
+/*
+ * The many-body tree of the last step that built one, for parity checks against the
+ * reference algorithm (barnesHut.js:21-47).  Any pointer may be NULL.
+ *   cube: origin x, y, z and size (4 floats); shift: LEVELS - log2(size)
+ *   codes: 2n u32, (high, low) Morton halves per node (barnesHut.js:352-366)
+ *   order: n u32, node index at each sorted position
+ *   leaf_depth: n u32, d3 depth of each sorted point's leaf cell
+ *   left, right: n-1 u32 children of each binary node; bit 31 set = sorted point
+ *   depth: n-1 u32, d3 depth of the cells each binary node stands for
+ *   kids: 8(n-1) u32, the d3 children of each binary node (0xffffffff = none)
+ *   com: 3(n-1) doubles, value: n-1 doubles (centre of mass, strength at `depth`)
+ */
+int fg_debug_tree(fg_ctx* ctx, float* cube, int32_t* shift, uint32_t* codes, uint32_t* order,
+                  uint32_t* leaf_depth, uint32_t* left, uint32_t* right, uint32_t* depth,
+                  uint32_t* kids, double* com, double* value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* D3FGPU_H */
