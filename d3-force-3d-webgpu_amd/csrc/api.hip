@@ -1,0 +1,762 @@
+// api.hip -- the C ABI (include/d3fgpu.h): one simulation context on one GPU.
+//
+// Replaces the reference engine (src/gpu/engine.js:73-428).  The state lives in HBM as
+// structure-of-arrays f32 (x, y, z, vx, vy, vz; two velocity buffers so that passes that
+// read other nodes' velocities -- link, small-graph collide -- write the other one), pins as
+// f32 targets plus a byte mask.  Each force descriptor gets a pass (passes/index.js:8-14)
+// that keeps its device arrays until its (id, version) changes (engine.js:314-330, 357).
+// A tick runs the passes in order, then the integrator (engine.js:376-383).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/d3fgpu.h"
+#include "common.h"
+#include "forces.h"
+#include "sort.h"
+#include "tree.h"
+
+using namespace fg;
+
+namespace {
+
+constexpr float BIG = 3e38f;  // stands for infinity on the device (passes/util.js:5-9)
+constexpr uint32_t GRID_MIN = 8192;  // collide through the grid from this many nodes (collide.js:21-22)
+constexpr double EXACT_THETA2 = 1e-6;  // all-pairs many-body below this theta^2 (manyBody.js:20-21)
+constexpr int MAX_PENDING = 3;          // busy() threshold (engine.js:11-13)
+
+float finite_f32(double v) {
+  if (!(v == v)) return 0.0f;  // NaN / undefined -> 0 (util.js:24-28)
+  if (v > BIG) return BIG;
+  if (v < -BIG) return -BIG;
+  return (float)v;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void alloc(size_t b) {
+    free();
+    if (b == 0) b = 16;
+    FG_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+  ~DevBuf() { free(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+void upload_f32(hipStream_t s, DevBuf& b, const double* src, size_t count) {
+  std::vector<float> tmp(count > 0 ? count : 1);
+  for (size_t i = 0; i < count; ++i) tmp[i] = src ? finite_f32(src[i]) : 0.0f;
+  b.alloc(sizeof(float) * tmp.size());
+  FG_HIP(hipMemcpyAsync(b.p, tmp.data(), sizeof(float) * count, hipMemcpyHostToDevice, s));
+  FG_HIP(hipStreamSynchronize(s));
+}
+
+struct Pass {
+  uint64_t id = 0, version = ~0ull;
+  int type = 0;
+  fg_force_desc d{};
+  DevBuf a, b;          // per-node arrays (strengths / radii / targets)
+  DevBuf row, ent;      // link CSR
+  uint32_t m = 0;
+  float rmax = 0;
+  std::unique_ptr<TreeBuffers> tree;
+  std::unique_ptr<GridWork> grid;
+  DevBuf center;
+};
+
+}  // namespace
+
+struct fg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n = 0;
+  int ndim = 2;
+  DevBuf pos[3], vel[2][3], fix[3], mask;
+  int cur = 0;  // which velocity buffer is current
+  bool pinned_any = false;
+  std::vector<std::unique_ptr<Pass>> passes;
+  uint32_t seed = 0;
+  uint64_t steps = 0;
+  uint64_t epoch = 1;
+  int radix_bits = 8;
+  std::string err;
+  // readback
+  float* staging = nullptr;
+  size_t staging_n = 0;
+  hipEvent_t rb_event = nullptr;
+  uint64_t rb_ticket = 0, rb_epoch = 0;
+  bool rb_inflight = false;
+  // AoS mirror for fg_buffers
+  DevBuf aos_pos, aos_vel;
+  bool mirror = false;
+  std::vector<hipEvent_t> pending;
+  fg_stats stats{};
+
+  float* P(int a) { return pos[a].as<float>(); }
+  float* V(int a) { return vel[cur][a].as<float>(); }
+  float* W(int a) { return vel[cur ^ 1][a].as<float>(); }
+};
+
+namespace {
+
+__global__ void pack_aos(uint32_t n, const float* x, const float* y, const float* z, float4* out) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  out[i] = make_float4(x[i], y[i], z[i], 0.0f);
+}
+
+void set_device(fg_ctx* c) { FG_HIP(hipSetDevice(c->device)); }
+
+int fail(fg_ctx* c, int status, const std::string& msg) {
+  if (c) c->err = msg;
+  return status;
+}
+
+template <typename F>
+int guarded(fg_ctx* c, F&& f) {
+  if (!c) return FG_ERR_INVALID;
+  try {
+    set_device(c);
+    return f();
+  } catch (const ArgError& e) {
+    return fail(c, e.status, e.what());
+  } catch (const HipError& e) {
+    return fail(c, e.code == hipErrorOutOfMemory ? FG_ERR_NOMEM : FG_ERR_HIP, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(c, FG_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(c, FG_ERR_HIP, e.what());
+  }
+}
+
+void zero3(hipStream_t s, DevBuf* b, uint32_t n) {
+  for (int a = 0; a < 3; ++a) FG_HIP(hipMemsetAsync(b[a].p, 0, sizeof(float) * (n ? n : 1), s));
+}
+
+// Link CSR from the link list (passes/link.js:65-101): for every node, its incident links.
+void build_links(fg_ctx* c, Pass& p, const fg_force_desc& d) {
+  uint32_t n = c->n, m = d.count;
+  if (m && !d.links) throw ArgError(FG_ERR_INVALID, "link: links array is null");
+  std::vector<uint32_t> row(n + 1, 0);
+  for (uint32_t k = 0; k < m; ++k) {
+    uint32_t s = d.links[2 * k], t = d.links[2 * k + 1];
+    if (s >= n || t >= n) throw ArgError(FG_ERR_INVALID, "link: node index out of range");
+    ++row[s + 1];
+    ++row[t + 1];
+  }
+  for (uint32_t i = 0; i < n; ++i) row[i + 1] += row[i];
+  std::vector<LinkEntry> ent(2 * (size_t)m > 0 ? 2 * (size_t)m : 1);
+  std::vector<uint32_t> cursor(row.begin(), row.end() - 1);
+  for (uint32_t k = 0; k < m; ++k) {
+    uint32_t s = d.links[2 * k], t = d.links[2 * k + 1];
+    float dist = d.distances ? finite_f32(d.distances[k]) : 30.0f;
+    float str = d.strengths ? finite_f32(d.strengths[k]) : 0.0f;
+    float bias = d.bias ? finite_f32(d.bias[k]) : 0.5f;
+    ent[cursor[s]++] = LinkEntry{t, 2 * k, dist, str, bias};
+    ent[cursor[t]++] = LinkEntry{s, 2 * k + 1, dist, str, bias};
+  }
+  p.row.alloc(sizeof(uint32_t) * row.size());
+  p.ent.alloc(sizeof(LinkEntry) * ent.size());
+  FG_HIP(hipMemcpyAsync(p.row.p, row.data(), sizeof(uint32_t) * row.size(), hipMemcpyHostToDevice, c->stream));
+  FG_HIP(hipMemcpyAsync(p.ent.p, ent.data(), sizeof(LinkEntry) * ent.size(), hipMemcpyHostToDevice, c->stream));
+  FG_HIP(hipStreamSynchronize(c->stream));
+  p.m = m;
+}
+
+void update_pass(fg_ctx* c, Pass& p, const fg_force_desc& d) {
+  bool fresh = p.version != d.version || p.id != d.id;
+  p.d = d;
+  p.d.links = nullptr;
+  p.d.strengths = p.d.radii = p.d.distances = p.d.bias = p.d.targets = nullptr;
+  uint32_t n = c->n;
+  switch (d.type) {
+    case FG_FORCE_MANYBODY:
+      if (fresh) upload_f32(c->stream, p.a, d.strengths, d.strengths ? n : 0);
+      if (!d.strengths && fresh) upload_f32(c->stream, p.a, nullptr, n);
+      break;
+    case FG_FORCE_LINK:
+      if (fresh) build_links(c, p, d);
+      break;
+    case FG_FORCE_COLLIDE:
+      if (fresh) {
+        upload_f32(c->stream, p.a, d.radii, n);
+        p.rmax = 0;
+        for (uint32_t i = 0; d.radii && i < n; ++i) p.rmax = std::max(p.rmax, finite_f32(d.radii[i]));
+      }
+      break;
+    case FG_FORCE_RADIAL:
+      if (fresh) {
+        upload_f32(c->stream, p.a, d.strengths, n);
+        upload_f32(c->stream, p.b, d.radii, n);
+      }
+      break;
+    case FG_FORCE_POSITION:
+      if (d.axis < 0 || d.axis > 2) throw ArgError(FG_ERR_INVALID, "position: axis must be 0, 1 or 2");
+      if (fresh) {
+        upload_f32(c->stream, p.a, d.strengths, n);
+        upload_f32(c->stream, p.b, d.targets, n);
+      }
+      break;
+    case FG_FORCE_CENTER:
+      if (!p.center.p) {
+        p.center.alloc(sizeof(CenterWork));
+        FG_HIP(hipMemset(p.center.p, 0, sizeof(CenterWork)));
+      }
+      break;
+    default:
+      throw ArgError(FG_ERR_UNSUPPORTED, "unknown force type " + std::to_string(d.type));
+  }
+  p.version = d.version;
+  p.id = d.id;
+}
+
+void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
+  const uint32_t n = c->n;
+  const int nd = c->ndim;
+  float* pos[3] = {c->P(0), c->P(1), c->P(2)};
+  switch (p.type) {
+    case FG_FORCE_MANYBODY: {
+      float* vel[3] = {c->V(0), c->V(1), c->V(2)};
+      if (!(p.d.theta2 > EXACT_THETA2)) {
+        launch_manybody_pairs(c->stream, n, nd, pos, p.a.as<float>(), vel, finite_f32(p.d.distance_min2),
+                              finite_f32(p.d.distance_max2), sp);
+      } else {
+        if (!p.tree) p.tree.reset(new TreeBuffers());
+        TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2};
+        tree_manybody(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), vel[0],
+                      vel[1], vel[2], tp, sp, c->radix_bits);
+      }
+      break;
+    }
+    case FG_FORCE_LINK:
+      for (int k = 0; k < p.d.iterations && p.m; ++k) {
+        const float* vin[3] = {c->V(0), c->V(1), c->V(2)};
+        float* vout[3] = {c->W(0), c->W(1), c->W(2)};
+        launch_link(c->stream, n, nd, pos, vin, vout, p.row.as<uint32_t>(), p.ent.as<LinkEntry>(), sp);
+        c->cur ^= 1;
+      }
+      break;
+    case FG_FORCE_COLLIDE:
+      if (!(p.rmax > 0)) break;
+      for (int k = 0; k < p.d.iterations; ++k) {
+        if (n >= GRID_MIN) {
+          if (!p.grid) p.grid.reset(new GridWork());
+          float* vel[3] = {c->V(0), c->V(1), c->V(2)};
+          launch_collide_grid(c->stream, *p.grid, n, nd, pos, vel, p.a.as<float>(), p.rmax,
+                              (float)p.d.strength, c->radix_bits == 11 ? 11 : 11, sp);
+        } else {
+          const float* vin[3] = {c->V(0), c->V(1), c->V(2)};
+          float* vout[3] = {c->W(0), c->W(1), c->W(2)};
+          launch_collide_pairs(c->stream, n, nd, pos, vin, vout, p.a.as<float>(), (float)p.d.strength, sp);
+          c->cur ^= 1;
+        }
+      }
+      break;
+    case FG_FORCE_CENTER:
+      launch_center(c->stream, n, nd, pos, p.center.as<CenterWork>(), p.d.x, p.d.y, p.d.z, p.d.strength);
+      break;
+    case FG_FORCE_RADIAL: {
+      float* vel[3] = {c->V(0), c->V(1), c->V(2)};
+      launch_radial(c->stream, n, nd, pos, vel, p.b.as<float>(), p.a.as<float>(), finite_f32(p.d.x),
+                    finite_f32(p.d.y), finite_f32(p.d.z), sp);
+      break;
+    }
+    case FG_FORCE_POSITION:
+      if (p.d.axis < nd)
+        launch_position(c->stream, n, c->P(p.d.axis), c->V(p.d.axis), p.b.as<float>(), p.a.as<float>(), sp);
+      break;
+    default: break;
+  }
+}
+
+void refresh_mirror(fg_ctx* c) {
+  if (!c->mirror || !c->n) return;
+  dim3 g(ceil_div(c->n, 256)), b(256);
+  hipLaunchKernelGGL(pack_aos, g, b, 0, c->stream, c->n, c->P(0), c->P(1), c->P(2), c->aos_pos.as<float4>());
+  hipLaunchKernelGGL(pack_aos, g, b, 0, c->stream, c->n, c->V(0), c->V(1), c->V(2), c->aos_vel.as<float4>());
+  FG_LAUNCH_CHECK();
+}
+
+void retire_pending(fg_ctx* c) {
+  while (!c->pending.empty() && hipEventQuery(c->pending.front()) == hipSuccess) {
+    (void)hipEventDestroy(c->pending.front());
+    c->pending.erase(c->pending.begin());
+  }
+}
+
+void ensure_staging(fg_ctx* c) {
+  size_t need = 6 * (size_t)(c->n ? c->n : 1);
+  if (c->staging && c->staging_n >= need) return;
+  if (c->staging) (void)hipHostFree(c->staging);
+  c->staging = nullptr;
+  FG_HIP(hipHostMalloc((void**)&c->staging, sizeof(float) * need, hipHostMallocDefault));
+  c->staging_n = need;
+}
+
+void copy_state_async(fg_ctx* c) {
+  ensure_staging(c);
+  size_t n = c->n;
+  for (int a = 0; a < 3; ++a) {
+    FG_HIP(hipMemcpyAsync(c->staging + a * n, c->P(a), sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    FG_HIP(hipMemcpyAsync(c->staging + (3 + a) * n, c->V(a), sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  }
+}
+
+void land(fg_ctx* c, double* pos, double* vel) {
+  size_t n = c->n;
+  for (size_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      bool on = a < c->ndim;
+      if (pos) pos[3 * i + a] = on ? (double)c->staging[a * n + i] : 0.0;
+      if (vel) vel[3 * i + a] = on ? (double)c->staging[(3 + a) * n + i] : 0.0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fg_abi_version(void) { return FG_ABI_VERSION; }
+
+int fg_device_count(int* count) {
+  if (!count) return FG_ERR_INVALID;
+  int k = 0;
+  if (hipGetDeviceCount(&k) != hipSuccess) k = 0;
+  *count = k;
+  return FG_OK;
+}
+
+int fg_create(fg_ctx** out, int device) {
+  if (!out) return FG_ERR_INVALID;
+  *out = nullptr;
+  int k = 0;
+  if (hipGetDeviceCount(&k) != hipSuccess || k <= 0) return FG_ERR_NODEVICE;
+  if (device < 0 || device >= k) return FG_ERR_INVALID;
+  fg_ctx* c = new (std::nothrow) fg_ctx();
+  if (!c) return FG_ERR_NOMEM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return FG_ERR_HIP;
+  }
+  if (hipEventCreateWithFlags(&c->rb_event, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FG_ERR_HIP;
+  }
+  *out = c;
+  return FG_OK;
+}
+
+int fg_destroy(fg_ctx* c) {
+  if (!c) return FG_ERR_INVALID;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->passes.clear();
+  for (hipEvent_t e : c->pending) (void)hipEventDestroy(e);
+  c->pending.clear();
+  if (c->staging) (void)hipHostFree(c->staging);
+  if (c->rb_event) (void)hipEventDestroy(c->rb_event);
+  for (int a = 0; a < 3; ++a) {
+    c->pos[a].free();
+    c->vel[0][a].free();
+    c->vel[1][a].free();
+    c->fix[a].free();
+  }
+  c->mask.free();
+  c->aos_pos.free();
+  c->aos_vel.free();
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FG_OK;
+}
+
+const char* fg_last_error(const fg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fg_fits(fg_ctx* c, uint32_t n, int32_t ndim, int32_t* fits) {
+  return guarded(c, [&] {
+    if (!fits || ndim < 1 || ndim > 3) return FG_ERR_INVALID;
+    size_t free_b = 0, total_b = 0;
+    FG_HIP(hipMemGetInfo(&free_b, &total_b));
+    // state, pins, tree (keys, sort scratch, leaves, links, records, sums), grid, link CSR
+    double per_node = 4.0 * 9 + 1 + 8 * 2 + 4 * 2 + 16 + 4 * 3 + 16 + 64 + 40 + 16 + 8 + 64;
+    *fits = (n < 0x7fffffffu && per_node * n < 0.8 * (double)free_b) ? 1 : 0;
+    return FG_OK;
+  });
+}
+
+int fg_set_nodes(fg_ctx* c, uint32_t n, int32_t ndim, const double* pos, const double* vel,
+                 const double* fix, const uint8_t* fixmask) {
+  return guarded(c, [&] {
+    if (ndim < 1 || ndim > 3) return fail(c, FG_ERR_INVALID, "ndim must be 1, 2 or 3");
+    if (n && !pos) return fail(c, FG_ERR_INVALID, "positions are required");
+    FG_HIP(hipStreamSynchronize(c->stream));
+    c->passes.clear();
+    c->n = n;
+    c->ndim = ndim;
+    c->cur = 0;
+    c->steps = 0;
+    c->epoch++;
+    c->rb_inflight = false;
+    c->stats = fg_stats{};
+    size_t b = sizeof(float) * (n ? n : 1);
+    for (int a = 0; a < 3; ++a) {
+      c->pos[a].alloc(b);
+      c->vel[0][a].alloc(b);
+      c->vel[1][a].alloc(b);
+      c->fix[a].alloc(b);
+    }
+    c->mask.alloc(n ? n : 1);
+    if (c->mirror) {
+      c->aos_pos.alloc(16 * (size_t)(n ? n : 1));
+      c->aos_vel.alloc(16 * (size_t)(n ? n : 1));
+    }
+    std::vector<float> h(6 * (size_t)(n ? n : 1), 0.0f);
+    std::vector<uint8_t> m(n ? n : 1, 0);
+    c->pinned_any = false;
+    for (uint32_t i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        bool on = a < ndim;
+        h[a * (size_t)n + i] = on ? finite_f32(pos[3 * (size_t)i + a]) : 0.0f;
+        h[(3 + a) * (size_t)n + i] = (on && vel) ? finite_f32(vel[3 * (size_t)i + a]) : 0.0f;
+      }
+    for (int a = 0; a < 3; ++a) {
+      FG_HIP(hipMemcpyAsync(c->pos[a].p, h.data() + a * (size_t)n, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+      FG_HIP(hipMemcpyAsync(c->vel[0][a].p, h.data() + (3 + a) * (size_t)n, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+      FG_HIP(hipStreamSynchronize(c->stream));
+    }
+    zero3(c->stream, c->vel[1], n);
+    std::fill(h.begin(), h.end(), 0.0f);
+    for (uint32_t i = 0; i < n; ++i) {
+      uint8_t mm = fixmask ? (uint8_t)(fixmask[i] & ((1u << ndim) - 1u)) : 0;
+      m[i] = mm;
+      if (mm) c->pinned_any = true;
+      for (int a = 0; a < 3; ++a)
+        if ((mm >> a) & 1) h[a * (size_t)n + i] = fix ? finite_f32(fix[3 * (size_t)i + a]) : 0.0f;
+    }
+    for (int a = 0; a < 3; ++a)
+      FG_HIP(hipMemcpyAsync(c->fix[a].p, h.data() + a * (size_t)n, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    FG_HIP(hipMemcpyAsync(c->mask.p, m.data(), n, hipMemcpyHostToDevice, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    refresh_mirror(c);
+    return FG_OK;
+  });
+}
+
+static int update_vec(fg_ctx* c, DevBuf* dst, uint32_t first, uint32_t count, const double* src) {
+  return guarded(c, [&] {
+    if (!src || (uint64_t)first + count > c->n) return fail(c, FG_ERR_INVALID, "range out of bounds");
+    std::vector<float> h(3 * (size_t)(count ? count : 1));
+    for (uint32_t i = 0; i < count; ++i)
+      for (int a = 0; a < 3; ++a) h[a * (size_t)count + i] = a < c->ndim ? finite_f32(src[3 * (size_t)i + a]) : 0.0f;
+    for (int a = 0; a < 3; ++a)
+      FG_HIP(hipMemcpyAsync(dst[a].as<float>() + first, h.data() + a * (size_t)count, sizeof(float) * count,
+                            hipMemcpyHostToDevice, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    c->epoch++;
+    return FG_OK;
+  });
+}
+
+int fg_update_positions(fg_ctx* c, uint32_t first, uint32_t count, const double* pos) {
+  if (!c) return FG_ERR_INVALID;
+  return update_vec(c, c->pos, first, count, pos);
+}
+
+int fg_update_velocities(fg_ctx* c, uint32_t first, uint32_t count, const double* vel) {
+  if (!c) return FG_ERR_INVALID;
+  return update_vec(c, c->vel[c->cur], first, count, vel);
+}
+
+int fg_update_pins(fg_ctx* c, uint32_t first, uint32_t count, const double* fix, const uint8_t* fixmask) {
+  return guarded(c, [&] {
+    if (!fixmask || (uint64_t)first + count > c->n) return fail(c, FG_ERR_INVALID, "range out of bounds");
+    std::vector<float> h(3 * (size_t)(count ? count : 1), 0.0f);
+    std::vector<uint8_t> m(count ? count : 1);
+    for (uint32_t i = 0; i < count; ++i) {
+      uint8_t mm = (uint8_t)(fixmask[i] & ((1u << c->ndim) - 1u));
+      m[i] = mm;
+      if (mm) c->pinned_any = true;
+      for (int a = 0; a < 3; ++a)
+        if ((mm >> a) & 1) h[a * (size_t)count + i] = fix ? finite_f32(fix[3 * (size_t)i + a]) : 0.0f;
+    }
+    for (int a = 0; a < 3; ++a)
+      FG_HIP(hipMemcpyAsync(c->fix[a].as<float>() + first, h.data() + a * (size_t)count, sizeof(float) * count,
+                            hipMemcpyHostToDevice, c->stream));
+    FG_HIP(hipMemcpyAsync(c->mask.as<uint8_t>() + first, m.data(), count, hipMemcpyHostToDevice, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    return FG_OK;
+  });
+}
+
+int fg_set_forces(fg_ctx* c, uint32_t k, const fg_force_desc* descs) {
+  return guarded(c, [&] {
+    if (k && !descs) return fail(c, FG_ERR_INVALID, "descriptors are null");
+    for (uint32_t i = 0; i < k; ++i)
+      if (descs[i].type < FG_FORCE_CENTER || descs[i].type > FG_FORCE_RADIAL)
+        return fail(c, FG_ERR_UNSUPPORTED, "unsupported force type " + std::to_string(descs[i].type));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    std::vector<std::unique_ptr<Pass>> next;
+    for (uint32_t i = 0; i < k; ++i) {
+      std::unique_ptr<Pass> p;
+      for (auto& q : c->passes)
+        if (q && q->id == descs[i].id && q->type == descs[i].type) {
+          p = std::move(q);
+          break;
+        }
+      if (!p) {
+        p.reset(new Pass());
+        p->type = descs[i].type;
+      }
+      update_pass(c, *p, descs[i]);
+      next.push_back(std::move(p));
+    }
+    c->passes = std::move(next);
+    return FG_OK;
+  });
+}
+
+int fg_set_seed(fg_ctx* c, uint32_t seed) {
+  if (!c) return FG_ERR_INVALID;
+  c->seed = seed;
+  return FG_OK;
+}
+
+int fg_enqueue(fg_ctx* c, uint32_t steps, const double* alphas, double decay) {
+  return guarded(c, [&] {
+    if (steps && !alphas) return fail(c, FG_ERR_INVALID, "alphas are null");
+    if (!c->n) return FG_OK;
+    const uint8_t* mask = c->pinned_any ? c->mask.as<uint8_t>() : nullptr;
+    for (uint32_t s = 0; s < steps; ++s) {
+      StepParams sp{(float)alphas[s], (float)decay, (uint32_t)c->steps, c->seed};
+      for (auto& p : c->passes) run_pass(c, *p, sp);
+      float* pos[3] = {c->P(0), c->P(1), c->P(2)};
+      float* vel[3] = {c->V(0), c->V(1), c->V(2)};
+      float* fix[3] = {c->fix[0].as<float>(), c->fix[1].as<float>(), c->fix[2].as<float>()};
+      launch_integrate(c->stream, c->n, c->ndim, pos, vel, mask, fix, sp);
+      c->steps++;
+    }
+    refresh_mirror(c);
+    hipEvent_t e;
+    FG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    FG_HIP(hipEventRecord(e, c->stream));
+    c->pending.push_back(e);
+    retire_pending(c);
+    return FG_OK;
+  });
+}
+
+int fg_apply_forces(fg_ctx* c, double alpha) {
+  return guarded(c, [&] {
+    if (!c->n) return FG_OK;
+    StepParams sp{(float)alpha, 1.0f, (uint32_t)c->steps, c->seed};
+    for (auto& p : c->passes) run_pass(c, *p, sp);
+    return FG_OK;
+  });
+}
+
+int fg_busy(fg_ctx* c, int32_t* busy) {
+  return guarded(c, [&] {
+    if (!busy) return FG_ERR_INVALID;
+    retire_pending(c);
+    *busy = (int32_t)c->pending.size() >= MAX_PENDING ? 1 : 0;
+    return FG_OK;
+  });
+}
+
+int fg_sync(fg_ctx* c) {
+  return guarded(c, [&] {
+    FG_HIP(hipStreamSynchronize(c->stream));
+    retire_pending(c);
+    return FG_OK;
+  });
+}
+
+int fg_readback(fg_ctx* c, double* pos, double* vel) {
+  return guarded(c, [&] {
+    if (!c->n) return FG_OK;
+    copy_state_async(c);
+    FG_HIP(hipStreamSynchronize(c->stream));
+    retire_pending(c);
+    land(c, pos, vel);
+    return FG_OK;
+  });
+}
+
+int fg_readback_start(fg_ctx* c, uint64_t* ticket) {
+  return guarded(c, [&] {
+    if (!ticket) return FG_ERR_INVALID;
+    if (c->rb_inflight) return fail(c, FG_ERR_STATE, "a readback is already in flight");
+    if (c->n) {
+      copy_state_async(c);
+      FG_HIP(hipEventRecord(c->rb_event, c->stream));
+    }
+    c->rb_inflight = true;
+    c->rb_epoch = c->epoch;
+    *ticket = ++c->rb_ticket;
+    return FG_OK;
+  });
+}
+
+int fg_readback_wait(fg_ctx* c, uint64_t ticket) {
+  return guarded(c, [&] {
+    if (!c->rb_inflight || ticket != c->rb_ticket) return fail(c, FG_ERR_STATE, "no such readback in flight");
+    if (c->n) FG_HIP(hipEventSynchronize(c->rb_event));
+    return FG_OK;
+  });
+}
+
+int fg_readback_fetch(fg_ctx* c, uint64_t ticket, double* pos, double* vel) {
+  return guarded(c, [&] {
+    if (!c->rb_inflight || ticket != c->rb_ticket) return fail(c, FG_ERR_STATE, "no such readback in flight");
+    if (c->n) FG_HIP(hipEventSynchronize(c->rb_event));
+    c->rb_inflight = false;
+    if (c->rb_epoch != c->epoch) return fail(c, FG_ERR_STATE, "the state was replaced after the readback started");
+    land(c, pos, vel);
+    return FG_OK;
+  });
+}
+
+int fg_buffers(fg_ctx* c, fg_buffer_info* out) {
+  return guarded(c, [&] {
+    if (!out) return FG_ERR_INVALID;
+    if (!c->mirror) {
+      c->mirror = true;
+      c->aos_pos.alloc(16 * (size_t)(c->n ? c->n : 1));
+      c->aos_vel.alloc(16 * (size_t)(c->n ? c->n : 1));
+      refresh_mirror(c);
+      FG_HIP(hipStreamSynchronize(c->stream));
+    }
+    out->positions = c->aos_pos.p;
+    out->velocities = c->aos_vel.p;
+    out->count = c->n;
+    out->stride = 16;
+    out->epoch = c->epoch;
+    return FG_OK;
+  });
+}
+
+int fg_get_stats(fg_ctx* c, fg_stats* out) {
+  return guarded(c, [&] {
+    if (!out) return FG_ERR_INVALID;
+    FG_HIP(hipStreamSynchronize(c->stream));
+    fg_stats s{};
+    s.steps = c->steps;
+    s.sort_bits = (uint32_t)c->radix_bits;
+    for (auto& p : c->passes) {
+      if (p->tree && p->tree->counters) {
+        uint32_t h[8];
+        FG_HIP(hipMemcpy(h, p->tree->counters, sizeof(h), hipMemcpyDeviceToHost));
+        s.interactions = (uint64_t)h[0] | ((uint64_t)h[1] << 32);
+        s.opened = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
+        s.walk_errors = h[4];
+        uint32_t e[4] = {0, 0, 0, 0};
+        if (p->tree->sort.errors) FG_HIP(hipMemcpy(e, p->tree->sort.errors, sizeof(e), hipMemcpyDeviceToHost));
+        s.walk_errors += e[0];
+        break;
+      }
+    }
+    *out = s;
+    return FG_OK;
+  });
+}
+
+int fg_radix_sort_pairs(fg_ctx* c, uint32_t n, const uint32_t* keys, const uint32_t* values,
+                        uint32_t first_bit, uint32_t bits, uint32_t* keys_out, uint32_t* values_out) {
+  return guarded(c, [&] {
+    if (n && (!keys || !values || !keys_out || !values_out)) return FG_ERR_INVALID;
+    if (first_bit >= 32 || bits == 0 || first_bit + bits > 32) return fail(c, FG_ERR_INVALID, "bit range outside the key");
+    if (!n) return FG_OK;
+    DevBuf k, v;
+    k.alloc(sizeof(uint32_t) * n);
+    v.alloc(sizeof(uint32_t) * n);
+    FG_HIP(hipMemcpyAsync(k.p, keys, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+    FG_HIP(hipMemcpyAsync(v.p, values, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+    SortWork w;
+    uint32_t *ok, *ov;
+    radix_sort_u32(c->stream, w, c->radix_bits, k.as<uint32_t>(), v.as<uint32_t>(), n, first_bit, bits, &ok, &ov);
+    FG_HIP(hipMemcpyAsync(keys_out, ok, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    FG_HIP(hipMemcpyAsync(values_out, ov, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    uint32_t e = 0;
+    FG_HIP(hipMemcpy(&e, w.errors, sizeof(e), hipMemcpyDeviceToHost));
+    if (e) return fail(c, FG_ERR_HIP, "sort: look-back wait bound reached");
+    return FG_OK;
+  });
+}
+
+int fg_debug_tree(fg_ctx* c, float* cube, int32_t* shift, uint32_t* codes, uint32_t* order,
+                  uint32_t* leaf_depth, uint32_t* left, uint32_t* right, uint32_t* depth,
+                  uint32_t* kids, double* com, double* value) {
+  return guarded(c, [&] {
+    TreeBuffers* t = nullptr;
+    for (auto& p : c->passes)
+      if (p->tree && p->tree->skeys) {
+        t = p->tree.get();
+        break;
+      }
+    if (!t) return fail(c, FG_ERR_STATE, "no many-body tree has been built");
+    FG_HIP(hipStreamSynchronize(c->stream));
+    uint32_t n = c->n, inner = n > 1 ? n - 1 : 0;
+    const int L = c->ndim == 3 ? 20 : 24, half = L / 2 * c->ndim;
+    if (cube || shift) {
+      CubeInfo ci;
+      FG_HIP(hipMemcpy(&ci, t->cube, sizeof(ci), hipMemcpyDeviceToHost));
+      if (cube) cube[0] = ci.origin[0], cube[1] = ci.origin[1], cube[2] = ci.origin[2], cube[3] = ci.size;
+      if (shift) *shift = ci.shift;
+    }
+    std::vector<uint32_t> ord(n ? n : 1);
+    FG_HIP(hipMemcpy(ord.data(), t->order, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (order) memcpy(order, ord.data(), sizeof(uint32_t) * n);
+    if (codes) {
+      std::vector<uint64_t> sk(n ? n : 1);
+      FG_HIP(hipMemcpy(sk.data(), t->skeys, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+      for (uint32_t s = 0; s < n; ++s) {
+        uint64_t key = sk[s];
+        uint32_t hi = (uint32_t)(key >> half), lo = (uint32_t)(key & ((1ull << half) - 1));
+        codes[2 * ord[s]] = hi << (32 - half);
+        codes[2 * ord[s] + 1] = lo << (32 - half);
+      }
+    }
+    if (leaf_depth) FG_HIP(hipMemcpy(leaf_depth, t->leaf_depth, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (inner && (left || right || depth)) {
+      std::vector<uint4> ln(inner);
+      FG_HIP(hipMemcpy(ln.data(), t->links, sizeof(uint4) * inner, hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < inner; ++i) {
+        if (left) left[i] = ln[i].x;
+        if (right) right[i] = ln[i].y;
+        if (depth) depth[i] = ln[i].z;
+      }
+    }
+    if (inner && (kids || com || value)) {
+      std::vector<NodeRec> r(inner);
+      FG_HIP(hipMemcpy(r.data(), t->rec, sizeof(NodeRec) * inner, hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < inner; ++i) {
+        if (kids) memcpy(kids + 8 * (size_t)i, r[i].kids, sizeof(r[i].kids));
+        if (com) com[3 * i] = r[i].cx, com[3 * i + 1] = r[i].cy, com[3 * i + 2] = r[i].cz;
+        if (value) value[i] = r[i].value;
+      }
+    }
+    return FG_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// common.h -- shared device helpers and host error plumbing of the physics backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+namespace fg {
+
+constexpr int WAVE = 64;
+
+// A failed HIP call inside the library: caught at the C-ABI boundary and turned into
+// FG_ERR_HIP with the message kept for fg_last_error.
+struct HipError : std::runtime_error {
+  hipError_t code;
+  HipError(hipError_t c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+
+struct ArgError : std::runtime_error {
+  int status;
+  ArgError(int s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+
+#define FG_HIP(call)                                                                        \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw ::fg::HipError(e_, std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+#define FG_LAUNCH_CHECK() FG_HIP(hipGetLastError())
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ---- device helpers ------------------------------------------------------------------
+
+// The reference jiggle hash (wgsl.js:28-39): pcg, then a value in [-5e-7, 5e-7).
+__host__ __device__ inline uint32_t pcg_hash(uint32_t v) {
+  uint32_t s = v * 747796405u + 2891336453u;
+  uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+  return (w >> 22u) ^ w;
+}
+
+__host__ __device__ inline float jiggle_hash(uint32_t a, uint32_t b, uint32_t axis, uint32_t step,
+                                             uint32_t seed) {
+  uint32_t h = pcg_hash(a ^ pcg_hash(b ^ pcg_hash(seed ^ pcg_hash(step * 3u + axis))));
+  return ((float)h * (1.0f / 4294967296.0f) - 0.5f) * 1e-6f;
+}
+
+// Displacement between two coincident points, antisymmetric in (i, j) (wgsl.js:45-52).
+__device__ inline void coincident(uint32_t i, uint32_t j, int ndim, uint32_t step, uint32_t seed,
+                                  float& dx, float& dy, float& dz) {
+  uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+  float rx = jiggle_hash(lo, hi, 0u, step, seed);
+  float ry = jiggle_hash(lo, hi, 1u, step, seed);
+  float rz = jiggle_hash(lo, hi, 2u, step, seed);
+  if (i > j) rx = -rx, ry = -ry, rz = -rz;
+  dx = rx;
+  dy = ndim > 1 ? ry : 0.0f;
+  dz = ndim > 2 ? rz : 0.0f;
+}
+
+// Floats as u32 with the same order (for atomicMin / atomicMax), barnesHut.js:132-138.
+__device__ inline uint32_t float_ordered(float f) {
+  uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ inline float float_unordered(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__device__ inline uint64_t wave_ballot(bool p) { return __ballot(p); }
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// Number of set bits below this lane in a 64-bit wave mask.
+__device__ inline uint32_t mask_rank(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Per-step values every kernel of a tick reads (Sim uniform, wgsl.js:13-22).
+struct StepParams {
+  float alpha;
+  float decay;
+  uint32_t step;
+  uint32_t seed;
+};
+
+}  // namespace fg

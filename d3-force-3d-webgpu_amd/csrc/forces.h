@@ -1,0 +1,63 @@
+// forces.h -- launchers of the streaming force passes (see forces.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+#include "sort.h"
+
+namespace fg {
+
+// One incident link of a node, as the node-centric link pass reads it.
+struct LinkEntry {
+  uint32_t other;   // the link's other end
+  uint32_t code;    // link index * 2 + (this node is the link's target)
+  float distance;
+  float strength;
+  float bias;
+};
+
+struct CenterWork {
+  static constexpr uint32_t MAX_BLOCKS = 1024;
+  double partial[MAX_BLOCKS][3];
+  float shift[4];
+  uint32_t done;
+};
+
+struct GridWork {
+  uint32_t cap = 0, bits = 0;
+  uint32_t* keys = nullptr;
+  uint32_t* vals = nullptr;
+  uint32_t* starts = nullptr;
+  float4* sorted = nullptr;
+  SortWork sort;
+  void reserve(uint32_t n);
+  void release();
+  ~GridWork() { release(); }
+};
+
+// pos/vel/fix: three device arrays each (x, y, z); unused axes may be null.
+void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
+                      const uint8_t* mask, const float* const* fix, StepParams step);
+void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* const* vin,
+                 float* const* vout, const uint32_t* row, const LinkEntry* ent,
+                 StepParams step);
+void launch_center(hipStream_t s, uint32_t n, int ndim, float* const* pos, CenterWork* w, double cx,
+                   double cy, double cz, double strength);
+void launch_radial(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
+                   const float* radius, const float* weight, float cx, float cy, float cz,
+                   StepParams step);
+void launch_position(hipStream_t s, uint32_t n, const float* p, float* v, const float* goal,
+                     const float* weight, StepParams step);
+void launch_manybody_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos,
+                           const float* strength, float* const* vel, float dmin2, float dmax2,
+                           StepParams step);
+void launch_collide_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos,
+                          const float* const* vin, float* const* vout, const float* radius,
+                          float strength, StepParams step);
+void launch_collide_grid(hipStream_t s, GridWork& g, uint32_t n, int ndim, float* const* pos,
+                         float* const* vel, const float* radius, float rmax, float strength,
+                         int radix_bits, StepParams step);
+
+}  // namespace fg

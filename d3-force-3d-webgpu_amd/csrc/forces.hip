@@ -1,0 +1,557 @@
+// forces.hip -- the streaming force passes and the integrator (SoA f32 state, wave64).
+//
+//   integrate         v *= decay; p += v; pinned axes p = fix, v = 0      (wgsl.js:67-84)
+//   link              node-centric gather over a CSR of incident links; the snapshot
+//                     pos + vel is formed on the fly and the result goes to the other
+//                     velocity buffer, so every node sees the same state (passes/link.js:17-47)
+//   center            block sums in FP64, the last block forms the shift; then the shift
+//                     (passes/center.js:11-72)
+//   radial, position  one thread per node (passes/radial.js:9-27, passes/position.js:7-21)
+//   manybody_pairs    exact all-pairs sum with LDS tiles (passes/manyBody.js:23-74)
+//   collide_pairs     exact all-pairs collide for small graphs (passes/collide.js:24-81)
+//   grid_*            collide through a hashed uniform grid (passes/collideGrid.js:6-22):
+//                     cells 2 * rmax wide, a slot per cell from a hashed block offset plus
+//                     the cell's Morton index inside its block, radix sort of (slot, node),
+//                     run starts, then a 3^d neighbourhood query per sorted node.
+#include "forces.h"
+
+namespace fg {
+
+// ---- integrate ----------------------------------------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(256) integrate_kernel(uint32_t n, float* __restrict__ x, float* __restrict__ y,
+                                                        float* __restrict__ z, float* __restrict__ vx,
+                                                        float* __restrict__ vy, float* __restrict__ vz,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const float* __restrict__ fx,
+                                                        const float* __restrict__ fy,
+                                                        const float* __restrict__ fz,
+                                                        StepParams step) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float decay = step.decay;
+  uint32_t m = mask ? mask[i] : 0u;
+  float v = vx[i] * decay, p = x[i] + v;
+  if (m & 1u) p = fx[i], v = 0.0f;
+  x[i] = p;
+  vx[i] = v;
+  if (D > 1) {
+    v = vy[i] * decay, p = y[i] + v;
+    if (m & 2u) p = fy[i], v = 0.0f;
+    y[i] = p;
+    vy[i] = v;
+  }
+  if (D > 2) {
+    v = vz[i] * decay, p = z[i] + v;
+    if (m & 4u) p = fz[i], v = 0.0f;
+    z[i] = p;
+    vz[i] = v;
+  }
+}
+
+void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
+                      const uint8_t* mask, const float* const* fix, StepParams step) {
+  if (!n) return;
+  dim3 g(ceil_div(n, 256)), b(256);
+  if (ndim == 1)
+    hipLaunchKernelGGL(integrate_kernel<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask, fix[0], fix[1], fix[2], step);
+  else if (ndim == 2)
+    hipLaunchKernelGGL(integrate_kernel<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask, fix[0], fix[1], fix[2], step);
+  else
+    hipLaunchKernelGGL(integrate_kernel<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask, fix[0], fix[1], fix[2], step);
+  FG_LAUNCH_CHECK();
+}
+
+// ---- link ---------------------------------------------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(256) link_kernel(uint32_t n, const float* __restrict__ x,
+                                                   const float* __restrict__ y,
+                                                   const float* __restrict__ z,
+                                                   const float* __restrict__ vx,
+                                                   const float* __restrict__ vy,
+                                                   const float* __restrict__ vz,
+                                                   float* __restrict__ ox, float* __restrict__ oy,
+                                                   float* __restrict__ oz,
+                                                   const uint32_t* __restrict__ row,
+                                                   const LinkEntry* __restrict__ ent,
+                                                   StepParams step) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float alpha = step.alpha;
+  const uint32_t sn = step.step, seed = step.seed;
+  float sx = x[i] + vx[i], sy = D > 1 ? y[i] + vy[i] : 0.f, sz = D > 2 ? z[i] + vz[i] : 0.f;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  const uint32_t e0 = row[i], e1 = row[i + 1];
+  for (uint32_t e = e0; e < e1; ++e) {
+    LinkEntry L = ent[e];
+    uint32_t j = L.other;
+    float tx = x[j] + vx[j], ty = D > 1 ? y[j] + vy[j] : 0.f, tz = D > 2 ? z[j] + vz[j] : 0.f;
+    bool target = L.code & 1u;
+    uint32_t id = L.code >> 1;
+    // d = snap[target] - snap[source], the same vector at both ends
+    float dx = target ? sx - tx : tx - sx;
+    float dy = target ? sy - ty : ty - sy;
+    float dz = target ? sz - tz : tz - sz;
+    if (dx == 0.f) dx = jiggle_hash(id, 0u, 0u, sn, seed);
+    if (D > 1 && dy == 0.f) dy = jiggle_hash(id, 0u, 1u, sn, seed);
+    if (D > 2 && dz == 0.f) dz = jiggle_hash(id, 0u, 2u, sn, seed);
+    float l = sqrtf(dx * dx + dy * dy + dz * dz);
+    float f = (l - L.distance) / l * alpha * L.strength;
+    float w = target ? -L.bias : 1.0f - L.bias;
+    ax += dx * f * w;
+    if (D > 1) ay += dy * f * w;
+    if (D > 2) az += dz * f * w;
+  }
+  ox[i] = vx[i] + ax;
+  if (D > 1) oy[i] = vy[i] + ay;
+  if (D > 2) oz[i] = vz[i] + az;
+}
+
+void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* const* vin,
+                 float* const* vout, const uint32_t* row, const LinkEntry* ent,
+                 StepParams step) {
+  if (!n) return;
+  dim3 g(ceil_div(n, 256)), b(256);
+  if (ndim == 1)
+    hipLaunchKernelGGL(link_kernel<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
+  else if (ndim == 2)
+    hipLaunchKernelGGL(link_kernel<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
+  else
+    hipLaunchKernelGGL(link_kernel<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
+  FG_LAUNCH_CHECK();
+}
+
+// ---- center -------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) center_reduce(uint32_t n, int ndim, const float* __restrict__ x,
+                                                     const float* __restrict__ y,
+                                                     const float* __restrict__ z,
+                                                     CenterWork* __restrict__ w, double cx, double cy,
+                                                     double cz, double strength) {
+  double s[3] = {0, 0, 0};
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    s[0] += (double)x[i];
+    if (ndim > 1) s[1] += (double)y[i];
+    if (ndim > 2) s[2] += (double)z[i];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s[a] += __shfl_xor(s[a], o, WAVE);
+  __shared__ double sh[4][3];
+  __shared__ uint32_t s_last;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) sh[wave][a] = s[a];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a) w->partial[blockIdx.x][a] = sh[0][a] + sh[1][a] + sh[2][a] + sh[3][a];
+    uint32_t done = __hip_atomic_fetch_add(&w->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = done + 1 == gridDim.x;
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  double t[3] = {0, 0, 0};
+  for (uint32_t b = 0; b < gridDim.x; ++b)
+    for (int a = 0; a < 3; ++a) t[a] += w->partial[b][a];
+  double c[3] = {cx, cy, cz};
+  for (int a = 0; a < 3; ++a) w->shift[a] = a < ndim ? (float)((t[a] / n - c[a]) * strength) : 0.0f;
+  w->done = 0;
+}
+
+__global__ void __launch_bounds__(256) center_apply(uint32_t n, int ndim, float* __restrict__ x,
+                                                    float* __restrict__ y, float* __restrict__ z,
+                                                    const CenterWork* __restrict__ w) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float sx = w->shift[0], sy = w->shift[1], sz = w->shift[2];
+  if (sx != 0.0f) x[i] -= sx;
+  if (ndim > 1 && sy != 0.0f) y[i] -= sy;
+  if (ndim > 2 && sz != 0.0f) z[i] -= sz;
+}
+
+void launch_center(hipStream_t s, uint32_t n, int ndim, float* const* pos, CenterWork* w, double cx,
+                   double cy, double cz, double strength) {
+  if (!n) return;
+  uint32_t blocks = ceil_div(n, 256);
+  uint32_t rb = blocks < CenterWork::MAX_BLOCKS ? blocks : CenterWork::MAX_BLOCKS;
+  hipLaunchKernelGGL(center_reduce, dim3(rb), dim3(256), 0, s, n, ndim, pos[0], pos[1], pos[2], w,
+                     cx, cy, cz, strength);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(center_apply, dim3(blocks), dim3(256), 0, s, n, ndim, pos[0], pos[1], pos[2],
+                     (const CenterWork*)w);
+  FG_LAUNCH_CHECK();
+}
+
+// ---- radial / position --------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) radial_kernel(uint32_t n, int ndim, const float* __restrict__ x,
+                                                     const float* __restrict__ y,
+                                                     const float* __restrict__ z,
+                                                     float* __restrict__ vx, float* __restrict__ vy,
+                                                     float* __restrict__ vz,
+                                                     const float* __restrict__ radius,
+                                                     const float* __restrict__ weight, float cx,
+                                                     float cy, float cz,
+                                                     StepParams step) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float dx = x[i] - cx, dy = ndim > 1 ? y[i] - cy : 0.f, dz = ndim > 2 ? z[i] - cz : 0.f;
+  if (dx == 0.f) dx = 1e-6f;
+  if (ndim > 1 && dy == 0.f) dy = 1e-6f;
+  if (ndim > 2 && dz == 0.f) dz = 1e-6f;
+  float r = sqrtf(dx * dx + dy * dy + dz * dz);
+  float k = (radius[i] - r) * weight[i] * step.alpha / r;
+  vx[i] += dx * k;
+  if (ndim > 1) vy[i] += dy * k;
+  if (ndim > 2) vz[i] += dz * k;
+}
+
+void launch_radial(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
+                   const float* radius, const float* weight, float cx, float cy, float cz,
+                   StepParams step) {
+  if (!n) return;
+  hipLaunchKernelGGL(radial_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, ndim, pos[0], pos[1],
+                     pos[2], vel[0], vel[1], vel[2], radius, weight, cx, cy, cz, step);
+  FG_LAUNCH_CHECK();
+}
+
+__global__ void __launch_bounds__(256) position_kernel(uint32_t n, const float* __restrict__ p,
+                                                       float* __restrict__ v,
+                                                       const float* __restrict__ goal,
+                                                       const float* __restrict__ weight,
+                                                       StepParams step) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  v[i] += (goal[i] - p[i]) * weight[i] * step.alpha;
+}
+
+void launch_position(hipStream_t s, uint32_t n, const float* p, float* v, const float* goal,
+                     const float* weight, StepParams step) {
+  if (!n) return;
+  hipLaunchKernelGGL(position_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, p, v, goal, weight,
+                     step);
+  FG_LAUNCH_CHECK();
+}
+
+// ---- exact many-body ----------------------------------------------------------------
+
+constexpr int PAIR_TILE = 256;
+
+template <int D>
+__global__ void __launch_bounds__(PAIR_TILE) manybody_pairs(uint32_t n, const float* __restrict__ x,
+                                                            const float* __restrict__ y,
+                                                            const float* __restrict__ z,
+                                                            const float* __restrict__ strength,
+                                                            float* __restrict__ vx,
+                                                            float* __restrict__ vy,
+                                                            float* __restrict__ vz, float dmin2,
+                                                            float dmax2,
+                                                            StepParams step) {
+  __shared__ float4 tile[PAIR_TILE];
+  const uint32_t i = blockIdx.x * PAIR_TILE + threadIdx.x;
+  const bool live = i < n;
+  const float alpha = step.alpha;
+  const uint32_t sn = step.step, seed = step.seed;
+  float mx = live ? x[i] : 0.f, my = live && D > 1 ? y[i] : 0.f, mz = live && D > 2 ? z[i] : 0.f;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  for (uint32_t base = 0; base < n; base += PAIR_TILE) {
+    uint32_t j = base + threadIdx.x;
+    tile[threadIdx.x] = j < n ? make_float4(x[j], D > 1 ? y[j] : 0.f, D > 2 ? z[j] : 0.f, strength[j])
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    uint32_t count = n - base < (uint32_t)PAIR_TILE ? n - base : (uint32_t)PAIR_TILE;
+    for (uint32_t k = 0; k < count; ++k) {
+      float4 q = tile[k];
+      float dx = q.x - mx, dy = q.y - my, dz = q.z - mz;
+      float l = dx * dx + dy * dy + dz * dz;
+      if (l >= dmax2 || base + k == i) continue;
+      if (l == 0.f) {
+        coincident(i, base + k, D, sn, seed, dx, dy, dz);
+        l = dx * dx + dy * dy + dz * dz;
+      }
+      if (l < dmin2) l = sqrtf(dmin2 * l);
+      float f = q.w * alpha * __builtin_amdgcn_rcpf(l);
+      ax += dx * f;
+      ay += dy * f;
+      az += dz * f;
+    }
+    __syncthreads();
+  }
+  if (live) {
+    vx[i] += ax;
+    if (D > 1) vy[i] += ay;
+    if (D > 2) vz[i] += az;
+  }
+}
+
+void launch_manybody_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos,
+                           const float* strength, float* const* vel, float dmin2, float dmax2,
+                           StepParams step) {
+  if (n < 2) return;
+  dim3 g(ceil_div(n, PAIR_TILE)), b(PAIR_TILE);
+  if (ndim == 1)
+    hipLaunchKernelGGL(manybody_pairs<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], strength, vel[0], vel[1], vel[2], dmin2, dmax2, step);
+  else if (ndim == 2)
+    hipLaunchKernelGGL(manybody_pairs<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], strength, vel[0], vel[1], vel[2], dmin2, dmax2, step);
+  else
+    hipLaunchKernelGGL(manybody_pairs<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], strength, vel[0], vel[1], vel[2], dmin2, dmax2, step);
+  FG_LAUNCH_CHECK();
+}
+
+// ---- collide, all pairs -------------------------------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(PAIR_TILE) collide_pairs(uint32_t n, const float* __restrict__ x,
+                                                           const float* __restrict__ y,
+                                                           const float* __restrict__ z,
+                                                           const float* __restrict__ vx,
+                                                           const float* __restrict__ vy,
+                                                           const float* __restrict__ vz,
+                                                           float* __restrict__ ox,
+                                                           float* __restrict__ oy,
+                                                           float* __restrict__ oz,
+                                                           const float* __restrict__ radius,
+                                                           float strength,
+                                                           StepParams step) {
+  __shared__ float4 tile[PAIR_TILE];
+  const uint32_t i = blockIdx.x * PAIR_TILE + threadIdx.x;
+  const bool live = i < n;
+  const uint32_t sn = step.step, seed = step.seed;
+  float mx = live ? x[i] + vx[i] : 0.f;
+  float my = live && D > 1 ? y[i] + vy[i] : 0.f;
+  float mz = live && D > 2 ? z[i] + vz[i] : 0.f;
+  float ri = live ? radius[i] : 0.f, ri2 = ri * ri;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  for (uint32_t base = 0; base < n; base += PAIR_TILE) {
+    uint32_t j = base + threadIdx.x;
+    tile[threadIdx.x] = j < n ? make_float4(x[j] + vx[j], D > 1 ? y[j] + vy[j] : 0.f,
+                                            D > 2 ? z[j] + vz[j] : 0.f, radius[j])
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    uint32_t count = n - base < (uint32_t)PAIR_TILE ? n - base : (uint32_t)PAIR_TILE;
+    for (uint32_t k = 0; k < count; ++k) {
+      float4 q = tile[k];
+      float rj = q.w, r = ri + rj;
+      float dx = mx - q.x, dy = my - q.y, dz = mz - q.z;
+      float l = dx * dx + dy * dy + dz * dz;
+      if (l >= r * r || base + k == i) continue;
+      if (l == 0.f) {
+        coincident(i, base + k, D, sn, seed, dx, dy, dz);
+        l = dx * dx + dy * dy + dz * dz;
+      }
+      float sl = sqrtf(l), rj2 = rj * rj;
+      float f = (r - sl) / sl * strength * rj2 / (ri2 + rj2);
+      ax += dx * f;
+      ay += dy * f;
+      az += dz * f;
+    }
+    __syncthreads();
+  }
+  if (live) {
+    ox[i] = vx[i] + ax;
+    if (D > 1) oy[i] = vy[i] + ay;
+    if (D > 2) oz[i] = vz[i] + az;
+  }
+}
+
+void launch_collide_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos,
+                          const float* const* vin, float* const* vout, const float* radius,
+                          float strength, StepParams step) {
+  if (!n) return;
+  dim3 g(ceil_div(n, PAIR_TILE)), b(PAIR_TILE);
+  if (ndim == 1)
+    hipLaunchKernelGGL(collide_pairs<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], radius, strength, step);
+  else if (ndim == 2)
+    hipLaunchKernelGGL(collide_pairs<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], radius, strength, step);
+  else
+    hipLaunchKernelGGL(collide_pairs<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], radius, strength, step);
+  FG_LAUNCH_CHECK();
+}
+
+// ---- collide, grid ------------------------------------------------------------------
+
+__device__ inline uint32_t spread3_3(uint32_t v) {  // 2 bits -> every third bit
+  v &= 3u;
+  return (v & 1u) | ((v & 2u) << 2);
+}
+__device__ inline uint32_t spread2_3(uint32_t v) {  // 3 bits -> every other bit
+  v &= 7u;
+  return (v & 1u) | ((v & 2u) << 1) | ((v & 4u) << 2);
+}
+
+// Slot of a cell: blocks of 4^3 cells (8^2 in 2D, 64 in 1D) take 64 consecutive slots, the
+// cells of a block in Morton order, from an offset hashed from the block's coordinates.
+template <int D>
+__device__ inline uint32_t grid_slot(int cx, int cy, int cz, uint32_t mask) {
+  uint32_t ux = (uint32_t)cx, uy = (uint32_t)cy, uz = (uint32_t)cz;
+  uint32_t bx, by, bz, m;
+  if (D == 3) {
+    bx = ux >> 2, by = uy >> 2, bz = uz >> 2;
+    m = (spread3_3(uz) << 2) | (spread3_3(uy) << 1) | spread3_3(ux);
+  } else if (D == 2) {
+    bx = ux >> 3, by = uy >> 3, bz = 0;
+    m = (spread2_3(uy) << 1) | spread2_3(ux);
+  } else {
+    bx = ux >> 6, by = 0, bz = 0;
+    m = ux & 63u;
+  }
+  uint32_t h = pcg_hash((bx * 73856093u) ^ (by * 19349663u) ^ (bz * 83492791u));
+  return ((h << 6) + m) & mask;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) grid_keys(uint32_t n, const float* __restrict__ x,
+                                                 const float* __restrict__ y,
+                                                 const float* __restrict__ z,
+                                                 const float* __restrict__ vx,
+                                                 const float* __restrict__ vy,
+                                                 const float* __restrict__ vz, float inv,
+                                                 uint32_t mask, uint32_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float sx = x[i] + vx[i];
+  float sy = D > 1 ? y[i] + vy[i] : 0.f;
+  float sz = D > 2 ? z[i] + vz[i] : 0.f;
+  keys[i] = grid_slot<D>((int)floorf(sx * inv), D > 1 ? (int)floorf(sy * inv) : 0,
+                         D > 2 ? (int)floorf(sz * inv) : 0, mask);
+  vals[i] = i;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) grid_begin(uint32_t n, const uint32_t* __restrict__ skeys,
+                                                  const uint32_t* __restrict__ svals,
+                                                  const float* __restrict__ x,
+                                                  const float* __restrict__ y,
+                                                  const float* __restrict__ z,
+                                                  const float* __restrict__ vx,
+                                                  const float* __restrict__ vy,
+                                                  const float* __restrict__ vz,
+                                                  const float* __restrict__ radius,
+                                                  uint32_t* __restrict__ starts,
+                                                  float4* __restrict__ sorted) {
+  uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  uint32_t key = skeys[s], i = svals[s];
+  if (s == 0 || skeys[s - 1] != key) starts[key] = s;
+  sorted[s] = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f,
+                          radius[i]);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) grid_query(uint32_t n, const uint32_t* __restrict__ skeys,
+                                                  const uint32_t* __restrict__ svals,
+                                                  const uint32_t* __restrict__ starts,
+                                                  const float4* __restrict__ sorted,
+                                                  float* __restrict__ vx, float* __restrict__ vy,
+                                                  float* __restrict__ vz, float inv,
+                                                  uint32_t mask, float strength,
+                                                  StepParams step) {
+  uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t sn = step.step, seed = step.seed;
+  float4 me = sorted[s];
+  uint32_t i = svals[s];
+  float ri2 = me.w * me.w;
+  int hx = (int)floorf(me.x * inv);
+  int hy = D > 1 ? (int)floorf(me.y * inv) : 0;
+  int hz = D > 2 ? (int)floorf(me.z * inv) : 0;
+  constexpr int RY = D > 1 ? 1 : 0, RZ = D > 2 ? 1 : 0;
+  uint32_t seen[27];
+  int count = 0;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  for (int dz = -RZ; dz <= RZ; ++dz)
+    for (int dy = -RY; dy <= RY; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        uint32_t key = grid_slot<D>(hx + dx, hy + dy, hz + dz, mask);
+        bool again = false;
+        for (int k = 0; k < count; ++k) again |= seen[k] == key;
+        if (again) continue;
+        seen[count++] = key;
+        uint32_t start = starts[key];
+        if (start >= n || skeys[start] != key || (start > 0 && skeys[start - 1] == key)) continue;
+        for (uint32_t t = start; t < n && skeys[t] == key; ++t) {
+          if (t == s) continue;
+          float4 q = sorted[t];
+          float rj = q.w, r = me.w + rj;
+          float ex = me.x - q.x, ey = me.y - q.y, ez = me.z - q.z;
+          float l = ex * ex + ey * ey + ez * ez;
+          if (l >= r * r) continue;
+          if (l == 0.f) {
+            coincident(i, svals[t], D, sn, seed, ex, ey, ez);
+            l = ex * ex + ey * ey + ez * ez;
+          }
+          float len = sqrtf(l), rj2 = rj * rj;
+          float f = (r - len) / len * strength * rj2 / (ri2 + rj2);
+          ax += ex * f;
+          ay += ey * f;
+          az += ez * f;
+        }
+      }
+  vx[i] += ax;
+  if (D > 1) vy[i] += ay;
+  if (D > 2) vz[i] += az;
+}
+
+void GridWork::reserve(uint32_t n_) {
+  uint32_t b = 8;
+  while ((1ull << b) < 2ull * n_) ++b;
+  if (n_ <= cap && bits == b && keys) return;
+  release();
+  uint32_t m = n_ > 1 ? n_ : 1;
+  bits = b;
+  FG_HIP(hipMalloc(&keys, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&starts, sizeof(uint32_t) << bits));
+  FG_HIP(hipMemset(starts, 0xff, sizeof(uint32_t) << bits));
+  FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
+  sort.reserve(m, sizeof(uint32_t));
+  cap = n_;
+}
+
+void GridWork::release() {
+  if (keys) (void)hipFree(keys);
+  if (vals) (void)hipFree(vals);
+  if (starts) (void)hipFree(starts);
+  if (sorted) (void)hipFree(sorted);
+  keys = vals = starts = nullptr;
+  sorted = nullptr;
+  sort.release();
+  cap = 0;
+}
+
+template <int D>
+void grid_run(hipStream_t s, GridWork& g, uint32_t n, float* const* pos, float* const* vel,
+              const float* radius, float inv, uint32_t mask, float strength, int radix_bits,
+              StepParams step) {
+  dim3 gr(ceil_div(n, 256)), b(256);
+  hipLaunchKernelGGL(grid_keys<D>, gr, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2],
+                     inv, mask, g.keys, g.vals);
+  FG_LAUNCH_CHECK();
+  uint32_t *sk, *sv;
+  radix_sort_u32(s, g.sort, radix_bits, g.keys, g.vals, n, 0, g.bits, &sk, &sv);
+  hipLaunchKernelGGL(grid_begin<D>, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv, pos[0],
+                     pos[1], pos[2], vel[0], vel[1], vel[2], radius, g.starts, g.sorted);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(grid_query<D>, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
+                     (const uint32_t*)g.starts, (const float4*)g.sorted, vel[0], vel[1], vel[2], inv,
+                     mask, strength, step);
+  FG_LAUNCH_CHECK();
+}
+
+void launch_collide_grid(hipStream_t s, GridWork& g, uint32_t n, int ndim, float* const* pos,
+                         float* const* vel, const float* radius, float rmax, float strength,
+                         int radix_bits, StepParams step) {
+  if (!n || !(rmax > 0)) return;
+  g.reserve(n);
+  float inv = 1.0f / (2.0f * rmax);
+  uint32_t mask = (uint32_t)((1ull << g.bits) - 1ull);
+  if (ndim == 1) grid_run<1>(s, g, n, pos, vel, radius, inv, mask, strength, radix_bits, step);
+  else if (ndim == 2) grid_run<2>(s, g, n, pos, vel, radius, inv, mask, strength, radix_bits, step);
+  else grid_run<3>(s, g, n, pos, vel, radius, inv, mask, strength, radix_bits, step);
+}
+
+}  // namespace fg

@@ -1,0 +1,37 @@
+// sort.h -- device radix sort of (key, u32 value) pairs (see sort.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fg {
+
+// Scratch of one sorter: the alternate key/value buffers, digit histograms, tile tickets,
+// look-back words and the epoch counter that keeps stale look-back words from matching.
+struct SortWork {
+  void* keys_alt = nullptr;
+  uint32_t* vals_alt = nullptr;
+  uint32_t* ghist = nullptr;
+  uint64_t* lookback = nullptr;
+  uint32_t* tickets = nullptr;
+  uint32_t* epoch = nullptr;
+  uint32_t* errors = nullptr;  // [0]: look-back waits that hit their bound
+  uint32_t cap_n = 0;
+  size_t cap_key_bytes = 0;
+  uint64_t lookback_words = 0;
+
+  void reserve(uint32_t n, size_t key_bytes);
+  void release();
+  ~SortWork() { release(); }
+};
+
+// Sorts bits [first_bit, first_bit + bits) of the keys, 8 or 11 bits per pass.  The sorted
+// pairs end up in (keys, vals) or in the work's alternate buffers: *out_* says which.
+void radix_sort_u32(hipStream_t stream, SortWork& w, int radix_bits, uint32_t* keys,
+                    uint32_t* vals, uint32_t n, uint32_t first_bit, uint32_t bits,
+                    uint32_t** out_keys, uint32_t** out_vals);
+void radix_sort_u64(hipStream_t stream, SortWork& w, int radix_bits, uint64_t* keys,
+                    uint32_t* vals, uint32_t n, uint32_t first_bit, uint32_t bits,
+                    uint64_t** out_keys, uint32_t** out_vals);
+
+}  // namespace fg

@@ -1,0 +1,71 @@
+// tree.h -- the many-body Barnes-Hut tree of one simulation (see tree.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+#include "sort.h"
+
+namespace fg {
+
+constexpr uint32_t REF_POINT = 0x80000000u;  // a child reference that is a sorted point
+constexpr uint32_t REF_NONE = 0xffffffffu;
+
+// What the walk reads of a binary node: its centre of charge (FP64), its strength at the
+// depth of its deepest cell, that depth, and its children in the d3 tree.
+struct alignas(64) NodeRec {
+  double cx, cy, cz;
+  float value;
+  uint32_t depth;
+  uint32_t kids[8];
+};
+
+// Sums the bottom-up reduction carries per node (FP64).
+struct NodeSum {
+  double a;       // sum of strength * c^leafDepth
+  double w;       // sum of the d3-children weights inside this node's cell group
+  double gx, gy, gz;  // weight * centre, same children
+};
+
+// Cube of the tree (d3's cover), and the running box used to find it.
+struct CubeInfo {
+  float origin[3];
+  float size;
+  int32_t shift;
+  uint32_t done;        // blocks of the bbox kernel that have finished
+  uint32_t box[6];      // order-preserving min x, y, z, max x, y, z
+};
+
+struct TreeBuffers {
+  uint32_t cap = 0;
+  uint64_t* keys = nullptr;     // compact Morton code per node, then per sorted point
+  uint32_t* vals = nullptr;     // node index, then the sorted order
+  uint64_t* skeys = nullptr;    // the sorted codes (points into keys or the sorter's buffer)
+  uint32_t* order = nullptr;    // node index at each sorted position
+  float4* leaves = nullptr;     // sorted: x, y, z, strength
+  uint32_t* leaf_depth = nullptr;
+  uint32_t* leaf_parent = nullptr;
+  uint4* links = nullptr;       // per binary node: left, right, depth, parent
+  uint32_t* flags = nullptr;
+  NodeRec* rec = nullptr;
+  NodeSum* sums = nullptr;
+  CubeInfo* cube = nullptr;
+  uint32_t* counters = nullptr; // [0] interactions lo, [1] hi, [2] opened lo, [3] hi, [4] walk errors
+  SortWork sort;
+
+  void reserve(uint32_t n);
+  void release();
+  ~TreeBuffers() { release(); }
+};
+
+struct TreeParams {
+  double theta2, dmin2, dmax2;
+};
+
+// Build the tree over the current positions and add the many-body force to the velocities.
+void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x,
+                   const float* y, const float* z, const float* strength, float* vx, float* vy,
+                   float* vz, const TreeParams& p, StepParams step, int radix_bits);
+
+}  // namespace fg

@@ -1,0 +1,601 @@
+// tree.hip -- the many-body force with a Barnes-Hut tree, on CDNA4.
+//
+// The tree is the one the reference GPU path builds (src/gpu/passes/barnesHut.js:6-47):
+// d3's cube (cover), LEVELS levels of Morton code in two halves, points sorted on the
+// whole code (ties by node index), a Karras 2012 binary radix tree over the sorted codes
+// whose nodes each stand for a chain of d3 cells, and d3's children of every node
+// (collapse).  Codes, order, topology and children come out bit-identical to that
+// algorithm.  The kernels are this backend's own:
+//
+//   tree_bbox     grid-stride min/max with order-preserving u32 atomics; the last block
+//                 to finish turns the box into the cube (one launch instead of two)
+//   tree_keys     exact cellOf -> compact code (high half << HALF | low half) per node
+//   (sort)        LSD radix sort of the 2*HALF-bit codes with the node index as value
+//   tree_leaves   sorted points' (x, y, z, strength) and leaf depths
+//   tree_build    Karras ranges and splits; parent links for the climb
+//   tree_reduce   one thread per point climbs to the root; the second of two siblings to
+//                 arrive at a node (atomic flag) combines them -- left, then right, so the
+//                 sums do not depend on timing -- in FP64, with d3's |value| weighting
+//                 (manyBody.js:42-71), and lists the node's d3 children (collapse)
+//   tree_walk     one wave per 64 consecutive sorted points: the wave walks the union of
+//                 its points' traversals from a stack in LDS of (node, lane mask); every
+//                 lane takes d3's own decision at every cell (barnesHut.js:719-746), so
+//                 each point gets exactly its d3 traversal; node records are read with
+//                 wave-uniform loads.
+#include <math.h>
+
+#include "tree.h"
+
+namespace fg {
+
+namespace {
+
+constexpr int LEVELS_OF[4] = {0, 24, 24, 20};
+
+template <int D>
+struct Dims {
+  static constexpr int LEVELS = LEVELS_OF[D];
+  static constexpr int HALF = LEVELS / 2 * D;  // bits of each half of the code
+  static constexpr int BITS = 2 * HALF;
+};
+
+// c^k for c = sqrt(4 / 2^D), exact up to the rounding of sqrt(2) (barnesHut.js:83-89).
+template <int D>
+__host__ __device__ inline double damp(int k) {
+  if (D == 2) return 1.0;
+  int t = D == 3 ? -k : k;  // exponent in halves
+  int q = t >> 1;           // floor(t / 2)
+  return ldexp((t & 1) ? 1.4142135623730951 : 1.0, q);
+}
+
+__device__ inline uint32_t spread3(uint32_t v) {
+  uint32_t x = v & 0x3ffu;
+  x = (x | (x << 16u)) & 0x030000ffu;
+  x = (x | (x << 8u)) & 0x0300f00fu;
+  x = (x | (x << 4u)) & 0x030c30c3u;
+  x = (x | (x << 2u)) & 0x09249249u;
+  return x;
+}
+
+__device__ inline uint32_t spread2(uint32_t v) {
+  uint32_t x = v & 0xffffu;
+  x = (x | (x << 8u)) & 0x00ff00ffu;
+  x = (x | (x << 4u)) & 0x0f0f0f0fu;
+  x = (x | (x << 2u)) & 0x33333333u;
+  x = (x | (x << 1u)) & 0x55555555u;
+  return x;
+}
+
+// Morton code of one half, right-aligned (the reference left-aligns it in a u32).
+template <int D>
+__device__ inline uint32_t morton(uint32_t qx, uint32_t qy, uint32_t qz) {
+  if (D == 3) return (spread3(qz) << 2) | (spread3(qy) << 1) | spread3(qx);
+  if (D == 2) return (spread2(qy) << 1) | spread2(qx);
+  return qx;
+}
+
+// floor((x - origin) / cell) without rounding (barnesHut.js:343-350).
+__device__ inline float cell_of(float x, float origin, int k) {
+  if (k >= 0) return floorf(ldexpf(x, k)) - ldexpf(origin, k);
+  return floorf(ldexpf(floorf(x) - origin, k));
+}
+
+// ---- bbox + cube --------------------------------------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
+                                                 const float* __restrict__ y,
+                                                 const float* __restrict__ z, uint32_t n,
+                                                 CubeInfo* __restrict__ cube) {
+  float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) lo[a] = fminf(lo[a], p[a]), hi[a] = fmaxf(hi[a], p[a]);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], o, WAVE));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], o, WAVE));
+    }
+  }
+  __shared__ uint32_t s_last;
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&cube->box[a], float_ordered(lo[a]));
+      atomicMax(&cube->box[3 + a], float_ordered(hi[a]));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t done = __hip_atomic_fetch_add(&cube->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = done + 1 == gridDim.x;
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  // the last block: d3's cover() on the box (barnesHut.js:169-181), then reset the box
+  float blo[3], bhi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    blo[a] = float_unordered(__hip_atomic_load(&cube->box[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    bhi[a] = float_unordered(__hip_atomic_load(&cube->box[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  float origin[3] = {floorf(blo[0]), floorf(blo[1]), floorf(blo[2])};
+  float size = 1.0f;
+  int log2size = 0;
+  for (int k = 0; k < 128; ++k) {
+    if (bhi[0] < origin[0] + size && bhi[1] < origin[1] + size && bhi[2] < origin[2] + size) break;
+    size *= 2.0f;
+    log2size += 1;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) cube->origin[a] = origin[a];
+  cube->size = size;
+  cube->shift = Dims<D>::LEVELS - log2size;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    cube->box[a] = 0xffffffffu;
+    cube->box[3 + a] = 0u;
+  }
+  cube->done = 0;
+}
+
+// ---- keys ---------------------------------------------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(256) tree_keys(const float* __restrict__ x,
+                                                 const float* __restrict__ y,
+                                                 const float* __restrict__ z, uint32_t n,
+                                                 const CubeInfo* __restrict__ cube,
+                                                 uint64_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  constexpr int L = Dims<D>::LEVELS, H = L / 2;
+  const int k = cube->shift;
+  const float top = (float)((1u << L) - 1u);
+  float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
+  uint32_t q[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float c = cell_of(p[a], cube->origin[a], k);
+    c = fminf(fmaxf(c, 0.0f), top);
+    q[a] = (uint32_t)c;
+  }
+  constexpr uint32_t lm = (1u << H) - 1u;
+  uint32_t hi = morton<D>(q[0] >> H, q[1] >> H, q[2] >> H);
+  uint32_t lo = morton<D>(q[0] & lm, q[1] & lm, q[2] & lm);
+  keys[i] = ((uint64_t)hi << Dims<D>::HALF) | lo;
+  vals[i] = i;
+}
+
+// Length of the common prefix of the codes at sorted positions a and b, counted as the
+// reference counts it: code bits, then 64 + common bits of the positions for equal codes
+// (barnesHut.js:104-117); -1 when b is out of range.
+template <int D>
+__device__ inline int delta(const uint64_t* __restrict__ k, int n, int a, int b) {
+  if (b < 0 || b >= n) return -1;
+  uint64_t ka = k[a], kb = k[b];
+  if (ka != kb) return __clzll(ka ^ kb) - (64 - Dims<D>::BITS);
+  return 64 + __clz((uint32_t)a ^ (uint32_t)b);
+}
+
+template <int D>
+__device__ inline uint32_t depth_of(int prefix) {
+  uint32_t d = (uint32_t)prefix / (uint32_t)D;
+  return d < (uint32_t)Dims<D>::LEVELS ? d : (uint32_t)Dims<D>::LEVELS;
+}
+
+// ---- leaves -------------------------------------------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(256) tree_leaves(const uint64_t* __restrict__ skeys,
+                                                   const uint32_t* __restrict__ order,
+                                                   const float* __restrict__ x,
+                                                   const float* __restrict__ y,
+                                                   const float* __restrict__ z,
+                                                   const float* __restrict__ strength, uint32_t n,
+                                                   float4* __restrict__ leaves,
+                                                   uint32_t* __restrict__ leaf_depth,
+                                                   uint32_t* __restrict__ flags) {
+  uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  uint32_t v = order[s];
+  leaves[s] = make_float4(x[v], D > 1 ? y[v] : 0.0f, D > 2 ? z[v] : 0.0f, strength[v]);
+  uint32_t depth = 0;
+  if (n > 1) {
+    int a = delta<D>(skeys, (int)n, (int)s, (int)s - 1), b = delta<D>(skeys, (int)n, (int)s, (int)s + 1);
+    depth = depth_of<D>(a > b ? a : b) + 1;
+  }
+  leaf_depth[s] = depth;
+  if (s + 1 < n) flags[s] = 0;
+}
+
+// ---- Karras build -------------------------------------------------------------------
+
+template <int D>
+__global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ skeys, uint32_t n,
+                                                  uint4* __restrict__ links,
+                                                  uint32_t* __restrict__ leaf_parent) {
+  int i = (int)(blockIdx.x * 256 + threadIdx.x);
+  const int N = (int)n;
+  if (i >= N - 1) return;
+  int before = delta<D>(skeys, N, i, i - 1), after = delta<D>(skeys, N, i, i + 1);
+  int d = after > before ? 1 : -1;
+  int floor_prefix = before < after ? before : after;
+  int span = 2;
+  while (span < (1 << 30) && delta<D>(skeys, N, i, i + span * d) > floor_prefix) span *= 2;
+  int len = 0;
+  for (int t = span >> 1; t >= 1; t >>= 1)
+    if (delta<D>(skeys, N, i, i + (len + t) * d) > floor_prefix) len += t;
+  int j = i + len * d;
+  int prefix = delta<D>(skeys, N, i, j);
+  int s = 0, stride = len;
+  for (;;) {
+    stride = (stride + 1) >> 1;
+    if (delta<D>(skeys, N, i, i + (s + stride) * d) > prefix) s += stride;
+    if (stride <= 1) break;
+  }
+  int split = i + s * d + (d < 0 ? d : 0);
+  int first = i < j ? i : j, last = i < j ? j : i;
+  uint32_t left = (uint32_t)split | (first == split ? REF_POINT : 0u);
+  uint32_t right = (uint32_t)(split + 1) | (last == split + 1 ? REF_POINT : 0u);
+  links[i].x = left;
+  links[i].y = right;
+  links[i].z = depth_of<D>(prefix);
+  if (left & REF_POINT) leaf_parent[split] = (uint32_t)i;
+  else links[split].w = (uint32_t)i;
+  if (right & REF_POINT) leaf_parent[split + 1] = (uint32_t)i;
+  else links[split + 1].w = (uint32_t)i;
+  if (i == 0) links[0].w = REF_NONE;
+}
+
+// ---- bottom-up reduction + collapse ---------------------------------------------------
+
+template <int D>
+__device__ inline void child_sum(uint32_t k, uint32_t parent_depth, const float4* leaves,
+                                 const uint4* links, const NodeRec* rec, const NodeSum* sums,
+                                 NodeSum& out) {
+  if (k & REF_POINT) {
+    // a point is a d3 child one level below its binary parent's cell
+    float4 q = leaves[k & ~REF_POINT];
+    double sv = (double)q.w;
+    out.a += sv * damp<D>((int)parent_depth + 1);
+    double w = fabs(sv);
+    out.w += w;
+    out.gx += w * (double)q.x;
+    if (D > 1) out.gy += w * (double)q.y;
+    if (D > 2) out.gz += w * (double)q.z;
+    return;
+  }
+  const NodeSum& c = sums[k];
+  uint32_t kd = links[k].z;
+  out.a += c.a;
+  if (kd > parent_depth) {
+    // a deeper cell: a d3 child, weighed by |its strength one level below this cell|
+    double w = fabs(c.a * damp<D>(-(int)(parent_depth + 1)));
+    out.w += w;
+    out.gx += w * rec[k].cx;
+    if (D > 1) out.gy += w * rec[k].cy;
+    if (D > 2) out.gz += w * rec[k].cz;
+  } else {
+    // the same cell: its children are this cell's children
+    out.w += c.w;
+    out.gx += c.gx;
+    if (D > 1) out.gy += c.gy;
+    if (D > 2) out.gz += c.gz;
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
+                                                   const uint32_t* __restrict__ leaf_parent,
+                                                   const uint4* __restrict__ links,
+                                                   uint32_t* flags, NodeRec* rec, NodeSum* sums) {
+  uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n || n < 2) return;
+  uint32_t p = leaf_parent[s];
+  for (uint32_t guard = 0; guard < n && p != REF_NONE; ++guard) {
+    uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;  // the sibling is still on its way: it will carry on
+    uint4 L = links[p];
+    NodeSum sum = {0, 0, 0, 0, 0};
+    child_sum<D>(L.x, L.z, leaves, links, rec, sums, sum);
+    child_sum<D>(L.y, L.z, leaves, links, rec, sums, sum);
+    NodeRec r;
+    if (sum.w > 0) {
+      r.cx = sum.gx / sum.w;
+      r.cy = D > 1 ? sum.gy / sum.w : 0.0;
+      r.cz = D > 2 ? sum.gz / sum.w : 0.0;
+    } else {
+      r.cx = r.cy = r.cz = 0.0;
+    }
+    double value = sum.a * damp<D>(-(int)L.z);
+    // d3 skips a cell whose strengths add up to exactly 0; sums in another order leave a
+    // residue of rounding, so a cancellation to 2^-40 of the cell's weight counts as 0
+    double scale = sum.w * damp<D>((int)L.z + 1);
+    if (fabs(sum.a) <= fabs(scale) * 0x1p-40 || !(sum.w > 0)) value = 0.0;
+    r.value = (float)value;
+    r.depth = L.z;
+    // collapse: look through binary children in the same d3 cell (barnesHut.js:641-671)
+    uint32_t list[8] = {L.x, L.y, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE};
+    int count = 2;
+#pragma unroll
+    for (int level = 1; level < D; ++level) {
+      uint32_t next[8] = {REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE};
+      int m = 0;
+      for (int c = 0; c < count; ++c) {
+        uint32_t k = list[c];
+        if (!(k & REF_POINT) && links[k].z == L.z) {
+          uint4 b = links[k];
+          next[m] = b.x;
+          next[m + 1] = b.y;
+          m += 2;
+        } else {
+          next[m] = k;
+          m += 1;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) list[c] = next[c];
+      count = m;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) r.kids[c] = list[c];
+    rec[p] = r;
+    sums[p] = sum;
+    p = L.w;
+  }
+}
+
+// ---- walk ---------------------------------------------------------------------------
+
+constexpr int WALK_WAVES = 4;
+constexpr int STACK = 256;
+
+struct WalkArgs {
+  double theta2, dmin2, dmax2;
+  uint32_t n;
+};
+
+template <int D>
+__global__ void __launch_bounds__(WALK_WAVES * WAVE)
+    tree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
+              const NodeRec* __restrict__ rec, const CubeInfo* __restrict__ cube,
+              float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
+              uint32_t* __restrict__ counters, StepParams step, WalkArgs A) {
+  __shared__ uint32_t s_node[WALK_WAVES][STACK];
+  __shared__ uint64_t s_mask[WALK_WAVES][STACK];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const uint32_t n = A.n;
+  const uint32_t s = (blockIdx.x * WALK_WAVES + wave) * WAVE + lane;
+  const bool live = s < n;
+  if (__ballot(live) == 0ull) return;
+  const float alpha = step.alpha;
+  const uint32_t step_no = step.step, seed = step.seed;
+  const double size = (double)cube->size;
+  const double theta2 = A.theta2, dmin2 = A.dmin2, dmax2 = A.dmax2;
+  const float dmin2f = (float)dmin2, dmax2f = (float)fmin(dmax2, 3.0e38);
+  float4 me4 = live ? leaves[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const double mx = me4.x, my = me4.y, mz = me4.z;
+  const uint32_t self_node = live ? order[s] : 0u;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  uint32_t inter = 0;
+
+  // One cell of node `cur`, for this lane: d3's decision at every depth of its chain.
+  // Returns true when the lane is done with the node (approximated or skipped).
+  auto visit = [&](uint32_t cur, uint32_t top, const NodeRec& r) -> bool {
+    if (top > r.depth) return false;  // points sharing the smallest cell: open
+    if (r.value == 0.0f) return true;  // d3 skips cells with no strength
+    double dx = r.cx - mx, dy = D > 1 ? r.cy - my : 0.0, dz = D > 2 ? r.cz - mz : 0.0;
+    double l = dx * dx + dy * dy + dz * dz;
+    double w = ldexp(size, -(int)top);
+    uint32_t Dd = top;
+    while (Dd <= r.depth && !(w * w < theta2 * l)) {
+      w *= 0.5;
+      ++Dd;
+    }
+    if (Dd > r.depth) return false;
+    if (l >= dmax2) return true;
+    float fx = (float)dx, fy = (float)dy, fz = (float)dz, fl = (float)l;
+    if (l == 0.0) {
+      coincident(self_node, n + cur, D, step_no, seed, fx, fy, fz);
+      fl = fx * fx + fy * fy + fz * fz;
+    }
+    if (fl < dmin2f) fl = sqrtf(dmin2f * fl);
+    float f = r.value * (float)damp<D>((int)(r.depth - Dd)) * alpha * __builtin_amdgcn_rcpf(fl);
+    ax += fx * f;
+    if (D > 1) ay += fy * f;
+    if (D > 2) az += fz * f;
+    ++inter;
+    return true;
+  };
+
+  uint32_t sp = 0;
+  uint64_t opened = 0;
+  uint32_t err = 0;
+  {
+    NodeRec r = rec[0];
+    bool done = live ? visit(0u, 0u, r) : true;
+    uint64_t open = __ballot(!done);
+    if (open) {
+      if (lane == 0) {
+        s_node[wave][0] = 0u;
+        s_mask[wave][0] = open;
+      }
+      sp = 1;
+    }
+  }
+  const uint32_t guard_max = 2u * n + 8u;
+  for (uint32_t guard = 0; sp > 0; ++guard) {
+    if (guard > guard_max) {
+      err = 1;
+      break;
+    }
+    --sp;
+    uint32_t e = __builtin_amdgcn_readfirstlane(s_node[wave][sp]);
+    uint64_t m = s_mask[wave][sp];
+    m = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
+        __builtin_amdgcn_readfirstlane((uint32_t)m);
+    const bool in = (m >> lane) & 1ull;
+    ++opened;
+    const NodeRec& pr = rec[e];
+    const uint32_t top = pr.depth + 1;
+#pragma unroll 1
+    for (int c = 0; c < (1 << D); ++c) {
+      uint32_t k = pr.kids[c];
+      if (k == REF_NONE) break;
+      if (k & REF_POINT) {
+        uint32_t j = k & ~REF_POINT;
+        float4 q = leaves[j];
+        if (in && j != s) {
+          float dx = q.x - me4.x, dy = D > 1 ? q.y - me4.y : 0.f, dz = D > 2 ? q.z - me4.z : 0.f;
+          float l = dx * dx + dy * dy + dz * dz;
+          if (l < dmax2f) {
+            if (l == 0.0f) {
+              coincident(self_node, order[j], D, step_no, seed, dx, dy, dz);
+              l = dx * dx + dy * dy + dz * dz;
+            }
+            if (l < dmin2f) l = sqrtf(dmin2f * l);
+            float f = q.w * alpha * __builtin_amdgcn_rcpf(l);
+            ax += dx * f;
+            if (D > 1) ay += dy * f;
+            if (D > 2) az += dz * f;
+            ++inter;
+          }
+        }
+        continue;
+      }
+      NodeRec r = rec[k];
+      bool done = in ? visit(k, top, r) : true;
+      uint64_t open = __ballot(!done);
+      if (open) {
+        if (sp >= STACK) {
+          err = 1;
+          continue;
+        }
+        if (lane == 0) {
+          s_node[wave][sp] = k;
+          s_mask[wave][sp] = open;
+        }
+        ++sp;
+      }
+    }
+  }
+  if (live) {
+    uint32_t o = self_node;
+    vx[o] += ax;
+    if (D > 1) vy[o] += ay;
+    if (D > 2) vz[o] += az;
+  }
+  // counters: one atomic per wave
+  uint32_t tot = inter;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, WAVE);
+  if (lane == 0) {
+    atomicAdd((unsigned long long*)&counters[0], (unsigned long long)tot);
+    atomicAdd((unsigned long long*)&counters[2], (unsigned long long)opened);
+    if (err) atomicAdd(&counters[4], 1u);
+  }
+}
+
+__global__ void tree_counters_clear(uint32_t* c) {
+  if (threadIdx.x < 8) c[threadIdx.x] = 0;
+}
+
+template <int D>
+void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float* y, const float* z,
+         const float* strength, float* vx, float* vy, float* vz, const TreeParams& p,
+         StepParams step, int radix_bits) {
+  const uint32_t blocks = ceil_div(n, 256);
+  hipLaunchKernelGGL(tree_counters_clear, dim3(1), dim3(64), 0, st, t.counters);
+  uint32_t bb = blocks < 512 ? blocks : 512;
+  hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL((tree_keys<D>), dim3(blocks), dim3(256), 0, st, x, y, z, n,
+                     (const CubeInfo*)t.cube, t.keys, t.vals);
+  FG_LAUNCH_CHECK();
+  uint64_t* sk;
+  uint32_t* sv;
+  radix_sort_u64(st, t.sort, radix_bits, t.keys, t.vals, n, 0, Dims<D>::BITS, &sk, &sv);
+  t.skeys = sk;
+  t.order = sv;
+  hipLaunchKernelGGL((tree_leaves<D>), dim3(blocks), dim3(256), 0, st, (const uint64_t*)sk,
+                     (const uint32_t*)sv, x, y, z, strength, n, t.leaves, t.leaf_depth, t.flags);
+  FG_LAUNCH_CHECK();
+  if (n < 2) return;
+  hipLaunchKernelGGL((tree_build<D>), dim3(ceil_div(n - 1, 256)), dim3(256), 0, st,
+                     (const uint64_t*)sk, n, t.links, t.leaf_parent);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL((tree_reduce<D>), dim3(blocks), dim3(256), 0, st, n,
+                     (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
+                     (const uint4*)t.links, t.flags, t.rec, t.sums);
+  FG_LAUNCH_CHECK();
+  WalkArgs A;
+  A.theta2 = p.theta2;
+  A.dmin2 = p.dmin2;
+  A.dmax2 = p.dmax2;
+  A.n = n;
+  hipLaunchKernelGGL((tree_walk<D>), dim3(ceil_div(n, WALK_WAVES * WAVE)), dim3(WALK_WAVES * WAVE),
+                     0, st, (const float4*)t.leaves, (const uint32_t*)sv, (const NodeRec*)t.rec,
+                     (const CubeInfo*)t.cube, vx, vy, vz, t.counters, step, A);
+  FG_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void TreeBuffers::reserve(uint32_t n) {
+  if (n <= cap && keys) return;
+  release();
+  uint32_t m = n > 2 ? n : 2;
+  FG_HIP(hipMalloc(&keys, sizeof(uint64_t) * m));
+  FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&leaves, sizeof(float4) * m));
+  FG_HIP(hipMalloc(&leaf_depth, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&leaf_parent, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&links, sizeof(uint4) * m));
+  FG_HIP(hipMalloc(&flags, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
+  FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
+  FG_HIP(hipMalloc(&cube, sizeof(CubeInfo)));
+  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 8));
+  CubeInfo c0 = {};
+  for (int a = 0; a < 3; ++a) c0.box[a] = 0xffffffffu, c0.box[3 + a] = 0u;
+  FG_HIP(hipMemcpy(cube, &c0, sizeof(c0), hipMemcpyHostToDevice));
+  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * 8));
+  sort.reserve(m, sizeof(uint64_t));
+  cap = n;
+}
+
+void TreeBuffers::release() {
+  void* ptrs[] = {keys, vals, leaves, leaf_depth, leaf_parent, links, flags, rec, sums, cube, counters};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  keys = nullptr;
+  vals = nullptr;
+  leaves = nullptr;
+  leaf_depth = leaf_parent = flags = counters = nullptr;
+  links = nullptr;
+  rec = nullptr;
+  sums = nullptr;
+  cube = nullptr;
+  skeys = nullptr;
+  order = nullptr;
+  sort.release();
+  cap = 0;
+}
+
+void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x,
+                   const float* y, const float* z, const float* strength, float* vx, float* vy,
+                   float* vz, const TreeParams& p, StepParams step, int radix_bits) {
+  if (n < 1) return;
+  t.reserve(n);
+  if (ndim == 1) run<1>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits);
+  else if (ndim == 2) run<2>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits);
+  else run<3>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits);
+}
+
+}  // namespace fg

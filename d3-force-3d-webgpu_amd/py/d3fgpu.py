@@ -1,0 +1,332 @@
+"""Python bindings of the MI355X physics backend's C ABI (include/d3fgpu.h), over ctypes.
+
+The same calls a ctypes-based host would make: one `Engine` wraps one fg_ctx (one
+simulation on one GPU) and mirrors the reference engine's methods (src/gpu/engine.js:
+setNodes, setForces, enqueue, readback, sync, buffers, fits, setSeed).  Loading fails
+loudly when lib/libd3fgpu.so has not been built: there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libd3fgpu.so")
+
+FG_OK = 0
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "unsupported",
+          -5: "bad state", -6: "no device"}
+
+CENTER, COLLIDE, LINK, MANYBODY, POSITION, RADIAL = 1, 2, 3, 4, 5, 6
+
+_d = C.POINTER(C.c_double)
+_u = C.POINTER(C.c_uint32)
+_f = C.POINTER(C.c_float)
+_u8 = C.POINTER(C.c_uint8)
+_i = C.POINTER(C.c_int32)
+
+
+class ForceDesc(C.Structure):
+    """fg_force_desc."""
+    _fields_ = [
+        ("type", C.c_int32), ("iterations", C.c_int32), ("id", C.c_uint64), ("version", C.c_uint64),
+        ("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("strength", C.c_double),
+        ("theta2", C.c_double), ("distance_min2", C.c_double), ("distance_max2", C.c_double),
+        ("axis", C.c_int32), ("count", C.c_uint32), ("links", _u), ("strengths", _d), ("radii", _d),
+        ("distances", _d), ("bias", _d), ("targets", _d),
+    ]
+
+
+class BufferInfo(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("velocities", C.c_void_p), ("count", C.c_uint32),
+                ("stride", C.c_uint32), ("epoch", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("interactions", C.c_uint64), ("opened", C.c_uint64),
+                ("long_runs", C.c_uint32), ("sort_bits", C.c_uint32), ("walk_errors", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+# every entry point of include/d3fgpu.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "fg_abi_version": (C.c_int, []),
+    "fg_device_count": (C.c_int, [_i]),
+    "fg_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "fg_destroy": (C.c_int, [C.c_void_p]),
+    "fg_last_error": (C.c_char_p, [C.c_void_p]),
+    "fg_fits": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, _i]),
+    "fg_set_nodes": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, _d, _d, _d, _u8]),
+    "fg_update_positions": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _d]),
+    "fg_update_velocities": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _d]),
+    "fg_update_pins": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _d, _u8]),
+    "fg_set_forces": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(ForceDesc)]),
+    "fg_set_seed": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "fg_enqueue": (C.c_int, [C.c_void_p, C.c_uint32, _d, C.c_double]),
+    "fg_apply_forces": (C.c_int, [C.c_void_p, C.c_double]),
+    "fg_busy": (C.c_int, [C.c_void_p, _i]),
+    "fg_sync": (C.c_int, [C.c_void_p]),
+    "fg_readback": (C.c_int, [C.c_void_p, _d, _d]),
+    "fg_readback_start": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "fg_readback_wait": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "fg_readback_fetch": (C.c_int, [C.c_void_p, C.c_uint64, _d, _d]),
+    "fg_buffers": (C.c_int, [C.c_void_p, C.POINTER(BufferInfo)]),
+    "fg_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "fg_radix_sort_pairs": (C.c_int, [C.c_void_p, C.c_uint32, _u, _u, C.c_uint32, C.c_uint32, _u, _u]),
+    "fg_debug_tree": (C.c_int, [C.c_void_p, _f, _i, _u, _u, _u, _u, _u, _u, _u, _d, _d]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded backend library (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"backend library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class FgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{ERRORS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def ptr(a, t=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def device_count():
+    k = C.c_int32(0)
+    lib().fg_device_count(C.byref(k))
+    return k.value
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None:
+        assert a.shape[0] == n
+    return a
+
+
+class Force:
+    """A force as its gpu() descriptor describes it; keeps its arrays alive."""
+    _next_id = 1
+
+    def __init__(self, type, **kw):
+        self.type = type
+        self.id = Force._next_id
+        Force._next_id += 1
+        self.version = 1
+        self.kw = kw
+
+    def bump(self, **kw):
+        self.kw.update(kw)
+        self.version += 1
+
+    def desc(self):
+        k = self.kw
+        d = ForceDesc()
+        d.type, d.id, d.version = self.type, self.id, self.version
+        d.iterations = int(k.get("iterations", 1))
+        d.x, d.y, d.z = (float(k.get(a, 0.0)) for a in "xyz")
+        d.strength = float(k.get("strength", 1.0))
+        d.theta2 = float(k.get("theta2", 0.81))
+        d.distance_min2 = float(k.get("distance_min2", 1.0))
+        d.distance_max2 = float(k.get("distance_max2", np.inf))
+        d.axis = int(k.get("axis", 0))
+        keep = []
+        for name in ("strengths", "radii", "distances", "bias", "targets"):
+            if k.get(name) is not None:
+                arr = _f64(k[name])
+                keep.append(arr)
+                setattr(d, name, ptr(arr))
+        if k.get("links") is not None:
+            links = np.ascontiguousarray(k["links"], dtype=np.uint32).reshape(-1)
+            keep.append(links)
+            d.links = ptr(links, C.c_uint32)
+            d.count = len(links) // 2
+        return d, keep
+
+
+def many_body(strengths, theta=0.9, distance_min=1.0, distance_max=np.inf):
+    return Force(MANYBODY, strengths=strengths, theta2=theta * theta, distance_min2=distance_min ** 2,
+                 distance_max2=distance_max ** 2)
+
+
+def link(src, dst, strengths, distances, bias, iterations=1):
+    links = np.stack([np.asarray(src, np.uint32), np.asarray(dst, np.uint32)], 1)
+    return Force(LINK, links=links, strengths=strengths, distances=distances, bias=bias, iterations=iterations)
+
+
+def collide(radii, strength=1.0, iterations=1):
+    return Force(COLLIDE, radii=radii, strength=strength, iterations=iterations)
+
+
+def center(x=0.0, y=0.0, z=0.0, strength=1.0):
+    return Force(CENTER, x=x, y=y, z=z, strength=strength)
+
+
+def radial(radii, strengths, x=0.0, y=0.0, z=0.0):
+    return Force(RADIAL, radii=radii, strengths=strengths, x=x, y=y, z=z)
+
+
+def position(axis, targets, strengths):
+    return Force(POSITION, axis=axis, targets=targets, strengths=strengths)
+
+
+class Engine:
+    """One simulation on one GPU (the reference's engine object)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib().fg_create(C.byref(h), device)
+        if rc != FG_OK:
+            raise FgError(rc, "fg_create failed")
+        self.h = h
+        self.n = 0
+        self.ndim = 2
+
+    def close(self):
+        if self.h:
+            lib().fg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != FG_OK:
+            raise FgError(rc, lib().fg_last_error(self.h).decode())
+
+    def fits(self, n, ndim):
+        out = C.c_int32(0)
+        self._check(lib().fg_fits(self.h, n, ndim, C.byref(out)))
+        return bool(out.value)
+
+    def set_nodes(self, pos, vel=None, fix=None, fixmask=None, ndim=3):
+        pos = _f64(pos)
+        n = len(pos)
+        pos3 = np.zeros((n, 3))
+        pos3[:, :pos.shape[1]] = pos
+        vel3 = None
+        if vel is not None:
+            vel = _f64(vel)
+            vel3 = np.zeros((n, 3))
+            vel3[:, :vel.shape[1]] = vel
+        fix3 = None if fix is None else np.ascontiguousarray(_f64(fix).reshape(n, 3))
+        mask = None if fixmask is None else np.ascontiguousarray(fixmask, dtype=np.uint8)
+        self._check(lib().fg_set_nodes(self.h, n, ndim, ptr(pos3), ptr(vel3), ptr(fix3), ptr(mask, C.c_uint8)))
+        self.n, self.ndim = n, ndim
+
+    def update_positions(self, first, pos):
+        pos = np.ascontiguousarray(_f64(pos).reshape(-1, 3))
+        self._check(lib().fg_update_positions(self.h, first, len(pos), ptr(pos)))
+
+    def update_velocities(self, first, vel):
+        vel = np.ascontiguousarray(_f64(vel).reshape(-1, 3))
+        self._check(lib().fg_update_velocities(self.h, first, len(vel), ptr(vel)))
+
+    def update_pins(self, first, fix, fixmask):
+        fix = np.ascontiguousarray(_f64(fix).reshape(-1, 3))
+        mask = np.ascontiguousarray(fixmask, dtype=np.uint8)
+        self._check(lib().fg_update_pins(self.h, first, len(mask), ptr(fix), ptr(mask, C.c_uint8)))
+
+    def set_forces(self, forces):
+        descs, keep = [], []
+        for f in forces:
+            d, k = f.desc()
+            descs.append(d)
+            keep.extend(k)
+        arr = (ForceDesc * max(1, len(descs)))(*descs)
+        self._check(lib().fg_set_forces(self.h, len(descs), arr))
+
+    def set_seed(self, seed):
+        self._check(lib().fg_set_seed(self.h, seed & 0xffffffff))
+
+    def enqueue(self, alphas, decay=0.6):
+        a = _f64(np.atleast_1d(alphas))
+        self._check(lib().fg_enqueue(self.h, len(a), ptr(a), decay))
+
+    def apply_forces(self, alpha):
+        self._check(lib().fg_apply_forces(self.h, alpha))
+
+    def busy(self):
+        b = C.c_int32(0)
+        self._check(lib().fg_busy(self.h, C.byref(b)))
+        return bool(b.value)
+
+    def sync(self):
+        self._check(lib().fg_sync(self.h))
+
+    def readback(self):
+        pos = np.zeros((self.n, 3))
+        vel = np.zeros((self.n, 3))
+        self._check(lib().fg_readback(self.h, ptr(pos), ptr(vel)))
+        return pos, vel
+
+    def readback_async(self):
+        t = C.c_uint64(0)
+        self._check(lib().fg_readback_start(self.h, C.byref(t)))
+        return t.value
+
+    def readback_wait(self, ticket):
+        self._check(lib().fg_readback_wait(self.h, ticket))
+
+    def readback_fetch(self, ticket):
+        pos = np.zeros((self.n, 3))
+        vel = np.zeros((self.n, 3))
+        self._check(lib().fg_readback_fetch(self.h, ticket, ptr(pos), ptr(vel)))
+        return pos, vel
+
+    def buffers(self):
+        b = BufferInfo()
+        self._check(lib().fg_buffers(self.h, C.byref(b)))
+        return b
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().fg_get_stats(self.h, C.byref(s)))
+        return s
+
+    def radix_sort_pairs(self, keys, values, first_bit=0, bits=32):
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        values = np.ascontiguousarray(values, dtype=np.uint32)
+        n = len(keys)
+        ko, vo = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        self._check(lib().fg_radix_sort_pairs(self.h, n, ptr(keys, C.c_uint32), ptr(values, C.c_uint32),
+                                              first_bit, bits, ptr(ko, C.c_uint32), ptr(vo, C.c_uint32)))
+        return ko, vo
+
+    def debug_tree(self):
+        n = self.n
+        inner = max(1, n - 1)
+        cube = np.zeros(4, np.float32)
+        shift = C.c_int32(0)
+        codes = np.zeros(2 * n, np.uint32)
+        order = np.zeros(n, np.uint32)
+        leaf_depth = np.zeros(n, np.uint32)
+        left, right, depth = (np.zeros(inner, np.uint32) for _ in range(3))
+        kids = np.zeros(8 * inner, np.uint32)
+        com = np.zeros(3 * inner)
+        value = np.zeros(inner)
+        u = C.c_uint32
+        self._check(lib().fg_debug_tree(self.h, ptr(cube, C.c_float), C.byref(shift), ptr(codes, u), ptr(order, u),
+                                        ptr(leaf_depth, u), ptr(left, u), ptr(right, u), ptr(depth, u),
+                                        ptr(kids, u), ptr(com), ptr(value)))
+        return dict(cube=cube, shift=shift.value, hi=codes[0::2].copy(), lo=codes[1::2].copy(), order=order,
+                    leaf_depth=leaf_depth, left=left[:n - 1], right=right[:n - 1], depth=depth[:n - 1],
+                    kids=kids.reshape(inner, 8)[:n - 1], com=com.reshape(inner, 3)[:n - 1], value=value[:n - 1])

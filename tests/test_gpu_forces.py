@@ -1,0 +1,368 @@
+"""Every force on the GPU against the FP64 oracle on identical f32-exact input
+(test/gpu-forces-test.js is the model).  One application of the force list (no
+integrator) is compared as the velocity increment it produces:
+
+  manyBody (theta > 0)   d3's own Barnes-Hut over d3-octree/quadtree/binarytree, FP64:
+                         rel-L2 <= 1e-4 over all nodes, and the per-node rule of
+                         gpu-helpers.js:63-84 with its outlier allowance
+  manyBody (theta = 0)   exact all-pairs sum, FP64
+  link, collide          the GPU's Jacobi semantics restated in FP64 (passes/link.js,
+                         passes/collide.js); d3's sequential order differs by design
+                         (docs/differences.md:54-65)
+  center, radial, x/y/z  closed forms, over several ticks with the integrator
+"""
+import numpy as np
+import pytest
+
+import d3fgpu
+import oracle as O
+from helpers import close_count, make_nodes, make_nodes_fast, rel_l2, tree_outliers
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_dv(engine, pos, dims, forces, alpha=1.0, vel=None):
+    engine.set_nodes(pos, vel=vel, ndim=dims)
+    engine.set_forces(forces)
+    engine.apply_forces(alpha)
+    p, v = engine.readback()
+    return p, v - (0 if vel is None else vel)
+
+
+def oracle_manybody(pos, dims, strengths, alpha=1.0, theta=0.9, dmin=1.0, dmax=np.inf):
+    s32 = np.asarray(strengths, np.float32).astype(np.float64)
+    dv, st = O.manybody(pos, np.zeros_like(pos), s32, alpha, theta2=theta * theta, dmin2=dmin * dmin,
+                        dmax2=dmax * dmax, ndim=dims)
+    return dv, st
+
+
+def report(name, err, **kw):
+    extra = " ".join(f"{k}={v}" for k, v in kw.items())
+    print(f"[parity] {name}: rel-L2 {err:.3e} {extra}", flush=True)
+
+
+# ---- many-body, Barnes-Hut ----------------------------------------------------------
+
+@pytest.mark.parametrize("dims,rtol", [(1, 2e-2), (2, 1e-3), (3, 1e-3)])
+def test_manybody_tree_500(engine_factory, dims, rtol):
+    pos = make_nodes(500, dims)
+    s = np.full(500, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [d3fgpu.many_body(s)])
+    ref, _ = oracle_manybody(pos, dims, s)
+    err = rel_l2(dv, ref)
+    report(f"manyBody tree {dims}D n=500", err)
+    assert close_count(dv, ref, rtol=rtol) <= tree_outliers(500)
+    if dims > 1:
+        assert err < 1e-4
+
+
+@pytest.mark.parametrize("theta", [0.5, 1.5])
+def test_manybody_tree_theta(engine_factory, theta):
+    pos = make_nodes(2000, 3)
+    s = np.full(2000, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s, theta=theta)])
+    ref, _ = oracle_manybody(pos, 3, s, theta=theta)
+    err = rel_l2(dv, ref)
+    report(f"manyBody tree theta={theta}", err)
+    assert close_count(dv, ref, rtol=1e-3) <= tree_outliers(2000)
+    assert err < 1e-4
+
+
+def test_manybody_tree_per_node_strength_dmin_dmax(engine_factory):
+    pos = make_nodes(2000, 3)
+    s = np.array([-10.0 - i % 7 for i in range(2000)])
+    f = d3fgpu.many_body(s, distance_min=3, distance_max=60)
+    _, dv = gpu_dv(engine_factory(), pos, 3, [f])
+    ref, _ = oracle_manybody(pos, 3, s, dmin=3, dmax=60)
+    err = rel_l2(dv, ref)
+    report("manyBody tree dmin/dmax", err)
+    assert close_count(dv, ref, rtol=1e-3) <= tree_outliers(2000)
+    assert err < 1e-4
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_manybody_tree_mixed_signs(engine_factory, dims):
+    pos = make_nodes(1000, dims)
+    s = np.array([-30.0 if i % 3 else 20.0 for i in range(1000)])
+    _, dv = gpu_dv(engine_factory(), pos, dims, [d3fgpu.many_body(s)])
+    ref, _ = oracle_manybody(pos, dims, s)
+    err = rel_l2(dv, ref)
+    report(f"manyBody tree mixed signs {dims}D", err)
+    assert close_count(dv, ref, rtol=1e-3) <= tree_outliers(1000)
+    assert err < 1e-4
+
+
+def test_manybody_tree_huge_cube(engine_factory):
+    pos = make_nodes(1000, 3)
+    pos[0, 0] = 1e5
+    pos[1, 1] = -2e5
+    s = np.full(1000, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s)])
+    ref, _ = oracle_manybody(pos, 3, s)
+    report("manyBody tree huge cube", rel_l2(dv, ref))
+    assert close_count(dv, ref, rtol=1e-3) <= tree_outliers(1000)
+
+
+def test_manybody_tree_far_from_origin(engine_factory):
+    pos = make_nodes(1000, 3)
+    pos[:, 0] = (pos[:, 0] + 5e4).astype(np.float32)
+    pos[:, 1] = (pos[:, 1] - 3e4).astype(np.float32)
+    pos[:, 2] = (pos[:, 2] + 1e4).astype(np.float32)
+    s = np.full(1000, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s)])
+    ref, _ = oracle_manybody(pos, 3, s)
+    report("manyBody tree far from origin", rel_l2(dv, ref))
+    assert close_count(dv, ref, rtol=1e-2) <= tree_outliers(1000)
+
+
+@pytest.mark.parametrize("n", [2, 3, 1025, 8192, 8193, 32769])
+def test_manybody_tree_sizes(engine_factory, n):
+    pos = make_nodes_fast(n, 3, 20 * np.cbrt(n))
+    s = np.full(n, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s)])
+    ref, _ = oracle_manybody(pos, 3, s)
+    err = rel_l2(dv, ref)
+    report(f"manyBody tree n={n}", err)
+    assert close_count(dv, ref, rtol=1e-3, atol=1e-3) <= tree_outliers(n)
+    assert err < 1e-4
+
+
+@pytest.mark.parametrize("dims,outlier", [(2, 120), (3, 120), (2, 2000), (3, 2000)])
+def test_manybody_tree_clumps(engine_factory, dims, outlier):
+    pos = make_nodes_fast(9000, dims, 2)
+    pos[0, 0] = outlier
+    s = np.full(9000, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [d3fgpu.many_body(s)])
+    ref, _ = oracle_manybody(pos, dims, s)
+    report(f"manyBody tree clump {dims}D outlier {outlier}", rel_l2(dv, ref))
+    assert close_count(dv, ref, rtol=1e-3) <= tree_outliers(9000)
+
+
+def test_manybody_tree_stats_match_oracle(engine_factory):
+    n = 20000
+    pos = make_nodes_fast(n, 3, 20 * np.cbrt(n))
+    s = np.full(n, -30.0)
+    e = engine_factory()
+    gpu_dv(e, pos, 3, [d3fgpu.many_body(s)])
+    st = e.stats()
+    _, ost = oracle_manybody(pos, 3, s)
+    print(f"[stats] interactions gpu={st.interactions} oracle={ost[0]} opened(wave)={st.opened}", flush=True)
+    assert st.walk_errors == 0
+    assert abs(int(st.interactions) - int(ost[0])) <= 0.01 * ost[0]
+
+
+# ---- many-body, exact ---------------------------------------------------------------
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_manybody_exact(engine_factory, dims):
+    pos = make_nodes(300, dims)
+    s = np.full(300, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [d3fgpu.many_body(s, theta=0)])
+    ref = O.manybody_exact(pos, np.zeros_like(pos), s, 1.0, ndim=dims)
+    report(f"manyBody exact {dims}D", rel_l2(dv, ref))
+    assert close_count(dv, ref) == 0
+    assert rel_l2(dv, ref) < 1e-5
+
+
+def test_manybody_exact_strength_dmin_dmax(engine_factory):
+    pos = make_nodes(200, 3)
+    s = np.array([15.0 if i % 3 == 0 else -20.0 - i % 7 for i in range(200)])
+    _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s, theta=0, distance_min=5, distance_max=150)])
+    ref = O.manybody_exact(pos, np.zeros_like(pos), s, 1.0, dmin2=25, dmax2=150 ** 2)
+    assert close_count(dv, ref, rtol=1e-3) == 0
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 8, 9, 127, 128, 129, 255, 257])
+def test_manybody_exact_small(engine_factory, n):
+    pos = make_nodes(n, 3)
+    s = np.full(n, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s, theta=0)])
+    ref = O.manybody_exact(pos, np.zeros_like(pos), s, 1.0)
+    assert close_count(dv, ref, rtol=1e-3, atol=1e-3) == 0
+
+
+# ---- link ---------------------------------------------------------------------------
+
+def oracle_link(pos, vel, dims, src, dst, strengths, distances, bias, iterations, alpha):
+    x, y, z = (O.dv(pos[:, a]) for a in range(3))
+    vx, vy, vz = (O.dv(vel[:, a]) for a in range(3))
+    u = lambda a: O.P(np.ascontiguousarray(a, np.uint32), O.C.c_uint32)
+    f32 = lambda a: O.dv(np.asarray(a, np.float32))
+    O.lib().or_link_jacobi(len(pos), dims, O.P(x), O.P(y), O.P(z), O.P(vx), O.P(vy), O.P(vz), len(src), u(src), u(dst),
+                           O.P(f32(strengths)), O.P(f32(distances)), O.P(f32(bias)), iterations, alpha)
+    return np.stack([vx, vy, vz], 1)
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_link_graph(engine_factory, dims):
+    n = 3000
+    pos = make_nodes_fast(n, dims, 400)
+    vel = make_nodes_fast(n, dims, 2, seed=9)
+    src, dst = O.make_graph(n, 42)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    bias, strengths, distances = O.link_defaults(n, src, dst)
+    f = d3fgpu.link(src, dst, strengths, distances, bias, iterations=2)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [f], alpha=0.7, vel=vel)
+    ref = oracle_link(pos, vel, dims, src, dst, strengths, distances, bias, 2, 0.7) - vel
+    err = rel_l2(dv, ref)
+    report(f"link {dims}D", err)
+    assert err < 1e-4
+    assert close_count(dv, ref, rtol=1e-3) == 0
+
+
+# ---- collide ------------------------------------------------------------------------
+
+def oracle_collide(pos, vel, dims, radii, strength, iterations):
+    x, y, z = (O.dv(pos[:, a]) for a in range(3))
+    vx, vy, vz = (O.dv(vel[:, a]) for a in range(3))
+    r = O.dv(np.asarray(radii, np.float32))
+    O.lib().or_collide_jacobi(len(pos), dims, O.P(x), O.P(y), O.P(z), O.P(vx), O.P(vy), O.P(vz), O.P(r),
+                              strength, iterations)
+    return np.stack([vx, vy, vz], 1)
+
+
+@pytest.mark.parametrize("n,dims", [(2000, 2), (2000, 3), (10000, 1), (10000, 2), (10000, 3)])
+def test_collide(engine_factory, n, dims):
+    pos = make_nodes_fast(n, dims, 12 * n ** (1 / dims))
+    radii = np.array([6 + i % 3 for i in range(n)], float)
+    f = d3fgpu.collide(radii, strength=0.9, iterations=1)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [f])
+    ref = oracle_collide(pos, np.zeros_like(pos), dims, radii, 0.9, 1)
+    err = rel_l2(dv, ref)
+    report(f"collide n={n} {dims}D", err, touched=int(np.sum(np.any(ref != 0, 1))))
+    assert err < 1e-4
+    assert close_count(dv, ref, rtol=1e-3) == 0
+
+
+def test_collide_grid_two_iterations_lognormal(engine_factory):
+    n, dims = 20000, 3
+    rng = np.random.default_rng(5)
+    pos = make_nodes_fast(n, dims, 10 * np.cbrt(n))
+    radii = np.exp(np.log(2) + 0.4 * rng.standard_normal(n))
+    f = d3fgpu.collide(radii, strength=1.0, iterations=2)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [f])
+    ref = oracle_collide(pos, np.zeros_like(pos), dims, radii, 1.0, 2)
+    err = rel_l2(dv, ref)
+    report("collide grid lognormal 2 iterations", err)
+    assert err < 1e-4
+
+
+# ---- center / radial / position / integrate, several ticks --------------------------
+
+def run_both(engine, pos, dims, forces_gpu, forces_oracle, ticks, fix=None, fixmask=None):
+    engine.set_nodes(pos, fix=fix, fixmask=fixmask, ndim=dims)
+    engine.set_forces(forces_gpu)
+    alpha, decay_a = 1.0, 1 - 0.001 ** (1 / 300)
+    alphas = []
+    for _ in range(ticks):
+        alpha += (0 - alpha) * decay_a
+        alphas.append(alpha)
+    engine.enqueue(alphas, 0.6)
+    gp, gv = engine.readback()
+    fixarr = None
+    if fix is not None:
+        fixarr = np.where(np.stack([(fixmask >> a) & 1 for a in range(3)], 1) == 1, fix, np.nan)
+    sim = O.Sim(pos, fix=fixarr, ndim=dims)
+    keep = [f for f in forces_oracle]
+    sim.set_forces([f[0] for f in keep], jacobi=True)
+    sim.tick(ticks)
+    op, ov = sim.state()
+    return gp, gv, op, ov
+
+
+def odesc(type, **kw):
+    keep = []
+    d = O.ForceDesc(type=type, iterations=kw.pop("iterations", 1))
+    for k, v in kw.items():
+        if isinstance(v, np.ndarray):
+            a = O.dv(np.asarray(v, np.float32))
+            keep.append(a)
+            setattr(d, k, O.P(a))
+        else:
+            setattr(d, k, v)
+    return d, keep
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_center(engine_factory, dims):
+    pos = make_nodes(1000, dims, 100)
+    pos[:, 0] += 40
+    if dims > 1:
+        pos[:, 1] -= 25
+    if dims > 2:
+        pos[:, 2] += 10
+    pos = pos.astype(np.float32).astype(np.float64)
+    c = (3.0, 4.0 if dims > 1 else 0.0, 5.0 if dims > 2 else 0.0)
+    gp, gv, op, ov = run_both(engine_factory(), pos, dims, [d3fgpu.center(*c, strength=0.7)],
+                              [odesc(1, x=c[0], y=c[1], z=c[2], strength=0.7)], 2)
+    assert close_count(gp, op, rtol=1e-3) == 0
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_radial(engine_factory, dims):
+    n = 200
+    pos = make_nodes(n, dims)
+    r = np.array([30.0 + i % 5 * 10 for i in range(n)])
+    w = np.full(n, 0.3)
+    c = (5.0, -5.0 if dims > 1 else 0.0, 2.0 if dims > 2 else 0.0)
+    gp, gv, op, ov = run_both(engine_factory(), pos, dims, [d3fgpu.radial(r, w, *c)],
+                              [odesc(6, radii=r, strengths=w, x=c[0], y=c[1], z=c[2])], 3)
+    assert close_count(np.hstack([gp, gv]), np.hstack([op, ov]), rtol=1e-3) == 0
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_position(engine_factory, dims):
+    n = 200
+    pos = make_nodes(n, dims)
+    gf = [d3fgpu.position(0, np.array([i % 10 for i in range(n)], float),
+                          np.array([0.2 if i % 4 else 0.05 for i in range(n)]))]
+    of = [odesc(5, axis=0, targets=np.array([i % 10 for i in range(n)], float),
+                strengths=np.array([0.2 if i % 4 else 0.05 for i in range(n)]))]
+    if dims > 1:
+        gf.append(d3fgpu.position(1, np.full(n, -20.0), np.full(n, 0.3)))
+        of.append(odesc(5, axis=1, targets=np.full(n, -20.0), strengths=np.full(n, 0.3)))
+    if dims > 2:
+        gf.append(d3fgpu.position(2, np.full(n, 15.0), np.full(n, 0.1)))
+        of.append(odesc(5, axis=2, targets=np.full(n, 15.0), strengths=np.full(n, 0.1)))
+    gp, gv, op, ov = run_both(engine_factory(), pos, dims, gf, of, 3)
+    assert close_count(np.hstack([gp, gv]), np.hstack([op, ov]), rtol=1e-3) == 0
+
+
+def test_pins_and_force_order(engine_factory):
+    n = 250
+    pos = make_nodes(n, 3, 120)
+    pos[:, 0] = (pos[:, 0] + 30).astype(np.float32)
+    fix = np.zeros((n, 3))
+    mask = np.zeros(n, np.uint8)
+    fix[0] = [5, -6, 7]
+    mask[0] = 7
+    fix[1, 0] = 100
+    mask[1] = 1
+    s = np.full(n, -15.0)
+    gf = [d3fgpu.many_body(s, theta=0), d3fgpu.position(0, np.full(n, 10.0), np.full(n, 0.05)),
+          d3fgpu.center(), d3fgpu.radial(np.full(n, 60.0), np.full(n, 0.02))]
+    of = [odesc(4, strengths=s, theta2=0.0, distance_min2=1.0, distance_max2=np.inf),
+          odesc(5, axis=0, targets=np.full(n, 10.0), strengths=np.full(n, 0.05)),
+          odesc(1, strength=1.0), odesc(6, radii=np.full(n, 60.0), strengths=np.full(n, 0.02))]
+    gp, gv, op, ov = run_both(engine_factory(), pos, 3, gf, of, 5, fix=fix, fixmask=mask)
+    assert gp[0].tolist() == [5.0, -6.0, 7.0]
+    assert gv[0].tolist() == [0.0, 0.0, 0.0]
+    assert gp[1, 0] == 100.0
+    assert close_count(np.hstack([gp, gv]), np.hstack([op, ov]), rtol=2e-3, atol=1e-3) == 0
+
+
+def test_coincident_nodes_separate(engine_factory):
+    e = engine_factory()
+    pos = np.tile([1.0, 2.0, 3.0], (10, 1))
+    e.set_nodes(pos, ndim=3)
+    e.set_forces([d3fgpu.many_body(np.full(10, -30.0)), d3fgpu.collide(np.full(10, 2.0))])
+    alpha, a = 1.0, []
+    for _ in range(30):
+        alpha += -alpha * (1 - 0.001 ** (1 / 300))
+        a.append(alpha)
+    e.enqueue(a, 0.6)
+    p, v = e.readback()
+    assert np.all(np.isfinite(p))
+    d = np.linalg.norm(p[:, None] - p[None], axis=2) + np.eye(10) * 1e9
+    assert d.min() > 0.5
