@@ -82,6 +82,62 @@ struct Pass {
   DevBuf center;
 };
 
+// HIP events around the phases of each tick, summed on request.
+struct EventTimer : PhaseTimer {
+  hipStream_t stream = nullptr;
+  struct Rec {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  hipEvent_t open[FG_PHASE_COUNT] = {};
+  std::vector<hipEvent_t> pool;
+  double ms[FG_PHASE_COUNT] = {};
+  uint64_t launches[FG_PHASE_COUNT] = {};
+
+  hipEvent_t get() {
+    hipEvent_t e;
+    if (!pool.empty()) {
+      e = pool.back();
+      pool.pop_back();
+    } else {
+      FG_HIP(hipEventCreate(&e));
+    }
+    return e;
+  }
+  void begin(int phase) override {
+    hipEvent_t e = get();
+    FG_HIP(hipEventRecord(e, stream));
+    open[phase] = e;
+  }
+  void end(int phase) override {
+    hipEvent_t e = get();
+    FG_HIP(hipEventRecord(e, stream));
+    recs.push_back(Rec{phase, open[phase], e});
+    open[phase] = nullptr;
+  }
+  void collect() {
+    for (auto& r : recs) {
+      float t = 0;
+      FG_HIP(hipEventSynchronize(r.b));
+      FG_HIP(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.phase] += t;
+      launches[r.phase] += 1;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    recs.clear();
+  }
+  void reset() {
+    collect();
+    for (int i = 0; i < FG_PHASE_COUNT; ++i) ms[i] = 0, launches[i] = 0;
+  }
+  ~EventTimer() {
+    for (auto& r : recs) (void)hipEventDestroy(r.a), (void)hipEventDestroy(r.b);
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 }  // namespace
 
 struct fg_ctx {
@@ -109,6 +165,8 @@ struct fg_ctx {
   bool mirror = false;
   std::vector<hipEvent_t> pending;
   fg_stats stats{};
+  std::unique_ptr<EventTimer> timer;
+  PhaseTimer* T() { return timer.get(); }
 
   float* P(int a) { return pos[a].as<float>(); }
   float* V(int a) { return vel[cur][a].as<float>(); }
@@ -236,17 +294,19 @@ void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
     case FG_FORCE_MANYBODY: {
       float* vel[3] = {c->V(0), c->V(1), c->V(2)};
       if (!(p.d.theta2 > EXACT_THETA2)) {
+        PhaseScope ps(c->T(), FG_PHASE_PAIRS);
         launch_manybody_pairs(c->stream, n, nd, pos, p.a.as<float>(), vel, finite_f32(p.d.distance_min2),
                               finite_f32(p.d.distance_max2), sp);
       } else {
         if (!p.tree) p.tree.reset(new TreeBuffers());
         TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2};
         tree_manybody(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), vel[0],
-                      vel[1], vel[2], tp, sp, c->radix_bits);
+                      vel[1], vel[2], tp, sp, c->radix_bits, c->T());
       }
       break;
     }
-    case FG_FORCE_LINK:
+    case FG_FORCE_LINK: {
+      PhaseScope ps(c->T(), FG_PHASE_LINK);
       for (int k = 0; k < p.d.iterations && p.m; ++k) {
         const float* vin[3] = {c->V(0), c->V(1), c->V(2)};
         float* vout[3] = {c->W(0), c->W(1), c->W(2)};
@@ -254,8 +314,10 @@ void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
         c->cur ^= 1;
       }
       break;
-    case FG_FORCE_COLLIDE:
+    }
+    case FG_FORCE_COLLIDE: {
       if (!(p.rmax > 0)) break;
+      PhaseScope ps(c->T(), FG_PHASE_COLLIDE);
       for (int k = 0; k < p.d.iterations; ++k) {
         if (n >= GRID_MIN) {
           if (!p.grid) p.grid.reset(new GridWork());
@@ -270,18 +332,24 @@ void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
         }
       }
       break;
-    case FG_FORCE_CENTER:
+    }
+    case FG_FORCE_CENTER: {
+      PhaseScope ps(c->T(), FG_PHASE_CENTER);
       launch_center(c->stream, n, nd, pos, p.center.as<CenterWork>(), p.d.x, p.d.y, p.d.z, p.d.strength);
       break;
+    }
     case FG_FORCE_RADIAL: {
+      PhaseScope ps(c->T(), FG_PHASE_RADIAL);
       float* vel[3] = {c->V(0), c->V(1), c->V(2)};
       launch_radial(c->stream, n, nd, pos, vel, p.b.as<float>(), p.a.as<float>(), finite_f32(p.d.x),
                     finite_f32(p.d.y), finite_f32(p.d.z), sp);
       break;
     }
     case FG_FORCE_POSITION:
-      if (p.d.axis < nd)
+      if (p.d.axis < nd) {
+        PhaseScope ps(c->T(), FG_PHASE_POSITION);
         launch_position(c->stream, n, c->P(p.d.axis), c->V(p.d.axis), p.b.as<float>(), p.a.as<float>(), sp);
+      }
       break;
     default: break;
   }
@@ -547,11 +615,15 @@ int fg_enqueue(fg_ctx* c, uint32_t steps, const double* alphas, double decay) {
     const uint8_t* mask = c->pinned_any ? c->mask.as<uint8_t>() : nullptr;
     for (uint32_t s = 0; s < steps; ++s) {
       StepParams sp{(float)alphas[s], (float)decay, (uint32_t)c->steps, c->seed};
+      PhaseScope tick(c->T(), FG_PHASE_TICK);
       for (auto& p : c->passes) run_pass(c, *p, sp);
       float* pos[3] = {c->P(0), c->P(1), c->P(2)};
       float* vel[3] = {c->V(0), c->V(1), c->V(2)};
       float* fix[3] = {c->fix[0].as<float>(), c->fix[1].as<float>(), c->fix[2].as<float>()};
-      launch_integrate(c->stream, c->n, c->ndim, pos, vel, mask, fix, sp);
+      {
+        PhaseScope ps(c->T(), FG_PHASE_INTEGRATE);
+        launch_integrate(c->stream, c->n, c->ndim, pos, vel, mask, fix, sp);
+      }
       c->steps++;
     }
     refresh_mirror(c);
@@ -675,6 +747,33 @@ int fg_get_stats(fg_ctx* c, fg_stats* out) {
       }
     }
     *out = s;
+    return FG_OK;
+  });
+}
+
+int fg_timing_enable(fg_ctx* c, int32_t on) {
+  return guarded(c, [&] {
+    if (on) {
+      if (!c->timer) {
+        c->timer.reset(new EventTimer());
+        c->timer->stream = c->stream;
+      }
+      c->timer->reset();
+    } else if (c->timer) {
+      FG_HIP(hipStreamSynchronize(c->stream));
+      c->timer.reset();
+    }
+    return FG_OK;
+  });
+}
+
+int fg_get_timing(fg_ctx* c, fg_timing* out) {
+  return guarded(c, [&] {
+    if (!out) return FG_ERR_INVALID;
+    *out = fg_timing{};
+    if (!c->timer) return FG_OK;
+    c->timer->collect();
+    for (int i = 0; i < FG_PHASE_COUNT; ++i) out->ms[i] = c->timer->ms[i], out->launches[i] = c->timer->launches[i];
     return FG_OK;
   });
 }

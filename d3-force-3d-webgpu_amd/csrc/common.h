@@ -88,4 +88,22 @@ struct StepParams {
   uint32_t seed;
 };
 
+// Records HIP events around phases of a tick on one stream (off when null).
+struct PhaseTimer {
+  virtual void begin(int phase) = 0;
+  virtual void end(int phase) = 0;
+  virtual ~PhaseTimer() {}
+};
+
+struct PhaseScope {
+  PhaseTimer* t;
+  int phase;
+  PhaseScope(PhaseTimer* t_, int p) : t(t_), phase(p) {
+    if (t) t->begin(phase);
+  }
+  ~PhaseScope() {
+    if (t) t->end(phase);
+  }
+};
+
 }  // namespace fg

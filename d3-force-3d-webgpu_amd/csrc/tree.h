@@ -66,6 +66,7 @@ struct TreeParams {
 // Build the tree over the current positions and add the many-body force to the velocities.
 void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x,
                    const float* y, const float* z, const float* strength, float* vx, float* vy,
-                   float* vz, const TreeParams& p, StepParams step, int radix_bits);
+                   float* vz, const TreeParams& p, StepParams step, int radix_bits,
+                   PhaseTimer* timer = nullptr);
 
 }  // namespace fg

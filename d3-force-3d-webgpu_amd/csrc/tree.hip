@@ -509,8 +509,9 @@ __global__ void tree_counters_clear(uint32_t* c) {
 template <int D>
 void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float* y, const float* z,
          const float* strength, float* vx, float* vy, float* vz, const TreeParams& p,
-         StepParams step, int radix_bits) {
+         StepParams step, int radix_bits, PhaseTimer* timer) {
   const uint32_t blocks = ceil_div(n, 256);
+  if (timer) timer->begin(1);
   hipLaunchKernelGGL(tree_counters_clear, dim3(1), dim3(64), 0, st, t.counters);
   uint32_t bb = blocks < 512 ? blocks : 512;
   hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube);
@@ -520,13 +521,18 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   FG_LAUNCH_CHECK();
   uint64_t* sk;
   uint32_t* sv;
+  if (timer) timer->end(1), timer->begin(2);
   radix_sort_u64(st, t.sort, radix_bits, t.keys, t.vals, n, 0, Dims<D>::BITS, &sk, &sv);
+  if (timer) timer->end(2), timer->begin(1);
   t.skeys = sk;
   t.order = sv;
   hipLaunchKernelGGL((tree_leaves<D>), dim3(blocks), dim3(256), 0, st, (const uint64_t*)sk,
                      (const uint32_t*)sv, x, y, z, strength, n, t.leaves, t.leaf_depth, t.flags);
   FG_LAUNCH_CHECK();
-  if (n < 2) return;
+  if (n < 2) {
+    if (timer) timer->end(1);
+    return;
+  }
   hipLaunchKernelGGL((tree_build<D>), dim3(ceil_div(n - 1, 256)), dim3(256), 0, st,
                      (const uint64_t*)sk, n, t.links, t.leaf_parent);
   FG_LAUNCH_CHECK();
@@ -534,6 +540,7 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
                      (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
                      (const uint4*)t.links, t.flags, t.rec, t.sums);
   FG_LAUNCH_CHECK();
+  if (timer) timer->end(1), timer->begin(3);
   WalkArgs A;
   A.theta2 = p.theta2;
   A.dmin2 = p.dmin2;
@@ -543,6 +550,7 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
                      0, st, (const float4*)t.leaves, (const uint32_t*)sv, (const NodeRec*)t.rec,
                      (const CubeInfo*)t.cube, vx, vy, vz, t.counters, step, A);
   FG_LAUNCH_CHECK();
+  if (timer) timer->end(3);
 }
 
 }  // namespace
@@ -590,12 +598,13 @@ void TreeBuffers::release() {
 
 void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x,
                    const float* y, const float* z, const float* strength, float* vx, float* vy,
-                   float* vz, const TreeParams& p, StepParams step, int radix_bits) {
+                   float* vz, const TreeParams& p, StepParams step, int radix_bits,
+                   PhaseTimer* timer) {
   if (n < 1) return;
   t.reserve(n);
-  if (ndim == 1) run<1>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits);
-  else if (ndim == 2) run<2>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits);
-  else run<3>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits);
+  if (ndim == 1) run<1>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer);
+  else if (ndim == 2) run<2>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer);
+  else run<3>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer);
 }
 
 }  // namespace fg

@@ -48,6 +48,13 @@ class Stats(C.Structure):
                 ("pad", C.c_uint32)]
 
 
+class Timing(C.Structure):
+    _fields_ = [("ms", C.c_double * 16), ("launches", C.c_uint64 * 16)]
+
+
+PHASES = ["tick", "tree_build", "tree_sort", "walk", "pairs", "link", "collide", "center", "radial",
+          "position", "integrate"]
+
 # every entry point of include/d3fgpu.h: name -> (restype, argtypes)
 SIGNATURES = {
     "fg_abi_version": (C.c_int, []),
@@ -72,6 +79,8 @@ SIGNATURES = {
     "fg_readback_fetch": (C.c_int, [C.c_void_p, C.c_uint64, _d, _d]),
     "fg_buffers": (C.c_int, [C.c_void_p, C.POINTER(BufferInfo)]),
     "fg_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "fg_timing_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fg_get_timing": (C.c_int, [C.c_void_p, C.POINTER(Timing)]),
     "fg_radix_sort_pairs": (C.c_int, [C.c_void_p, C.c_uint32, _u, _u, C.c_uint32, C.c_uint32, _u, _u]),
     "fg_debug_tree": (C.c_int, [C.c_void_p, _f, _i, _u, _u, _u, _u, _u, _u, _u, _d, _d]),
 }
@@ -301,6 +310,14 @@ class Engine:
         s = Stats()
         self._check(lib().fg_get_stats(self.h, C.byref(s)))
         return s
+
+    def timing_enable(self, on=True):
+        self._check(lib().fg_timing_enable(self.h, 1 if on else 0))
+
+    def timing(self):
+        t = Timing()
+        self._check(lib().fg_get_timing(self.h, C.byref(t)))
+        return {name: (t.ms[i], int(t.launches[i])) for i, name in enumerate(PHASES)}
 
     def radix_sort_pairs(self, keys, values, first_bit=0, bits=32):
         keys = np.ascontiguousarray(keys, dtype=np.uint32)

@@ -179,6 +179,34 @@ int fg_buffers(fg_ctx* ctx, fg_buffer_info* out);
 
 int fg_get_stats(fg_ctx* ctx, fg_stats* out);
 
+/* ---- timing (HIP events on the context's stream) -------------------------------- */
+
+/* Phases the timer reports, each the sum of its launches' event-to-event times. */
+enum fg_phase {
+  FG_PHASE_TICK = 0,       /* a whole tick: every pass and the integrator */
+  FG_PHASE_TREE_BUILD = 1, /* bbox, keys, leaves, Karras build, reduction (sort excluded) */
+  FG_PHASE_TREE_SORT = 2,  /* the radix sort of the Morton codes */
+  FG_PHASE_WALK = 3,       /* the tree walk kernel */
+  FG_PHASE_PAIRS = 4,      /* the exact all-pairs many-body kernel */
+  FG_PHASE_LINK = 5,
+  FG_PHASE_COLLIDE = 6,
+  FG_PHASE_CENTER = 7,
+  FG_PHASE_RADIAL = 8,
+  FG_PHASE_POSITION = 9,
+  FG_PHASE_INTEGRATE = 10,
+  FG_PHASE_COUNT = 16
+};
+
+typedef struct fg_timing {
+  double ms[FG_PHASE_COUNT];        /* summed milliseconds since timing was enabled */
+  uint64_t launches[FG_PHASE_COUNT];
+} fg_timing;
+
+/* Start (on != 0, clearing the sums) or stop recording events around every phase. */
+int fg_timing_enable(fg_ctx* ctx, int32_t on);
+/* Wait for queued work and return the sums. */
+int fg_get_timing(fg_ctx* ctx, fg_timing* out);
+
 /* ---- kernels on their own (the reference tests drive sort.js the same way,
  *      test/gpu-sort-test.js:9-38) ----------------------------------------------- */
 
