@@ -161,3 +161,44 @@ def test_gpu_tree_cells_match_d3_tree_sums():
     c = np.sqrt(0.5)
     want = np.sum(s[g["order"]] * c ** (lvl - g["depth"][0]))
     assert abs(g["value"][0] - want) < 1e-9 * abs(want)
+
+
+def layout_rms(n, theta, dims, ticks=300):
+    """The benchmark graph (examples/benchmark.html:86-93) laid out with link(30) +
+    manyBody + center for 300 ticks from d3's start positions; RMS distance from 0."""
+    x, y, z = O.phyllotaxis(n, dims)
+    pos = np.stack([x, y, z], 1)
+    src, dst = O.make_graph(n, 42)
+    bias, st, dist = O.link_defaults(n, src, dst)
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, np.float64)
+        keep.append(a)
+        return O.P(a)
+
+    links = np.ascontiguousarray(np.stack([src, dst], 1).reshape(-1), np.uint32)
+    keep.append(links)
+    d = [O.ForceDesc(type=3, iterations=1, count=len(src), links=O.P(links, O.C.c_uint32), strengths=arr(st),
+                     distances=arr(dist), bias=arr(bias)),
+         O.ForceDesc(type=4, strengths=arr(np.full(n, -30.0)), theta2=theta * theta, distance_min2=1.0,
+                     distance_max2=np.inf),
+         O.ForceDesc(type=1, strength=1.0)]
+    s = O.Sim(pos, ndim=dims)
+    s.set_forces(d)
+    s.tick(ticks)
+    p, _ = s.state()
+    return float(np.sqrt(np.mean(np.sum(p ** 2, 1))))
+
+
+@pytest.mark.parametrize("theta,want", [(0.9, 372), (0.5, 418)])
+def test_layout_size_matches_published_3d(theta, want):
+    """docs/differences.md:35-40: RMS radius of a 1,000-node random graph after 300 ticks,
+    d3-force-3d CPU, 3D: 372 (theta 0.9), 418 (0.5).  Pins the octree Barnes-Hut with d3's
+    sqrt(4/numChildren) damping, the sequential link force and the tick, in aggregate."""
+    assert abs(layout_rms(1000, theta, 3) - want) < 1.0
+
+
+def test_layout_size_matches_published_2d():
+    """docs/differences.md:37: 2D CPU 552 (c = 1 in 2D, so theta does not shape it)."""
+    assert abs(layout_rms(1000, 0.9, 2) - 552) < 1.0
