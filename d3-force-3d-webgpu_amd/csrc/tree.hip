@@ -436,10 +436,12 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
       break;
     }
     --sp;
-    uint32_t e = __builtin_amdgcn_readfirstlane(s_node[wave][sp]);
+    uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane(s_node[wave][sp]);
     uint64_t m = s_mask[wave][sp];
-    m = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
-        __builtin_amdgcn_readfirstlane((uint32_t)m);
+    // readfirstlane yields an int: widen both halves as unsigned (a sign-extended low
+    // word would set every lane of the high half)
+    m = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
+        (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)m);
     const bool in = (m >> lane) & 1ull;
     ++opened;
     const NodeRec& pr = rec[e];
