@@ -80,6 +80,32 @@ __device__ inline uint32_t mask_rank(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Cross-workgroup hand-off without cache maintenance: data goes through coherent (sc1)
+// loads and stores, the writer waits for its stores to complete before the flag atomic
+// that publishes them, and the reader issues its loads only after that atomic returned.
+// An acquire/release atomic at agent scope would instead write back and invalidate the
+// whole L2 at every use.
+template <typename T>
+__device__ inline void store_coherent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline T load_coherent(const T* p) {
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Every earlier memory operation of this wave has completed; later ones stay after it.
+__device__ inline void wait_all_memory() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
+}
+// Publish: counts completed stores; returns the previous count.
+__device__ inline uint32_t publish_count(uint32_t* flag) {
+  wait_all_memory();
+  uint32_t old = __hip_atomic_fetch_add(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return old;
+}
+
 // Per-step values every kernel of a tick reads (Sim uniform, wgsl.js:13-22).
 struct StepParams {
   float alpha;

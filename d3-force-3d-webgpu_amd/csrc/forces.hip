@@ -147,15 +147,22 @@ __global__ void __launch_bounds__(256) center_reduce(uint32_t n, int ndim, const
     for (int a = 0; a < 3; ++a) sh[wave][a] = s[a];
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int a = 0; a < 3; ++a) w->partial[blockIdx.x][a] = sh[0][a] + sh[1][a] + sh[2][a] + sh[3][a];
-    uint32_t done = __hip_atomic_fetch_add(&w->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = done + 1 == gridDim.x;
+    for (int a = 0; a < 3; ++a) store_coherent(&w->partial[blockIdx.x][a], sh[0][a] + sh[1][a] + sh[2][a] + sh[3][a]);
+    s_last = publish_count(&w->done) + 1 == gridDim.x;
   }
   __syncthreads();
-  if (!s_last || threadIdx.x != 0) return;
+  if (!s_last) return;
+  // the last block: sum the partials in block order (deterministic), 256 threads wide
+  __shared__ double tsum[256][3];
   double t[3] = {0, 0, 0};
-  for (uint32_t b = 0; b < gridDim.x; ++b)
-    for (int a = 0; a < 3; ++a) t[a] += w->partial[b][a];
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256)
+    for (int a = 0; a < 3; ++a) t[a] += load_coherent(&w->partial[b][a]);
+  for (int a = 0; a < 3; ++a) tsum[threadIdx.x][a] = t[a];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int a = 0; a < 3; ++a) t[a] = 0;
+  for (int k = 0; k < 256; ++k)
+    for (int a = 0; a < 3; ++a) t[a] += tsum[k][a];
   double c[3] = {cx, cy, cz};
   for (int a = 0; a < 3; ++a) w->shift[a] = a < ndim ? (float)((t[a] / n - c[a]) * strength) : 0.0f;
   w->done = 0;

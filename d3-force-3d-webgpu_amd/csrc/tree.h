@@ -21,6 +21,13 @@ struct alignas(64) NodeRec {
   uint32_t kids[8];
 };
 
+// The first half of a NodeRec: what a parent's walk step reads of each child.
+struct alignas(32) NodeHdr {
+  double cx, cy, cz;
+  float value;
+  uint32_t depth;
+};
+
 // Sums the bottom-up reduction carries per node (FP64).
 struct NodeSum {
   double a;       // sum of strength * c^leafDepth

@@ -110,10 +110,7 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t done = __hip_atomic_fetch_add(&cube->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = done + 1 == gridDim.x;
-  }
+  if (threadIdx.x == 0) s_last = publish_count(&cube->done) + 1 == gridDim.x;
   __syncthreads();
   if (!s_last || threadIdx.x != 0) return;
   // the last block: d3's cover() on the box (barnesHut.js:169-181), then reset the box
@@ -271,23 +268,54 @@ __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, const float4
     if (D > 2) out.gz += w * (double)q.z;
     return;
   }
-  const NodeSum& c = sums[k];
+  // written in this kernel by the thread that completed node k: coherent loads
+  double ca = load_coherent(&sums[k].a);
   uint32_t kd = links[k].z;
-  out.a += c.a;
+  out.a += ca;
   if (kd > parent_depth) {
     // a deeper cell: a d3 child, weighed by |its strength one level below this cell|
-    double w = fabs(c.a * damp<D>(-(int)(parent_depth + 1)));
+    double w = fabs(ca * damp<D>(-(int)(parent_depth + 1)));
     out.w += w;
-    out.gx += w * rec[k].cx;
-    if (D > 1) out.gy += w * rec[k].cy;
-    if (D > 2) out.gz += w * rec[k].cz;
+    out.gx += w * load_coherent(&rec[k].cx);
+    if (D > 1) out.gy += w * load_coherent(&rec[k].cy);
+    if (D > 2) out.gz += w * load_coherent(&rec[k].cz);
   } else {
     // the same cell: its children are this cell's children
-    out.w += c.w;
-    out.gx += c.gx;
-    if (D > 1) out.gy += c.gy;
-    if (D > 2) out.gz += c.gz;
+    out.w += load_coherent(&sums[k].w);
+    out.gx += load_coherent(&sums[k].gx);
+    if (D > 1) out.gy += load_coherent(&sums[k].gy);
+    if (D > 2) out.gz += load_coherent(&sums[k].gz);
   }
+}
+
+// d3's children of binary node p (barnesHut.js:641-671): its two children, with every
+// internal child in the same d3 cell replaced by its own two children, DIMS - 1 times.
+template <int D>
+__device__ inline void collapse_into(uint4 L, const uint4* links, uint32_t* kids) {
+  uint32_t m = 0;
+  const uint32_t pair[2] = {L.x, L.y};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    uint32_t a = pair[c];
+    if (D > 1 && !(a & REF_POINT) && links[a].z == L.z) {
+      uint4 A = links[a];
+      const uint32_t pa[2] = {A.x, A.y};
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        uint32_t b = pa[e];
+        if (D > 2 && !(b & REF_POINT) && links[b].z == L.z) {
+          uint4 B = links[b];
+          kids[m++] = B.x;
+          kids[m++] = B.y;
+        } else {
+          kids[m++] = b;
+        }
+      }
+    } else {
+      kids[m++] = a;
+    }
+  }
+  for (; m < 8; ++m) kids[m] = REF_NONE;
 }
 
 template <int D>
@@ -299,54 +327,35 @@ __global__ void __launch_bounds__(256) tree_reduce(uint32_t n, const float4* __r
   if (s >= n || n < 2) return;
   uint32_t p = leaf_parent[s];
   for (uint32_t guard = 0; guard < n && p != REF_NONE; ++guard) {
-    uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;  // the sibling is still on its way: it will carry on
+    if (publish_count(&flags[p]) == 0) return;  // the sibling is still on its way
     uint4 L = links[p];
     NodeSum sum = {0, 0, 0, 0, 0};
     child_sum<D>(L.x, L.z, leaves, links, rec, sums, sum);
     child_sum<D>(L.y, L.z, leaves, links, rec, sums, sum);
-    NodeRec r;
+    double cx = 0.0, cy = 0.0, cz = 0.0;
     if (sum.w > 0) {
-      r.cx = sum.gx / sum.w;
-      r.cy = D > 1 ? sum.gy / sum.w : 0.0;
-      r.cz = D > 2 ? sum.gz / sum.w : 0.0;
-    } else {
-      r.cx = r.cy = r.cz = 0.0;
+      cx = sum.gx / sum.w;
+      cy = D > 1 ? sum.gy / sum.w : 0.0;
+      cz = D > 2 ? sum.gz / sum.w : 0.0;
     }
     double value = sum.a * damp<D>(-(int)L.z);
     // d3 skips a cell whose strengths add up to exactly 0; sums in another order leave a
     // residue of rounding, so a cancellation to 2^-40 of the cell's weight counts as 0
     double scale = sum.w * damp<D>((int)L.z + 1);
     if (fabs(sum.a) <= fabs(scale) * 0x1p-40 || !(sum.w > 0)) value = 0.0;
+    NodeRec& r = rec[p];
     r.value = (float)value;
     r.depth = L.z;
-    // collapse: look through binary children in the same d3 cell (barnesHut.js:641-671)
-    uint32_t list[8] = {L.x, L.y, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE};
-    int count = 2;
-#pragma unroll
-    for (int level = 1; level < D; ++level) {
-      uint32_t next[8] = {REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE, REF_NONE};
-      int m = 0;
-      for (int c = 0; c < count; ++c) {
-        uint32_t k = list[c];
-        if (!(k & REF_POINT) && links[k].z == L.z) {
-          uint4 b = links[k];
-          next[m] = b.x;
-          next[m + 1] = b.y;
-          m += 2;
-        } else {
-          next[m] = k;
-          m += 1;
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < 8; ++c) list[c] = next[c];
-      count = m;
-    }
-#pragma unroll
-    for (int c = 0; c < 8; ++c) r.kids[c] = list[c];
-    rec[p] = r;
-    sums[p] = sum;
+    collapse_into<D>(L, links, r.kids);
+    // read by the parent's thread in this kernel: coherent stores
+    store_coherent(&r.cx, cx);
+    store_coherent(&r.cy, cy);
+    store_coherent(&r.cz, cz);
+    store_coherent(&sums[p].a, sum.a);
+    store_coherent(&sums[p].w, sum.w);
+    store_coherent(&sums[p].gx, sum.gx);
+    store_coherent(&sums[p].gy, sum.gy);
+    store_coherent(&sums[p].gz, sum.gz);
     p = L.w;
   }
 }
@@ -354,11 +363,22 @@ __global__ void __launch_bounds__(256) tree_reduce(uint32_t n, const float4* __r
 // ---- walk ---------------------------------------------------------------------------
 
 constexpr int WALK_WAVES = 4;
-constexpr int STACK = 256;
+constexpr int STACK = 192;  // per wave; 7 * 21 + 8 = 155 in 3D for distinct codes
 
 struct WalkArgs {
   double theta2, dmin2, dmax2;
   uint32_t n;
+};
+
+// Per wave, in LDS: the stack of nodes to open, each as its depth, the lanes that must
+// open it and its d3 children (so that opening it needs no load of its own record), and a staging
+// area where one coalesced load puts the 64-byte records of all children of the node
+// being opened (points: their float4).
+struct WalkLDS {
+  uint32_t depth[STACK];
+  uint64_t mask[STACK];
+  uint32_t kids[STACK][8];
+  uint32_t stage[8][16];
 };
 
 template <int D>
@@ -367,9 +387,10 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
               const NodeRec* __restrict__ rec, const CubeInfo* __restrict__ cube,
               float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
               uint32_t* __restrict__ counters, StepParams step, WalkArgs A) {
-  __shared__ uint32_t s_node[WALK_WAVES][STACK];
-  __shared__ uint64_t s_mask[WALK_WAVES][STACK];
+  __shared__ WalkLDS lds[WALK_WAVES];
+  constexpr int NK = 1 << D;
   const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  WalkLDS& L = lds[wave];
   const uint32_t n = A.n;
   const uint32_t s = (blockIdx.x * WALK_WAVES + wave) * WAVE + lane;
   const bool live = s < n;
@@ -385,20 +406,21 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   float ax = 0.f, ay = 0.f, az = 0.f;
   uint32_t inter = 0;
 
-  // One cell of node `cur`, for this lane: d3's decision at every depth of its chain.
-  // Returns true when the lane is done with the node (approximated or skipped).
-  auto visit = [&](uint32_t cur, uint32_t top, const NodeRec& r) -> bool {
-    if (top > r.depth) return false;  // points sharing the smallest cell: open
-    if (r.value == 0.0f) return true;  // d3 skips cells with no strength
-    double dx = r.cx - mx, dy = D > 1 ? r.cy - my : 0.0, dz = D > 2 ? r.cz - mz : 0.0;
+  // d3's decision for this lane at every cell of node `cur`'s chain, from depth `top`
+  // to the node's depth (barnesHut.js:719-746). True when the lane is done with the node.
+  auto visit = [&](uint32_t cur, uint32_t top, double cx, double cy, double cz, float value,
+                   uint32_t depth) -> bool {
+    if (top > depth) return false;  // points sharing the smallest cell: open
+    if (value == 0.0f) return true;  // d3 skips cells with no strength
+    double dx = cx - mx, dy = D > 1 ? cy - my : 0.0, dz = D > 2 ? cz - mz : 0.0;
     double l = dx * dx + dy * dy + dz * dz;
     double w = ldexp(size, -(int)top);
     uint32_t Dd = top;
-    while (Dd <= r.depth && !(w * w < theta2 * l)) {
+    while (Dd <= depth && !(w * w < theta2 * l)) {
       w *= 0.5;
       ++Dd;
     }
-    if (Dd > r.depth) return false;
+    if (Dd > depth) return false;
     if (l >= dmax2) return true;
     float fx = (float)dx, fy = (float)dy, fz = (float)dz, fl = (float)l;
     if (l == 0.0) {
@@ -406,7 +428,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
       fl = fx * fx + fy * fy + fz * fz;
     }
     if (fl < dmin2f) fl = sqrtf(dmin2f * fl);
-    float f = r.value * (float)damp<D>((int)(r.depth - Dd)) * alpha * __builtin_amdgcn_rcpf(fl);
+    float f = value * (float)damp<D>((int)(depth - Dd)) * alpha * __builtin_amdgcn_rcpf(fl);
     ax += fx * f;
     if (D > 1) ay += fy * f;
     if (D > 2) az += fz * f;
@@ -418,41 +440,61 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   uint64_t opened = 0;
   uint32_t err = 0;
   {
-    NodeRec r = rec[0];
-    bool done = live ? visit(0u, 0u, r) : true;
+    const NodeRec& r = rec[0];
+    bool done = live ? visit(0u, 0u, r.cx, r.cy, r.cz, r.value, r.depth) : true;
     uint64_t open = __ballot(!done);
     if (open) {
       if (lane == 0) {
-        s_node[wave][0] = 0u;
-        s_mask[wave][0] = open;
+        L.depth[0] = r.depth;
+        L.mask[0] = open;
       }
+      if (lane < 8) L.kids[0][lane] = r.kids[lane];
       sp = 1;
     }
   }
   const uint32_t guard_max = 2u * n + 8u;
+  // lane t stages dwords 2(t & 7), +1 of child t >> 3
+  const uint32_t my_c = lane >> 3, my_w = lane & 7;
   for (uint32_t guard = 0; sp > 0; ++guard) {
     if (guard > guard_max) {
       err = 1;
       break;
     }
     --sp;
-    uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane(s_node[wave][sp]);
-    uint64_t m = s_mask[wave][sp];
-    // readfirstlane yields an int: widen both halves as unsigned (a sign-extended low
-    // word would set every lane of the high half)
+    uint64_t m = L.mask[sp];
     m = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
         (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)m);
     const bool in = (m >> lane) & 1ull;
     ++opened;
-    const NodeRec& pr = rec[e];
-    const uint32_t top = pr.depth + 1;
-#pragma unroll 1
-    for (int c = 0; c < (1 << D); ++c) {
-      uint32_t k = pr.kids[c];
+    // one coalesced load of every child's record into the staging area
+    {
+      const uint32_t k = my_c < (uint32_t)NK ? L.kids[sp][my_c] : REF_NONE;
+      uint2 v = make_uint2(0u, 0u);
+      if (k != REF_NONE) {
+        if (k & REF_POINT) {
+          if (my_w < 2) v = ((const uint2*)(leaves + (k & ~REF_POINT)))[my_w];
+        } else {
+          v = ((const uint2*)(rec + k))[my_w];
+        }
+      }
+      if (my_c < (uint32_t)NK) {
+        L.stage[my_c][2 * my_w] = v.x;
+        L.stage[my_c][2 * my_w + 1] = v.y;
+      }
+    }
+    // the children's cell chains start one level below the popped node's cell
+    const uint32_t top_depth = (uint32_t)__builtin_amdgcn_readfirstlane(L.depth[sp]) + 1;
+    // stage -> each child in turn
+    uint32_t kids_here[NK];
+#pragma unroll
+    for (int c = 0; c < NK; ++c) kids_here[c] = L.kids[sp][c];
+#pragma unroll
+    for (int c = 0; c < NK; ++c) {
+      const uint32_t k = (uint32_t)__builtin_amdgcn_readfirstlane(kids_here[c]);
       if (k == REF_NONE) break;
       if (k & REF_POINT) {
-        uint32_t j = k & ~REF_POINT;
-        float4 q = leaves[j];
+        const uint32_t j = k & ~REF_POINT;
+        const float4 q = *(const float4*)&L.stage[c][0];
         if (in && j != s) {
           float dx = q.x - me4.x, dy = D > 1 ? q.y - me4.y : 0.f, dz = D > 2 ? q.z - me4.z : 0.f;
           float l = dx * dx + dy * dy + dz * dz;
@@ -471,8 +513,8 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
         }
         continue;
       }
-      NodeRec r = rec[k];
-      bool done = in ? visit(k, top, r) : true;
+      const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
+      bool done = in ? visit(k, top_depth, h.cx, h.cy, h.cz, h.value, h.depth) : true;
       uint64_t open = __ballot(!done);
       if (open) {
         if (sp >= STACK) {
@@ -480,9 +522,10 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
           continue;
         }
         if (lane == 0) {
-          s_node[wave][sp] = k;
-          s_mask[wave][sp] = open;
+          L.depth[sp] = h.depth;
+          L.mask[sp] = open;
         }
+        if (lane < 8) L.kids[sp][lane] = L.stage[c][8 + lane];
         ++sp;
       }
     }

@@ -14,10 +14,17 @@ def make_nodes(n, dims, size=200.0, seed=12345):
 
 
 def make_nodes_fast(n, dims, size=200.0, seed=12345):
-    """Same distribution as make_nodes for large n (numpy PCG), f32-exact."""
+    """Same distribution as make_nodes for large n (numpy PCG), f32-exact and without
+    coincident points (those take the hash jiggle, which the FP64 oracle does not model)."""
     rng = np.random.default_rng(seed)
     out = np.zeros((n, 3))
     out[:, :dims] = ((rng.random((n, dims)) - 0.5) * size).astype(np.float32)
+    for _ in range(100):
+        _, first = np.unique(out, axis=0, return_index=True)
+        if len(first) == n:
+            break
+        dup = np.setdiff1d(np.arange(n), first)
+        out[dup, 0] = np.nextafter(out[dup, 0].astype(np.float32), np.float32(np.inf)).astype(np.float64)
     return out
 
 
