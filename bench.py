@@ -146,7 +146,8 @@ def run_gpu(n, steps, warmup, light=False):
     phases = {name: round(v[0] / k, 4) for name, v in t.items() if v[1]}
     walk_ms = t["walk"][0] / max(1, t["walk"][1])
     return dict(ms=ms, phases=phases, walk_ms=walk_ms, walk_launches=t["walk"][1], interactions=int(st.interactions),
-                opened=int(st.opened), walk_errors=int(st.walk_errors), links=m, finite=finite, timed_ticks=k)
+                opened=int(st.opened), lane_visits=int(st.lane_visits), walk_errors=int(st.walk_errors), links=m,
+                finite=finite, timed_ticks=k)
 
 
 def cpu_baseline(n, ticks=1):
@@ -237,6 +238,8 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms_per_tick": r["phases"],
         "walk_errors": r["walk_errors"],
+        "walk_waves_openings": r["opened"],
+        "walk_lane_utilisation": round(r["lane_visits"] / max(1, 64 * r["opened"]), 4),
         "finite": r["finite"],
     }
     if ref is not None:

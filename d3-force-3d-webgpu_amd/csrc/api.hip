@@ -739,6 +739,7 @@ int fg_get_stats(fg_ctx* c, fg_stats* out) {
         FG_HIP(hipMemcpy(h, p->tree->counters, sizeof(h), hipMemcpyDeviceToHost));
         s.interactions = (uint64_t)h[0] | ((uint64_t)h[1] << 32);
         s.opened = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
+        s.lane_visits = (uint64_t)h[6] | ((uint64_t)h[7] << 32);
         s.walk_errors = h[4];
         uint32_t e[4] = {0, 0, 0, 0};
         if (p->tree->sort.errors) FG_HIP(hipMemcpy(e, p->tree->sort.errors, sizeof(e), hipMemcpyDeviceToHost));

@@ -54,11 +54,12 @@ struct TreeBuffers {
   uint32_t* leaf_depth = nullptr;
   uint32_t* leaf_parent = nullptr;
   uint4* links = nullptr;       // per binary node: left, right, depth, parent
+  uint2* span = nullptr;        // per binary node: first and last sorted point it covers
   uint32_t* flags = nullptr;
   NodeRec* rec = nullptr;
   NodeSum* sums = nullptr;
   CubeInfo* cube = nullptr;
-  uint32_t* counters = nullptr; // [0] interactions lo, [1] hi, [2] opened lo, [3] hi, [4] walk errors
+  uint32_t* counters = nullptr; // u64 interactions [0,1], opened [2,3], walk errors [4], lane visits [6,7]
   SortWork sort;
 
   void reserve(uint32_t n);

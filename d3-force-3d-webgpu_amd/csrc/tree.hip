@@ -102,12 +102,17 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
     }
   }
   __shared__ uint32_t s_last;
-  if ((threadIdx.x & (WAVE - 1)) == 0) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(&cube->box[a], float_ordered(lo[a]));
-      atomicMax(&cube->box[3 + a], float_ordered(hi[a]));
-    }
+  __shared__ float s_lo[4][3], s_hi[4][3];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) s_lo[wave][a] = lo[a], s_hi[wave][a] = hi[a];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int a = threadIdx.x;
+    float l = fminf(fminf(s_lo[0][a], s_lo[1][a]), fminf(s_lo[2][a], s_lo[3][a]));
+    float h = fmaxf(fmaxf(s_hi[0][a], s_hi[1][a]), fmaxf(s_hi[2][a], s_hi[3][a]));
+    atomicMin(&cube->box[a], float_ordered(l));
+    atomicMax(&cube->box[3 + a], float_ordered(h));
   }
   __syncthreads();
   if (threadIdx.x == 0) s_last = publish_count(&cube->done) + 1 == gridDim.x;
@@ -215,7 +220,7 @@ __global__ void __launch_bounds__(256) tree_leaves(const uint64_t* __restrict__ 
 
 template <int D>
 __global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ skeys, uint32_t n,
-                                                  uint4* __restrict__ links,
+                                                  uint4* __restrict__ links, uint2* __restrict__ spans,
                                                   uint32_t* __restrict__ leaf_parent) {
   int i = (int)(blockIdx.x * 256 + threadIdx.x);
   const int N = (int)n;
@@ -243,6 +248,7 @@ __global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ s
   links[i].x = left;
   links[i].y = right;
   links[i].z = depth_of<D>(prefix);
+  spans[i] = make_uint2((uint32_t)first, (uint32_t)last);
   if (left & REF_POINT) leaf_parent[split] = (uint32_t)i;
   else links[split].w = (uint32_t)i;
   if (right & REF_POINT) leaf_parent[split + 1] = (uint32_t)i;
@@ -252,10 +258,22 @@ __global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ s
 
 // ---- bottom-up reduction + collapse ---------------------------------------------------
 
+// Loads and stores of a node's sums: plain ones between threads of one workgroup (the
+// node and both its children lie within the workgroup's points), coherent ones otherwise.
+template <typename T>
+__device__ inline T ld(const T* p, bool coherent) {
+  return coherent ? load_coherent(p) : *p;
+}
+template <typename T>
+__device__ inline void st(T* p, T v, bool coherent) {
+  if (coherent) store_coherent(p, v);
+  else *p = v;
+}
+
 template <int D>
-__device__ inline void child_sum(uint32_t k, uint32_t parent_depth, const float4* leaves,
-                                 const uint4* links, const NodeRec* rec, const NodeSum* sums,
-                                 NodeSum& out) {
+__device__ inline void child_sum(uint32_t k, uint32_t parent_depth, bool coherent,
+                                 const float4* leaves, const uint4* links, const NodeRec* rec,
+                                 const NodeSum* sums, NodeSum& out) {
   if (k & REF_POINT) {
     // a point is a d3 child one level below its binary parent's cell
     float4 q = leaves[k & ~REF_POINT];
@@ -268,23 +286,22 @@ __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, const float4
     if (D > 2) out.gz += w * (double)q.z;
     return;
   }
-  // written in this kernel by the thread that completed node k: coherent loads
-  double ca = load_coherent(&sums[k].a);
+  double ca = ld(&sums[k].a, coherent);
   uint32_t kd = links[k].z;
   out.a += ca;
   if (kd > parent_depth) {
     // a deeper cell: a d3 child, weighed by |its strength one level below this cell|
     double w = fabs(ca * damp<D>(-(int)(parent_depth + 1)));
     out.w += w;
-    out.gx += w * load_coherent(&rec[k].cx);
-    if (D > 1) out.gy += w * load_coherent(&rec[k].cy);
-    if (D > 2) out.gz += w * load_coherent(&rec[k].cz);
+    out.gx += w * ld(&rec[k].cx, coherent);
+    if (D > 1) out.gy += w * ld(&rec[k].cy, coherent);
+    if (D > 2) out.gz += w * ld(&rec[k].cz, coherent);
   } else {
     // the same cell: its children are this cell's children
-    out.w += load_coherent(&sums[k].w);
-    out.gx += load_coherent(&sums[k].gx);
-    if (D > 1) out.gy += load_coherent(&sums[k].gy);
-    if (D > 2) out.gz += load_coherent(&sums[k].gz);
+    out.w += ld(&sums[k].w, coherent);
+    out.gx += ld(&sums[k].gx, coherent);
+    if (D > 1) out.gy += ld(&sums[k].gy, coherent);
+    if (D > 2) out.gz += ld(&sums[k].gz, coherent);
   }
 }
 
@@ -318,20 +335,42 @@ __device__ inline void collapse_into(uint4 L, const uint4* links, uint32_t* kids
   for (; m < 8; ++m) kids[m] = REF_NONE;
 }
 
+constexpr uint32_t REDUCE_BLOCK = 512;
+
+// One thread per point climbs towards the root; the second of two siblings to reach a
+// node combines them.  Inside the subtrees that lie within this workgroup's points the
+// hand-off is between waves of one workgroup (plain memory, workgroup-scope atomics);
+// above them it is coherent across the device.
 template <int D>
-__global__ void __launch_bounds__(256) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
-                                                   const uint32_t* __restrict__ leaf_parent,
-                                                   const uint4* __restrict__ links,
-                                                   uint32_t* flags, NodeRec* rec, NodeSum* sums) {
-  uint32_t s = blockIdx.x * 256 + threadIdx.x;
+__global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
+                                                            const uint32_t* __restrict__ leaf_parent,
+                                                            const uint4* __restrict__ links,
+                                                            const uint2* __restrict__ span,
+                                                            uint32_t* flags, NodeRec* rec, NodeSum* sums) {
+  const uint32_t s = blockIdx.x * REDUCE_BLOCK + threadIdx.x;
+  const uint32_t lo = blockIdx.x * REDUCE_BLOCK, hi = lo + REDUCE_BLOCK - 1;
   if (s >= n || n < 2) return;
   uint32_t p = leaf_parent[s];
+  bool local = true;  // whether node p lies within this workgroup's points
+  {
+    uint2 sp = span[p];
+    local = sp.x >= lo && sp.y <= hi;
+  }
   for (uint32_t guard = 0; guard < n && p != REF_NONE; ++guard) {
-    if (publish_count(&flags[p]) == 0) return;  // the sibling is still on its way
+    uint32_t old;
+    if (local) {
+      old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      old = publish_count(&flags[p]);
+    }
+    if (old == 0) return;  // the sibling is still on its way
+    // children of a local node were written by this workgroup; children of a shared
+    // node were written coherently (their parent, p, is not local to their writer)
+    const bool coherent_in = !local;
     uint4 L = links[p];
     NodeSum sum = {0, 0, 0, 0, 0};
-    child_sum<D>(L.x, L.z, leaves, links, rec, sums, sum);
-    child_sum<D>(L.y, L.z, leaves, links, rec, sums, sum);
+    child_sum<D>(L.x, L.z, coherent_in, leaves, links, rec, sums, sum);
+    child_sum<D>(L.y, L.z, coherent_in, leaves, links, rec, sums, sum);
     double cx = 0.0, cy = 0.0, cz = 0.0;
     if (sum.w > 0) {
       cx = sum.gx / sum.w;
@@ -347,16 +386,24 @@ __global__ void __launch_bounds__(256) tree_reduce(uint32_t n, const float4* __r
     r.value = (float)value;
     r.depth = L.z;
     collapse_into<D>(L, links, r.kids);
-    // read by the parent's thread in this kernel: coherent stores
-    store_coherent(&r.cx, cx);
-    store_coherent(&r.cy, cy);
-    store_coherent(&r.cz, cz);
-    store_coherent(&sums[p].a, sum.a);
-    store_coherent(&sums[p].w, sum.w);
-    store_coherent(&sums[p].gx, sum.gx);
-    store_coherent(&sums[p].gy, sum.gy);
-    store_coherent(&sums[p].gz, sum.gz);
-    p = L.w;
+    const uint32_t parent = L.w;
+    bool parent_local = false;
+    if (parent != REF_NONE) {
+      uint2 ps = span[parent];
+      parent_local = ps.x >= lo && ps.y <= hi;
+    }
+    // the parent's thread reads these: coherently unless it is in this workgroup
+    const bool coherent_out = !parent_local;
+    st(&r.cx, cx, coherent_out);
+    st(&r.cy, cy, coherent_out);
+    st(&r.cz, cz, coherent_out);
+    st(&sums[p].a, sum.a, coherent_out);
+    st(&sums[p].w, sum.w, coherent_out);
+    st(&sums[p].gx, sum.gx, coherent_out);
+    st(&sums[p].gy, sum.gy, coherent_out);
+    st(&sums[p].gz, sum.gz, coherent_out);
+    p = parent;
+    local = parent_local;
   }
 }
 
@@ -437,7 +484,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   };
 
   uint32_t sp = 0;
-  uint64_t opened = 0;
+  uint64_t opened = 0, lane_visits = 0;
   uint32_t err = 0;
   {
     const NodeRec& r = rec[0];
@@ -466,6 +513,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
         (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)m);
     const bool in = (m >> lane) & 1ull;
     ++opened;
+    lane_visits += (uint64_t)__popcll(m);
     // one coalesced load of every child's record into the staging area
     {
       const uint32_t k = my_c < (uint32_t)NK ? L.kids[sp][my_c] : REF_NONE;
@@ -543,6 +591,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   if (lane == 0) {
     atomicAdd((unsigned long long*)&counters[0], (unsigned long long)tot);
     atomicAdd((unsigned long long*)&counters[2], (unsigned long long)opened);
+    atomicAdd((unsigned long long*)&counters[6], (unsigned long long)lane_visits);
     if (err) atomicAdd(&counters[4], 1u);
   }
 }
@@ -579,11 +628,11 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
     return;
   }
   hipLaunchKernelGGL((tree_build<D>), dim3(ceil_div(n - 1, 256)), dim3(256), 0, st,
-                     (const uint64_t*)sk, n, t.links, t.leaf_parent);
+                     (const uint64_t*)sk, n, t.links, t.span, t.leaf_parent);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL((tree_reduce<D>), dim3(blocks), dim3(256), 0, st, n,
+  hipLaunchKernelGGL((tree_reduce<D>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st, n,
                      (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
-                     (const uint4*)t.links, t.flags, t.rec, t.sums);
+                     (const uint4*)t.links, (const uint2*)t.span, t.flags, t.rec, t.sums);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(3);
   WalkArgs A;
@@ -610,6 +659,7 @@ void TreeBuffers::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&leaf_depth, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&leaf_parent, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&links, sizeof(uint4) * m));
+  FG_HIP(hipMalloc(&span, sizeof(uint2) * m));
   FG_HIP(hipMalloc(&flags, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
@@ -624,7 +674,7 @@ void TreeBuffers::reserve(uint32_t n) {
 }
 
 void TreeBuffers::release() {
-  void* ptrs[] = {keys, vals, leaves, leaf_depth, leaf_parent, links, flags, rec, sums, cube, counters};
+  void* ptrs[] = {keys, vals, leaves, leaf_depth, leaf_parent, links, span, flags, rec, sums, cube, counters};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = nullptr;
@@ -632,6 +682,7 @@ void TreeBuffers::release() {
   leaves = nullptr;
   leaf_depth = leaf_parent = flags = counters = nullptr;
   links = nullptr;
+  span = nullptr;
   rec = nullptr;
   sums = nullptr;
   cube = nullptr;

@@ -44,8 +44,7 @@ class BufferInfo(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("interactions", C.c_uint64), ("opened", C.c_uint64),
-                ("long_runs", C.c_uint32), ("sort_bits", C.c_uint32), ("walk_errors", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("lane_visits", C.c_uint64), ("sort_bits", C.c_uint32), ("walk_errors", C.c_uint32)]
 
 
 class Timing(C.Structure):

@@ -92,10 +92,9 @@ typedef struct fg_stats {
   uint64_t steps;            /* steps run since set_nodes */
   uint64_t interactions;     /* body-source evaluations in the last walk (0 if not counted) */
   uint64_t opened;           /* wave-level node openings in the last walk */
-  uint32_t long_runs;        /* runs of > 64 equal high halves sorted separately */
+  uint64_t lane_visits;      /* sum over openings of the lanes (points) that opened the node */
   uint32_t sort_bits;        /* bits per radix digit used by the tree sort */
   uint32_t walk_errors;      /* walks that hit their stack or iteration bound (must be 0) */
-  uint32_t pad;
 } fg_stats;
 
 /* ---- library ------------------------------------------------------------------- */

@@ -26,3 +26,17 @@ def engine_factory():
     yield make
     for e in made:
         e.close()
+
+
+def pytest_sessionfinish(session, exitstatus):
+    """Write the rel-L2 figures the GPU parity tests measured to gpurun_out/parity.json."""
+    import json
+    try:
+        from helpers import PARITY
+    except Exception:
+        return
+    if PARITY:
+        out = os.path.join(ROOT, "gpurun_out")
+        os.makedirs(out, exist_ok=True)
+        with open(os.path.join(out, "parity.json"), "w") as f:
+            json.dump(PARITY, f, indent=1)

@@ -28,6 +28,9 @@ def make_nodes_fast(n, dims, size=200.0, seed=12345):
     return out
 
 
+PARITY = []  # (name, rel-L2, extra) measured by the GPU parity tests of this session
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)

@@ -16,7 +16,7 @@ import pytest
 
 import d3fgpu
 import oracle as O
-from helpers import close_count, make_nodes, make_nodes_fast, rel_l2, tree_outliers
+from helpers import PARITY, close_count, make_nodes, make_nodes_fast, rel_l2, tree_outliers
 
 pytestmark = pytest.mark.gpu
 
@@ -38,6 +38,7 @@ def oracle_manybody(pos, dims, strengths, alpha=1.0, theta=0.9, dmin=1.0, dmax=n
 
 def report(name, err, **kw):
     extra = " ".join(f"{k}={v}" for k, v in kw.items())
+    PARITY.append({"case": name, "rel_l2": err, **kw})
     print(f"[parity] {name}: rel-L2 {err:.3e} {extra}", flush=True)
 
 
