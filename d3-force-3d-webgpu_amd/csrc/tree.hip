@@ -415,18 +415,79 @@ constexpr int STACK = 192;  // per wave; 7 * 21 + 8 = 155 in 3D for distinct cod
 struct WalkArgs {
   double theta2, dmin2, dmax2;
   uint32_t n;
+  uint32_t packed_max;  // openings by at most this many points use the packed mode
 };
 
 // Per wave, in LDS: the stack of nodes to open, each as its depth, the lanes that must
-// open it and its d3 children (so that opening it needs no load of its own record), and a staging
-// area where one coalesced load puts the 64-byte records of all children of the node
-// being opened (points: their float4).
+// open it and its d3 children (so that opening it needs no load of its own record); a
+// staging area where one coalesced load puts the 64-byte records of all children of the
+// node being opened (points: their float4); the wave's points and their force sums from
+// the packed mode.
 struct WalkLDS {
   uint32_t depth[STACK];
   uint64_t mask[STACK];
   uint32_t kids[STACK][8];
   uint32_t stage[8][16];
+  float4 body[WAVE];      // x, y, z, sorted index (as bits)
+  float acc[3][WAVE];
+  uint32_t list[WAVE];    // the points opening the current node, in lane order
+  uint64_t ballots[8];
 };
+
+// The force of one cell (a node's chain from depth `top`) or point on one body, with d3's
+// rules (barnesHut.js:705-746, manyBody.js:73-115).  Returns whether the body is done
+// with the node (false: it must open it).  Adds the force to f.
+template <int D>
+__device__ inline bool cell_force(double mx, double my, double mz, uint32_t self_node, uint32_t n,
+                                  uint32_t cur, uint32_t top, double cx, double cy, double cz,
+                                  float value, uint32_t depth, double size, double theta2,
+                                  double dmax2, float dmin2f, float alpha, uint32_t step_no,
+                                  uint32_t seed, float& fx, float& fy, float& fz, uint32_t& inter) {
+  if (top > depth) return false;  // points sharing the smallest cell: open
+  if (value == 0.0f) return true;  // d3 skips cells with no strength
+  double dx = cx - mx, dy = D > 1 ? cy - my : 0.0, dz = D > 2 ? cz - mz : 0.0;
+  double l = dx * dx + dy * dy + dz * dz;
+  double w = ldexp(size, -(int)top);
+  uint32_t Dd = top;
+  while (Dd <= depth && !(w * w < theta2 * l)) {
+    w *= 0.5;
+    ++Dd;
+  }
+  if (Dd > depth) return false;
+  if (l >= dmax2) return true;
+  float ex = (float)dx, ey = (float)dy, ez = (float)dz, fl = (float)l;
+  if (l == 0.0) {
+    coincident(self_node, n + cur, D, step_no, seed, ex, ey, ez);
+    fl = ex * ex + ey * ey + ez * ez;
+  }
+  if (fl < dmin2f) fl = sqrtf(dmin2f * fl);
+  float f = value * (float)damp<D>((int)(depth - Dd)) * alpha * __builtin_amdgcn_rcpf(fl);
+  fx += ex * f;
+  if (D > 1) fy += ey * f;
+  if (D > 2) fz += ez * f;
+  ++inter;
+  return true;
+}
+
+template <int D>
+__device__ inline void point_force(float mx, float my, float mz, uint32_t self_node, float4 q,
+                                   const uint32_t* order, uint32_t j, float dmax2f, float dmin2f,
+                                   float alpha, uint32_t step_no, uint32_t seed, float& fx,
+                                   float& fy, float& fz, uint32_t& inter) {
+  float dx = q.x - mx, dy = D > 1 ? q.y - my : 0.f, dz = D > 2 ? q.z - mz : 0.f;
+  float l = dx * dx + dy * dy + dz * dz;
+  if (!(l < dmax2f)) return;
+  if (l == 0.0f) {
+    coincident(self_node, order[j], D, step_no, seed, dx, dy, dz);
+    l = dx * dx + dy * dy + dz * dz;
+  }
+  if (l < dmin2f) l = sqrtf(dmin2f * l);
+  float f = q.w * alpha * __builtin_amdgcn_rcpf(l);
+  fx += dx * f;
+  if (D > 1) fy += dy * f;
+  if (D > 2) fz += dz * f;
+  ++inter;
+}
 
 template <int D>
 __global__ void __launch_bounds__(WALK_WAVES * WAVE)
@@ -439,56 +500,33 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
   WalkLDS& L = lds[wave];
   const uint32_t n = A.n;
-  const uint32_t s = (blockIdx.x * WALK_WAVES + wave) * WAVE + lane;
+  const uint32_t base = (blockIdx.x * WALK_WAVES + wave) * WAVE;
+  const uint32_t s = base + lane;
   const bool live = s < n;
   if (__ballot(live) == 0ull) return;
   const float alpha = step.alpha;
   const uint32_t step_no = step.step, seed = step.seed;
   const double size = (double)cube->size;
-  const double theta2 = A.theta2, dmin2 = A.dmin2, dmax2 = A.dmax2;
-  const float dmin2f = (float)dmin2, dmax2f = (float)fmin(dmax2, 3.0e38);
+  const double theta2 = A.theta2, dmax2 = A.dmax2;
+  const float dmin2f = (float)A.dmin2, dmax2f = (float)fmin(dmax2, 3.0e38);
   float4 me4 = live ? leaves[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const double mx = me4.x, my = me4.y, mz = me4.z;
   const uint32_t self_node = live ? order[s] : 0u;
+  L.body[lane] = make_float4(me4.x, me4.y, me4.z, __uint_as_float(self_node));
+  L.acc[0][lane] = 0.f;
+  L.acc[1][lane] = 0.f;
+  L.acc[2][lane] = 0.f;
   float ax = 0.f, ay = 0.f, az = 0.f;
   uint32_t inter = 0;
-
-  // d3's decision for this lane at every cell of node `cur`'s chain, from depth `top`
-  // to the node's depth (barnesHut.js:719-746). True when the lane is done with the node.
-  auto visit = [&](uint32_t cur, uint32_t top, double cx, double cy, double cz, float value,
-                   uint32_t depth) -> bool {
-    if (top > depth) return false;  // points sharing the smallest cell: open
-    if (value == 0.0f) return true;  // d3 skips cells with no strength
-    double dx = cx - mx, dy = D > 1 ? cy - my : 0.0, dz = D > 2 ? cz - mz : 0.0;
-    double l = dx * dx + dy * dy + dz * dz;
-    double w = ldexp(size, -(int)top);
-    uint32_t Dd = top;
-    while (Dd <= depth && !(w * w < theta2 * l)) {
-      w *= 0.5;
-      ++Dd;
-    }
-    if (Dd > depth) return false;
-    if (l >= dmax2) return true;
-    float fx = (float)dx, fy = (float)dy, fz = (float)dz, fl = (float)l;
-    if (l == 0.0) {
-      coincident(self_node, n + cur, D, step_no, seed, fx, fy, fz);
-      fl = fx * fx + fy * fy + fz * fz;
-    }
-    if (fl < dmin2f) fl = sqrtf(dmin2f * fl);
-    float f = value * (float)damp<D>((int)(depth - Dd)) * alpha * __builtin_amdgcn_rcpf(fl);
-    ax += fx * f;
-    if (D > 1) ay += fy * f;
-    if (D > 2) az += fz * f;
-    ++inter;
-    return true;
-  };
 
   uint32_t sp = 0;
   uint64_t opened = 0, lane_visits = 0;
   uint32_t err = 0;
   {
     const NodeRec& r = rec[0];
-    bool done = live ? visit(0u, 0u, r.cx, r.cy, r.cz, r.value, r.depth) : true;
+    bool done = live ? cell_force<D>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, r.cx, r.cy, r.cz,
+                                     r.value, r.depth, size, theta2, dmax2, dmin2f, alpha, step_no,
+                                     seed, ax, ay, az, inter)
+                     : true;
     uint64_t open = __ballot(!done);
     if (open) {
       if (lane == 0) {
@@ -512,8 +550,9 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
     m = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
         (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)m);
     const bool in = (m >> lane) & 1ull;
+    const uint32_t k_active = (uint32_t)__popcll(m);
     ++opened;
-    lane_visits += (uint64_t)__popcll(m);
+    lane_visits += k_active;
     // one coalesced load of every child's record into the staging area
     {
       const uint32_t k = my_c < (uint32_t)NK ? L.kids[sp][my_c] : REF_NONE;
@@ -531,58 +570,110 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
       }
     }
     // the children's cell chains start one level below the popped node's cell
-    const uint32_t top_depth = (uint32_t)__builtin_amdgcn_readfirstlane(L.depth[sp]) + 1;
-    // stage -> each child in turn
+    const uint32_t top = (uint32_t)__builtin_amdgcn_readfirstlane(L.depth[sp]) + 1;
     uint32_t kids_here[NK];
 #pragma unroll
-    for (int c = 0; c < NK; ++c) kids_here[c] = L.kids[sp][c];
+    for (int c = 0; c < NK; ++c) kids_here[c] = (uint32_t)__builtin_amdgcn_readfirstlane(L.kids[sp][c]);
+    uint32_t nk = 0;
 #pragma unroll
-    for (int c = 0; c < NK; ++c) {
-      const uint32_t k = (uint32_t)__builtin_amdgcn_readfirstlane(kids_here[c]);
-      if (k == REF_NONE) break;
-      if (k & REF_POINT) {
-        const uint32_t j = k & ~REF_POINT;
-        const float4 q = *(const float4*)&L.stage[c][0];
-        if (in && j != s) {
-          float dx = q.x - me4.x, dy = D > 1 ? q.y - me4.y : 0.f, dz = D > 2 ? q.z - me4.z : 0.f;
-          float l = dx * dx + dy * dy + dz * dz;
-          if (l < dmax2f) {
-            if (l == 0.0f) {
-              coincident(self_node, order[j], D, step_no, seed, dx, dy, dz);
-              l = dx * dx + dy * dy + dz * dz;
-            }
-            if (l < dmin2f) l = sqrtf(dmin2f * l);
-            float f = q.w * alpha * __builtin_amdgcn_rcpf(l);
-            ax += dx * f;
-            if (D > 1) ay += dy * f;
-            if (D > 2) az += dz * f;
-            ++inter;
+    for (int c = 0; c < NK; ++c) nk += kids_here[c] != REF_NONE ? 1u : 0u;
+
+    uint32_t open_bits = 0;  // bit c: this lane's point must open child c
+    if (k_active <= A.packed_max) {
+      // packed: item t of round r is (point list[t / NK], child t % NK), NK items per point
+      if (in) L.list[mask_rank(m)] = lane;
+      const uint32_t rounds = (k_active * NK + WAVE - 1) / WAVE;
+      for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t t = r * WAVE + lane;
+        const uint32_t i = t / NK, c = t % NK;
+        bool open_item = false;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        if (i < k_active && c < nk) {
+          const uint32_t b = L.list[i];
+          const float4 bd = L.body[b];
+          const uint32_t bnode = __float_as_uint(bd.w);
+          uint32_t k = REF_NONE;
+#pragma unroll
+          for (int q = 0; q < NK; ++q)
+            if ((uint32_t)q == c) k = kids_here[q];
+          if (k & REF_POINT) {
+            const uint32_t j = k & ~REF_POINT;
+            if (j != base + b)
+              point_force<D>(bd.x, bd.y, bd.z, bnode, *(const float4*)&L.stage[c][0], order, j,
+                             dmax2f, dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
+          } else {
+            const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
+            open_item = !cell_force<D>(bd.x, bd.y, bd.z, bnode, n, k, top, h.cx, h.cy, h.cz, h.value,
+                                       h.depth, size, theta2, dmax2, dmin2f, alpha, step_no, seed,
+                                       fx, fy, fz, inter);
           }
         }
-        continue;
+        // sum the NK items of each point (aligned groups of NK lanes)
+#pragma unroll
+        for (int o = NK / 2; o > 0; o >>= 1) {
+          fx += __shfl_xor(fx, o, WAVE);
+          if (D > 1) fy += __shfl_xor(fy, o, WAVE);
+          if (D > 2) fz += __shfl_xor(fz, o, WAVE);
+        }
+        if (c == 0 && i < k_active) {
+          const uint32_t b = L.list[i];
+          L.acc[0][b] += fx;
+          if (D > 1) L.acc[1][b] += fy;
+          if (D > 2) L.acc[2][b] += fz;
+        }
+        const uint64_t bal = __ballot(open_item);
+        if (lane == 0) L.ballots[r] = bal;
       }
-      const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
-      bool done = in ? visit(k, top_depth, h.cx, h.cy, h.cz, h.value, h.depth) : true;
-      uint64_t open = __ballot(!done);
-      if (open) {
-        if (sp >= STACK) {
-          err = 1;
+      if (in) {
+        const uint32_t i = mask_rank(m);
+        const uint64_t bal = L.ballots[(i * NK) / WAVE];
+        open_bits = (uint32_t)(bal >> ((i * NK) % WAVE)) & ((1u << NK) - 1u);
+      }
+    } else {
+      // one lane per point, children in turn
+#pragma unroll
+      for (int c = 0; c < NK; ++c) {
+        const uint32_t k = kids_here[c];
+        if (k == REF_NONE) break;
+        if (k & REF_POINT) {
+          const uint32_t j = k & ~REF_POINT;
+          if (in && j != s)
+            point_force<D>(me4.x, me4.y, me4.z, self_node, *(const float4*)&L.stage[c][0], order, j,
+                           dmax2f, dmin2f, alpha, step_no, seed, ax, ay, az, inter);
           continue;
         }
-        if (lane == 0) {
-          L.depth[sp] = h.depth;
-          L.mask[sp] = open;
-        }
-        if (lane < 8) L.kids[sp][lane] = L.stage[c][8 + lane];
-        ++sp;
+        const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
+        bool done = in ? cell_force<D>(me4.x, me4.y, me4.z, self_node, n, k, top, h.cx, h.cy, h.cz,
+                                       h.value, h.depth, size, theta2, dmax2, dmin2f, alpha, step_no,
+                                       seed, ax, ay, az, inter)
+                       : true;
+        if (!done) open_bits |= 1u << c;
       }
+    }
+    // push the children some lane must open
+#pragma unroll
+    for (int c = 0; c < NK; ++c) {
+      const uint32_t k = kids_here[c];
+      if (k == REF_NONE) break;
+      const uint64_t open = __ballot(in && ((open_bits >> c) & 1u));
+      if (!open) continue;
+      if (sp >= STACK) {
+        err = 1;
+        continue;
+      }
+      if (lane == 0) {
+        L.depth[sp] = ((const NodeHdr*)&L.stage[c][0])->depth;
+        L.mask[sp] = open;
+      }
+      if (lane < 8) L.kids[sp][lane] = L.stage[c][8 + lane];
+      ++sp;
     }
   }
   if (live) {
     uint32_t o = self_node;
-    vx[o] += ax;
-    if (D > 1) vy[o] += ay;
-    if (D > 2) vz[o] += az;
+    vx[o] += ax + L.acc[0][lane];
+    if (D > 1) vy[o] += ay + L.acc[1][lane];
+    if (D > 2) vz[o] += az + L.acc[2][lane];
   }
   // counters: one atomic per wave
   uint32_t tot = inter;
@@ -640,6 +731,7 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   A.dmin2 = p.dmin2;
   A.dmax2 = p.dmax2;
   A.n = n;
+  A.packed_max = 24;
   hipLaunchKernelGGL((tree_walk<D>), dim3(ceil_div(n, WALK_WAVES * WAVE)), dim3(WALK_WAVES * WAVE),
                      0, st, (const float4*)t.leaves, (const uint32_t*)sv, (const NodeRec*)t.rec,
                      (const CubeInfo*)t.cube, vx, vy, vz, t.counters, step, A);
