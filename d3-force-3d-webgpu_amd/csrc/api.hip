@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <string>
 #include <vector>
@@ -153,6 +154,7 @@ struct fg_ctx {
   uint64_t steps = 0;
   uint64_t epoch = 1;
   int radix_bits = 8;
+  uint32_t walk_packed_max = 24;
   std::string err;
   // readback
   float* staging = nullptr;
@@ -299,7 +301,7 @@ void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
                               finite_f32(p.d.distance_max2), sp);
       } else {
         if (!p.tree) p.tree.reset(new TreeBuffers());
-        TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2};
+        TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2, c->walk_packed_max};
         tree_manybody(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), vel[0],
                       vel[1], vel[2], tp, sp, c->radix_bits, c->T());
       }
@@ -421,6 +423,9 @@ int fg_create(fg_ctx** out, int device) {
   fg_ctx* c = new (std::nothrow) fg_ctx();
   if (!c) return FG_ERR_NOMEM;
   c->device = device;
+  // experiment overrides (the tuned defaults apply otherwise)
+  if (const char* v = getenv("D3FGPU_WALK_PACKED_MAX")) c->walk_packed_max = (uint32_t)atoi(v);
+  if (const char* v = getenv("D3FGPU_RADIX_BITS")) c->radix_bits = atoi(v) == 11 ? 11 : 8;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return FG_ERR_HIP;

@@ -731,7 +731,7 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   A.dmin2 = p.dmin2;
   A.dmax2 = p.dmax2;
   A.n = n;
-  A.packed_max = 24;
+  A.packed_max = p.walk_packed_max;
   hipLaunchKernelGGL((tree_walk<D>), dim3(ceil_div(n, WALK_WAVES * WAVE)), dim3(WALK_WAVES * WAVE),
                      0, st, (const float4*)t.leaves, (const uint32_t*)sv, (const NodeRec*)t.rec,
                      (const CubeInfo*)t.cube, vx, vy, vz, t.counters, step, A);
