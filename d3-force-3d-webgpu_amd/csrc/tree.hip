@@ -431,37 +431,55 @@ struct WalkLDS {
   float4 body[WAVE];      // x, y, z, sorted index (as bits)
   float acc[3][WAVE];
   uint32_t list[WAVE];    // the points opening the current node, in lane order
+  uint32_t cur[8];        // the children of the node being opened
   uint64_t ballots[8];
 };
 
-// The force of one cell (a node's chain from depth `top`) or point on one body, with d3's
-// rules (barnesHut.js:705-746, manyBody.js:73-115).  Returns whether the body is done
-// with the node (false: it must open it).  Adds the force to f.
+// The jiggled displacement of two coincident bodies: out of line, it is rare.
+__device__ __noinline__ float4 coincident_far(uint32_t i, uint32_t j, int ndim, uint32_t step_no,
+                                              uint32_t seed) {
+  float dx, dy, dz;
+  coincident(i, j, ndim, step_no, seed, dx, dy, dz);
+  return make_float4(dx, dy, dz, dx * dx + dy * dy + dz * dz);
+}
+
+// c^k in f32 for k >= 0 (the damping of a cell k levels above the node's deepest cell).
+template <int D>
+__device__ inline float dampf(uint32_t k) {
+  if (D == 2) return 1.0f;
+  const float odd = (k & 1u) ? (D == 3 ? 0.70710678118654752f : 1.41421356237309505f) : 1.0f;
+  return ldexpf(odd, D == 3 ? -(int)(k >> 1) : (int)(k >> 1));
+}
+
+// The force of one node (its cell chain from depth `top`, whose first width squared is
+// w2top) on one body, with d3's rules (barnesHut.js:719-746, manyBody.js:73-99).
+// Returns whether the body is done with the node (false: it must open it).
 template <int D>
 __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self_node, uint32_t n,
-                                  uint32_t cur, uint32_t top, double cx, double cy, double cz,
-                                  float value, uint32_t depth, double size, double theta2,
-                                  double dmax2, float dmin2f, float alpha, uint32_t step_no,
-                                  uint32_t seed, float& fx, float& fy, float& fz, uint32_t& inter) {
-  if (top > depth) return false;  // points sharing the smallest cell: open
-  if (value == 0.0f) return true;  // d3 skips cells with no strength
-  double dx = cx - mx, dy = D > 1 ? cy - my : 0.0, dz = D > 2 ? cz - mz : 0.0;
-  double l = dx * dx + dy * dy + dz * dz;
-  double w = ldexp(size, -(int)top);
+                                  uint32_t cur, uint32_t top, double w2top, const NodeHdr& h,
+                                  double theta2, double dmax2, float dmin2f, float alpha,
+                                  uint32_t step_no, uint32_t seed, float& fx, float& fy, float& fz,
+                                  uint32_t& inter) {
+  if (top > h.depth) return false;  // points sharing the smallest cell: open
+  if (h.value == 0.0f) return true;  // d3 skips cells with no strength
+  const double dx = h.cx - mx, dy = D > 1 ? h.cy - my : 0.0, dz = D > 2 ? h.cz - mz : 0.0;
+  const double l = dx * dx + dy * dy + dz * dz;
+  const double reach = theta2 * l;
+  double w2 = w2top;
   uint32_t Dd = top;
-  while (Dd <= depth && !(w * w < theta2 * l)) {
-    w *= 0.5;
+  while (Dd <= h.depth && !(w2 < reach)) {
+    w2 *= 0.25;
     ++Dd;
   }
-  if (Dd > depth) return false;
+  if (Dd > h.depth) return false;
   if (l >= dmax2) return true;
   float ex = (float)dx, ey = (float)dy, ez = (float)dz, fl = (float)l;
   if (l == 0.0) {
-    coincident(self_node, n + cur, D, step_no, seed, ex, ey, ez);
-    fl = ex * ex + ey * ey + ez * ez;
+    const float4 j = coincident_far(self_node, n + cur, D, step_no, seed);
+    ex = j.x, ey = j.y, ez = j.z, fl = j.w;
   }
   if (fl < dmin2f) fl = sqrtf(dmin2f * fl);
-  float f = value * (float)damp<D>((int)(depth - Dd)) * alpha * __builtin_amdgcn_rcpf(fl);
+  const float f = h.value * dampf<D>(h.depth - Dd) * alpha * __builtin_amdgcn_rcpf(fl);
   fx += ex * f;
   if (D > 1) fy += ey * f;
   if (D > 2) fz += ez * f;
@@ -469,6 +487,7 @@ __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self
   return true;
 }
 
+// The force of one point on one body (barnesHut.js:705-717, manyBody.js:99-113).
 template <int D>
 __device__ inline void point_force(float mx, float my, float mz, uint32_t self_node, float4 q,
                                    const uint32_t* order, uint32_t j, float dmax2f, float dmin2f,
@@ -478,11 +497,11 @@ __device__ inline void point_force(float mx, float my, float mz, uint32_t self_n
   float l = dx * dx + dy * dy + dz * dz;
   if (!(l < dmax2f)) return;
   if (l == 0.0f) {
-    coincident(self_node, order[j], D, step_no, seed, dx, dy, dz);
-    l = dx * dx + dy * dy + dz * dz;
+    const float4 jj = coincident_far(self_node, order[j], D, step_no, seed);
+    dx = jj.x, dy = jj.y, dz = jj.z, l = jj.w;
   }
   if (l < dmin2f) l = sqrtf(dmin2f * l);
-  float f = q.w * alpha * __builtin_amdgcn_rcpf(l);
+  const float f = q.w * alpha * __builtin_amdgcn_rcpf(l);
   fx += dx * f;
   if (D > 1) fy += dy * f;
   if (D > 2) fz += dz * f;
@@ -509,7 +528,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   const double size = (double)cube->size;
   const double theta2 = A.theta2, dmax2 = A.dmax2;
   const float dmin2f = (float)A.dmin2, dmax2f = (float)fmin(dmax2, 3.0e38);
-  float4 me4 = live ? leaves[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 me4 = live ? leaves[s] : make_float4(0.f, 0.f, 0.f, 0.f);
   const uint32_t self_node = live ? order[s] : 0u;
   L.body[lane] = make_float4(me4.x, me4.y, me4.z, __uint_as_float(self_node));
   L.acc[0][lane] = 0.f;
@@ -523,9 +542,9 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   uint32_t err = 0;
   {
     const NodeRec& r = rec[0];
-    bool done = live ? cell_force<D>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, r.cx, r.cy, r.cz,
-                                     r.value, r.depth, size, theta2, dmax2, dmin2f, alpha, step_no,
-                                     seed, ax, ay, az, inter)
+    const NodeHdr h = *(const NodeHdr*)&r;
+    bool done = live ? cell_force<D>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, size * size, h,
+                                     theta2, dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter)
                      : true;
     uint64_t open = __ballot(!done);
     if (open) {
@@ -553,7 +572,9 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
     const uint32_t k_active = (uint32_t)__popcll(m);
     ++opened;
     lane_visits += k_active;
-    // one coalesced load of every child's record into the staging area
+    // the popped node's children, and one coalesced load of their records into staging
+    uint32_t my_kid = REF_NONE;
+    if (lane < (uint32_t)NK) my_kid = L.kids[sp][lane];
     {
       const uint32_t k = my_c < (uint32_t)NK ? L.kids[sp][my_c] : REF_NONE;
       uint2 v = make_uint2(0u, 0u);
@@ -569,20 +590,19 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
         L.stage[my_c][2 * my_w + 1] = v.y;
       }
     }
+    if (lane < (uint32_t)NK) L.cur[lane] = my_kid;
+    const uint32_t nk = (uint32_t)__popcll(__ballot(lane < (uint32_t)NK && my_kid != REF_NONE));
     // the children's cell chains start one level below the popped node's cell
     const uint32_t top = (uint32_t)__builtin_amdgcn_readfirstlane(L.depth[sp]) + 1;
-    uint32_t kids_here[NK];
-#pragma unroll
-    for (int c = 0; c < NK; ++c) kids_here[c] = (uint32_t)__builtin_amdgcn_readfirstlane(L.kids[sp][c]);
-    uint32_t nk = 0;
-#pragma unroll
-    for (int c = 0; c < NK; ++c) nk += kids_here[c] != REF_NONE ? 1u : 0u;
+    const double wt = ldexp(size, -(int)top);
+    const double w2top = wt * wt;
 
     uint32_t open_bits = 0;  // bit c: this lane's point must open child c
     if (k_active <= A.packed_max) {
       // packed: item t of round r is (point list[t / NK], child t % NK), NK items per point
       if (in) L.list[mask_rank(m)] = lane;
       const uint32_t rounds = (k_active * NK + WAVE - 1) / WAVE;
+#pragma unroll 1
       for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t t = r * WAVE + lane;
         const uint32_t i = t / NK, c = t % NK;
@@ -592,10 +612,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
           const uint32_t b = L.list[i];
           const float4 bd = L.body[b];
           const uint32_t bnode = __float_as_uint(bd.w);
-          uint32_t k = REF_NONE;
-#pragma unroll
-          for (int q = 0; q < NK; ++q)
-            if ((uint32_t)q == c) k = kids_here[q];
+          const uint32_t k = L.cur[c];
           if (k & REF_POINT) {
             const uint32_t j = k & ~REF_POINT;
             if (j != base + b)
@@ -603,9 +620,8 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
                              dmax2f, dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           } else {
             const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
-            open_item = !cell_force<D>(bd.x, bd.y, bd.z, bnode, n, k, top, h.cx, h.cy, h.cz, h.value,
-                                       h.depth, size, theta2, dmax2, dmin2f, alpha, step_no, seed,
-                                       fx, fy, fz, inter);
+            open_item = !cell_force<D>(bd.x, bd.y, bd.z, bnode, n, k, top, w2top, h, theta2, dmax2,
+                                       dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           }
         }
         // sum the NK items of each point (aligned groups of NK lanes)
@@ -631,30 +647,26 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
       }
     } else {
       // one lane per point, children in turn
-#pragma unroll
-      for (int c = 0; c < NK; ++c) {
-        const uint32_t k = kids_here[c];
-        if (k == REF_NONE) break;
+#pragma unroll 1
+      for (uint32_t c = 0; c < nk; ++c) {
+        const uint32_t k = L.cur[c];
+        if (!in) continue;
         if (k & REF_POINT) {
           const uint32_t j = k & ~REF_POINT;
-          if (in && j != s)
+          if (j != s)
             point_force<D>(me4.x, me4.y, me4.z, self_node, *(const float4*)&L.stage[c][0], order, j,
                            dmax2f, dmin2f, alpha, step_no, seed, ax, ay, az, inter);
           continue;
         }
         const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
-        bool done = in ? cell_force<D>(me4.x, me4.y, me4.z, self_node, n, k, top, h.cx, h.cy, h.cz,
-                                       h.value, h.depth, size, theta2, dmax2, dmin2f, alpha, step_no,
-                                       seed, ax, ay, az, inter)
-                       : true;
-        if (!done) open_bits |= 1u << c;
+        if (!cell_force<D>(me4.x, me4.y, me4.z, self_node, n, k, top, w2top, h, theta2, dmax2,
+                           dmin2f, alpha, step_no, seed, ax, ay, az, inter))
+          open_bits |= 1u << c;
       }
     }
     // push the children some lane must open
-#pragma unroll
-    for (int c = 0; c < NK; ++c) {
-      const uint32_t k = kids_here[c];
-      if (k == REF_NONE) break;
+#pragma unroll 1
+    for (uint32_t c = 0; c < nk; ++c) {
       const uint64_t open = __ballot(in && ((open_bits >> c) & 1u));
       if (!open) continue;
       if (sp >= STACK) {
@@ -670,7 +682,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
     }
   }
   if (live) {
-    uint32_t o = self_node;
+    const uint32_t o = self_node;
     vx[o] += ax + L.acc[0][lane];
     if (D > 1) vy[o] += ay + L.acc[1][lane];
     if (D > 2) vz[o] += az + L.acc[2][lane];
