@@ -154,7 +154,7 @@ struct fg_ctx {
   uint64_t steps = 0;
   uint64_t epoch = 1;
   int radix_bits = 8;
-  uint32_t walk_packed_max = 64;
+  uint32_t walk_packed_max = 24;
   std::string err;
   // readback
   float* staging = nullptr;

@@ -69,7 +69,7 @@ struct TreeBuffers {
 
 struct TreeParams {
   double theta2, dmin2, dmax2;
-  uint32_t walk_packed_max = 64;  // openings by at most this many points run packed
+  uint32_t walk_packed_max = 24;  // openings by at most this many points run packed
 };
 
 // Build the tree over the current positions and add the many-body force to the velocities.

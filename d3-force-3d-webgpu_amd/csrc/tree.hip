@@ -432,7 +432,7 @@ struct WalkLDS {
   float acc[3][WAVE];
   uint32_t list[WAVE];    // the points opening the current node, in lane order
   uint32_t cur[8];        // the children of the node being opened
-  uint64_t ballots[9];
+  uint64_t ballots[8];
 };
 
 // The jiggled displacement of two coincident bodies: out of line, it is rare.
@@ -599,18 +599,16 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
 
     uint32_t open_bits = 0;  // bit c: this lane's point must open child c
     if (k_active <= A.packed_max) {
-      // packed: item t is (point list[t / nk], child t % nk), spread over the wave's lanes
+      // packed: item t of round r is (point list[t / NK], child t % NK), NK items per point
       if (in) L.list[mask_rank(m)] = lane;
-      const uint32_t inv = (65536u + nk - 1u) / nk;  // t / nk = (t * inv) >> 16 for t < 2^13
-      const uint32_t items = k_active * nk;
-      const uint32_t rounds = (items + WAVE - 1) / WAVE;
+      const uint32_t rounds = (k_active * NK + WAVE - 1) / WAVE;
 #pragma unroll 1
       for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t t = r * WAVE + lane;
-        const uint32_t i = (t * inv) >> 16, c = t - i * nk;
+        const uint32_t i = t / NK, c = t % NK;
         bool open_item = false;
-        if (t < items) {
-          float fx = 0.f, fy = 0.f, fz = 0.f;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        if (i < k_active && c < nk) {
           const uint32_t b = L.list[i];
           const float4 bd = L.body[b];
           const uint32_t bnode = __float_as_uint(bd.w);
@@ -625,19 +623,27 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
             open_item = !cell_force<D>(bd.x, bd.y, bd.z, bnode, n, k, top, w2top, h, theta2, dmax2,
                                        dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           }
-          // the items of one point add up in LDS (the lanes of one instruction in lane order)
-          atomicAdd(&L.acc[0][b], fx);
-          if (D > 1) atomicAdd(&L.acc[1][b], fy);
-          if (D > 2) atomicAdd(&L.acc[2][b], fz);
+        }
+        // sum the NK items of each point (aligned groups of NK lanes)
+#pragma unroll
+        for (int o = NK / 2; o > 0; o >>= 1) {
+          fx += __shfl_xor(fx, o, WAVE);
+          if (D > 1) fy += __shfl_xor(fy, o, WAVE);
+          if (D > 2) fz += __shfl_xor(fz, o, WAVE);
+        }
+        if (c == 0 && i < k_active) {
+          const uint32_t b = L.list[i];
+          L.acc[0][b] += fx;
+          if (D > 1) L.acc[1][b] += fy;
+          if (D > 2) L.acc[2][b] += fz;
         }
         const uint64_t bal = __ballot(open_item);
         if (lane == 0) L.ballots[r] = bal;
       }
       if (in) {
-        const uint32_t p0 = mask_rank(m) * nk, o = p0 & (WAVE - 1);
-        const uint64_t lo_bits = L.ballots[p0 >> 6] >> o;
-        const uint64_t hi_bits = (o + nk > WAVE) ? (L.ballots[(p0 >> 6) + 1] << (WAVE - o)) : 0ull;
-        open_bits = (uint32_t)(lo_bits | hi_bits) & ((1u << nk) - 1u);
+        const uint32_t i = mask_rank(m);
+        const uint64_t bal = L.ballots[(i * NK) / WAVE];
+        open_bits = (uint32_t)(bal >> ((i * NK) % WAVE)) & ((1u << NK) - 1u);
       }
     } else {
       // one lane per point, children in turn
