@@ -188,6 +188,30 @@ def cpu_baseline(n, ticks=1):
     return 1e3 * dt / ticks
 
 
+def copy_calibration():
+    """A device copy of 256 MiB, three dispatches: what FETCH_SIZE and WRITE_SIZE read for a
+    known number of bytes (the scale factor of the walk's counter readings)."""
+    import torch
+    a = torch.ones(64 * 1024 * 1024, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copied_bytes_per_dispatch": a.numel() * 4, "dispatches": 3}))
+
+
+def walk_traffic(n):
+    """HBM bytes per walk launch from the committed counter summary (1M nodes only)."""
+    path = os.path.join(ROOT, "profiles", "tree_walk_1M_3d_hbm_counters.txt")
+    if n != 1_000_000 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,7 +220,12 @@ def main():
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ref-mix", action="store_true", help="skip the link+charge+center run")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs one simulation per GPU on one GPU; "
                                    "--gpus above 1 is not supported (see the sharding notes)"}))
@@ -232,7 +261,7 @@ def main():
                    "nodes": n, "links": r["links"], "dims": 3},
         "roofline": {"bound": "mfma", "kernel": "tree_walk", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": walk_traffic(n),
                      "walk_ms": round(r["walk_ms"], 4), "interactions": r["interactions"],
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "cpu_baseline": cpu,
