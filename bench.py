@@ -93,6 +93,55 @@ def make_graph(n, seed):
     return src, dst
 
 
+def ba_graph(n, m, seed):
+    """Barabasi-Albert graph: node i >= 1 attaches m links, each to an endpoint drawn uniformly
+    from the endpoints of all earlier links (linear preferential attachment; node 1 attaches to
+    node 0).  Vectorised: a link whose draw lands on an earlier link's target takes that link's
+    target, resolved by pointer jumping."""
+    if n < 2:
+        return np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+    rng = np.random.default_rng(seed)
+    M = m * (n - 1)
+    e = np.arange(M, dtype=np.int64)
+    src = 1 + e // m
+    slots = 2 * m * (src - 1)                      # endpoints of the earlier nodes' links
+    q = np.floor(rng.random(M) * slots).astype(np.int64)
+    tgt = np.zeros(M, np.int64)
+    done = slots == 0
+    ptr = q.copy()
+    while not done.all():
+        todo = ~done
+        even = todo & (ptr % 2 == 0)
+        tgt[even] = src[ptr[even] // 2]
+        done |= even
+        todo = ~done
+        par = ptr[todo] // 2
+        ready = done[par]
+        idx = np.flatnonzero(todo)
+        tgt[idx[ready]] = tgt[par[ready]]
+        done[idx[ready]] = True
+        ptr[idx[~ready]] = q[par[~ready]]
+        first = idx[~ready][slots[par[~ready]] == 0]   # the earlier link was node 1's: target 0
+        tgt[first] = 0
+        done[first] = True
+    return src.astype(np.uint32), tgt.astype(np.uint32)
+
+
+def phyllotaxis(n, ndim):
+    """d3's initial positions in 1, 2 or 3 dimensions (src/simulation.js:85-100)."""
+    i = np.arange(n, dtype=np.float64)
+    out = np.zeros((n, 3))
+    if ndim == 3:
+        out[:] = phyllotaxis3(n)
+    elif ndim == 2:
+        r = 10 * np.sqrt(0.5 + i)
+        a = i * (np.pi * (3 - np.sqrt(5)))
+        out[:, 0], out[:, 1] = r * np.cos(a), r * np.sin(a)
+    else:
+        out[:, 0] = 10 * i
+    return out
+
+
 def link_defaults(n, src, dst):
     """src/link.js:65-97: bias, default strength 1 / min(count), distance 30."""
     count = np.bincount(src, minlength=n).astype(np.float64) + np.bincount(dst, minlength=n)
@@ -148,6 +197,98 @@ def run_gpu(n, steps, warmup, light=False):
     return dict(ms=ms, phases=phases, walk_ms=walk_ms, walk_launches=t["walk"][1], interactions=int(st.interactions),
                 opened=int(st.opened), lane_visits=int(st.lane_visits), walk_errors=int(st.walk_errors), links=m,
                 finite=finite, timed_ticks=k)
+
+
+def batch_specs(jobs, seed=2024):
+    """The batched workload: `jobs` layouts, sizes floor(exp(U(ln 500, ln 50000))), a quarter
+    2D and the rest 3D, Barabasi-Albert graphs with m = 2 seeded per job."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(500), np.log(50000), jobs))).astype(int)
+    dims = np.array([2] * (jobs // 4) + [3] * (jobs - jobs // 4))
+    rng.shuffle(dims)
+    return [(int(n), int(d), seed + 1 + k) for k, (n, d) in enumerate(zip(sizes, dims))]
+
+
+def batch_forces(n, seed):
+    """link (d3 defaults) + manyBody(-30, theta 0.9) + collide(radius 3) + center."""
+    import d3fgpu
+    src, dst = ba_graph(n, 2, seed)
+    bias, strengths, distances = link_defaults(n, src, dst)
+    return [d3fgpu.link(src, dst, strengths, distances, bias), d3fgpu.many_body(np.full(n, -30.0), theta=0.9),
+            d3fgpu.collide(np.full(n, 3.0)), d3fgpu.center()]
+
+
+def run_batched(specs, cache=None):
+    """All layouts resident in one batch, ticked together until every one is done."""
+    import d3fgpu
+    jobs = [d3fgpu.Job(phyllotaxis(n, d)[:, :d], batch_forces(n, sd), ndim=d, seed=sd) for n, d, sd in specs]
+    t0 = time.perf_counter()
+    b = d3fgpu.Batch(0)
+    ids = [b.submit(j) for j in jobs]
+    t_submit = time.perf_counter() - t0
+    tune = b.autotune(cache)
+    t_tune = time.perf_counter() - t0 - t_submit
+    ticks = 0
+    while True:
+        b.step(50)
+        ticks += 50
+        if all(j["status"] == d3fgpu.JOB_DONE for j in b.jobs()):
+            break
+    out = {jid: b.read(jid)[0] for jid in ids}
+    wall = time.perf_counter() - t0
+    st = b.stats()
+    b.close()
+    return dict(wall=wall, submit_s=t_submit, tune_s=t_tune, ticks=ticks, tune=tune, stats=st,
+                layouts=[out[j] for j in ids])
+
+
+def run_plain_jobs(specs):
+    """The single-simulation engine, job after job, each to alpha < alphaMin."""
+    import d3fgpu
+    e = d3fgpu.Engine(0)
+    outs = []
+    t0 = time.perf_counter()
+    for n, d, sd in specs:
+        e.set_nodes(phyllotaxis(n, d), ndim=d)
+        e.set_forces(batch_forces(n, sd))
+        e.set_seed(sd)
+        a, alphas = 1.0, []
+        while True:
+            a += (0.0 - a) * (1 - 0.001 ** (1 / 300))
+            alphas.append(a)
+            if a < 0.001:
+                break
+        e.enqueue(alphas, 0.6)
+        outs.append(e.readback()[0])
+    wall = time.perf_counter() - t0
+    e.close()
+    return dict(wall=wall, layouts=outs)
+
+
+def bench_batch(jobs, with_plain=True):
+    specs = batch_specs(jobs)
+    nodes = sum(n for n, _, _ in specs)
+    log(f"batched: {jobs} layouts, {nodes} nodes")
+    bt = run_batched(specs, cache=os.path.join(ROOT, "gpurun_out", "batch_tune.txt"))
+    log(f"batched: {bt['wall']:.2f}s for {jobs} layouts ({bt['ticks']} ticks stepped, tune {bt['tune']})")
+    out = {"jobs": jobs, "nodes": nodes, "dims": "1:3 2D:3D", "graph": "Barabasi-Albert m=2",
+           "forces": "link + manyBody(theta 0.9) + collide(radius 3) + center",
+           "layouts_per_s": round(jobs / bt["wall"], 3), "wall_s": round(bt["wall"], 3),
+           "submit_s": round(bt["submit_s"], 3), "autotune_s": round(bt["tune_s"], 3),
+           "ticks_stepped": bt["ticks"], "tune": bt["tune"],
+           "launches_per_tick": bt["stats"]["launches_per_tick"], "walk_errors": bt["stats"]["walk_errors"]}
+    if with_plain:
+        pl = run_plain_jobs(specs)
+        log(f"plain job by job: {pl['wall']:.2f}s")
+        errs = []
+        for (n, d, sd), a, c in zip(specs, bt["layouts"], pl["layouts"]):
+            p0 = phyllotaxis(n, d)
+            den = np.linalg.norm(c - p0)
+            errs.append(float(np.linalg.norm(a - c) / (den if den > 0 else 1.0)))
+        out.update({"plain_layouts_per_s": round(jobs / pl["wall"], 3), "plain_wall_s": round(pl["wall"], 3),
+                    "speedup_vs_plain": round(pl["wall"] / bt["wall"], 2),
+                    "layout_rel_l2_vs_plain": {"max": max(errs), "median": float(np.median(errs))}})
+    return out
 
 
 def cpu_baseline(n, ticks=1):
@@ -220,6 +361,9 @@ def main():
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ref-mix", action="store_true", help="skip the link+charge+center run")
+    ap.add_argument("--batch-jobs", type=int, default=256, help="layouts in the batched leg (0: skip)")
+    ap.add_argument("--no-plain-jobs", action="store_true", help="batched leg without the job-by-job run")
+    ap.add_argument("--only-batch", action="store_true", help="run only the batched leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -231,6 +375,9 @@ def main():
                                    "--gpus above 1 is not supported (see the sharding notes)"}))
         sys.exit(2)
     n = args.nodes
+    if args.only_batch:
+        print(json.dumps({"batched": bench_batch(args.batch_jobs, not args.no_plain_jobs)}))
+        return
     r = run_gpu(n, args.steps, args.warmup)
     ref = None
     if not args.no_ref_mix:
@@ -271,6 +418,8 @@ def main():
         "walk_lane_utilisation": round(r["lane_visits"] / max(1, 64 * r["opened"]), 4),
         "finite": r["finite"],
     }
+    if args.batch_jobs > 0:
+        out["batched"] = bench_batch(args.batch_jobs, not args.no_plain_jobs)
     if ref is not None:
         out["ref_mix_ms_per_tick"] = round(ref["ms"], 4)
         out["ref_mix_note"] = ("link + manyBody + center only, the mix of the published 2.6 ms/tick "

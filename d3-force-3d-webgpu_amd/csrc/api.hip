@@ -19,6 +19,7 @@
 #include "../../include/d3fgpu.h"
 #include "common.h"
 #include "forces.h"
+#include "host.h"
 #include "sort.h"
 #include "tree.h"
 
@@ -26,41 +27,9 @@ using namespace fg;
 
 namespace {
 
-constexpr float BIG = 3e38f;  // stands for infinity on the device (passes/util.js:5-9)
 constexpr uint32_t GRID_MIN = 8192;  // collide through the grid from this many nodes (collide.js:21-22)
 constexpr double EXACT_THETA2 = 1e-6;  // all-pairs many-body below this theta^2 (manyBody.js:20-21)
 constexpr int MAX_PENDING = 3;          // busy() threshold (engine.js:11-13)
-
-float finite_f32(double v) {
-  if (!(v == v)) return 0.0f;  // NaN / undefined -> 0 (util.js:24-28)
-  if (v > BIG) return BIG;
-  if (v < -BIG) return -BIG;
-  return (float)v;
-}
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  void alloc(size_t b) {
-    free();
-    if (b == 0) b = 16;
-    FG_HIP(hipMalloc(&p, b));
-    bytes = b;
-  }
-  void free() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <typename T>
-  T* as() const {
-    return (T*)p;
-  }
-  ~DevBuf() { free(); }
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-};
 
 void upload_f32(hipStream_t s, DevBuf& b, const double* src, size_t count) {
   std::vector<float> tmp(count > 0 ? count : 1);
@@ -211,34 +180,16 @@ void zero3(hipStream_t s, DevBuf* b, uint32_t n) {
   for (int a = 0; a < 3; ++a) FG_HIP(hipMemsetAsync(b[a].p, 0, sizeof(float) * (n ? n : 1), s));
 }
 
-// Link CSR from the link list (passes/link.js:65-101): for every node, its incident links.
 void build_links(fg_ctx* c, Pass& p, const fg_force_desc& d) {
-  uint32_t n = c->n, m = d.count;
-  if (m && !d.links) throw ArgError(FG_ERR_INVALID, "link: links array is null");
-  std::vector<uint32_t> row(n + 1, 0);
-  for (uint32_t k = 0; k < m; ++k) {
-    uint32_t s = d.links[2 * k], t = d.links[2 * k + 1];
-    if (s >= n || t >= n) throw ArgError(FG_ERR_INVALID, "link: node index out of range");
-    ++row[s + 1];
-    ++row[t + 1];
-  }
-  for (uint32_t i = 0; i < n; ++i) row[i + 1] += row[i];
-  std::vector<LinkEntry> ent(2 * (size_t)m > 0 ? 2 * (size_t)m : 1);
-  std::vector<uint32_t> cursor(row.begin(), row.end() - 1);
-  for (uint32_t k = 0; k < m; ++k) {
-    uint32_t s = d.links[2 * k], t = d.links[2 * k + 1];
-    float dist = d.distances ? finite_f32(d.distances[k]) : 30.0f;
-    float str = d.strengths ? finite_f32(d.strengths[k]) : 0.0f;
-    float bias = d.bias ? finite_f32(d.bias[k]) : 0.5f;
-    ent[cursor[s]++] = LinkEntry{t, 2 * k, dist, str, bias};
-    ent[cursor[t]++] = LinkEntry{s, 2 * k + 1, dist, str, bias};
-  }
+  std::vector<uint32_t> row;
+  std::vector<LinkEntry> ent;
+  link_csr(c->n, d, row, ent);
   p.row.alloc(sizeof(uint32_t) * row.size());
   p.ent.alloc(sizeof(LinkEntry) * ent.size());
   FG_HIP(hipMemcpyAsync(p.row.p, row.data(), sizeof(uint32_t) * row.size(), hipMemcpyHostToDevice, c->stream));
   FG_HIP(hipMemcpyAsync(p.ent.p, ent.data(), sizeof(LinkEntry) * ent.size(), hipMemcpyHostToDevice, c->stream));
   FG_HIP(hipStreamSynchronize(c->stream));
-  p.m = m;
+  p.m = d.count;
 }
 
 void update_pass(fg_ctx* c, Pass& p, const fg_force_desc& d) {
@@ -401,6 +352,33 @@ void land(fg_ctx* c, double* pos, double* vel) {
 }
 
 }  // namespace
+
+namespace fg {
+
+void link_csr(uint32_t n, const fg_force_desc& d, std::vector<uint32_t>& row, std::vector<LinkEntry>& ent) {
+  const uint32_t m = d.count;
+  if (m && !d.links) throw ArgError(FG_ERR_INVALID, "link: links array is null");
+  row.assign((size_t)n + 1, 0);
+  for (uint32_t k = 0; k < m; ++k) {
+    uint32_t s = d.links[2 * k], t = d.links[2 * k + 1];
+    if (s >= n || t >= n) throw ArgError(FG_ERR_INVALID, "link: node index out of range");
+    ++row[s + 1];
+    ++row[t + 1];
+  }
+  for (uint32_t i = 0; i < n; ++i) row[i + 1] += row[i];
+  ent.assign(2 * (size_t)m > 0 ? 2 * (size_t)m : 1, LinkEntry{});
+  std::vector<uint32_t> cursor(row.begin(), row.end() - 1);
+  for (uint32_t k = 0; k < m; ++k) {
+    uint32_t s = d.links[2 * k], t = d.links[2 * k + 1];
+    float dist = d.distances ? finite_f32(d.distances[k]) : 30.0f;
+    float str = d.strengths ? finite_f32(d.strengths[k]) : 0.0f;
+    float bias = d.bias ? finite_f32(d.bias[k]) : 0.5f;
+    ent[cursor[s]++] = LinkEntry{t, 2 * k, dist, str, bias};
+    ent[cursor[t]++] = LinkEntry{s, 2 * k + 1, dist, str, bias};
+  }
+}
+
+}  // namespace fg
 
 extern "C" {
 

@@ -14,6 +14,7 @@
 //                     the cell's Morton index inside its block, radix sort of (slot, node),
 //                     run starts, then a 3^d neighbourhood query per sorted node.
 #include "forces.h"
+#include "batch.h"
 
 namespace fg {
 
@@ -65,22 +66,15 @@ void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, fl
 
 // ---- link ---------------------------------------------------------------------------
 
+// One node's incident links: the snapshot pos + vel of both ends, d3's spring along the
+// link, and the node's share of it (bias for the target, 1 - bias for the source).
 template <int D>
-__global__ void __launch_bounds__(256) link_kernel(uint32_t n, const float* __restrict__ x,
-                                                   const float* __restrict__ y,
-                                                   const float* __restrict__ z,
-                                                   const float* __restrict__ vx,
-                                                   const float* __restrict__ vy,
-                                                   const float* __restrict__ vz,
-                                                   float* __restrict__ ox, float* __restrict__ oy,
-                                                   float* __restrict__ oz,
-                                                   const uint32_t* __restrict__ row,
-                                                   const LinkEntry* __restrict__ ent,
-                                                   StepParams step) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const float alpha = step.alpha;
-  const uint32_t sn = step.step, seed = step.seed;
+__device__ inline void link_node(uint32_t i, const float* __restrict__ x, const float* __restrict__ y,
+                                 const float* __restrict__ z, const float* __restrict__ vx,
+                                 const float* __restrict__ vy, const float* __restrict__ vz,
+                                 float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
+                                 const uint32_t* __restrict__ row, const LinkEntry* __restrict__ ent,
+                                 float alpha, uint32_t sn, uint32_t seed) {
   float sx = x[i] + vx[i], sy = D > 1 ? y[i] + vy[i] : 0.f, sz = D > 2 ? z[i] + vz[i] : 0.f;
   float ax = 0.f, ay = 0.f, az = 0.f;
   const uint32_t e0 = row[i], e1 = row[i + 1];
@@ -107,6 +101,23 @@ __global__ void __launch_bounds__(256) link_kernel(uint32_t n, const float* __re
   ox[i] = vx[i] + ax;
   if (D > 1) oy[i] = vy[i] + ay;
   if (D > 2) oz[i] = vz[i] + az;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) link_kernel(uint32_t n, const float* __restrict__ x,
+                                                   const float* __restrict__ y,
+                                                   const float* __restrict__ z,
+                                                   const float* __restrict__ vx,
+                                                   const float* __restrict__ vy,
+                                                   const float* __restrict__ vz,
+                                                   float* __restrict__ ox, float* __restrict__ oy,
+                                                   float* __restrict__ oz,
+                                                   const uint32_t* __restrict__ row,
+                                                   const LinkEntry* __restrict__ ent,
+                                                   StepParams step) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  link_node<D>(i, x, y, z, vx, vy, vz, ox, oy, oz, row, ent, step.alpha, step.step, step.seed);
 }
 
 void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* const* vin,
@@ -390,9 +401,10 @@ __device__ inline uint32_t spread2_3(uint32_t v) {  // 3 bits -> every other bit
 }
 
 // Slot of a cell: blocks of 4^3 cells (8^2 in 2D, 64 in 1D) take 64 consecutive slots, the
-// cells of a block in Morton order, from an offset hashed from the block's coordinates.
+// cells of a block in Morton order, from an offset hashed from the block's coordinates
+// (and, in a batch, the job's segment; 0 for a single simulation).
 template <int D>
-__device__ inline uint32_t grid_slot(int cx, int cy, int cz, uint32_t mask) {
+__device__ inline uint32_t grid_slot(int cx, int cy, int cz, uint32_t seg, uint32_t mask) {
   uint32_t ux = (uint32_t)cx, uy = (uint32_t)cy, uz = (uint32_t)cz;
   uint32_t bx, by, bz, m;
   if (D == 3) {
@@ -405,8 +417,63 @@ __device__ inline uint32_t grid_slot(int cx, int cy, int cz, uint32_t mask) {
     bx = ux >> 6, by = 0, bz = 0;
     m = ux & 63u;
   }
-  uint32_t h = pcg_hash((bx * 73856093u) ^ (by * 19349663u) ^ (bz * 83492791u));
+  uint32_t h = pcg_hash((bx * 73856093u) ^ (by * 19349663u) ^ (bz * 83492791u) ^ (seg * 2654435761u));
   return ((h << 6) + m) & mask;
+}
+
+template <int D>
+__device__ inline int3 grid_cell(float4 q, float inv) {
+  return make_int3((int)floorf(q.x * inv), D > 1 ? (int)floorf(q.y * inv) : 0,
+                   D > 2 ? (int)floorf(q.z * inv) : 0);
+}
+
+// Collide sorted point s (snapshot `me`, node index `node`) with the points of its 3^D
+// neighbour cells (passes/collide.js:34-80 on candidates from passes/collideGrid.js).
+// Cells are visited in a fixed order and, within a cell, points in sorted order; a point
+// found under a cell's slot counts only if it lies in that cell, so slots shared by
+// several cells -- or, in a batch (SEG), by other jobs -- change the work but never the
+// sum.  svals[t] - node_off is the node index of sorted point t.
+template <int D, bool SEG>
+__device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node, uint32_t n,
+                                          const uint32_t* __restrict__ skeys,
+                                          const uint32_t* __restrict__ svals,
+                                          const uint32_t* __restrict__ starts,
+                                          const float4* __restrict__ sorted,
+                                          const uint32_t* __restrict__ sseg, uint32_t seg,
+                                          uint32_t node_off, float inv, uint32_t mask,
+                                          float strength, uint32_t sn, uint32_t seed, float& ax,
+                                          float& ay, float& az) {
+  const float ri2 = me.w * me.w;
+  const int3 h = grid_cell<D>(me, inv);
+  constexpr int RY = D > 1 ? 1 : 0, RZ = D > 2 ? 1 : 0;
+  for (int dz = -RZ; dz <= RZ; ++dz)
+    for (int dy = -RY; dy <= RY; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int cx = h.x + dx, cy = h.y + dy, cz = h.z + dz;
+        const uint32_t key = grid_slot<D>(cx, cy, cz, seg, mask);
+        const uint32_t start = starts[key];
+        if (start >= n || skeys[start] != key || (start > 0 && skeys[start - 1] == key)) continue;
+        for (uint32_t t = start; t < n && skeys[t] == key; ++t) {
+          if (t == s) continue;
+          if (SEG && sseg[t] != seg) continue;
+          const float4 q = sorted[t];
+          const int3 c = grid_cell<D>(q, inv);
+          if (c.x != cx || c.y != cy || c.z != cz) continue;
+          const float rj = q.w, r = me.w + rj;
+          float ex = me.x - q.x, ey = me.y - q.y, ez = me.z - q.z;
+          float l = ex * ex + ey * ey + ez * ez;
+          if (l >= r * r) continue;
+          if (l == 0.f) {
+            coincident(node, svals[t] - node_off, D, sn, seed, ex, ey, ez);
+            l = ex * ex + ey * ey + ez * ez;
+          }
+          const float len = sqrtf(l), rj2 = rj * rj;
+          const float f = (r - len) / len * strength * rj2 / (ri2 + rj2);
+          ax += ex * f;
+          ay += ey * f;
+          az += ez * f;
+        }
+      }
 }
 
 template <int D>
@@ -420,11 +487,9 @@ __global__ void __launch_bounds__(256) grid_keys(uint32_t n, const float* __rest
                                                  uint32_t* __restrict__ vals) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  float sx = x[i] + vx[i];
-  float sy = D > 1 ? y[i] + vy[i] : 0.f;
-  float sz = D > 2 ? z[i] + vz[i] : 0.f;
-  keys[i] = grid_slot<D>((int)floorf(sx * inv), D > 1 ? (int)floorf(sy * inv) : 0,
-                         D > 2 ? (int)floorf(sz * inv) : 0, mask);
+  const float4 q = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f, 0.f);
+  const int3 c = grid_cell<D>(q, inv);
+  keys[i] = grid_slot<D>(c.x, c.y, c.z, 0u, mask);
   vals[i] = i;
 }
 
@@ -459,45 +524,11 @@ __global__ void __launch_bounds__(256) grid_query(uint32_t n, const uint32_t* __
                                                   StepParams step) {
   uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
-  const uint32_t sn = step.step, seed = step.seed;
-  float4 me = sorted[s];
-  uint32_t i = svals[s];
-  float ri2 = me.w * me.w;
-  int hx = (int)floorf(me.x * inv);
-  int hy = D > 1 ? (int)floorf(me.y * inv) : 0;
-  int hz = D > 2 ? (int)floorf(me.z * inv) : 0;
-  constexpr int RY = D > 1 ? 1 : 0, RZ = D > 2 ? 1 : 0;
-  uint32_t seen[27];
-  int count = 0;
+  const float4 me = sorted[s];
+  const uint32_t i = svals[s];
   float ax = 0.f, ay = 0.f, az = 0.f;
-  for (int dz = -RZ; dz <= RZ; ++dz)
-    for (int dy = -RY; dy <= RY; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        uint32_t key = grid_slot<D>(hx + dx, hy + dy, hz + dz, mask);
-        bool again = false;
-        for (int k = 0; k < count; ++k) again |= seen[k] == key;
-        if (again) continue;
-        seen[count++] = key;
-        uint32_t start = starts[key];
-        if (start >= n || skeys[start] != key || (start > 0 && skeys[start - 1] == key)) continue;
-        for (uint32_t t = start; t < n && skeys[t] == key; ++t) {
-          if (t == s) continue;
-          float4 q = sorted[t];
-          float rj = q.w, r = me.w + rj;
-          float ex = me.x - q.x, ey = me.y - q.y, ez = me.z - q.z;
-          float l = ex * ex + ey * ey + ez * ez;
-          if (l >= r * r) continue;
-          if (l == 0.f) {
-            coincident(i, svals[t], D, sn, seed, ex, ey, ez);
-            l = ex * ex + ey * ey + ez * ez;
-          }
-          float len = sqrtf(l), rj2 = rj * rj;
-          float f = (r - len) / len * strength * rj2 / (ri2 + rj2);
-          ax += ex * f;
-          ay += ey * f;
-          az += ez * f;
-        }
-      }
+  grid_collide_point<D, false>(s, me, i, n, skeys, svals, starts, sorted, nullptr, 0u, 0u, inv, mask,
+                               strength, step.step, step.seed, ax, ay, az);
   vx[i] += ax;
   if (D > 1) vy[i] += ay;
   if (D > 2) vz[i] += az;
@@ -560,5 +591,354 @@ void launch_collide_grid(hipStream_t s, GridWork& g, uint32_t n, int ndim, float
   else if (ndim == 2) grid_run<2>(s, g, n, pos, vel, radius, inv, mask, strength, radix_bits, step);
   else grid_run<3>(s, g, n, pos, vel, radius, inv, mask, strength, radix_bits, step);
 }
+
+}  // namespace fg
+
+// ---- batched passes -------------------------------------------------------------------
+//
+// The same per-node work over the node space (or grid space) of a batch: each index finds
+// its job through its segment, works on the job's own arrays with job-local indices and
+// the job's own alpha, step and seed, and leaves jobs that are not running alone.
+
+namespace fg {
+
+namespace {
+
+struct NodeRef {
+  const BJob* B;
+  uint32_t v;  // the node's index within its job
+};
+
+__device__ inline NodeRef node_ref(uint32_t i, const uint32_t* seg_of, const uint32_t* seg_job,
+                                   const BJob* jobs, uint32_t BJob::*off) {
+  const BJob* B = jobs + seg_job[seg_of[i]];
+  return NodeRef{B, i - B->*off};
+}
+
+// d3's alpha step (simulation.js:54-73) for every job, in FP64 without contraction so the
+// sequence is the host's bit for bit; also this tick's step number and whether it runs.
+__global__ void __launch_bounds__(256) badvance(BJob* jobs, uint32_t njobs) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= njobs) return;
+  BJob& B = jobs[j];
+  if (B.status == BJOB_RUNNING) {
+    // two roundings, as JavaScript computes it: no fused multiply-add
+#pragma clang fp contract(off)
+    const double a = B.alpha + (B.alpha_target - B.alpha) * B.alpha_decay;
+    B.alpha = a;
+    B.alpha_f = (float)a;
+    B.step = B.ticks;
+    B.ticks += 1;
+    B.active = 1;
+    if (a < B.alpha_min || (B.max_ticks && B.ticks >= B.max_ticks)) B.status = BJOB_FINISHING;
+  } else {
+    if (B.status == BJOB_FINISHING) B.status = BJOB_DONE;
+    B.active = 0;
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) blink(uint32_t total, const uint32_t* __restrict__ seg_of,
+                                             const uint32_t* __restrict__ seg_job,
+                                             const BJob* __restrict__ jobs, uint32_t iteration) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
+  const BJob& B = *r.B;
+  if (!B.active || !(B.forces & BF_LINK) || iteration >= B.link_iters) return;
+  const uint32_t c = iteration & 1u;
+  link_node<D>(r.v, B.p[0], B.p[1], B.p[2], B.v[c][0], B.v[c][1], B.v[c][2], B.v[c ^ 1][0],
+               B.v[c ^ 1][1], B.v[c ^ 1][2], B.link_row, B.link_ent, B.alpha_f, B.step, B.seed);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) bgrid_keys(uint32_t total, const uint32_t* __restrict__ seg_of,
+                                                  const uint32_t* __restrict__ seg_job,
+                                                  const BJob* __restrict__ jobs, uint32_t mask,
+                                                  uint32_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const uint32_t seg = seg_of[i];
+  const BJob& B = jobs[seg_job[seg]];
+  const uint32_t v = i - B.grid_off, c = B.vcur;
+  const float4 q = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
+                               D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, 0.f);
+  const int3 cell = grid_cell<D>(q, B.coll_inv);
+  keys[i] = grid_slot<D>(cell.x, cell.y, cell.z, seg, mask);
+  vals[i] = i;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) bgrid_begin(uint32_t total, const uint32_t* __restrict__ skeys,
+                                                   const uint32_t* __restrict__ svals,
+                                                   const uint32_t* __restrict__ seg_of,
+                                                   const uint32_t* __restrict__ seg_job,
+                                                   const BJob* __restrict__ jobs,
+                                                   uint32_t* __restrict__ starts,
+                                                   float4* __restrict__ sorted,
+                                                   uint32_t* __restrict__ sseg) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= total) return;
+  const uint32_t key = skeys[s], i = svals[s];
+  if (s == 0 || skeys[s - 1] != key) starts[key] = s;
+  const uint32_t seg = seg_of[i];
+  const BJob& B = jobs[seg_job[seg]];
+  const uint32_t v = i - B.grid_off, c = B.vcur;
+  sorted[s] = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
+                          D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, B.radius[v]);
+  sseg[s] = seg;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) bgrid_query(uint32_t total, const uint32_t* __restrict__ skeys,
+                                                   const uint32_t* __restrict__ svals,
+                                                   const uint32_t* __restrict__ starts,
+                                                   const float4* __restrict__ sorted,
+                                                   const uint32_t* __restrict__ sseg,
+                                                   const uint32_t* __restrict__ seg_job,
+                                                   const BJob* __restrict__ jobs, uint32_t mask,
+                                                   uint32_t iteration) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= total) return;
+  const uint32_t seg = sseg[s];
+  const BJob& B = jobs[seg_job[seg]];
+  if (!B.active || iteration >= B.coll_iters) return;
+  const uint32_t v = svals[s] - B.grid_off;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  grid_collide_point<D, true>(s, sorted[s], v, total, skeys, svals, starts, sorted, sseg, seg,
+                              B.grid_off, B.coll_inv, mask, B.coll_strength, B.step, B.seed, ax, ay,
+                              az);
+  float* const* vel = B.v[B.vcur];
+  vel[0][v] += ax;
+  if (D > 1) vel[1][v] += ay;
+  if (D > 2) vel[2][v] += az;
+}
+
+// Centre: FP64 sums per tile (job-relative tiles, fixed order inside a tile); the last tile
+// of a job adds its job's tile sums in order and forms the shift (passes/center.js:11-72).
+__global__ void __launch_bounds__(256) bcenter_reduce(const BTile* __restrict__ tiles,
+                                                      const uint32_t* __restrict__ seg_job,
+                                                      BJob* jobs, double* partial) {
+  const BTile t = tiles[blockIdx.x];
+  BJob* B = jobs + seg_job[t.seg];
+  if (!B->active || !(B->forces & BF_CENTER)) return;
+  const uint32_t nd = B->ndim, off = B->node_off;
+  double s[3] = {0, 0, 0};
+  for (uint32_t i = t.first + threadIdx.x; i < t.first + t.count; i += 256) {
+    const uint32_t v = i - off;
+    s[0] += (double)B->p[0][v];
+    if (nd > 1) s[1] += (double)B->p[1][v];
+    if (nd > 2) s[2] += (double)B->p[2][v];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s[a] += __shfl_xor(s[a], o, WAVE);
+  __shared__ double sh[4][3];
+  __shared__ uint32_t s_last;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) sh[wave][a] = s[a];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a)
+      store_coherent(&partial[3 * blockIdx.x + a], sh[0][a] + sh[1][a] + sh[2][a] + sh[3][a]);
+    s_last = publish_count(&B->center_done) + 1 == B->node_tiles;
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  double tot[3] = {0, 0, 0};
+  for (uint32_t k = 0; k < B->node_tiles; ++k)
+    for (int a = 0; a < 3; ++a) tot[a] += load_coherent(&partial[3 * (B->node_tiles_first + k) + a]);
+  for (int a = 0; a < 3; ++a)
+    B->center_shift[a] = a < (int)nd ? (float)((tot[a] / B->n - B->center[a]) * B->center_strength) : 0.0f;
+  B->center_done = 0;
+}
+
+__global__ void __launch_bounds__(256) bcenter_apply(uint32_t total, const uint32_t* __restrict__ seg_of,
+                                                     const uint32_t* __restrict__ seg_job,
+                                                     const BJob* __restrict__ jobs) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
+  const BJob& B = *r.B;
+  if (!B.active || !(B.forces & BF_CENTER)) return;
+  for (uint32_t a = 0; a < B.ndim; ++a)
+    if (B.center_shift[a] != 0.0f) B.p[a][r.v] -= B.center_shift[a];
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) bradial(uint32_t total, const uint32_t* __restrict__ seg_of,
+                                               const uint32_t* __restrict__ seg_job,
+                                               const BJob* __restrict__ jobs) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
+  const BJob& B = *r.B;
+  if (!B.active || !(B.forces & BF_RADIAL)) return;
+  const uint32_t v = r.v;
+  float dx = B.p[0][v] - B.radial_c[0];
+  float dy = D > 1 ? B.p[1][v] - B.radial_c[1] : 0.f;
+  float dz = D > 2 ? B.p[2][v] - B.radial_c[2] : 0.f;
+  if (dx == 0.f) dx = 1e-6f;
+  if (D > 1 && dy == 0.f) dy = 1e-6f;
+  if (D > 2 && dz == 0.f) dz = 1e-6f;
+  const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+  const float k = (B.radial_r[v] - rr) * B.radial_w[v] * B.alpha_f / rr;
+  float* const* vel = B.v[B.vcur];
+  vel[0][v] += dx * k;
+  if (D > 1) vel[1][v] += dy * k;
+  if (D > 2) vel[2][v] += dz * k;
+}
+
+__global__ void __launch_bounds__(256) bposition(uint32_t total, const uint32_t* __restrict__ seg_of,
+                                                 const uint32_t* __restrict__ seg_job,
+                                                 const BJob* __restrict__ jobs, int axis) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
+  const BJob& B = *r.B;
+  if (!B.active || !(B.forces & (BF_POS_X << axis)) || axis >= (int)B.ndim) return;
+  const uint32_t v = r.v;
+  B.v[B.vcur][axis][v] += (B.pos_goal[axis][v] - B.p[axis][v]) * B.pos_weight[axis][v] * B.alpha_f;
+}
+
+// v *= decay; p += v; pinned axes p = fix, v = 0 (wgsl.js:67-84).  Reads the job's current
+// velocity set and writes set 0, so every tick of a job starts from set 0.
+template <int D>
+__global__ void __launch_bounds__(256) bintegrate(uint32_t total, const uint32_t* __restrict__ seg_of,
+                                                  const uint32_t* __restrict__ seg_job,
+                                                  const BJob* __restrict__ jobs) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
+  const BJob& B = *r.B;
+  if (!B.active) return;
+  const uint32_t v = r.v, c = B.vcur;
+  const uint32_t m = B.mask ? B.mask[v] : 0u;
+  const float decay = B.decay;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float vv = B.v[c][a][v] * decay, p = B.p[a][v] + vv;
+    if ((m >> a) & 1u) p = B.fix[a][v], vv = 0.0f;
+    B.p[a][v] = p;
+    B.v[0][a][v] = vv;
+  }
+}
+
+}  // namespace
+
+void BatchGrid::reserve(uint32_t n_) {
+  uint32_t b = 8;
+  while ((1ull << b) < 2ull * n_) ++b;
+  if (n_ <= cap && bits == b && keys) return;
+  release();
+  uint32_t m = n_ > 1 ? n_ : 1;
+  bits = b;
+  FG_HIP(hipMalloc(&keys, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&starts, sizeof(uint32_t) << bits));
+  FG_HIP(hipMemset(starts, 0xff, sizeof(uint32_t) << bits));
+  FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
+  FG_HIP(hipMalloc(&sseg, sizeof(uint32_t) * m));
+  sort.reserve(m, sizeof(uint32_t));
+  cap = n_;
+}
+
+void BatchGrid::release() {
+  void* ptrs[] = {keys, vals, starts, sorted, sseg};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  keys = vals = starts = sseg = nullptr;
+  sorted = nullptr;
+  sort.release();
+  cap = 0;
+}
+
+void BatchWork::reserve(uint32_t tiles) {
+  if (tiles <= center_cap && center_partial) return;
+  release();
+  uint32_t m = tiles > 1 ? tiles : 1;
+  FG_HIP(hipMalloc(&center_partial, sizeof(double) * 3 * m));
+  center_cap = m;
+}
+
+void BatchWork::release() {
+  if (center_partial) (void)hipFree(center_partial);
+  center_partial = nullptr;
+  center_cap = 0;
+}
+
+void batch_advance(hipStream_t s, BJob* jobs, uint32_t njobs, uint32_t*) {
+  if (!njobs) return;
+  hipLaunchKernelGGL(badvance, dim3(ceil_div(njobs, 256)), dim3(256), 0, s, jobs, njobs);
+  FG_LAUNCH_CHECK();
+}
+
+#define FG_BY_DIM(ndim, KERNEL, ...)                                                   \
+  do {                                                                                 \
+    if ((ndim) == 1) hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__);                       \
+    else if ((ndim) == 2) hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__);                  \
+    else hipLaunchKernelGGL(KERNEL<3>, __VA_ARGS__);                                   \
+    FG_LAUNCH_CHECK();                                                                 \
+  } while (0)
+
+void batch_link(hipStream_t s, int ndim, const BSpace& sp, BJob* jobs, uint32_t iteration) {
+  if (!sp.total) return;
+  FG_BY_DIM(ndim, blink, dim3(ceil_div(sp.total, 256)), dim3(256), 0, s, sp.total,
+            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, iteration);
+}
+
+void batch_collide(hipStream_t s, int ndim, BatchGrid& g, const BSpace& sp, BJob* jobs,
+                   uint32_t iteration, int radix_bits) {
+  const uint32_t n = sp.total;
+  if (!n) return;
+  g.reserve(n);
+  const uint32_t mask = (uint32_t)((1ull << g.bits) - 1ull);
+  dim3 gr(ceil_div(n, 256)), b(256);
+  FG_BY_DIM(ndim, bgrid_keys, gr, b, 0, s, n, (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job,
+            (const BJob*)jobs, mask, g.keys, g.vals);
+  uint32_t *sk, *sv;
+  radix_sort_u32(s, g.sort, radix_bits, g.keys, g.vals, n, 0, g.bits, &sk, &sv);
+  FG_BY_DIM(ndim, bgrid_begin, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
+            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, g.starts,
+            g.sorted, g.sseg);
+  FG_BY_DIM(ndim, bgrid_query, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
+            (const uint32_t*)g.starts, (const float4*)g.sorted, (const uint32_t*)g.sseg,
+            (const uint32_t*)sp.seg_job, (const BJob*)jobs, mask, iteration);
+}
+
+void batch_center(hipStream_t s, int, BatchWork& w, const BSpace& sp, BJob* jobs) {
+  if (!sp.total || !sp.tiles) return;
+  w.reserve(sp.tiles);
+  hipLaunchKernelGGL(bcenter_reduce, dim3(sp.tiles), dim3(256), 0, s, (const BTile*)sp.tile,
+                     (const uint32_t*)sp.seg_job, jobs, w.center_partial);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bcenter_apply, dim3(ceil_div(sp.total, 256)), dim3(256), 0, s, sp.total,
+                     (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs);
+  FG_LAUNCH_CHECK();
+}
+
+void batch_radial(hipStream_t s, int ndim, const BSpace& sp, BJob* jobs) {
+  if (!sp.total) return;
+  FG_BY_DIM(ndim, bradial, dim3(ceil_div(sp.total, 256)), dim3(256), 0, s, sp.total,
+            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs);
+}
+
+void batch_position(hipStream_t s, const BSpace& sp, BJob* jobs, int axis) {
+  if (!sp.total) return;
+  hipLaunchKernelGGL(bposition, dim3(ceil_div(sp.total, 256)), dim3(256), 0, s, sp.total,
+                     (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, axis);
+  FG_LAUNCH_CHECK();
+}
+
+void batch_integrate(hipStream_t s, int ndim, const BSpace& sp, BJob* jobs) {
+  if (!sp.total) return;
+  FG_BY_DIM(ndim, bintegrate, dim3(ceil_div(sp.total, 256)), dim3(256), 0, s, sp.total,
+            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs);
+}
+
+#undef FG_BY_DIM
 
 }  // namespace fg

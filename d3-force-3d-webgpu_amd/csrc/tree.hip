@@ -25,6 +25,9 @@
 #include <math.h>
 
 #include "tree.h"
+#include "batch.h"
+
+#include <string>
 
 namespace fg {
 
@@ -82,6 +85,59 @@ __device__ inline float cell_of(float x, float origin, int k) {
 
 // ---- bbox + cube --------------------------------------------------------------------
 
+// Reduces the 256 threads' boxes and folds the block's box into the u32 box words with
+// order-preserving atomics (barnesHut.js:140-167).
+__device__ inline void block_box(float lo[3], float hi[3], uint32_t* box) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], o, WAVE));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], o, WAVE));
+    }
+  }
+  __shared__ float s_lo[4][3], s_hi[4][3];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) s_lo[wave][a] = lo[a], s_hi[wave][a] = hi[a];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int a = threadIdx.x;
+    float l = fminf(fminf(s_lo[0][a], s_lo[1][a]), fminf(s_lo[2][a], s_lo[3][a]));
+    float h = fmaxf(fmaxf(s_hi[0][a], s_hi[1][a]), fmaxf(s_hi[2][a], s_hi[3][a]));
+    atomicMin(&box[a], float_ordered(l));
+    atomicMax(&box[3 + a], float_ordered(h));
+  }
+  __syncthreads();
+}
+
+// d3's cover() of the box (barnesHut.js:169-181): origin = floor(min), size = the smallest
+// power of two with max < origin + size; shift = LEVELS - log2(size).  Resets the box.
+template <int D>
+__device__ inline void cover_box(uint32_t* box, float origin[3], float& size, int32_t& shift) {
+  float blo[3], bhi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    blo[a] = float_unordered(__hip_atomic_load(&box[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    bhi[a] = float_unordered(__hip_atomic_load(&box[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) origin[a] = floorf(blo[a]);
+  size = 1.0f;
+  int log2size = 0;
+  for (int k = 0; k < 128; ++k) {
+    if (bhi[0] < origin[0] + size && bhi[1] < origin[1] + size && bhi[2] < origin[2] + size) break;
+    size *= 2.0f;
+    log2size += 1;
+  }
+  shift = Dims<D>::LEVELS - log2size;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    box[a] = 0xffffffffu;
+    box[3 + a] = 0u;
+  }
+}
+
 template <int D>
 __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
                                                  const float* __restrict__ y,
@@ -93,59 +149,42 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
 #pragma unroll
     for (int a = 0; a < 3; ++a) lo[a] = fminf(lo[a], p[a]), hi[a] = fmaxf(hi[a], p[a]);
   }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      lo[a] = fminf(lo[a], __shfl_xor(lo[a], o, WAVE));
-      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], o, WAVE));
-    }
-  }
+  block_box(lo, hi, cube->box);
   __shared__ uint32_t s_last;
-  __shared__ float s_lo[4][3], s_hi[4][3];
-  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-  if (lane == 0)
-    for (int a = 0; a < 3; ++a) s_lo[wave][a] = lo[a], s_hi[wave][a] = hi[a];
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    const int a = threadIdx.x;
-    float l = fminf(fminf(s_lo[0][a], s_lo[1][a]), fminf(s_lo[2][a], s_lo[3][a]));
-    float h = fmaxf(fmaxf(s_hi[0][a], s_hi[1][a]), fmaxf(s_hi[2][a], s_hi[3][a]));
-    atomicMin(&cube->box[a], float_ordered(l));
-    atomicMax(&cube->box[3 + a], float_ordered(h));
-  }
-  __syncthreads();
   if (threadIdx.x == 0) s_last = publish_count(&cube->done) + 1 == gridDim.x;
   __syncthreads();
   if (!s_last || threadIdx.x != 0) return;
-  // the last block: d3's cover() on the box (barnesHut.js:169-181), then reset the box
-  float blo[3], bhi[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    blo[a] = float_unordered(__hip_atomic_load(&cube->box[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    bhi[a] = float_unordered(__hip_atomic_load(&cube->box[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  }
-  float origin[3] = {floorf(blo[0]), floorf(blo[1]), floorf(blo[2])};
-  float size = 1.0f;
-  int log2size = 0;
-  for (int k = 0; k < 128; ++k) {
-    if (bhi[0] < origin[0] + size && bhi[1] < origin[1] + size && bhi[2] < origin[2] + size) break;
-    size *= 2.0f;
-    log2size += 1;
-  }
+  // the last block turns the box into the cube
+  float origin[3], size;
+  int32_t shift;
+  cover_box<D>(cube->box, origin, size, shift);
 #pragma unroll
   for (int a = 0; a < 3; ++a) cube->origin[a] = origin[a];
   cube->size = size;
-  cube->shift = Dims<D>::LEVELS - log2size;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    cube->box[a] = 0xffffffffu;
-    cube->box[3 + a] = 0u;
-  }
+  cube->shift = shift;
   cube->done = 0;
 }
 
 // ---- keys ---------------------------------------------------------------------------
+
+// The compact code of a point: exact cellOf per axis, high half << HALF | low half
+// (barnesHut.js:343-366).
+template <int D>
+__device__ inline uint64_t point_key(const float p[3], const float origin[3], int k) {
+  constexpr int L = Dims<D>::LEVELS, H = L / 2;
+  const float top = (float)((1u << L) - 1u);
+  uint32_t q[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float c = cell_of(p[a], origin[a], k);
+    c = fminf(fmaxf(c, 0.0f), top);
+    q[a] = (uint32_t)c;
+  }
+  constexpr uint32_t lm = (1u << H) - 1u;
+  uint32_t hi = morton<D>(q[0] >> H, q[1] >> H, q[2] >> H);
+  uint32_t lo = morton<D>(q[0] & lm, q[1] & lm, q[2] & lm);
+  return ((uint64_t)hi << Dims<D>::HALF) | lo;
+}
 
 template <int D>
 __global__ void __launch_bounds__(256) tree_keys(const float* __restrict__ x,
@@ -156,21 +195,8 @@ __global__ void __launch_bounds__(256) tree_keys(const float* __restrict__ x,
                                                  uint32_t* __restrict__ vals) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  constexpr int L = Dims<D>::LEVELS, H = L / 2;
-  const int k = cube->shift;
-  const float top = (float)((1u << L) - 1u);
-  float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
-  uint32_t q[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    float c = cell_of(p[a], cube->origin[a], k);
-    c = fminf(fmaxf(c, 0.0f), top);
-    q[a] = (uint32_t)c;
-  }
-  constexpr uint32_t lm = (1u << H) - 1u;
-  uint32_t hi = morton<D>(q[0] >> H, q[1] >> H, q[2] >> H);
-  uint32_t lo = morton<D>(q[0] & lm, q[1] & lm, q[2] & lm);
-  keys[i] = ((uint64_t)hi << Dims<D>::HALF) | lo;
+  const float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
+  keys[i] = point_key<D>(p, cube->origin, cube->shift);
   vals[i] = i;
 }
 
@@ -218,13 +244,11 @@ __global__ void __launch_bounds__(256) tree_leaves(const uint64_t* __restrict__ 
 
 // ---- Karras build -------------------------------------------------------------------
 
+// Karras 2012 node i of the sorted codes k[0, N): the range [first, last] it covers, its
+// split and the common prefix of its range (barnesHut.js:587-624).
 template <int D>
-__global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ skeys, uint32_t n,
-                                                  uint4* __restrict__ links, uint2* __restrict__ spans,
-                                                  uint32_t* __restrict__ leaf_parent) {
-  int i = (int)(blockIdx.x * 256 + threadIdx.x);
-  const int N = (int)n;
-  if (i >= N - 1) return;
+__device__ inline void karras(const uint64_t* __restrict__ skeys, int N, int i, int& first,
+                              int& last, int& split, int& prefix) {
   int before = delta<D>(skeys, N, i, i - 1), after = delta<D>(skeys, N, i, i + 1);
   int d = after > before ? 1 : -1;
   int floor_prefix = before < after ? before : after;
@@ -234,26 +258,46 @@ __global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ s
   for (int t = span >> 1; t >= 1; t >>= 1)
     if (delta<D>(skeys, N, i, i + (len + t) * d) > floor_prefix) len += t;
   int j = i + len * d;
-  int prefix = delta<D>(skeys, N, i, j);
+  prefix = delta<D>(skeys, N, i, j);
   int s = 0, stride = len;
   for (;;) {
     stride = (stride + 1) >> 1;
     if (delta<D>(skeys, N, i, i + (s + stride) * d) > prefix) s += stride;
     if (stride <= 1) break;
   }
-  int split = i + s * d + (d < 0 ? d : 0);
-  int first = i < j ? i : j, last = i < j ? j : i;
-  uint32_t left = (uint32_t)split | (first == split ? REF_POINT : 0u);
-  uint32_t right = (uint32_t)(split + 1) | (last == split + 1 ? REF_POINT : 0u);
-  links[i].x = left;
-  links[i].y = right;
-  links[i].z = depth_of<D>(prefix);
-  spans[i] = make_uint2((uint32_t)first, (uint32_t)last);
-  if (left & REF_POINT) leaf_parent[split] = (uint32_t)i;
-  else links[split].w = (uint32_t)i;
-  if (right & REF_POINT) leaf_parent[split + 1] = (uint32_t)i;
-  else links[split + 1].w = (uint32_t)i;
-  if (i == 0) links[0].w = REF_NONE;
+  split = i + s * d + (d < 0 ? d : 0);
+  first = i < j ? i : j;
+  last = i < j ? j : i;
+}
+
+// Writes binary node `node` (= off + i) of a tree whose sorted points are [off, off + N).
+template <int D>
+__device__ inline void karras_write(const uint64_t* __restrict__ skeys, int N, int i, uint32_t off,
+                                    uint4* __restrict__ links, uint2* __restrict__ spans,
+                                    uint32_t* __restrict__ leaf_parent) {
+  int first, last, split, prefix;
+  karras<D>(skeys, N, i, first, last, split, prefix);
+  const uint32_t node = off + (uint32_t)i, a = off + (uint32_t)split, b = a + 1u;
+  uint32_t left = a | (first == split ? REF_POINT : 0u);
+  uint32_t right = b | (last == split + 1 ? REF_POINT : 0u);
+  links[node].x = left;
+  links[node].y = right;
+  links[node].z = depth_of<D>(prefix);
+  spans[node] = make_uint2(off + (uint32_t)first, off + (uint32_t)last);
+  if (left & REF_POINT) leaf_parent[a] = node;
+  else links[a].w = node;
+  if (right & REF_POINT) leaf_parent[b] = node;
+  else links[b].w = node;
+  if (i == 0) links[node].w = REF_NONE;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ skeys, uint32_t n,
+                                                  uint4* __restrict__ links, uint2* __restrict__ spans,
+                                                  uint32_t* __restrict__ leaf_parent) {
+  int i = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (i >= (int)n - 1) return;
+  karras_write<D>(skeys, (int)n, i, 0u, links, spans, leaf_parent);
 }
 
 // ---- bottom-up reduction + collapse ---------------------------------------------------
@@ -410,25 +454,20 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
 // ---- walk ---------------------------------------------------------------------------
 
 constexpr int WALK_WAVES = 4;
-constexpr int STACK = 192;  // per wave; 7 * 21 + 8 = 155 in 3D for distinct codes
-
-struct WalkArgs {
-  double theta2, dmin2, dmax2;
-  uint32_t n;
-  uint32_t packed_max;  // openings by at most this many points use the packed mode
-};
+constexpr int WALK_STACK = 192;  // per wave; 7 * 21 + 8 = 155 in 3D for distinct codes
 
 // Per wave, in LDS: the stack of nodes to open, each as its depth, the lanes that must
 // open it and its d3 children (so that opening it needs no load of its own record); a
 // staging area where one coalesced load puts the 64-byte records of all children of the
 // node being opened (points: their float4); the wave's points and their force sums from
 // the packed mode.
+template <int STACK>
 struct WalkLDS {
   uint32_t depth[STACK];
   uint64_t mask[STACK];
   uint32_t kids[STACK][8];
   uint32_t stage[8][16];
-  float4 body[WAVE];      // x, y, z, sorted index (as bits)
+  float4 body[WAVE];      // x, y, z, node index (as bits)
   float acc[3][WAVE];
   uint32_t list[WAVE];    // the points opening the current node, in lane order
   uint32_t cur[8];        // the children of the node being opened
@@ -453,10 +492,11 @@ __device__ inline float dampf(uint32_t k) {
 
 // The force of one node (its cell chain from depth `top`, whose first width squared is
 // w2top) on one body, with d3's rules (barnesHut.js:719-746, manyBody.js:73-99).
+// `cell` is the node's index within its tree (the jiggle's stand-in index is n + cell).
 // Returns whether the body is done with the node (false: it must open it).
 template <int D>
 __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self_node, uint32_t n,
-                                  uint32_t cur, uint32_t top, double w2top, const NodeHdr& h,
+                                  uint32_t cell, uint32_t top, double w2top, const NodeHdr& h,
                                   double theta2, double dmax2, float dmin2f, float alpha,
                                   uint32_t step_no, uint32_t seed, float& fx, float& fy, float& fz,
                                   uint32_t& inter) {
@@ -475,7 +515,7 @@ __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self
   if (l >= dmax2) return true;
   float ex = (float)dx, ey = (float)dy, ez = (float)dz, fl = (float)l;
   if (l == 0.0) {
-    const float4 j = coincident_far(self_node, n + cur, D, step_no, seed);
+    const float4 j = coincident_far(self_node, n + cell, D, step_no, seed);
     ex = j.x, ey = j.y, ez = j.z, fl = j.w;
   }
   if (fl < dmin2f) fl = sqrtf(dmin2f * fl);
@@ -487,17 +527,18 @@ __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self
   return true;
 }
 
-// The force of one point on one body (barnesHut.js:705-717, manyBody.js:99-113).
+// The force of one point (node index `other`) on one body (barnesHut.js:705-717,
+// manyBody.js:99-113).
 template <int D>
 __device__ inline void point_force(float mx, float my, float mz, uint32_t self_node, float4 q,
-                                   const uint32_t* order, uint32_t j, float dmax2f, float dmin2f,
-                                   float alpha, uint32_t step_no, uint32_t seed, float& fx,
-                                   float& fy, float& fz, uint32_t& inter) {
+                                   const uint32_t* order, uint32_t j, uint32_t off, float dmax2f,
+                                   float dmin2f, float alpha, uint32_t step_no, uint32_t seed,
+                                   float& fx, float& fy, float& fz, uint32_t& inter) {
   float dx = q.x - mx, dy = D > 1 ? q.y - my : 0.f, dz = D > 2 ? q.z - mz : 0.f;
   float l = dx * dx + dy * dy + dz * dz;
   if (!(l < dmax2f)) return;
   if (l == 0.0f) {
-    const float4 jj = coincident_far(self_node, order[j], D, step_no, seed);
+    const float4 jj = coincident_far(self_node, order[j] - off, D, step_no, seed);
     dx = jj.x, dy = jj.y, dz = jj.z, l = jj.w;
   }
   if (l < dmin2f) l = sqrtf(dmin2f * l);
@@ -508,28 +549,40 @@ __device__ inline void point_force(float mx, float my, float mz, uint32_t self_n
   ++inter;
 }
 
-template <int D>
-__global__ void __launch_bounds__(WALK_WAVES * WAVE)
-    tree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
-              const NodeRec* __restrict__ rec, const CubeInfo* __restrict__ cube,
-              float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
-              uint32_t* __restrict__ counters, StepParams step, WalkArgs A) {
-  __shared__ WalkLDS lds[WALK_WAVES];
+// One tree as a walk sees it: a single simulation (off 0) or one job of a batch, whose
+// points and nodes are [off, off + n) of the shared arrays.
+struct WalkTree {
+  uint32_t off, n;
+  double size, theta2, dmax2;
+  float dmin2f, dmax2f, alpha;
+  uint32_t step_no, seed;
+  float *vx, *vy, *vz;  // the velocities, indexed by the node index within the tree
+};
+
+struct WalkCount {
+  uint64_t opened = 0, lane_visits = 0;
+  uint32_t inter = 0, err = 0;
+};
+
+// One wave walks the union of the traversals of sorted points [base, base + count) of one
+// tree (count <= 64) and adds their forces to the velocities.
+template <int D, int STACK>
+__device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __restrict__ leaves,
+                                          const uint32_t* __restrict__ order,
+                                          const NodeRec* __restrict__ rec, const WalkTree& T,
+                                          uint32_t base, uint32_t count, uint32_t packed_max,
+                                          uint32_t lane, WalkCount& C) {
   constexpr int NK = 1 << D;
-  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-  WalkLDS& L = lds[wave];
-  const uint32_t n = A.n;
-  const uint32_t base = (blockIdx.x * WALK_WAVES + wave) * WAVE;
+  const uint32_t off = T.off, n = T.n;
   const uint32_t s = base + lane;
-  const bool live = s < n;
-  if (__ballot(live) == 0ull) return;
-  const float alpha = step.alpha;
-  const uint32_t step_no = step.step, seed = step.seed;
-  const double size = (double)cube->size;
-  const double theta2 = A.theta2, dmax2 = A.dmax2;
-  const float dmin2f = (float)A.dmin2, dmax2f = (float)fmin(dmax2, 3.0e38);
+  const bool live = lane < count;
+  const float alpha = T.alpha;
+  const uint32_t step_no = T.step_no, seed = T.seed;
+  const double size = T.size;
+  const double theta2 = T.theta2, dmax2 = T.dmax2;
+  const float dmin2f = T.dmin2f, dmax2f = T.dmax2f;
   const float4 me4 = live ? leaves[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint32_t self_node = live ? order[s] : 0u;
+  const uint32_t self_node = live ? order[s] - off : 0u;
   L.body[lane] = make_float4(me4.x, me4.y, me4.z, __uint_as_float(self_node));
   L.acc[0][lane] = 0.f;
   L.acc[1][lane] = 0.f;
@@ -541,7 +594,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   uint64_t opened = 0, lane_visits = 0;
   uint32_t err = 0;
   {
-    const NodeRec& r = rec[0];
+    const NodeRec& r = rec[off];
     const NodeHdr h = *(const NodeHdr*)&r;
     bool done = live ? cell_force<D>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, size * size, h,
                                      theta2, dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter)
@@ -598,7 +651,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
     const double w2top = wt * wt;
 
     uint32_t open_bits = 0;  // bit c: this lane's point must open child c
-    if (k_active <= A.packed_max) {
+    if (k_active <= packed_max) {
       // packed: item t of round r is (point list[t / NK], child t % NK), NK items per point
       if (in) L.list[mask_rank(m)] = lane;
       const uint32_t rounds = (k_active * NK + WAVE - 1) / WAVE;
@@ -616,12 +669,12 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
           if (k & REF_POINT) {
             const uint32_t j = k & ~REF_POINT;
             if (j != base + b)
-              point_force<D>(bd.x, bd.y, bd.z, bnode, *(const float4*)&L.stage[c][0], order, j,
+              point_force<D>(bd.x, bd.y, bd.z, bnode, *(const float4*)&L.stage[c][0], order, j, off,
                              dmax2f, dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           } else {
             const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
-            open_item = !cell_force<D>(bd.x, bd.y, bd.z, bnode, n, k, top, w2top, h, theta2, dmax2,
-                                       dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
+            open_item = !cell_force<D>(bd.x, bd.y, bd.z, bnode, n, k - off, top, w2top, h, theta2,
+                                       dmax2, dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           }
         }
         // sum the NK items of each point (aligned groups of NK lanes)
@@ -655,11 +708,11 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
           const uint32_t j = k & ~REF_POINT;
           if (j != s)
             point_force<D>(me4.x, me4.y, me4.z, self_node, *(const float4*)&L.stage[c][0], order, j,
-                           dmax2f, dmin2f, alpha, step_no, seed, ax, ay, az, inter);
+                           off, dmax2f, dmin2f, alpha, step_no, seed, ax, ay, az, inter);
           continue;
         }
         const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
-        if (!cell_force<D>(me4.x, me4.y, me4.z, self_node, n, k, top, w2top, h, theta2, dmax2,
+        if (!cell_force<D>(me4.x, me4.y, me4.z, self_node, n, k - off, top, w2top, h, theta2, dmax2,
                            dmin2f, alpha, step_no, seed, ax, ay, az, inter))
           open_bits |= 1u << c;
       }
@@ -669,7 +722,7 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
     for (uint32_t c = 0; c < nk; ++c) {
       const uint64_t open = __ballot(in && ((open_bits >> c) & 1u));
       if (!open) continue;
-      if (sp >= STACK) {
+      if (sp >= (uint32_t)STACK) {
         err = 1;
         continue;
       }
@@ -683,24 +736,325 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   }
   if (live) {
     const uint32_t o = self_node;
-    vx[o] += ax + L.acc[0][lane];
-    if (D > 1) vy[o] += ay + L.acc[1][lane];
-    if (D > 2) vz[o] += az + L.acc[2][lane];
+    T.vx[o] += ax + L.acc[0][lane];
+    if (D > 1) T.vy[o] += ay + L.acc[1][lane];
+    if (D > 2) T.vz[o] += az + L.acc[2][lane];
   }
-  // counters: one atomic per wave
-  uint32_t tot = inter;
+  C.opened += opened;
+  C.lane_visits += lane_visits;
+  C.inter += inter;
+  C.err |= err;
+}
+
+// The wave's counters: one atomic each.
+__device__ inline void walk_flush(const WalkCount& C, uint32_t lane, uint32_t* counters) {
+  uint32_t tot = C.inter;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, WAVE);
   if (lane == 0) {
     atomicAdd((unsigned long long*)&counters[0], (unsigned long long)tot);
-    atomicAdd((unsigned long long*)&counters[2], (unsigned long long)opened);
-    atomicAdd((unsigned long long*)&counters[6], (unsigned long long)lane_visits);
-    if (err) atomicAdd(&counters[4], 1u);
+    atomicAdd((unsigned long long*)&counters[2], (unsigned long long)C.opened);
+    atomicAdd((unsigned long long*)&counters[6], (unsigned long long)C.lane_visits);
+    if (C.err) atomicAdd(&counters[4], 1u);
   }
+}
+
+struct WalkArgs {
+  double theta2, dmin2, dmax2;
+  uint32_t n;
+  uint32_t packed_max;  // openings by at most this many points use the packed mode
+};
+
+// One simulation: wave w of the grid walks sorted points [64 w, 64 w + 64).
+template <int D>
+__global__ void __launch_bounds__(WALK_WAVES * WAVE)
+    tree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
+              const NodeRec* __restrict__ rec, const CubeInfo* __restrict__ cube,
+              float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
+              uint32_t* __restrict__ counters, StepParams step, WalkArgs A) {
+  __shared__ WalkLDS<WALK_STACK> lds[WALK_WAVES];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const uint32_t base = (blockIdx.x * WALK_WAVES + wave) * WAVE;
+  if (base >= A.n) return;
+  WalkTree T;
+  T.off = 0;
+  T.n = A.n;
+  T.size = (double)cube->size;
+  T.theta2 = A.theta2;
+  T.dmax2 = A.dmax2;
+  T.dmin2f = (float)A.dmin2;
+  T.dmax2f = (float)fmin(A.dmax2, 3.0e38);
+  T.alpha = step.alpha;
+  T.step_no = step.step;
+  T.seed = step.seed;
+  T.vx = vx;
+  T.vy = vy;
+  T.vz = vz;
+  const uint32_t count = A.n - base < (uint32_t)WAVE ? A.n - base : (uint32_t)WAVE;
+  WalkCount C;
+  walk_tile<D, WALK_STACK>(lds[wave], leaves, order, rec, T, base, count, A.packed_max, lane, C);
+  walk_flush(C, lane, counters);
 }
 
 __global__ void tree_counters_clear(uint32_t* c) {
   if (threadIdx.x < 8) c[threadIdx.x] = 0;
+}
+
+// ---- batched trees ------------------------------------------------------------------
+//
+// The trees of many jobs at once over the tree space of a batch (jobs with a many-body
+// force, each a segment [off, off + n)).  The codes of all jobs are sorted together, then
+// stably by segment, which leaves every job's points in its own segment in the order of a
+// sort of that job alone; build, reduction and walk then work on job-relative indices, so
+// each tree is the one the single-simulation path builds for that job.
+
+template <int D>
+__global__ void __launch_bounds__(256) btree_bbox(const BTile* __restrict__ tiles,
+                                                  const uint32_t* __restrict__ seg_job, BJob* jobs) {
+  const BTile t = tiles[blockIdx.x];
+  BJob* B = jobs + seg_job[t.seg];
+  if (!B->active) return;
+  const uint32_t off = B->tree_off;
+  const float* x = B->p[0];
+  const float* y = B->p[1];
+  const float* z = B->p[2];
+  float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  for (uint32_t i = t.first + threadIdx.x; i < t.first + t.count; i += 256) {
+    const uint32_t v = i - off;
+    float p[3] = {x[v], D > 1 ? y[v] : 0.0f, D > 2 ? z[v] : 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) lo[a] = fminf(lo[a], p[a]), hi[a] = fmaxf(hi[a], p[a]);
+  }
+  block_box(lo, hi, B->cube_box);
+  __shared__ uint32_t s_last;
+  if (threadIdx.x == 0) s_last = publish_count(&B->cube_done) + 1 == B->tree_tiles;
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  float origin[3], size;
+  int32_t shift;
+  cover_box<D>(B->cube_box, origin, size, shift);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) B->cube_origin[a] = origin[a];
+  B->cube_size = size;
+  B->cube_shift = shift;
+  B->cube_done = 0;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) btree_keys(uint32_t total, const uint32_t* __restrict__ seg_of,
+                                                  const uint32_t* __restrict__ seg_job,
+                                                  const BJob* __restrict__ jobs,
+                                                  uint64_t* __restrict__ keys,
+                                                  uint64_t* __restrict__ codes,
+                                                  uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const BJob& B = jobs[seg_job[seg_of[i]]];
+  const uint32_t v = i - B.tree_off;
+  const float p[3] = {B.p[0][v], D > 1 ? B.p[1][v] : 0.0f, D > 2 ? B.p[2][v] : 0.0f};
+  const uint64_t key = point_key<D>(p, B.cube_origin, B.cube_shift);
+  keys[i] = key;
+  codes[i] = key;
+  vals[i] = i;
+}
+
+// The segment of each code-sorted entry: the key of the last, stable, sort pass.
+__global__ void __launch_bounds__(256) btree_digit(uint32_t total, const uint32_t* __restrict__ sv,
+                                                   const uint32_t* __restrict__ seg_of,
+                                                   uint32_t* __restrict__ digit) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s < total) digit[s] = seg_of[sv[s]];
+}
+
+__global__ void __launch_bounds__(256) btree_gather(uint32_t total, const uint32_t* __restrict__ order,
+                                                    const uint64_t* __restrict__ codes,
+                                                    uint64_t* __restrict__ skeys) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s < total) skeys[s] = codes[order[s]];
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) btree_leaves(uint32_t total, const uint64_t* __restrict__ skeys,
+                                                    const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ seg_of,
+                                                    const uint32_t* __restrict__ seg_job,
+                                                    const BJob* __restrict__ jobs,
+                                                    float4* __restrict__ leaves,
+                                                    uint32_t* __restrict__ leaf_depth,
+                                                    uint32_t* __restrict__ flags) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= total) return;
+  const BJob& B = jobs[seg_job[seg_of[s]]];
+  const uint32_t off = B.tree_off, n = B.n;
+  const uint32_t v = order[s] - off;
+  leaves[s] = make_float4(B.p[0][v], D > 1 ? B.p[1][v] : 0.0f, D > 2 ? B.p[2][v] : 0.0f,
+                          B.mb_strength[v]);
+  const int ls = (int)(s - off);
+  int a = delta<D>(skeys + off, (int)n, ls, ls - 1), b = delta<D>(skeys + off, (int)n, ls, ls + 1);
+  leaf_depth[s] = depth_of<D>(a > b ? a : b) + 1;
+  flags[s] = 0;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) btree_build(uint32_t total, const uint64_t* __restrict__ skeys,
+                                                   const uint32_t* __restrict__ seg_of,
+                                                   const uint32_t* __restrict__ seg_job,
+                                                   const BJob* __restrict__ jobs,
+                                                   uint4* __restrict__ links, uint2* __restrict__ spans,
+                                                   uint32_t* __restrict__ leaf_parent) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const BJob& B = jobs[seg_job[seg_of[i]]];
+  const uint32_t off = B.tree_off, k = i - off;
+  if (k + 1 >= B.n) return;
+  karras_write<D>(skeys + off, (int)B.n, (int)k, off, links, spans, leaf_parent);
+}
+
+// Persistent walk: every wave takes 64-point tiles of any job from one queue until it is
+// empty; the per-tile work is the single-simulation walk's.
+template <int D, int STACK, int WAVES>
+__global__ void __launch_bounds__(WAVES * WAVE)
+    btree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
+               const NodeRec* __restrict__ rec, const uint2* __restrict__ tiles, uint32_t ntiles,
+               const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
+               uint32_t* __restrict__ counters, uint32_t packed_max) {
+  __shared__ WalkLDS<STACK> lds[WAVES];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  WalkCount C;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&counters[5], 1u);
+    t = (uint32_t)__builtin_amdgcn_readfirstlane(t);
+    if (t >= ntiles) break;
+    const uint2 tile = tiles[t];
+    const BJob& B = jobs[seg_job[tile.x]];
+    if (!B.active) continue;
+    WalkTree T;
+    T.off = B.tree_off;
+    T.n = B.n;
+    T.size = (double)B.cube_size;
+    T.theta2 = B.theta2;
+    T.dmax2 = B.dmax2;
+    T.dmin2f = (float)B.dmin2;
+    T.dmax2f = (float)fmin(B.dmax2, 3.0e38);
+    T.alpha = B.alpha_f;
+    T.step_no = B.step;
+    T.seed = B.seed;
+    T.vx = B.v[B.vcur][0];
+    T.vy = B.v[B.vcur][1];
+    T.vz = B.v[B.vcur][2];
+    const uint32_t end = T.off + T.n;
+    const uint32_t count = end - tile.y < (uint32_t)WAVE ? end - tile.y : (uint32_t)WAVE;
+    __builtin_amdgcn_wave_barrier();
+    walk_tile<D, STACK>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, C);
+  }
+  walk_flush(C, lane, counters);
+}
+
+template <int D, int STACK, int WAVES>
+void launch_bwalk(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const BatchTune& tune,
+                  int cus) {
+  auto kern = btree_walk<D, STACK, WAVES>;
+  int per_cu = 0;
+  FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WAVES * WAVE, 0));
+  if (per_cu < 1) per_cu = 1;
+  if (tune.walk_blocks_per_cu > 0 && tune.walk_blocks_per_cu < per_cu) per_cu = tune.walk_blocks_per_cu;
+  uint32_t blocks = (uint32_t)(per_cu * (cus > 0 ? cus : 1));
+  const uint32_t need = ceil_div(sp.walk_tiles, WAVES);
+  if (blocks > need) blocks = need;
+  if (!blocks) return;
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(WAVES * WAVE), 0, st, (const float4*)t.leaves,
+                     (const uint32_t*)t.order, (const NodeRec*)t.rec, (const uint2*)sp.walk,
+                     sp.walk_tiles, (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.counters,
+                     tune.walk_packed_max);
+  FG_LAUNCH_CHECK();
+}
+
+// The walk kernel for the tuned workgroup size and stack depth; the depth is taken as the
+// nearest instantiated one at least as deep (2D: 128 or 192; 3D: 160, 192 or 256).
+template <int D>
+void launch_bwalk_tuned(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs,
+                        const BatchTune& tune, int cus) {
+  const int w = tune.walk_waves == 2 || tune.walk_waves == 8 ? tune.walk_waves : 4;
+  const int k = tune.walk_stack;
+  if (D == 1) {
+    launch_bwalk<D, 192, 4>(st, t, sp, jobs, tune, cus);
+  } else if (D == 2) {
+    if (k <= 128) {
+      if (w == 2) launch_bwalk<D, 128, 2>(st, t, sp, jobs, tune, cus);
+      else if (w == 4) launch_bwalk<D, 128, 4>(st, t, sp, jobs, tune, cus);
+      else launch_bwalk<D, 128, 8>(st, t, sp, jobs, tune, cus);
+    } else {
+      if (w == 2) launch_bwalk<D, 192, 2>(st, t, sp, jobs, tune, cus);
+      else if (w == 4) launch_bwalk<D, 192, 4>(st, t, sp, jobs, tune, cus);
+      else launch_bwalk<D, 192, 8>(st, t, sp, jobs, tune, cus);
+    }
+  } else {
+    if (k <= 160) {
+      if (w == 2) launch_bwalk<D, 160, 2>(st, t, sp, jobs, tune, cus);
+      else if (w == 4) launch_bwalk<D, 160, 4>(st, t, sp, jobs, tune, cus);
+      else launch_bwalk<D, 160, 8>(st, t, sp, jobs, tune, cus);
+    } else if (k <= 192) {
+      if (w == 2) launch_bwalk<D, 192, 2>(st, t, sp, jobs, tune, cus);
+      else if (w == 4) launch_bwalk<D, 192, 4>(st, t, sp, jobs, tune, cus);
+      else launch_bwalk<D, 192, 8>(st, t, sp, jobs, tune, cus);
+    } else {
+      if (w == 2) launch_bwalk<D, 256, 2>(st, t, sp, jobs, tune, cus);
+      else if (w == 4) launch_bwalk<D, 256, 4>(st, t, sp, jobs, tune, cus);
+      else launch_bwalk<D, 256, 8>(st, t, sp, jobs, tune, cus);
+    }
+  }
+}
+
+template <int D>
+void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const BatchTune& tune,
+               int cus, PhaseTimer* timer) {
+  const uint32_t total = sp.total;
+  if (!total) return;
+  t.reserve(total);
+  const uint32_t blocks = ceil_div(total, 256);
+  if (timer) timer->begin(1);
+  hipLaunchKernelGGL(tree_counters_clear, dim3(1), dim3(64), 0, st, t.counters);
+  hipLaunchKernelGGL((btree_bbox<D>), dim3(sp.tiles), dim3(256), 0, st, (const BTile*)sp.tile,
+                     (const uint32_t*)sp.seg_job, jobs);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL((btree_keys<D>), dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)sp.seg_of,
+                     (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.keys, t.codes, t.vals);
+  FG_LAUNCH_CHECK();
+  if (timer) timer->end(1), timer->begin(2);
+  uint64_t* sk;
+  uint32_t* sv;
+  radix_sort_u64(st, t.sort, tune.radix_bits, t.keys, t.vals, total, 0, Dims<D>::BITS, &sk, &sv);
+  uint32_t* order = sv;
+  if (sp.segs > 1) {
+    hipLaunchKernelGGL(btree_digit, dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)sv,
+                       (const uint32_t*)sp.seg_of, t.digit);
+    FG_LAUNCH_CHECK();
+    uint32_t bits = 1;
+    while ((1u << bits) < sp.segs) ++bits;
+    uint32_t* sd;
+    radix_sort_u32(st, t.sort_seg, tune.radix_bits, t.digit, sv, total, 0, bits, &sd, &order);
+  }
+  if (timer) timer->end(2), timer->begin(1);
+  hipLaunchKernelGGL(btree_gather, dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)order,
+                     (const uint64_t*)t.codes, t.skeys);
+  FG_LAUNCH_CHECK();
+  t.order = order;
+  hipLaunchKernelGGL((btree_leaves<D>), dim3(blocks), dim3(256), 0, st, total, (const uint64_t*)t.skeys,
+                     (const uint32_t*)order, (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job,
+                     (const BJob*)jobs, t.leaves, t.leaf_depth, t.flags);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL((btree_build<D>), dim3(blocks), dim3(256), 0, st, total, (const uint64_t*)t.skeys,
+                     (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.links,
+                     t.span, t.leaf_parent);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL((tree_reduce<D>), dim3(ceil_div(total, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st,
+                     total, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent, (const uint4*)t.links,
+                     (const uint2*)t.span, t.flags, (NodeRec*)t.rec, (NodeSum*)t.sums);
+  FG_LAUNCH_CHECK();
+  if (timer) timer->end(1), timer->begin(3);
+  launch_bwalk_tuned<D>(st, t, sp, jobs, tune, cus);
+  if (timer) timer->end(3);
 }
 
 template <int D>
@@ -805,6 +1159,56 @@ void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const fl
   if (ndim == 1) run<1>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer);
   else if (ndim == 2) run<2>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer);
   else run<3>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer);
+}
+
+}  // namespace fg
+
+namespace fg {
+
+void BatchTree::reserve(uint32_t n) {
+  if (n <= cap && keys) return;
+  release();
+  uint32_t m = n > 2 ? n : 2;
+  FG_HIP(hipMalloc(&keys, sizeof(uint64_t) * m));
+  FG_HIP(hipMalloc(&codes, sizeof(uint64_t) * m));
+  FG_HIP(hipMalloc(&skeys, sizeof(uint64_t) * m));
+  FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&digit, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&leaves, sizeof(float4) * m));
+  FG_HIP(hipMalloc(&leaf_depth, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&leaf_parent, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&links, sizeof(uint4) * m));
+  FG_HIP(hipMalloc(&span, sizeof(uint2) * m));
+  FG_HIP(hipMalloc(&flags, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
+  FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
+  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 8));
+  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * 8));
+  sort.reserve(m, sizeof(uint64_t));
+  sort_seg.reserve(m, sizeof(uint32_t));
+  cap = n;
+}
+
+void BatchTree::release() {
+  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, leaf_depth, leaf_parent, links, span, flags, rec, sums, counters};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  keys = codes = skeys = nullptr;
+  vals = digit = order = leaf_depth = leaf_parent = flags = counters = nullptr;
+  leaves = nullptr;
+  links = nullptr;
+  span = nullptr;
+  rec = sums = nullptr;
+  sort.release();
+  sort_seg.release();
+  cap = 0;
+}
+
+void batch_manybody(hipStream_t s, int ndim, BatchTree& t, const BSpace& sp, BJob* jobs,
+                    const BatchTune& tune, int cus, PhaseTimer* timer) {
+  if (ndim == 1) run_batch<1>(s, t, sp, jobs, tune, cus, timer);
+  else if (ndim == 2) run_batch<2>(s, t, sp, jobs, tune, cus, timer);
+  else run_batch<3>(s, t, sp, jobs, tune, cus, timer);
 }
 
 }  // namespace fg

@@ -51,6 +51,33 @@ class Timing(C.Structure):
     _fields_ = [("ms", C.c_double * 16), ("launches", C.c_uint64 * 16)]
 
 
+class JobDesc(C.Structure):
+    """fg_job_desc."""
+    _fields_ = [("n", C.c_uint32), ("ndim", C.c_int32), ("pos", _d), ("vel", _d), ("fix", _d), ("fixmask", _u8),
+                ("nforces", C.c_uint32), ("forces", C.POINTER(ForceDesc)), ("alpha", C.c_double),
+                ("alpha_min", C.c_double), ("alpha_decay", C.c_double), ("alpha_target", C.c_double),
+                ("velocity_decay", C.c_double), ("seed", C.c_uint32), ("max_ticks", C.c_uint32)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_uint32), ("ticks", C.c_uint32), ("alpha", C.c_double),
+                ("n", C.c_uint32), ("ndim", C.c_int32)]
+
+
+class BatchTune(C.Structure):
+    _fields_ = [("radix_bits", C.c_int32), ("walk_waves", C.c_int32), ("walk_stack", C.c_int32),
+                ("walk_blocks_per_cu", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("jobs_running", C.c_uint32), ("jobs_done", C.c_uint32), ("nodes", C.c_uint64),
+                ("bytes", C.c_uint64), ("budget", C.c_uint64), ("ticks", C.c_uint64),
+                ("graph_builds", C.c_uint64), ("interactions", C.c_uint64), ("walk_errors", C.c_uint32),
+                ("launches_per_tick", C.c_uint32)]
+
+
+JOB_RUNNING, JOB_DONE = 1, 3
+
 PHASES = ["tick", "tree_build", "tree_sort", "walk", "pairs", "link", "collide", "center", "radial",
           "position", "integrate"]
 
@@ -82,6 +109,21 @@ SIGNATURES = {
     "fg_get_timing": (C.c_int, [C.c_void_p, C.POINTER(Timing)]),
     "fg_radix_sort_pairs": (C.c_int, [C.c_void_p, C.c_uint32, _u, _u, C.c_uint32, C.c_uint32, _u, _u]),
     "fg_debug_tree": (C.c_int, [C.c_void_p, _f, _i, _u, _u, _u, _u, _u, _u, _u, _d, _d]),
+    "fg_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_uint64]),
+    "fg_batch_destroy": (C.c_int, [C.c_void_p]),
+    "fg_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "fg_batch_job_bytes": (C.c_int, [C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
+    "fg_batch_submit": (C.c_int, [C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
+    "fg_batch_step": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "fg_batch_sync": (C.c_int, [C.c_void_p]),
+    "fg_batch_jobs": (C.c_int, [C.c_void_p, C.POINTER(JobInfo), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "fg_batch_read": (C.c_int, [C.c_void_p, C.c_uint64, _d, _d]),
+    "fg_batch_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "fg_batch_set_tune": (C.c_int, [C.c_void_p, C.POINTER(BatchTune)]),
+    "fg_batch_get_tune": (C.c_int, [C.c_void_p, C.POINTER(BatchTune)]),
+    "fg_batch_autotune": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(BatchTune)]),
+    "fg_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
+    "fg_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
 }
 
 _lib = None
@@ -346,3 +388,146 @@ class Engine:
         return dict(cube=cube, shift=shift.value, hi=codes[0::2].copy(), lo=codes[1::2].copy(), order=order,
                     leaf_depth=leaf_depth, left=left[:n - 1], right=right[:n - 1], depth=depth[:n - 1],
                     kids=kids.reshape(inner, 8)[:n - 1], com=com.reshape(inner, 3)[:n - 1], value=value[:n - 1])
+
+
+ALPHA_DECAY = 1 - 0.001 ** (1 / 300)
+
+
+class Job:
+    """One layout for a batch: nodes, forces (in the order link, manyBody, collide, center,
+    radial, x, y, z) and d3's schedule."""
+
+    def __init__(self, pos, forces, ndim=None, vel=None, fix=None, fixmask=None, alpha=1.0, alpha_min=0.001,
+                 alpha_decay=ALPHA_DECAY, alpha_target=0.0, velocity_decay=0.4, seed=0, max_ticks=0):
+        pos = _f64(pos)
+        self.n = len(pos)
+        self.ndim = ndim or pos.shape[1]
+        self.pos3 = np.zeros((self.n, 3))
+        self.pos3[:, :pos.shape[1]] = pos
+        self.vel3 = None
+        if vel is not None:
+            vel = _f64(vel)
+            self.vel3 = np.zeros((self.n, 3))
+            self.vel3[:, :vel.shape[1]] = vel
+        self.fix3 = None if fix is None else np.ascontiguousarray(_f64(fix).reshape(self.n, 3))
+        self.mask = None if fixmask is None else np.ascontiguousarray(fixmask, dtype=np.uint8)
+        self.forces = list(forces)
+        self.alpha, self.alpha_min, self.alpha_decay = alpha, alpha_min, alpha_decay
+        self.alpha_target, self.velocity_decay = alpha_target, velocity_decay
+        self.seed, self.max_ticks = seed, max_ticks
+
+    def desc(self):
+        """(fg_job_desc, objects that must stay alive while it is used)."""
+        descs, keep = [], [self]
+        for f in self.forces:
+            d, k = f.desc()
+            descs.append(d)
+            keep.extend(k)
+        arr = (ForceDesc * max(1, len(descs)))(*descs)
+        keep.append(arr)
+        j = JobDesc(n=self.n, ndim=self.ndim, pos=ptr(self.pos3), vel=ptr(self.vel3), fix=ptr(self.fix3),
+                    fixmask=ptr(self.mask, C.c_uint8), nforces=len(descs), forces=arr, alpha=self.alpha,
+                    alpha_min=self.alpha_min, alpha_decay=self.alpha_decay, alpha_target=self.alpha_target,
+                    velocity_decay=self.velocity_decay, seed=self.seed & 0xffffffff, max_ticks=self.max_ticks)
+        return j, keep
+
+    def bytes(self):
+        j, keep = self.desc()
+        out = C.c_uint64(0)
+        rc = lib().fg_batch_job_bytes(C.byref(j), C.byref(out))
+        if rc != FG_OK:
+            raise FgError(rc, "fg_batch_job_bytes failed")
+        return out.value
+
+
+class Batch:
+    """Many layouts resident on one GPU, ticked together (fg_batch_*)."""
+
+    def __init__(self, device=0, budget=0):
+        h = C.c_void_p()
+        rc = lib().fg_batch_create(C.byref(h), device, budget)
+        if rc != FG_OK:
+            raise FgError(rc, "fg_batch_create failed")
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if self.h:
+            lib().fg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != FG_OK:
+            raise FgError(rc, lib().fg_batch_last_error(self.h).decode())
+
+    def submit(self, job):
+        """The job's id; raises FgError with status -3 when the budget has no room."""
+        j, keep = job.desc()
+        out = C.c_uint64(0)
+        self._check(lib().fg_batch_submit(self.h, C.byref(j), C.byref(out)))
+        self.sizes[out.value] = job.n
+        return out.value
+
+    def try_submit(self, job):
+        """The job's id, or None when the budget has no room now."""
+        try:
+            return self.submit(job)
+        except FgError as e:
+            if e.status == -3:
+                return None
+            raise
+
+    def step(self, ticks=1):
+        self._check(lib().fg_batch_step(self.h, ticks))
+
+    def sync(self):
+        self._check(lib().fg_batch_sync(self.h))
+
+    def jobs(self):
+        cap = 4096
+        arr = (JobInfo * cap)()
+        k = C.c_uint32(0)
+        self._check(lib().fg_batch_jobs(self.h, arr, cap, C.byref(k)))
+        return [dict(id=arr[i].id, status=arr[i].status, ticks=arr[i].ticks, alpha=arr[i].alpha, n=arr[i].n,
+                     ndim=arr[i].ndim) for i in range(min(k.value, cap))]
+
+    def read(self, job_id):
+        n = self.sizes[job_id]
+        pos, vel = np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(lib().fg_batch_read(self.h, job_id, ptr(pos), ptr(vel)))
+        return pos, vel
+
+    def release(self, job_id):
+        self._check(lib().fg_batch_release(self.h, job_id))
+        self.sizes.pop(job_id, None)
+
+    def set_tune(self, radix_bits=8, walk_waves=4, walk_stack=192, walk_blocks_per_cu=0):
+        t = BatchTune(radix_bits, walk_waves, walk_stack, walk_blocks_per_cu)
+        self._check(lib().fg_batch_set_tune(self.h, C.byref(t)))
+
+    def tune(self):
+        t = BatchTune()
+        self._check(lib().fg_batch_get_tune(self.h, C.byref(t)))
+        return dict(radix_bits=t.radix_bits, walk_waves=t.walk_waves, walk_stack=t.walk_stack,
+                    walk_blocks_per_cu=t.walk_blocks_per_cu)
+
+    def autotune(self, cache_path=None):
+        t = BatchTune()
+        path = cache_path.encode() if cache_path else None
+        self._check(lib().fg_batch_autotune(self.h, path, C.byref(t)))
+        return dict(radix_bits=t.radix_bits, walk_waves=t.walk_waves, walk_stack=t.walk_stack,
+                    walk_blocks_per_cu=t.walk_blocks_per_cu)
+
+    def use_graphs(self, on=True):
+        self._check(lib().fg_batch_use_graphs(self.h, 1 if on else 0))
+
+    def stats(self):
+        s = BatchStats()
+        self._check(lib().fg_batch_get_stats(self.h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in BatchStats._fields_}

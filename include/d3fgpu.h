@@ -234,6 +234,159 @@ int fg_debug_tree(fg_ctx* ctx, float* cube, int32_t* shift, uint32_t* codes, uin
                   uint32_t* leaf_depth, uint32_t* left, uint32_t* right, uint32_t* depth,
                   uint32_t* kids, double* com, double* value);
 
+
+/* ---- batches: many independent layouts advanced by one launch sequence ------------
+ *
+ * No counterpart in the reference (it runs one simulation per engine).  A batch holds
+ * jobs of any size and dimensionality resident on one GPU; each tick of the batch ticks
+ * every running job once, with one sequence of launches per dimensionality (segmented
+ * sort and tree, one persistent walk over a queue of 64-point tiles of all jobs), the
+ * tick captured in a hipGraph.  Each job keeps d3's own schedule (alpha, alphaMin,
+ * alphaDecay, alphaTarget, velocityDecay; simulation.js:54-73), advanced on the device,
+ * and finishes after the tick whose alpha falls below alphaMin.  A job's result does not
+ * depend on the other jobs: alone or in any mix it advances bit for bit the same.
+ *
+ * A job's forces are applied in the order link, manyBody, collide, center, radial, x, y,
+ * z (each at most once); descriptors in another order are refused.  Collide always runs
+ * on the grid, manyBody always on the tree.
+ */
+typedef struct fg_batch fg_batch;
+
+typedef struct fg_job_desc {
+  uint32_t n;                  /* nodes (>= 1) */
+  int32_t ndim;                /* 1, 2 or 3 */
+  const double* pos;           /* 3n, required */
+  const double* vel;           /* 3n or NULL (zero) */
+  const double* fix;           /* 3n or NULL */
+  const uint8_t* fixmask;      /* n or NULL: bit a = axis a pinned */
+  uint32_t nforces;
+  const fg_force_desc* forces;
+  double alpha;                /* starting alpha (1) */
+  double alpha_min;            /* 0.001 */
+  double alpha_decay;          /* 1 - 0.001^(1/300) */
+  double alpha_target;         /* 0 */
+  double velocity_decay;       /* d3's velocityDecay (0.4): velocities are multiplied by 1 - it */
+  uint32_t seed;               /* jiggle seed */
+  uint32_t max_ticks;          /* stop after this many ticks as well (0: no limit) */
+} fg_job_desc;
+
+/* Job states */
+#define FG_JOB_RUNNING 1
+#define FG_JOB_DONE 3
+
+typedef struct fg_job_info {
+  uint64_t id;
+  uint32_t status;   /* FG_JOB_RUNNING or FG_JOB_DONE */
+  uint32_t ticks;    /* ticks run */
+  double alpha;      /* alpha of the last tick run */
+  uint32_t n;
+  int32_t ndim;
+} fg_job_info;
+
+/* Tunables of the batched tick (fg_batch_autotune chooses them). */
+typedef struct fg_batch_tune {
+  int32_t radix_bits;          /* 8 or 11 bits per radix digit */
+  int32_t walk_waves;          /* waves per walk workgroup: 2, 4 or 8 */
+  int32_t walk_stack;          /* walk stack entries per wave: 128 (2D), 160, 192, 256 (3D) */
+  int32_t walk_blocks_per_cu;  /* persistent walk workgroups per CU, 0 = as many as fit */
+} fg_batch_tune;
+
+typedef struct fg_batch_stats {
+  uint32_t jobs_running, jobs_done;
+  uint64_t nodes;              /* resident nodes */
+  uint64_t bytes;              /* device bytes held for resident jobs and the tick's scratch */
+  uint64_t budget;             /* the HBM budget */
+  uint64_t ticks;              /* batched ticks run */
+  uint64_t graph_builds;       /* tick graphs captured (one per layout or tune change) */
+  uint64_t interactions;       /* body-source evaluations of the last tick's walks */
+  uint32_t walk_errors;        /* walks that hit their stack or iteration bound (must be 0) */
+  uint32_t launches_per_tick;  /* kernel launches in one tick */
+} fg_batch_stats;
+
+/* hbm_budget: device bytes the batch may hold (0 = 80% of the free memory). */
+int fg_batch_create(fg_batch** out, int device, uint64_t hbm_budget);
+int fg_batch_destroy(fg_batch* b);
+const char* fg_batch_last_error(const fg_batch* b);
+/* Device bytes a job costs while resident (its state and arrays plus its share of the
+ * tick's scratch); what admission charges against the budget. */
+int fg_batch_job_bytes(const fg_job_desc* job, uint64_t* bytes);
+/* Make a job resident (copies every array).  FG_ERR_NOMEM: the budget has no room now. */
+int fg_batch_submit(fg_batch* b, const fg_job_desc* job, uint64_t* id);
+/* Queue `ticks` ticks of every running job; returns at once. */
+int fg_batch_step(fg_batch* b, uint32_t ticks);
+int fg_batch_sync(fg_batch* b);
+/* Waits for queued ticks; the resident jobs in submission order. */
+int fg_batch_jobs(fg_batch* b, fg_job_info* out, uint32_t cap, uint32_t* count);
+/* Waits for queued ticks; a job's state, 3n doubles each (either may be NULL). */
+int fg_batch_read(fg_batch* b, uint64_t id, double* pos, double* vel);
+/* Removes a job (running or done) and frees its memory. */
+int fg_batch_release(fg_batch* b, uint64_t id);
+int fg_batch_set_tune(fg_batch* b, const fg_batch_tune* t);
+int fg_batch_get_tune(fg_batch* b, fg_batch_tune* t);
+/* Times candidate tunables on the resident jobs (their state is saved and restored, so
+ * results are unchanged), keeps the fastest that runs without walk errors and records it
+ * in the text file cache_path under the batch's shape (NULL: no file).  A shape found in
+ * the file is taken from it without timing. */
+int fg_batch_autotune(fg_batch* b, const char* cache_path, fg_batch_tune* chosen);
+/* Capture ticks in a hipGraph (on != 0, the default) or launch them one by one. */
+int fg_batch_use_graphs(fg_batch* b, int32_t on);
+int fg_batch_get_stats(fg_batch* b, fg_batch_stats* out);
+
+/* ---- serving: a request loop over a batch (continuous batching) -----------------
+ *
+ * No counterpart in the reference.  A server owns a batch and a worker thread.  Submitted
+ * jobs are copied and queued; between rounds of ticks the worker admits queued jobs in
+ * arrival order while they fit the HBM budget (a job that cannot fit even an empty batch
+ * fails), so jobs arriving while others run join the running batch as memory frees up.
+ * After every round it reads back and releases the jobs that finished (alpha <
+ * alphaMin), which wakes their waiters.  Latency is finish time minus submit time.
+ */
+typedef struct fg_server fg_server;
+
+typedef struct fg_serve_config {
+  uint64_t hbm_budget;       /* device bytes for resident jobs and scratch (0: 80% of free) */
+  uint32_t ticks_per_round;  /* ticks between admission and retirement passes (0: 10) */
+  int32_t autotune;          /* nonzero: tune each new batch shape (fg_batch_autotune) */
+  const char* tune_cache;    /* the autotuner's cache file (NULL: none) */
+} fg_serve_config;
+
+#define FG_SERVE_QUEUED 0
+#define FG_SERVE_RUNNING 1
+#define FG_SERVE_FINISHED 2
+#define FG_SERVE_FAILED 3
+
+typedef struct fg_serve_result {
+  uint64_t id;
+  uint32_t status;           /* FG_SERVE_* */
+  uint32_t n;
+  int32_t ndim;
+  uint32_t ticks;            /* ticks the job ran */
+  double submitted_s, admitted_s, finished_s;  /* on the server's clock (seconds) */
+} fg_serve_result;
+
+typedef struct fg_serve_stats {
+  uint64_t submitted, admitted, finished, failed;
+  uint32_t queued, running, peak_running;
+  double p50_latency_s, p99_latency_s, mean_latency_s, max_latency_s;
+  double layouts_per_s;      /* finished jobs / (last finish - first submission) */
+  uint64_t rounds, ticks;
+  uint64_t peak_bytes, budget;
+} fg_serve_stats;
+
+int fg_serve_create(fg_server** out, int device, const fg_serve_config* cfg);
+/* Stops the loop; jobs not yet finished are dropped. */
+int fg_serve_destroy(fg_server* s);
+const char* fg_serve_last_error(const fg_server* s);
+/* Copies the job and queues it; returns at once with its id. */
+int fg_serve_submit(fg_server* s, const fg_job_desc* job, uint64_t* id);
+/* Blocks until the job finished (or timeout_s passed: FG_ERR_STATE), then copies its
+ * final state (3n doubles each, either may be NULL) and forgets it. */
+int fg_serve_wait(fg_server* s, uint64_t id, double timeout_s, double* pos, double* vel,
+                  fg_serve_result* result);
+/* Blocks until no job is queued or running (or timeout_s passed: FG_ERR_STATE). */
+int fg_serve_drain(fg_server* s, double timeout_s);
+int fg_serve_get_stats(fg_server* s, fg_serve_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,221 @@
+"""The batch engine (fg_batch_*): many layouts ticked together.
+
+  * forces of every job in a mixed batch (1D/2D/3D, sizes 300..6000) against the FP64
+    oracle from identical f32 input, to the single-simulation bars (rel-L2 <= 1e-4 for
+    the tree force);
+  * a job advances bit for bit the same alone and in a mix (the serving guarantee),
+    with and without graph capture and under any tunables;
+  * the batch and the single-simulation engine agree job by job over many ticks
+    (bit for bit for link + manyBody; within rounding where center and collide sum in
+    another order);
+  * jobs finish after exactly the tick d3's schedule says (alpha < alphaMin), and the
+    HBM budget refuses what does not fit.
+"""
+import numpy as np
+import pytest
+
+import d3fgpu
+import oracle as O
+from bench import ba_graph, link_defaults, phyllotaxis
+from helpers import PARITY, close_count, make_nodes_fast, rel_l2, tree_outliers
+
+pytestmark = pytest.mark.gpu
+
+DECAY = d3fgpu.ALPHA_DECAY
+
+
+def layout_forces(n, ndim, seed, collide=True, center=True, manybody=True, link=True, radius=3.0):
+    """Link (BA m=2, d3's defaults) + manyBody(-30) + collide(radius) + center."""
+    fs = []
+    if link and n > 1:
+        src, dst = ba_graph(n, 2, seed)
+        bias, strengths, distances = link_defaults(n, src, dst)
+        fs.append(d3fgpu.link(src, dst, strengths, distances, bias))
+    if manybody:
+        fs.append(d3fgpu.many_body(np.full(n, -30.0)))
+    if collide:
+        fs.append(d3fgpu.collide(np.full(n, radius)))
+    if center:
+        fs.append(d3fgpu.center())
+    return fs
+
+
+def make_job(n, ndim, seed, **kw):
+    return d3fgpu.Job(phyllotaxis(n, ndim)[:, :ndim], layout_forces(n, ndim, seed, **kw), ndim=ndim,
+                      seed=seed)
+
+
+def run_alone(job, ticks, **tune):
+    b = d3fgpu.Batch(0)
+    if tune:
+        b.set_tune(**tune)
+    jid = b.submit(job)
+    b.step(ticks)
+    out = b.read(jid)
+    b.close()
+    return out
+
+
+def test_batch_forces_vs_oracle():
+    """One tick from zero velocity with no velocity decay: v = the force increment."""
+    sizes = [(300, 3), (1500, 2), (6000, 3), (700, 1), (4000, 2)]
+    b = d3fgpu.Batch(0)
+    jobs = {}
+    for k, (n, nd) in enumerate(sizes):
+        pos = make_nodes_fast(n, nd, seed=100 + k)
+        s = np.full(n, -30.0) * (1 + 0.5 * np.sin(np.arange(n)))
+        job = d3fgpu.Job(pos[:, :nd], [d3fgpu.many_body(s)], ndim=nd, velocity_decay=0.0, seed=k)
+        jobs[b.submit(job)] = (pos, s, nd)
+    b.step(1)
+    alpha = 1.0 + (0.0 - 1.0) * DECAY
+    for jid, (pos, s, nd) in jobs.items():
+        _, v = b.read(jid)
+        ref, _ = O.manybody(pos, np.zeros_like(pos), s.astype(np.float32).astype(np.float64), alpha,
+                            theta2=0.81, dmin2=1.0, dmax2=np.inf, ndim=nd)
+        err = rel_l2(v[:, :nd], ref[:, :nd])
+        PARITY.append({"case": f"batch manyBody {nd}D n={len(pos)}", "rel_l2": err})
+        print(f"[parity] batch manyBody {nd}D n={len(pos)}: rel-L2 {err:.3e}", flush=True)
+        assert close_count(v, ref, rtol=2e-2 if nd == 1 else 1e-3) <= tree_outliers(len(pos))
+        if nd > 1:
+            assert err < 1e-4
+    st = b.stats()
+    assert st["walk_errors"] == 0
+    b.close()
+
+
+def test_batch_alone_equals_mix():
+    """Each job of a mixed batch ends bit for bit where it ends when run alone."""
+    specs = [(2500, 3, 1), (800, 2, 2), (6000, 3, 3), (1200, 2, 4), (400, 3, 5), (3000, 1, 6)]
+    ticks = 25
+    mix = d3fgpu.Batch(0)
+    ids = [mix.submit(make_job(n, nd, sd)) for n, nd, sd in specs]
+    mix.step(ticks)
+    for jid, (n, nd, sd) in zip(ids, specs):
+        p_mix, v_mix = mix.read(jid)
+        p_one, v_one = run_alone(make_job(n, nd, sd), ticks)
+        assert np.all(np.isfinite(p_mix))
+        assert np.array_equal(p_mix, p_one), f"job n={n} {nd}D differs alone vs mix"
+        assert np.array_equal(v_mix, v_one)
+    mix.close()
+
+
+def test_batch_graphs_and_tunables_change_nothing():
+    specs = [(3000, 3, 11), (900, 2, 12), (5000, 3, 13)]
+    ticks = 12
+    results = []
+    for graphs, tune in [(True, {}), (False, {}),
+                         (True, dict(radix_bits=11, walk_waves=8, walk_stack=256)),
+                         (True, dict(radix_bits=8, walk_waves=2, walk_stack=160))]:
+        b = d3fgpu.Batch(0)
+        b.use_graphs(graphs)
+        if tune:
+            b.set_tune(**tune)
+        ids = [b.submit(make_job(n, nd, sd)) for n, nd, sd in specs]
+        b.step(ticks)
+        results.append([b.read(j)[0] for j in ids])
+        assert b.stats()["walk_errors"] == 0
+        b.close()
+    for other in results[1:]:
+        for a, c in zip(results[0], other):
+            assert np.array_equal(a, c)
+
+
+def plain_run(n, nd, seed, ticks, **kw):
+    e = d3fgpu.Engine(0)
+    e.set_nodes(phyllotaxis(n, nd), ndim=nd)
+    e.set_forces(layout_forces(n, nd, seed, **kw))
+    e.set_seed(seed)
+    alphas, a = [], 1.0
+    for _ in range(ticks):
+        a += (0.0 - a) * DECAY
+        alphas.append(a)
+    e.enqueue(alphas, 0.6)
+    p, v = e.readback()
+    e.close()
+    return p, v
+
+
+@pytest.mark.parametrize("n,nd", [(3000, 3), (2000, 2)])
+def test_batch_equals_single_simulation_link_manybody(n, nd):
+    ticks = 30
+    p1, v1 = plain_run(n, nd, 21, ticks, collide=False, center=False)
+    p2, v2 = run_alone(make_job(n, nd, 21, collide=False, center=False), ticks)
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(v1, v2)
+
+
+@pytest.mark.parametrize("n,nd", [(9000, 3), (3000, 2)])
+def test_batch_close_to_single_simulation_full_mix(n, nd):
+    """center sums and collide visits in another order: agreement to rounding, over 10 ticks."""
+    ticks = 10
+    p1, _ = plain_run(n, nd, 31, ticks)
+    p2, _ = run_alone(make_job(n, nd, 31), ticks)
+    err = rel_l2(p2 - phyllotaxis(n, nd), p1 - phyllotaxis(n, nd))
+    print(f"[parity] batch vs single, {nd}D n={n}, displacement after {ticks} ticks: rel-L2 {err:.2e}")
+    assert err < 1e-4
+
+
+def test_batch_finish_at_alpha_min():
+    """Jobs stop after the tick whose alpha falls below alphaMin (simulation.js:54-73)."""
+    cases = [(1.0, 0.001, DECAY), (0.3, 0.01, 0.05), (1.0, 0.2, 0.1), (0.5, 0.001, 0.0228)]
+    b = d3fgpu.Batch(0)
+    ids = []
+    want = []
+    for k, (a0, amin, dec) in enumerate(cases):
+        job = make_job(200 + 50 * k, 2 + k % 2, 40 + k)
+        job.alpha, job.alpha_min, job.alpha_decay = a0, amin, dec
+        ids.append(b.submit(job))
+        a, t = a0, 0
+        while True:
+            a += (0.0 - a) * dec
+            t += 1
+            if a < amin:
+                break
+        want.append((t, a))
+    b.step(max(t for t, _ in want) + 5)
+    info = {j["id"]: j for j in b.jobs()}
+    for jid, (t, a) in zip(ids, want):
+        assert info[jid]["status"] == d3fgpu.JOB_DONE
+        assert info[jid]["ticks"] == t
+        assert info[jid]["alpha"] == a
+    b.close()
+
+
+def test_batch_budget_and_release():
+    job = make_job(20000, 3, 51)
+    need = job.bytes()
+    b = d3fgpu.Batch(0, budget=int(need * 1.5))
+    j1 = b.submit(job)
+    assert b.try_submit(make_job(20000, 3, 52)) is None
+    b.step(2)
+    b.release(j1)
+    j2 = b.try_submit(make_job(20000, 3, 52))
+    assert j2 is not None
+    b.step(2)
+    p, _ = b.read(j2)
+    assert np.all(np.isfinite(p))
+    st = b.stats()
+    assert st["bytes"] <= st["budget"]
+    b.close()
+
+
+def test_batch_autotune_keeps_results(tmp_path):
+    specs = [(4000, 3, 61), (1500, 2, 62)]
+    ticks = 8
+    b = d3fgpu.Batch(0)
+    ids = [b.submit(make_job(n, nd, sd)) for n, nd, sd in specs]
+    cache = str(tmp_path / "tune.txt")
+    chosen = b.autotune(cache)
+    assert chosen["radix_bits"] in (8, 11) and chosen["walk_waves"] in (2, 4, 8)
+    b.step(ticks)
+    got = [b.read(j)[0] for j in ids]
+    b.close()
+    for (n, nd, sd), p in zip(specs, got):
+        q, _ = run_alone(make_job(n, nd, sd), ticks)
+        assert np.array_equal(p, q)
+    # the cache answers the same shape without timing
+    b = d3fgpu.Batch(0)
+    for n, nd, sd in specs:
+        b.submit(make_job(n, nd, sd))
+    assert b.autotune(cache) == chosen
+    b.close()
