@@ -291,6 +291,95 @@ def bench_batch(jobs, with_plain=True):
     return out
 
 
+def serve_specs(jobs, seed, lo=1000, hi=500_000):
+    """The serving stream's jobs: sizes log-uniform in [lo, hi], 70% 3D, Barabasi-Albert with
+    m in {1, 2, 3}, each force of {link, manyBody, collide, center, radial} present with
+    probability 0.6 (at least one), in that order."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(jobs):
+        n = int(np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        d = 3 if rng.random() < 0.7 else 2
+        m = int(rng.integers(1, 4))
+        mix = rng.random(5) < 0.6
+        if not mix.any():
+            mix[int(rng.integers(0, 5))] = True
+        out.append(dict(n=n, ndim=d, m=m, mix=[bool(x) for x in mix], seed=seed * 1000 + k))
+    return out
+
+
+def serve_job(spec):
+    """A d3fgpu.Job for one serving spec."""
+    import d3fgpu
+    n, d, sd = spec["n"], spec["ndim"], spec["seed"]
+    fs = []
+    if spec["mix"][0] and n > 1:
+        src, dst = ba_graph(n, spec["m"], sd)
+        bias, strengths, distances = link_defaults(n, src, dst)
+        fs.append(d3fgpu.link(src, dst, strengths, distances, bias))
+    if spec["mix"][1]:
+        fs.append(d3fgpu.many_body(np.full(n, -30.0), theta=0.9))
+    if spec["mix"][2]:
+        fs.append(d3fgpu.collide(np.full(n, 3.0)))
+    if spec["mix"][3]:
+        fs.append(d3fgpu.center())
+    if spec["mix"][4]:
+        fs.append(d3fgpu.radial(np.full(n, 10.0 * n ** (1.0 / d)), np.full(n, 0.05)))
+    return d3fgpu.Job(phyllotaxis(n, d)[:, :d], fs, ndim=d, seed=sd)
+
+
+def serve_run(jobs, rate, budget, seed=7):
+    """Poisson arrivals at `rate` jobs/s (inf: all at once) into a server; every job runs to
+    alpha < alphaMin.  Returns the server's latency and throughput figures."""
+    import threading
+    import d3fgpu
+    srv = d3fgpu.Server(0, budget=budget, ticks_per_round=10)
+    rng = np.random.default_rng(seed)
+    gaps = np.zeros(len(jobs)) if not np.isfinite(rate) else rng.exponential(1.0 / rate, len(jobs))
+    ids = []
+    t0 = time.perf_counter()
+    due = 0.0
+    for job, gap in zip(jobs, gaps):
+        due += gap
+        delay = due - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(job))
+    for j in ids:
+        srv.wait(j, timeout=900.0)
+    st = srv.stats()
+    srv.close()
+    return st
+
+
+def bench_serve(jobs, budget_gb=1.0):
+    specs = serve_specs(jobs, seed=5)
+    log(f"serving: building {jobs} jobs ({sum(s['n'] for s in specs)} nodes)")
+    built = [serve_job(s) for s in specs]
+    budget = int(budget_gb * 2 ** 30)
+    runs = []
+    capacity = None
+    for frac in (None, 0.8, 0.5):
+        # all jobs at once first (the capacity), then Poisson arrivals at fractions of it
+        rate = float("inf") if frac is None else frac * capacity
+        st = serve_run(built, rate, budget)
+        if frac is None:
+            capacity = st["layouts_per_s"]
+        log(f"serving at {rate} jobs/s: p50 {st['p50_latency_s']:.3f}s p99 {st['p99_latency_s']:.3f}s "
+            f"{st['layouts_per_s']:.2f} layouts/s, peak running {st['peak_running']}")
+        runs.append({"arrival_rate_per_s": None if not np.isfinite(rate) else round(rate, 3),
+                     "load": "all at once" if frac is None else f"{frac:.0%} of capacity",
+                     "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4),
+                     "mean_latency_s": round(st["mean_latency_s"], 4),
+                     "layouts_per_s": round(st["layouts_per_s"], 3), "finished": st["finished"],
+                     "failed": st["failed"], "peak_running": st["peak_running"],
+                     "peak_bytes_gb": round(st["peak_bytes"] / 2 ** 30, 3)})
+    return {"jobs": jobs, "nodes": sum(s["n"] for s in specs), "sizes": "log-uniform [1e3, 5e5]",
+            "dims": "70% 3D", "graph": "Barabasi-Albert m in {1,2,3}",
+            "forces": "each of link/manyBody/collide/center/radial with p=0.6",
+            "hbm_budget_gb": budget_gb, "ticks_per_round": 10, "runs": runs}
+
+
 def cpu_baseline(n, ticks=1):
     """The FP64 restatement of d3-force-3d's tick (d3 semantics: sequential link and
     collide, d3-octree Barnes-Hut), one thread, on `ticks` ticks of the same workload."""
@@ -364,6 +453,8 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=256, help="layouts in the batched leg (0: skip)")
     ap.add_argument("--no-plain-jobs", action="store_true", help="batched leg without the job-by-job run")
     ap.add_argument("--only-batch", action="store_true", help="run only the batched leg")
+    ap.add_argument("--serve-jobs", type=int, default=48, help="jobs per serving run (0: skip)")
+    ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -375,6 +466,9 @@ def main():
                                    "--gpus above 1 is not supported (see the sharding notes)"}))
         sys.exit(2)
     n = args.nodes
+    if args.only_serve:
+        print(json.dumps({"serving": bench_serve(args.serve_jobs)}))
+        return
     if args.only_batch:
         print(json.dumps({"batched": bench_batch(args.batch_jobs, not args.no_plain_jobs)}))
         return
@@ -420,6 +514,8 @@ def main():
     }
     if args.batch_jobs > 0:
         out["batched"] = bench_batch(args.batch_jobs, not args.no_plain_jobs)
+    if args.serve_jobs > 0:
+        out["serving"] = bench_serve(args.serve_jobs)
     if ref is not None:
         out["ref_mix_ms_per_tick"] = round(ref["ms"], 4)
         out["ref_mix_note"] = ("link + manyBody + center only, the mix of the published 2.6 ms/tick "

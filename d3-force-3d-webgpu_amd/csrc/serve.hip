@@ -128,29 +128,42 @@ void worker_loop(fg_server* s) {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->running.empty(); });
       if (s->stop) return;
-      // admission in arrival order; the head waits for memory rather than being overtaken
-      while (!s->queue.empty()) {
-        Request* r = s->queue.front();
-        uint64_t bid = 0;
-        int rc = fg_batch_submit(s->batch, &r->desc, &bid);
-        if (rc == FG_ERR_NOMEM && !s->running.empty()) break;
-        s->queue.pop_front();
-        if (rc != FG_OK) {
-          r->status = FG_SERVE_FAILED;
-          r->error = fg_batch_last_error(s->batch);
-          r->t_finish = s->now();
-          s->failed++;
-          drop_inputs(*r);
-          s->cv_done.notify_all();
-          continue;
-        }
-        r->status = FG_SERVE_RUNNING;
-        r->batch_id = bid;
-        r->t_admit = s->now();
-        s->running[bid] = r;
-        s->admitted++;
-        admit_now.push_back(r);
+    }
+    // admission in arrival order; the head waits for memory rather than being overtaken.
+    // Only this thread takes from the queue, so its head stays put while the job uploads
+    // outside the lock (submitters are never held up by an upload).
+    while (true) {
+      Request* r = nullptr;
+      bool busy = false;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop || s->queue.empty()) break;
+        r = s->queue.front();
+        busy = !s->running.empty();
       }
+      uint64_t bid = 0;
+      const int rc = fg_batch_submit(s->batch, &r->desc, &bid);
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (rc == FG_ERR_NOMEM && busy) break;
+      s->queue.pop_front();
+      if (rc != FG_OK) {
+        r->status = FG_SERVE_FAILED;
+        r->error = fg_batch_last_error(s->batch);
+        r->t_finish = s->now();
+        s->failed++;
+        drop_inputs(*r);
+        s->cv_done.notify_all();
+        continue;
+      }
+      r->status = FG_SERVE_RUNNING;
+      r->batch_id = bid;
+      r->t_admit = s->now();
+      s->running[bid] = r;
+      s->admitted++;
+      admit_now.push_back(r);
+    }
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
       s->peak_running = std::max(s->peak_running, (uint32_t)s->running.size());
       if (s->running.empty()) continue;
     }

@@ -78,6 +78,28 @@ class BatchStats(C.Structure):
 
 JOB_RUNNING, JOB_DONE = 1, 3
 
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("ticks_per_round", C.c_uint32), ("autotune", C.c_int32),
+                ("tune_cache", C.c_char_p)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_uint32), ("n", C.c_uint32), ("ndim", C.c_int32),
+                ("ticks", C.c_uint32), ("submitted_s", C.c_double), ("admitted_s", C.c_double),
+                ("finished_s", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("finished", C.c_uint64),
+                ("failed", C.c_uint64), ("queued", C.c_uint32), ("running", C.c_uint32),
+                ("peak_running", C.c_uint32), ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double),
+                ("mean_latency_s", C.c_double), ("max_latency_s", C.c_double), ("layouts_per_s", C.c_double),
+                ("rounds", C.c_uint64), ("ticks", C.c_uint64), ("peak_bytes", C.c_uint64), ("budget", C.c_uint64)]
+
+
+SERVE_QUEUED, SERVE_RUNNING, SERVE_FINISHED, SERVE_FAILED = 0, 1, 2, 3
+
 PHASES = ["tick", "tree_build", "tree_sort", "walk", "pairs", "link", "collide", "center", "radial",
           "position", "integrate"]
 
@@ -124,6 +146,13 @@ SIGNATURES = {
     "fg_batch_autotune": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(BatchTune)]),
     "fg_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "fg_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    "fg_serve_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(ServeConfig)]),
+    "fg_serve_destroy": (C.c_int, [C.c_void_p]),
+    "fg_serve_last_error": (C.c_char_p, [C.c_void_p]),
+    "fg_serve_submit": (C.c_int, [C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
+    "fg_serve_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, _d, _d, C.POINTER(ServeResult)]),
+    "fg_serve_drain": (C.c_int, [C.c_void_p, C.c_double]),
+    "fg_serve_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServeStats)]),
 }
 
 _lib = None
@@ -531,3 +560,56 @@ class Batch:
         s = BatchStats()
         self._check(lib().fg_batch_get_stats(self.h, C.byref(s)))
         return {name: getattr(s, name) for name, _ in BatchStats._fields_}
+
+
+class Server:
+    """A request loop over a batch (fg_serve_*): submit layouts at any time, wait for them."""
+
+    def __init__(self, device=0, budget=0, ticks_per_round=10, autotune=False, tune_cache=None):
+        self._cache = tune_cache.encode() if tune_cache else None
+        cfg = ServeConfig(budget, ticks_per_round, 1 if autotune else 0, self._cache)
+        h = C.c_void_p()
+        rc = lib().fg_serve_create(C.byref(h), device, C.byref(cfg))
+        if rc != FG_OK:
+            raise FgError(rc, "fg_serve_create failed")
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if self.h:
+            lib().fg_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != FG_OK:
+            raise FgError(rc, lib().fg_serve_last_error(self.h).decode())
+
+    def submit(self, job):
+        j, keep = job.desc()
+        out = C.c_uint64(0)
+        self._check(lib().fg_serve_submit(self.h, C.byref(j), C.byref(out)))
+        self.sizes[out.value] = job.n
+        return out.value
+
+    def wait(self, job_id, timeout=600.0):
+        """(positions, velocities, result) of a finished job."""
+        n = self.sizes[job_id]
+        pos, vel = np.zeros((n, 3)), np.zeros((n, 3))
+        r = ServeResult()
+        self._check(lib().fg_serve_wait(self.h, job_id, timeout, ptr(pos), ptr(vel), C.byref(r)))
+        self.sizes.pop(job_id, None)
+        return pos, vel, {name: getattr(r, name) for name, _ in ServeResult._fields_}
+
+    def drain(self, timeout=600.0):
+        self._check(lib().fg_serve_drain(self.h, timeout))
+
+    def stats(self):
+        st = ServeStats()
+        self._check(lib().fg_serve_get_stats(self.h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServeStats._fields_}
