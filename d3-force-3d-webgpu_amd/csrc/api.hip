@@ -718,8 +718,9 @@ int fg_get_stats(fg_ctx* c, fg_stats* out) {
     s.sort_bits = (uint32_t)c->radix_bits;
     for (auto& p : c->passes) {
       if (p->tree && p->tree->counters) {
-        uint32_t h[8];
+        uint32_t h[16];
         FG_HIP(hipMemcpy(h, p->tree->counters, sizeof(h), hipMemcpyDeviceToHost));
+        s.walk_max_stack = h[8];
         s.interactions = (uint64_t)h[0] | ((uint64_t)h[1] << 32);
         s.opened = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
         s.lane_visits = (uint64_t)h[6] | ((uint64_t)h[7] << 32);

@@ -120,7 +120,7 @@ constexpr uint32_t BTILE = 2048;  // points per reduction tile
 struct BatchTune {
   int radix_bits = 8;          // 8 or 11 bits per radix digit
   int walk_waves = 4;          // waves per walk workgroup: 2, 4 or 8
-  int walk_stack = 192;        // stack entries per wave: 128, 160, 192 or 256
+  int walk_stack = 48;         // LDS stack entries per wave: 32, 48, 64 or 96
   int walk_blocks_per_cu = 0;  // persistent walk grid: blocks per CU (0: as many as fit)
   uint32_t walk_packed_max = 24;
 };
@@ -143,6 +143,8 @@ struct BatchTree {
   void* rec = nullptr;   // NodeRec[cap]
   void* sums = nullptr;  // NodeSum[cap]
   uint32_t* counters = nullptr;  // interactions [0,1], opened [2,3], errors [4], queue [5], lane visits [6,7]
+  void* spill = nullptr;         // the walk's stack entries beyond LDS, per resident wave
+  size_t spill_bytes = 0;
   SortWork sort, sort_seg;
   void reserve(uint32_t n);
   void release();

@@ -560,15 +560,16 @@ JobHost* find_job(fg_batch* b, uint64_t id) {
   return nullptr;
 }
 
-uint32_t walk_errors(fg_batch* b, uint64_t* interactions) {
-  uint32_t e = 0;
+uint32_t walk_errors(fg_batch* b, uint64_t* interactions, uint32_t* max_stack = nullptr) {
+  uint32_t e = 0, deepest = 0;
   uint64_t inter = 0;
   for (Group& g : b->groups) {
     if (!g.tree.total || !g.btree.counters) continue;
-    uint32_t h[8];
+    uint32_t h[16];
     FG_HIP(hipMemcpy(h, g.btree.counters, sizeof(h), hipMemcpyDeviceToHost));
     inter += (uint64_t)h[0] | ((uint64_t)h[1] << 32);
     e += h[4];
+    deepest = std::max(deepest, h[8]);
     uint32_t se[4] = {0, 0, 0, 0};
     if (g.btree.sort.errors) {
       FG_HIP(hipMemcpy(se, g.btree.sort.errors, sizeof(se), hipMemcpyDeviceToHost));
@@ -576,6 +577,7 @@ uint32_t walk_errors(fg_batch* b, uint64_t* interactions) {
     }
   }
   if (interactions) *interactions = inter;
+  if (max_stack) *max_stack = deepest;
   return e;
 }
 
@@ -876,8 +878,9 @@ int fg_batch_set_tune(fg_batch* b, const fg_batch_tune* t) {
     if (t->radix_bits != 8 && t->radix_bits != 11) return bfail(b, FG_ERR_INVALID, "radix_bits must be 8 or 11");
     if (t->walk_waves != 2 && t->walk_waves != 4 && t->walk_waves != 8)
       return bfail(b, FG_ERR_INVALID, "walk_waves must be 2, 4 or 8");
-    if (t->walk_stack != 128 && t->walk_stack != 160 && t->walk_stack != 192 && t->walk_stack != 256)
-      return bfail(b, FG_ERR_INVALID, "walk_stack must be 128, 160, 192 or 256");
+    if (t->walk_stack != 16 && t->walk_stack != 32 && t->walk_stack != 48 && t->walk_stack != 64 &&
+        t->walk_stack != 96)
+      return bfail(b, FG_ERR_INVALID, "walk_stack must be 16, 32, 48, 64 or 96");
     FG_HIP(hipStreamSynchronize(b->stream));
     b->tune.radix_bits = t->radix_bits;
     b->tune.walk_waves = t->walk_waves;
@@ -924,7 +927,7 @@ int fg_batch_autotune(fg_batch* b, const char* cache_path, fg_batch_tune* chosen
           t.walk_waves = w;
           consider(t);
         }
-      for (int k : {128, 160, 192, 256})
+      for (int k : {32, 48, 64})
         if (k != best.walk_stack) {
           BatchTune t = best;
           t.walk_stack = k;
@@ -971,7 +974,7 @@ int fg_batch_get_stats(fg_batch* b, fg_batch_stats* out) {
     s.budget = b->budget;
     s.ticks = b->ticks;
     s.graph_builds = b->graph_builds;
-    s.walk_errors = walk_errors(b, &s.interactions);
+    s.walk_errors = walk_errors(b, &s.interactions, &s.walk_max_stack);
     s.launches_per_tick = (uint32_t)b->graph_nodes;
     *out = s;
     return FG_OK;

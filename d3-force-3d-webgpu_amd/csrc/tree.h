@@ -59,10 +59,13 @@ struct TreeBuffers {
   NodeRec* rec = nullptr;
   NodeSum* sums = nullptr;
   CubeInfo* cube = nullptr;
-  uint32_t* counters = nullptr; // u64 interactions [0,1], opened [2,3], walk errors [4], lane visits [6,7]
+  uint32_t* counters = nullptr; // u64 interactions [0,1], opened [2,3], walk errors [4], tile queue [5], lane visits [6,7]
+  void* spill = nullptr;        // the walk's stack entries beyond LDS, per resident wave
+  size_t spill_bytes = 0;
   SortWork sort;
 
   void reserve(uint32_t n);
+  void reserve_spill(size_t bytes);
   void release();
   ~TreeBuffers() { release(); }
 };
@@ -71,6 +74,9 @@ struct TreeParams {
   double theta2, dmin2, dmax2;
   uint32_t walk_packed_max = 24;  // openings by at most this many points run packed
 };
+
+// Bytes of walk spill area for `cus` CUs at an LDS stack of `stack` entries.
+size_t spill_bytes_for(int cus, int stack);
 
 // Build the tree over the current positions and add the many-body force to the velocities.
 void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x,

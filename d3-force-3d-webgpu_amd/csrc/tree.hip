@@ -454,18 +454,21 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
 // ---- walk ---------------------------------------------------------------------------
 
 constexpr int WALK_WAVES = 4;
-constexpr int WALK_STACK = 192;  // per wave; 7 * 21 + 8 = 155 in 3D for distinct codes
+constexpr int WALK_STACK = 48;   // stack entries per wave in LDS (a walk at 1M points peaks near 30)
+constexpr int WALK_DEPTH = 192;  // stack entries per wave in all; 7 * 21 + 8 = 155 in 3D for distinct codes
 
-// Per wave, in LDS: the stack of nodes to open, each as its depth, the lanes that must
-// open it and its d3 children (so that opening it needs no load of its own record); a
-// staging area where one coalesced load puts the 64-byte records of all children of the
+// Per wave, in LDS: the top STACK entries of the stack of nodes to open, each as its
+// depth, the lanes that must open it and its d3 children (so that opening it needs no load
+// of its own record) -- entry STACK is where an entry popped from the spill area lands;
+// a staging area where one coalesced load puts the 64-byte records of all children of the
 // node being opened (points: their float4); the wave's points and their force sums from
-// the packed mode.
+// the packed mode.  Entries from STACK to WALK_DEPTH live in the wave's spill area in
+// global memory, which deep walks (many coincident points) reach.
 template <int STACK>
 struct WalkLDS {
-  uint32_t depth[STACK];
-  uint64_t mask[STACK];
-  uint32_t kids[STACK][8];
+  uint32_t depth[STACK + 1];
+  uint64_t mask[STACK + 1];
+  uint32_t kids[STACK + 1][8];
   uint32_t stage[8][16];
   float4 body[WAVE];      // x, y, z, node index (as bits)
   float acc[3][WAVE];
@@ -549,6 +552,13 @@ __device__ inline void point_force(float mx, float my, float mz, uint32_t self_n
   ++inter;
 }
 
+// A stack entry beyond the LDS part.
+struct alignas(16) SpillEntry {
+  uint64_t mask;
+  uint32_t depth, pad;
+  uint32_t kids[8];
+};
+
 // One tree as a walk sees it: a single simulation (off 0) or one job of a batch, whose
 // points and nodes are [off, off + n) of the shared arrays.
 struct WalkTree {
@@ -561,7 +571,7 @@ struct WalkTree {
 
 struct WalkCount {
   uint64_t opened = 0, lane_visits = 0;
-  uint32_t inter = 0, err = 0;
+  uint32_t inter = 0, err = 0, max_sp = 0;
 };
 
 // One wave walks the union of the traversals of sorted points [base, base + count) of one
@@ -571,7 +581,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
                                           const uint32_t* __restrict__ order,
                                           const NodeRec* __restrict__ rec, const WalkTree& T,
                                           uint32_t base, uint32_t count, uint32_t packed_max,
-                                          uint32_t lane, WalkCount& C) {
+                                          uint32_t lane, SpillEntry* spill, WalkCount& C) {
   constexpr int NK = 1 << D;
   const uint32_t off = T.off, n = T.n;
   const uint32_t s = base + lane;
@@ -590,7 +600,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
   float ax = 0.f, ay = 0.f, az = 0.f;
   uint32_t inter = 0;
 
-  uint32_t sp = 0;
+  uint32_t sp = 0, max_sp = 0;
   uint64_t opened = 0, lane_visits = 0;
   uint32_t err = 0;
   {
@@ -618,7 +628,20 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
       break;
     }
     --sp;
-    uint64_t m = L.mask[sp];
+    uint32_t e = sp;  // the LDS entry to open
+    if (sp >= (uint32_t)STACK) {
+      // from the spill area: this wave's own earlier stores, completed before the loads
+      const SpillEntry* E = spill + (sp - STACK);
+      wait_all_memory();
+      if (lane == 0) {
+        L.mask[STACK] = load_coherent(&E->mask);
+        L.depth[STACK] = load_coherent(&E->depth);
+      }
+      if (lane < 8) L.kids[STACK][lane] = load_coherent(&E->kids[lane]);
+      __builtin_amdgcn_wave_barrier();
+      e = STACK;
+    }
+    uint64_t m = L.mask[e];
     m = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
         (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)m);
     const bool in = (m >> lane) & 1ull;
@@ -627,9 +650,9 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
     lane_visits += k_active;
     // the popped node's children, and one coalesced load of their records into staging
     uint32_t my_kid = REF_NONE;
-    if (lane < (uint32_t)NK) my_kid = L.kids[sp][lane];
+    if (lane < (uint32_t)NK) my_kid = L.kids[e][lane];
     {
-      const uint32_t k = my_c < (uint32_t)NK ? L.kids[sp][my_c] : REF_NONE;
+      const uint32_t k = my_c < (uint32_t)NK ? L.kids[e][my_c] : REF_NONE;
       uint2 v = make_uint2(0u, 0u);
       if (k != REF_NONE) {
         if (k & REF_POINT) {
@@ -646,7 +669,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
     if (lane < (uint32_t)NK) L.cur[lane] = my_kid;
     const uint32_t nk = (uint32_t)__popcll(__ballot(lane < (uint32_t)NK && my_kid != REF_NONE));
     // the children's cell chains start one level below the popped node's cell
-    const uint32_t top = (uint32_t)__builtin_amdgcn_readfirstlane(L.depth[sp]) + 1;
+    const uint32_t top = (uint32_t)__builtin_amdgcn_readfirstlane(L.depth[e]) + 1;
     const double wt = ldexp(size, -(int)top);
     const double w2top = wt * wt;
 
@@ -722,17 +745,28 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
     for (uint32_t c = 0; c < nk; ++c) {
       const uint64_t open = __ballot(in && ((open_bits >> c) & 1u));
       if (!open) continue;
-      if (sp >= (uint32_t)STACK) {
+      if (sp >= (uint32_t)WALK_DEPTH) {
         err = 1;
         continue;
       }
-      if (lane == 0) {
-        L.depth[sp] = ((const NodeHdr*)&L.stage[c][0])->depth;
-        L.mask[sp] = open;
+      const uint32_t cdepth = ((const NodeHdr*)&L.stage[c][0])->depth;
+      if (sp < (uint32_t)STACK) {
+        if (lane == 0) {
+          L.depth[sp] = cdepth;
+          L.mask[sp] = open;
+        }
+        if (lane < 8) L.kids[sp][lane] = L.stage[c][8 + lane];
+      } else {
+        SpillEntry* E = spill + (sp - STACK);
+        if (lane == 0) {
+          store_coherent(&E->mask, open);
+          store_coherent(&E->depth, cdepth);
+        }
+        if (lane < 8) store_coherent(&E->kids[lane], L.stage[c][8 + lane]);
       }
-      if (lane < 8) L.kids[sp][lane] = L.stage[c][8 + lane];
       ++sp;
     }
+    max_sp = sp > max_sp ? sp : max_sp;
   }
   if (live) {
     const uint32_t o = self_node;
@@ -744,6 +778,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
   C.lane_visits += lane_visits;
   C.inter += inter;
   C.err |= err;
+  C.max_sp = max_sp > C.max_sp ? max_sp : C.max_sp;
 }
 
 // The wave's counters: one atomic each.
@@ -756,6 +791,7 @@ __device__ inline void walk_flush(const WalkCount& C, uint32_t lane, uint32_t* c
     atomicAdd((unsigned long long*)&counters[2], (unsigned long long)C.opened);
     atomicAdd((unsigned long long*)&counters[6], (unsigned long long)C.lane_visits);
     if (C.err) atomicAdd(&counters[4], 1u);
+    atomicMax(&counters[8], C.max_sp);
   }
 }
 
@@ -765,17 +801,19 @@ struct WalkArgs {
   uint32_t packed_max;  // openings by at most this many points use the packed mode
 };
 
-// One simulation: wave w of the grid walks sorted points [64 w, 64 w + 64).
-template <int D>
-__global__ void __launch_bounds__(WALK_WAVES * WAVE)
+// One simulation: every wave takes 64-point tiles [64 t, 64 t + 64) of the sorted points
+// from a queue (counters[5]) until none is left; `spill` holds WALK_DEPTH - STACK entries
+// per resident wave.
+template <int D, int STACK, int WAVES>
+__global__ void __launch_bounds__(WAVES * WAVE)
     tree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
               const NodeRec* __restrict__ rec, const CubeInfo* __restrict__ cube,
               float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
-              uint32_t* __restrict__ counters, StepParams step, WalkArgs A) {
-  __shared__ WalkLDS<WALK_STACK> lds[WALK_WAVES];
+              uint32_t* __restrict__ counters, SpillEntry* __restrict__ spill, StepParams step,
+              WalkArgs A) {
+  __shared__ WalkLDS<STACK> lds[WAVES];
   const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-  const uint32_t base = (blockIdx.x * WALK_WAVES + wave) * WAVE;
-  if (base >= A.n) return;
+  SpillEntry* my_spill = spill + (size_t)(blockIdx.x * WAVES + wave) * (WALK_DEPTH - STACK);
   WalkTree T;
   T.off = 0;
   T.n = A.n;
@@ -790,14 +828,45 @@ __global__ void __launch_bounds__(WALK_WAVES * WAVE)
   T.vx = vx;
   T.vy = vy;
   T.vz = vz;
-  const uint32_t count = A.n - base < (uint32_t)WAVE ? A.n - base : (uint32_t)WAVE;
+  const uint32_t tiles = (A.n + WAVE - 1) / WAVE;
   WalkCount C;
-  walk_tile<D, WALK_STACK>(lds[wave], leaves, order, rec, T, base, count, A.packed_max, lane, C);
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&counters[5], 1u);
+    t = (uint32_t)__builtin_amdgcn_readfirstlane(t);
+    if (t >= tiles) break;
+    const uint32_t base = t * WAVE;
+    const uint32_t count = A.n - base < (uint32_t)WAVE ? A.n - base : (uint32_t)WAVE;
+    __builtin_amdgcn_wave_barrier();
+    walk_tile<D, STACK>(lds[wave], leaves, order, rec, T, base, count, A.packed_max, lane, my_spill, C);
+  }
   walk_flush(C, lane, counters);
 }
 
+// Resident workgroups of a persistent walk kernel on this device (all CUs).
+template <typename K>
+uint32_t resident_blocks(K kern, int threads, int cus) {
+  int per_cu = 0;
+  FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0));
+  if (per_cu < 1) per_cu = 1;
+  return (uint32_t)per_cu * (uint32_t)(cus > 0 ? cus : 1);
+}
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  FG_HIP(hipGetDevice(&dev));
+  FG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus;
+}
+
+// Bytes of spill area a persistent walk needs: WALK_DEPTH - STACK entries for each of the
+// at most 32 waves per CU.
+inline size_t spill_bytes(int cus, int stack) {
+  return (size_t)(cus > 0 ? cus : 1) * 32 * (size_t)(WALK_DEPTH - stack) * sizeof(SpillEntry);
+}
+
 __global__ void tree_counters_clear(uint32_t* c) {
-  if (threadIdx.x < 8) c[threadIdx.x] = 0;
+  if (threadIdx.x < 16) c[threadIdx.x] = 0;
 }
 
 // ---- batched trees ------------------------------------------------------------------
@@ -917,9 +986,10 @@ __global__ void __launch_bounds__(WAVES * WAVE)
     btree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
                const NodeRec* __restrict__ rec, const uint2* __restrict__ tiles, uint32_t ntiles,
                const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
-               uint32_t* __restrict__ counters, uint32_t packed_max) {
+               uint32_t* __restrict__ counters, SpillEntry* __restrict__ spill, uint32_t packed_max) {
   __shared__ WalkLDS<STACK> lds[WAVES];
   const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  SpillEntry* my_spill = spill + (size_t)(blockIdx.x * WAVES + wave) * (WALK_DEPTH - STACK);
   WalkCount C;
   for (;;) {
     uint32_t t = 0;
@@ -946,7 +1016,7 @@ __global__ void __launch_bounds__(WAVES * WAVE)
     const uint32_t end = T.off + T.n;
     const uint32_t count = end - tile.y < (uint32_t)WAVE ? end - tile.y : (uint32_t)WAVE;
     __builtin_amdgcn_wave_barrier();
-    walk_tile<D, STACK>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, C);
+    walk_tile<D, STACK>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
   }
   walk_flush(C, lane, counters);
 }
@@ -958,52 +1028,39 @@ void launch_bwalk(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, co
   int per_cu = 0;
   FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WAVES * WAVE, 0));
   if (per_cu < 1) per_cu = 1;
+  if (per_cu > 32 / WAVES) per_cu = 32 / WAVES;  // the spill area covers 32 waves per CU
   if (tune.walk_blocks_per_cu > 0 && tune.walk_blocks_per_cu < per_cu) per_cu = tune.walk_blocks_per_cu;
   uint32_t blocks = (uint32_t)(per_cu * (cus > 0 ? cus : 1));
   const uint32_t need = ceil_div(sp.walk_tiles, WAVES);
   if (blocks > need) blocks = need;
   if (!blocks) return;
+  if (t.spill_bytes < spill_bytes(cus, STACK)) throw ArgError(-5, "walk: spill area too small");
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(WAVES * WAVE), 0, st, (const float4*)t.leaves,
                      (const uint32_t*)t.order, (const NodeRec*)t.rec, (const uint2*)sp.walk,
                      sp.walk_tiles, (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.counters,
-                     tune.walk_packed_max);
+                     (SpillEntry*)t.spill, tune.walk_packed_max);
   FG_LAUNCH_CHECK();
 }
 
-// The walk kernel for the tuned workgroup size and stack depth; the depth is taken as the
-// nearest instantiated one at least as deep (2D: 128 or 192; 3D: 160, 192 or 256).
+// The walk kernel for the tuned workgroup size and LDS stack depth (16, 32, 48, 64 or 96
+// entries; deeper entries spill to global memory either way).
 template <int D>
 void launch_bwalk_tuned(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs,
                         const BatchTune& tune, int cus) {
   const int w = tune.walk_waves == 2 || tune.walk_waves == 8 ? tune.walk_waves : 4;
   const int k = tune.walk_stack;
-  if (D == 1) {
-    launch_bwalk<D, 192, 4>(st, t, sp, jobs, tune, cus);
-  } else if (D == 2) {
-    if (k <= 128) {
-      if (w == 2) launch_bwalk<D, 128, 2>(st, t, sp, jobs, tune, cus);
-      else if (w == 4) launch_bwalk<D, 128, 4>(st, t, sp, jobs, tune, cus);
-      else launch_bwalk<D, 128, 8>(st, t, sp, jobs, tune, cus);
-    } else {
-      if (w == 2) launch_bwalk<D, 192, 2>(st, t, sp, jobs, tune, cus);
-      else if (w == 4) launch_bwalk<D, 192, 4>(st, t, sp, jobs, tune, cus);
-      else launch_bwalk<D, 192, 8>(st, t, sp, jobs, tune, cus);
-    }
-  } else {
-    if (k <= 160) {
-      if (w == 2) launch_bwalk<D, 160, 2>(st, t, sp, jobs, tune, cus);
-      else if (w == 4) launch_bwalk<D, 160, 4>(st, t, sp, jobs, tune, cus);
-      else launch_bwalk<D, 160, 8>(st, t, sp, jobs, tune, cus);
-    } else if (k <= 192) {
-      if (w == 2) launch_bwalk<D, 192, 2>(st, t, sp, jobs, tune, cus);
-      else if (w == 4) launch_bwalk<D, 192, 4>(st, t, sp, jobs, tune, cus);
-      else launch_bwalk<D, 192, 8>(st, t, sp, jobs, tune, cus);
-    } else {
-      if (w == 2) launch_bwalk<D, 256, 2>(st, t, sp, jobs, tune, cus);
-      else if (w == 4) launch_bwalk<D, 256, 4>(st, t, sp, jobs, tune, cus);
-      else launch_bwalk<D, 256, 8>(st, t, sp, jobs, tune, cus);
-    }
-  }
+#define FG_BWALK_W(S)                                                     \
+  do {                                                                    \
+    if (w == 2) launch_bwalk<D, S, 2>(st, t, sp, jobs, tune, cus);        \
+    else if (w == 4) launch_bwalk<D, S, 4>(st, t, sp, jobs, tune, cus);   \
+    else launch_bwalk<D, S, 8>(st, t, sp, jobs, tune, cus);               \
+  } while (0)
+  if (k <= 16) FG_BWALK_W(16);
+  else if (k <= 32) FG_BWALK_W(32);
+  else if (k <= 48) FG_BWALK_W(48);
+  else if (k <= 64) FG_BWALK_W(64);
+  else FG_BWALK_W(96);
+#undef FG_BWALK_W
 }
 
 template <int D>
@@ -1098,9 +1155,16 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   A.dmax2 = p.dmax2;
   A.n = n;
   A.packed_max = p.walk_packed_max;
-  hipLaunchKernelGGL((tree_walk<D>), dim3(ceil_div(n, WALK_WAVES * WAVE)), dim3(WALK_WAVES * WAVE),
-                     0, st, (const float4*)t.leaves, (const uint32_t*)sv, (const NodeRec*)t.rec,
-                     (const CubeInfo*)t.cube, vx, vy, vz, t.counters, step, A);
+  auto kern = tree_walk<D, WALK_STACK, WALK_WAVES>;
+  const int cus = device_cus();
+  uint32_t wblocks = resident_blocks(kern, WALK_WAVES * WAVE, cus);
+  if (wblocks > (uint32_t)cus * (32 / WALK_WAVES)) wblocks = (uint32_t)cus * (32 / WALK_WAVES);
+  const uint32_t need = ceil_div(ceil_div(n, WAVE), WALK_WAVES);
+  if (wblocks > need) wblocks = need;
+  if (t.spill_bytes < spill_bytes(cus, WALK_STACK)) t.reserve_spill(spill_bytes(cus, WALK_STACK));
+  hipLaunchKernelGGL(kern, dim3(wblocks), dim3(WALK_WAVES * WAVE), 0, st, (const float4*)t.leaves,
+                     (const uint32_t*)sv, (const NodeRec*)t.rec, (const CubeInfo*)t.cube, vx, vy, vz,
+                     t.counters, (SpillEntry*)t.spill, step, A);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(3);
 }
@@ -1122,16 +1186,28 @@ void TreeBuffers::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
   FG_HIP(hipMalloc(&cube, sizeof(CubeInfo)));
-  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 8));
+  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 16));
   CubeInfo c0 = {};
   for (int a = 0; a < 3; ++a) c0.box[a] = 0xffffffffu, c0.box[3 + a] = 0u;
   FG_HIP(hipMemcpy(cube, &c0, sizeof(c0), hipMemcpyHostToDevice));
-  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * 8));
+  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * 16));
   sort.reserve(m, sizeof(uint64_t));
   cap = n;
 }
 
+void TreeBuffers::reserve_spill(size_t bytes) {
+  if (spill && spill_bytes >= bytes) return;
+  if (spill) (void)hipFree(spill);
+  spill = nullptr;
+  spill_bytes = 0;
+  FG_HIP(hipMalloc(&spill, bytes));
+  spill_bytes = bytes;
+}
+
 void TreeBuffers::release() {
+  if (spill) (void)hipFree(spill);
+  spill = nullptr;
+  spill_bytes = 0;
   void* ptrs[] = {keys, vals, leaves, leaf_depth, leaf_parent, links, span, flags, rec, sums, cube, counters};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1165,6 +1241,8 @@ void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const fl
 
 namespace fg {
 
+size_t spill_bytes_for(int cus, int stack) { return spill_bytes(cus, stack); }
+
 void BatchTree::reserve(uint32_t n) {
   if (n <= cap && keys) return;
   release();
@@ -1182,15 +1260,19 @@ void BatchTree::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&flags, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
-  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 8));
-  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * 8));
+  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 16));
+  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * 16));
   sort.reserve(m, sizeof(uint64_t));
   sort_seg.reserve(m, sizeof(uint32_t));
+  if (!spill) {
+    spill_bytes = spill_bytes_for(device_cus(), 16);
+    FG_HIP(hipMalloc(&spill, spill_bytes));
+  }
   cap = n;
 }
 
 void BatchTree::release() {
-  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, leaf_depth, leaf_parent, links, span, flags, rec, sums, counters};
+  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, leaf_depth, leaf_parent, links, span, flags, rec, sums, counters, spill};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = codes = skeys = nullptr;
@@ -1198,7 +1280,8 @@ void BatchTree::release() {
   leaves = nullptr;
   links = nullptr;
   span = nullptr;
-  rec = sums = nullptr;
+  rec = sums = spill = nullptr;
+  spill_bytes = 0;
   sort.release();
   sort_seg.release();
   cap = 0;

@@ -44,7 +44,8 @@ class BufferInfo(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("interactions", C.c_uint64), ("opened", C.c_uint64),
-                ("lane_visits", C.c_uint64), ("sort_bits", C.c_uint32), ("walk_errors", C.c_uint32)]
+                ("lane_visits", C.c_uint64), ("sort_bits", C.c_uint32), ("walk_errors", C.c_uint32),
+                ("walk_max_stack", C.c_uint32)]
 
 
 class Timing(C.Structure):
@@ -73,7 +74,7 @@ class BatchStats(C.Structure):
     _fields_ = [("jobs_running", C.c_uint32), ("jobs_done", C.c_uint32), ("nodes", C.c_uint64),
                 ("bytes", C.c_uint64), ("budget", C.c_uint64), ("ticks", C.c_uint64),
                 ("graph_builds", C.c_uint64), ("interactions", C.c_uint64), ("walk_errors", C.c_uint32),
-                ("launches_per_tick", C.c_uint32)]
+                ("launches_per_tick", C.c_uint32), ("walk_max_stack", C.c_uint32)]
 
 
 JOB_RUNNING, JOB_DONE = 1, 3
@@ -536,7 +537,7 @@ class Batch:
         self._check(lib().fg_batch_release(self.h, job_id))
         self.sizes.pop(job_id, None)
 
-    def set_tune(self, radix_bits=8, walk_waves=4, walk_stack=192, walk_blocks_per_cu=0):
+    def set_tune(self, radix_bits=8, walk_waves=4, walk_stack=48, walk_blocks_per_cu=0):
         t = BatchTune(radix_bits, walk_waves, walk_stack, walk_blocks_per_cu)
         self._check(lib().fg_batch_set_tune(self.h, C.byref(t)))
 

@@ -95,6 +95,7 @@ typedef struct fg_stats {
   uint64_t lane_visits;      /* sum over openings of the lanes (points) that opened the node */
   uint32_t sort_bits;        /* bits per radix digit used by the tree sort */
   uint32_t walk_errors;      /* walks that hit their stack or iteration bound (must be 0) */
+  uint32_t walk_max_stack;   /* deepest stack any wave of the last walk used */
 } fg_stats;
 
 /* ---- library ------------------------------------------------------------------- */
@@ -287,7 +288,8 @@ typedef struct fg_job_info {
 typedef struct fg_batch_tune {
   int32_t radix_bits;          /* 8 or 11 bits per radix digit */
   int32_t walk_waves;          /* waves per walk workgroup: 2, 4 or 8 */
-  int32_t walk_stack;          /* walk stack entries per wave: 128 (2D), 160, 192, 256 (3D) */
+  int32_t walk_stack;          /* walk stack entries per wave kept in LDS: 16, 32, 48, 64 or 96
+                                  (deeper entries spill to global memory, up to 192) */
   int32_t walk_blocks_per_cu;  /* persistent walk workgroups per CU, 0 = as many as fit */
 } fg_batch_tune;
 
@@ -301,6 +303,7 @@ typedef struct fg_batch_stats {
   uint64_t interactions;       /* body-source evaluations of the last tick's walks */
   uint32_t walk_errors;        /* walks that hit their stack or iteration bound (must be 0) */
   uint32_t launches_per_tick;  /* kernel launches in one tick */
+  uint32_t walk_max_stack;     /* deepest stack any wave of the last tick's walks used */
 } fg_batch_stats;
 
 /* hbm_budget: device bytes the batch may hold (0 = 80% of the free memory). */

@@ -104,8 +104,8 @@ def test_batch_graphs_and_tunables_change_nothing():
     ticks = 12
     results = []
     for graphs, tune in [(True, {}), (False, {}),
-                         (True, dict(radix_bits=11, walk_waves=8, walk_stack=256)),
-                         (True, dict(radix_bits=8, walk_waves=2, walk_stack=160))]:
+                         (True, dict(radix_bits=11, walk_waves=8, walk_stack=96)),
+                         (True, dict(radix_bits=8, walk_waves=2, walk_stack=32))]:
         b = d3fgpu.Batch(0)
         b.use_graphs(graphs)
         if tune:
@@ -219,3 +219,22 @@ def test_batch_autotune_keeps_results(tmp_path):
         b.submit(make_job(n, nd, sd))
     assert b.autotune(cache) == chosen
     b.close()
+
+
+def test_batch_walk_spill():
+    """LDS stack of 16 entries: walks deeper than that keep going in the global spill area
+    and give the same forces as a 48-entry LDS stack."""
+    n, nd = 60000, 3
+    out = []
+    for stack in (48, 16):
+        b = d3fgpu.Batch(0)
+        b.set_tune(walk_stack=stack)
+        jid = b.submit(make_job(n, nd, 71, collide=False))
+        b.step(3)
+        st = b.stats()
+        assert st["walk_errors"] == 0
+        if stack == 16:
+            assert st["walk_max_stack"] > 16
+        out.append(b.read(jid))
+        b.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
