@@ -45,6 +45,7 @@ struct Pass {
   fg_force_desc d{};
   DevBuf a, b;          // per-node arrays (strengths / radii / targets)
   DevBuf row, ent;      // link CSR
+  DevBuf snap;          // link: the snapshot pos + vel (float4 per node)
   uint32_t m = 0;
   float rmax = 0;
   std::unique_ptr<TreeBuffers> tree;
@@ -260,10 +261,12 @@ void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
     }
     case FG_FORCE_LINK: {
       PhaseScope ps(c->T(), FG_PHASE_LINK);
+      if (p.d.iterations > 0 && p.m && p.snap.bytes < sizeof(float4) * (size_t)n) p.snap.alloc(sizeof(float4) * (size_t)n);
       for (int k = 0; k < p.d.iterations && p.m; ++k) {
         const float* vin[3] = {c->V(0), c->V(1), c->V(2)};
         float* vout[3] = {c->W(0), c->W(1), c->W(2)};
-        launch_link(c->stream, n, nd, pos, vin, vout, p.row.as<uint32_t>(), p.ent.as<LinkEntry>(), sp);
+        launch_link(c->stream, n, nd, pos, vin, vout, p.row.as<uint32_t>(), p.ent.as<LinkEntry>(),
+                    p.snap.as<float4>(), sp);
         c->cur ^= 1;
       }
       break;

@@ -176,7 +176,8 @@ struct BatchWork {
 void batch_advance(hipStream_t s, BJob* jobs, uint32_t njobs, uint32_t* walk_counters);
 void batch_manybody(hipStream_t s, int ndim, BatchTree& t, const BSpace& sp, BJob* jobs,
                     const BatchTune& tune, int cus, PhaseTimer* timer);
-void batch_link(hipStream_t s, int ndim, const BSpace& sp, BJob* jobs, uint32_t iteration);
+// snap: float4 scratch for every index of the node space
+void batch_link(hipStream_t s, int ndim, const BSpace& sp, BJob* jobs, uint32_t iteration, float4* snap);
 void batch_collide(hipStream_t s, int ndim, BatchGrid& g, const BSpace& sp, BJob* jobs,
                    uint32_t iteration, int radix_bits);
 void batch_center(hipStream_t s, int ndim, BatchWork& w, const BSpace& sp, BJob* jobs);

@@ -38,7 +38,7 @@ namespace {
 constexpr uint32_t MAX_SLOTS = 4096;
 
 // Bytes of scratch per point of each space (what admission charges a job for its share).
-constexpr uint64_t NODE_SCRATCH = 8;    // segment of the node, tiles
+constexpr uint64_t NODE_SCRATCH = 24;   // segment of the node, tiles, link snapshot
 constexpr uint64_t TREE_SCRATCH = 240;  // codes, sort buffers, leaves, links, records, sums
 constexpr uint64_t GRID_SCRATCH = 64;   // slots, sort buffers, snapshot, table
 
@@ -60,6 +60,7 @@ struct Group {
   DevBuf node_seg_of, node_seg_job, node_tile;
   DevBuf tree_seg_of, tree_seg_job, tree_tile, tree_walk;
   DevBuf grid_seg_of, grid_seg_job;
+  DevBuf snap;  // link snapshot, one float4 per node-space index
   BatchTree btree;
   BatchGrid bgrid;
   BatchWork work;
@@ -482,6 +483,7 @@ void rebuild_layout(fg_batch* b) {
     g.tree.walk_tiles = (uint32_t)walk.size();
     g.tree.walk = g.tree_walk.as<uint2>();
     // scratch sized before any capture
+    if (g.max_link && g.snap.bytes < sizeof(float4) * (size_t)nt) g.snap.alloc(sizeof(float4) * (size_t)nt);
     if (g.tree.total) g.btree.reserve(g.tree.total);
     if (g.grid.total) g.bgrid.reserve(g.grid.total);
     if (g.node.tiles) g.work.reserve(g.node.tiles);
@@ -506,7 +508,8 @@ void encode_tick(fg_batch* b) {
   for (int gi = 2; gi >= 0; --gi) {
     Group& g = b->groups[gi];
     if (!g.node.total) continue;
-    for (uint32_t it = 0; it < g.max_link; ++it) batch_link(b->stream, g.ndim, g.node, b->table, it);
+    for (uint32_t it = 0; it < g.max_link; ++it)
+      batch_link(b->stream, g.ndim, g.node, b->table, it, g.snap.as<float4>());
     if (g.tree.total) batch_manybody(b->stream, g.ndim, g.btree, g.tree, b->table, b->tune, b->cus, nullptr);
     for (uint32_t it = 0; it < g.max_coll; ++it)
       batch_collide(b->stream, g.ndim, g.bgrid, g.grid, b->table, it, b->tune.radix_bits);

@@ -40,8 +40,9 @@ struct GridWork {
 // pos/vel/fix: three device arrays each (x, y, z); unused axes may be null.
 void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
                       const uint8_t* mask, const float* const* fix, StepParams step);
+// snap: n float4 of scratch (the snapshot pos + vel the links read)
 void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* const* vin,
-                 float* const* vout, const uint32_t* row, const LinkEntry* ent,
+                 float* const* vout, const uint32_t* row, const LinkEntry* ent, float4* snap,
                  StepParams step);
 void launch_center(hipStream_t s, uint32_t n, int ndim, float* const* pos, CenterWork* w, double cx,
                    double cy, double cz, double strength);

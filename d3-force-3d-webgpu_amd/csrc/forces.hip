@@ -1,9 +1,10 @@
 // forces.hip -- the streaming force passes and the integrator (SoA f32 state, wave64).
 //
 //   integrate         v *= decay; p += v; pinned axes p = fix, v = 0      (wgsl.js:67-84)
-//   link              node-centric gather over a CSR of incident links; the snapshot
-//                     pos + vel is formed on the fly and the result goes to the other
-//                     velocity buffer, so every node sees the same state (passes/link.js:17-47)
+//   link              a snapshot pass (pos + vel as one float4 per node), then a
+//                     node-centric gather over a CSR of incident links; the result goes to
+//                     the other velocity buffer, so every node sees the same state
+//                     (passes/link.js:17-47)
 //   center            block sums in FP64, the last block forms the shift; then the shift
 //                     (passes/center.js:11-72)
 //   radial, position  one thread per node (passes/radial.js:9-27, passes/position.js:7-21)
@@ -66,34 +67,48 @@ void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, fl
 
 // ---- link ---------------------------------------------------------------------------
 
-// One node's incident links: the snapshot pos + vel of both ends, d3's spring along the
-// link, and the node's share of it (bias for the target, 1 - bias for the source).
+// The snapshot every link reads: pos + vel of each node as one float4 (wgsl.js:56-65), so
+// that a link's other end is one 16-byte gather rather than six.
 template <int D>
-__device__ inline void link_node(uint32_t i, const float* __restrict__ x, const float* __restrict__ y,
-                                 const float* __restrict__ z, const float* __restrict__ vx,
-                                 const float* __restrict__ vy, const float* __restrict__ vz,
-                                 float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz,
+__global__ void __launch_bounds__(256) snapshot_kernel(uint32_t n, const float* __restrict__ x,
+                                                       const float* __restrict__ y,
+                                                       const float* __restrict__ z,
+                                                       const float* __restrict__ vx,
+                                                       const float* __restrict__ vy,
+                                                       const float* __restrict__ vz,
+                                                       float4* __restrict__ snap) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  snap[i] = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f, 0.f);
+}
+
+// One node's incident links: d3's spring along each link between the snapshots of its
+// ends, and the node's share of it (bias for the target, 1 - bias for the source).
+template <int D>
+__device__ inline void link_node(uint32_t i, const float4* __restrict__ snap,
+                                 const float* __restrict__ vx, const float* __restrict__ vy,
+                                 const float* __restrict__ vz, float* __restrict__ ox,
+                                 float* __restrict__ oy, float* __restrict__ oz,
                                  const uint32_t* __restrict__ row, const LinkEntry* __restrict__ ent,
                                  float alpha, uint32_t sn, uint32_t seed) {
-  float sx = x[i] + vx[i], sy = D > 1 ? y[i] + vy[i] : 0.f, sz = D > 2 ? z[i] + vz[i] : 0.f;
+  const float4 me = snap[i];
   float ax = 0.f, ay = 0.f, az = 0.f;
   const uint32_t e0 = row[i], e1 = row[i + 1];
   for (uint32_t e = e0; e < e1; ++e) {
-    LinkEntry L = ent[e];
-    uint32_t j = L.other;
-    float tx = x[j] + vx[j], ty = D > 1 ? y[j] + vy[j] : 0.f, tz = D > 2 ? z[j] + vz[j] : 0.f;
-    bool target = L.code & 1u;
-    uint32_t id = L.code >> 1;
+    const LinkEntry L = ent[e];
+    const float4 t = snap[L.other];
+    const bool target = L.code & 1u;
+    const uint32_t id = L.code >> 1;
     // d = snap[target] - snap[source], the same vector at both ends
-    float dx = target ? sx - tx : tx - sx;
-    float dy = target ? sy - ty : ty - sy;
-    float dz = target ? sz - tz : tz - sz;
+    float dx = target ? me.x - t.x : t.x - me.x;
+    float dy = target ? me.y - t.y : t.y - me.y;
+    float dz = target ? me.z - t.z : t.z - me.z;
     if (dx == 0.f) dx = jiggle_hash(id, 0u, 0u, sn, seed);
     if (D > 1 && dy == 0.f) dy = jiggle_hash(id, 0u, 1u, sn, seed);
     if (D > 2 && dz == 0.f) dz = jiggle_hash(id, 0u, 2u, sn, seed);
-    float l = sqrtf(dx * dx + dy * dy + dz * dz);
-    float f = (l - L.distance) / l * alpha * L.strength;
-    float w = target ? -L.bias : 1.0f - L.bias;
+    const float l = sqrtf(dx * dx + (D > 1 ? dy * dy : 0.f) + (D > 2 ? dz * dz : 0.f));
+    const float f = (l - L.distance) / l * alpha * L.strength;
+    const float w = target ? -L.bias : 1.0f - L.bias;
     ax += dx * f * w;
     if (D > 1) ay += dy * f * w;
     if (D > 2) az += dz * f * w;
@@ -104,9 +119,7 @@ __device__ inline void link_node(uint32_t i, const float* __restrict__ x, const 
 }
 
 template <int D>
-__global__ void __launch_bounds__(256) link_kernel(uint32_t n, const float* __restrict__ x,
-                                                   const float* __restrict__ y,
-                                                   const float* __restrict__ z,
+__global__ void __launch_bounds__(256) link_kernel(uint32_t n, const float4* __restrict__ snap,
                                                    const float* __restrict__ vx,
                                                    const float* __restrict__ vy,
                                                    const float* __restrict__ vz,
@@ -117,20 +130,24 @@ __global__ void __launch_bounds__(256) link_kernel(uint32_t n, const float* __re
                                                    StepParams step) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  link_node<D>(i, x, y, z, vx, vy, vz, ox, oy, oz, row, ent, step.alpha, step.step, step.seed);
+  link_node<D>(i, snap, vx, vy, vz, ox, oy, oz, row, ent, step.alpha, step.step, step.seed);
 }
 
 void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* const* vin,
-                 float* const* vout, const uint32_t* row, const LinkEntry* ent,
+                 float* const* vout, const uint32_t* row, const LinkEntry* ent, float4* snap,
                  StepParams step) {
   if (!n) return;
   dim3 g(ceil_div(n, 256)), b(256);
-  if (ndim == 1)
-    hipLaunchKernelGGL(link_kernel<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
-  else if (ndim == 2)
-    hipLaunchKernelGGL(link_kernel<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
-  else
-    hipLaunchKernelGGL(link_kernel<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
+  if (ndim == 1) {
+    hipLaunchKernelGGL(snapshot_kernel<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], snap);
+    hipLaunchKernelGGL(link_kernel<1>, g, b, 0, s, n, (const float4*)snap, vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
+  } else if (ndim == 2) {
+    hipLaunchKernelGGL(snapshot_kernel<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], snap);
+    hipLaunchKernelGGL(link_kernel<2>, g, b, 0, s, n, (const float4*)snap, vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
+  } else {
+    hipLaunchKernelGGL(snapshot_kernel<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], snap);
+    hipLaunchKernelGGL(link_kernel<3>, g, b, 0, s, n, (const float4*)snap, vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
+  }
   FG_LAUNCH_CHECK();
 }
 
@@ -638,17 +655,33 @@ __global__ void __launch_bounds__(256) badvance(BJob* jobs, uint32_t njobs) {
 }
 
 template <int D>
+__global__ void __launch_bounds__(256) bsnapshot(uint32_t total, const uint32_t* __restrict__ seg_of,
+                                                 const uint32_t* __restrict__ seg_job,
+                                                 const BJob* __restrict__ jobs, uint32_t iteration,
+                                                 float4* __restrict__ snap) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
+  const BJob& B = *r.B;
+  if (!B.active || !(B.forces & BF_LINK) || iteration >= B.link_iters) return;
+  const uint32_t c = iteration & 1u, v = r.v;
+  snap[i] = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
+                        D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, 0.f);
+}
+
+template <int D>
 __global__ void __launch_bounds__(256) blink(uint32_t total, const uint32_t* __restrict__ seg_of,
                                              const uint32_t* __restrict__ seg_job,
-                                             const BJob* __restrict__ jobs, uint32_t iteration) {
+                                             const BJob* __restrict__ jobs, uint32_t iteration,
+                                             const float4* __restrict__ snap) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
   const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
   const BJob& B = *r.B;
   if (!B.active || !(B.forces & BF_LINK) || iteration >= B.link_iters) return;
   const uint32_t c = iteration & 1u;
-  link_node<D>(r.v, B.p[0], B.p[1], B.p[2], B.v[c][0], B.v[c][1], B.v[c][2], B.v[c ^ 1][0],
-               B.v[c ^ 1][1], B.v[c ^ 1][2], B.link_row, B.link_ent, B.alpha_f, B.step, B.seed);
+  link_node<D>(r.v, snap + B.node_off, B.v[c][0], B.v[c][1], B.v[c][2], B.v[c ^ 1][0], B.v[c ^ 1][1],
+               B.v[c ^ 1][2], B.link_row, B.link_ent, B.alpha_f, B.step, B.seed);
 }
 
 template <int D>
@@ -884,10 +917,13 @@ void batch_advance(hipStream_t s, BJob* jobs, uint32_t njobs, uint32_t*) {
     FG_LAUNCH_CHECK();                                                                 \
   } while (0)
 
-void batch_link(hipStream_t s, int ndim, const BSpace& sp, BJob* jobs, uint32_t iteration) {
+void batch_link(hipStream_t s, int ndim, const BSpace& sp, BJob* jobs, uint32_t iteration, float4* snap) {
   if (!sp.total) return;
+  FG_BY_DIM(ndim, bsnapshot, dim3(ceil_div(sp.total, 256)), dim3(256), 0, s, sp.total,
+            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, iteration, snap);
   FG_BY_DIM(ndim, blink, dim3(ceil_div(sp.total, 256)), dim3(256), 0, s, sp.total,
-            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, iteration);
+            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, iteration,
+            (const float4*)snap);
 }
 
 void batch_collide(hipStream_t s, int ndim, BatchGrid& g, const BSpace& sp, BJob* jobs,

@@ -302,24 +302,27 @@ __global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ s
 
 // ---- bottom-up reduction + collapse ---------------------------------------------------
 
-// Loads and stores of a node's sums: plain ones between threads of one workgroup (the
-// node and both its children lie within the workgroup's points), coherent ones otherwise.
-template <typename T>
-__device__ inline T ld(const T* p, bool coherent) {
-  return coherent ? load_coherent(p) : *p;
-}
-template <typename T>
-__device__ inline void st(T* p, T v, bool coherent) {
-  if (coherent) store_coherent(p, v);
-  else *p = v;
-}
+// A node's sums and centre as the reduction hands them from child to parent.
+struct NodeAcc {
+  double a, w, gx, gy, gz;  // NodeSum
+  double cx, cy, cz;        // centre (0 when the node has no weight)
+};
 
+// Sums of one workgroup's local nodes (node index - first point of the workgroup) in LDS.
+struct ReduceLDS {
+  double a[512], w[512], gx[512], gy[512], gz[512], cx[512], cy[512], cz[512];
+  uint32_t flag[512];
+};
+
+// Adds child k of a node at depth parent_depth to `out`: a point (a d3 child one level
+// below the parent's cell), a deeper node (a d3 child, weighed by |its strength one level
+// below the parent's cell|) or a node in the same cell (its children are the parent's:
+// its sums pass through).  Local children are read from LDS, others coherently.
 template <int D>
-__device__ inline void child_sum(uint32_t k, uint32_t parent_depth, bool coherent,
-                                 const float4* leaves, const uint4* links, const NodeRec* rec,
-                                 const NodeSum* sums, NodeSum& out) {
+__device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo, bool local,
+                                 const ReduceLDS& S, const float4* leaves, const uint4* links,
+                                 const NodeRec* rec, const NodeSum* sums, NodeSum& out) {
   if (k & REF_POINT) {
-    // a point is a d3 child one level below its binary parent's cell
     float4 q = leaves[k & ~REF_POINT];
     double sv = (double)q.w;
     out.a += sv * damp<D>((int)parent_depth + 1);
@@ -330,22 +333,21 @@ __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, bool coheren
     if (D > 2) out.gz += w * (double)q.z;
     return;
   }
-  double ca = ld(&sums[k].a, coherent);
-  uint32_t kd = links[k].z;
+  const uint32_t kd = links[k].z;
+  const uint32_t q = k - lo;
+  const double ca = local ? S.a[q] : load_coherent(&sums[k].a);
   out.a += ca;
   if (kd > parent_depth) {
-    // a deeper cell: a d3 child, weighed by |its strength one level below this cell|
     double w = fabs(ca * damp<D>(-(int)(parent_depth + 1)));
     out.w += w;
-    out.gx += w * ld(&rec[k].cx, coherent);
-    if (D > 1) out.gy += w * ld(&rec[k].cy, coherent);
-    if (D > 2) out.gz += w * ld(&rec[k].cz, coherent);
+    out.gx += w * (local ? S.cx[q] : load_coherent(&rec[k].cx));
+    if (D > 1) out.gy += w * (local ? S.cy[q] : load_coherent(&rec[k].cy));
+    if (D > 2) out.gz += w * (local ? S.cz[q] : load_coherent(&rec[k].cz));
   } else {
-    // the same cell: its children are this cell's children
-    out.w += ld(&sums[k].w, coherent);
-    out.gx += ld(&sums[k].gx, coherent);
-    if (D > 1) out.gy += ld(&sums[k].gy, coherent);
-    if (D > 2) out.gz += ld(&sums[k].gz, coherent);
+    out.w += local ? S.w[q] : load_coherent(&sums[k].w);
+    out.gx += local ? S.gx[q] : load_coherent(&sums[k].gx);
+    if (D > 1) out.gy += local ? S.gy[q] : load_coherent(&sums[k].gy);
+    if (D > 2) out.gz += local ? S.gz[q] : load_coherent(&sums[k].gz);
   }
 }
 
@@ -382,17 +384,21 @@ __device__ inline void collapse_into(uint4 L, const uint4* links, uint32_t* kids
 constexpr uint32_t REDUCE_BLOCK = 512;
 
 // One thread per point climbs towards the root; the second of two siblings to reach a
-// node combines them.  Inside the subtrees that lie within this workgroup's points the
-// hand-off is between waves of one workgroup (plain memory, workgroup-scope atomics);
-// above them it is coherent across the device.
+// node combines them.  Nodes whose points all lie within this workgroup's 512 points are
+// combined through LDS (LDS flag, LDS sums); above them the hand-off is through global
+// memory, coherent across the device.  Every node's record goes to global memory for the
+// walk; a node whose parent is not local also leaves its sums there, coherently.
 template <int D>
 __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
                                                             const uint32_t* __restrict__ leaf_parent,
                                                             const uint4* __restrict__ links,
                                                             const uint2* __restrict__ span,
                                                             uint32_t* flags, NodeRec* rec, NodeSum* sums) {
+  __shared__ ReduceLDS S;
   const uint32_t s = blockIdx.x * REDUCE_BLOCK + threadIdx.x;
   const uint32_t lo = blockIdx.x * REDUCE_BLOCK, hi = lo + REDUCE_BLOCK - 1;
+  S.flag[threadIdx.x] = 0;
+  __syncthreads();
   if (s >= n || n < 2) return;
   uint32_t p = leaf_parent[s];
   bool local = true;  // whether node p lies within this workgroup's points
@@ -403,18 +409,17 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
   for (uint32_t guard = 0; guard < n && p != REF_NONE; ++guard) {
     uint32_t old;
     if (local) {
-      old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      old = __hip_atomic_fetch_add(&S.flag[p - lo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     } else {
       old = publish_count(&flags[p]);
     }
     if (old == 0) return;  // the sibling is still on its way
-    // children of a local node were written by this workgroup; children of a shared
-    // node were written coherently (their parent, p, is not local to their writer)
-    const bool coherent_in = !local;
-    uint4 L = links[p];
+    // a local node's children are local; a shared node's children left their sums in
+    // global memory (their parent, p, is not local to their writer)
+    const uint4 L = links[p];
     NodeSum sum = {0, 0, 0, 0, 0};
-    child_sum<D>(L.x, L.z, coherent_in, leaves, links, rec, sums, sum);
-    child_sum<D>(L.y, L.z, coherent_in, leaves, links, rec, sums, sum);
+    child_sum<D>(L.x, L.z, lo, local, S, leaves, links, rec, sums, sum);
+    child_sum<D>(L.y, L.z, lo, local, S, leaves, links, rec, sums, sum);
     double cx = 0.0, cy = 0.0, cz = 0.0;
     if (sum.w > 0) {
       cx = sum.gx / sum.w;
@@ -436,16 +441,29 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
       uint2 ps = span[parent];
       parent_local = ps.x >= lo && ps.y <= hi;
     }
-    // the parent's thread reads these: coherently unless it is in this workgroup
-    const bool coherent_out = !parent_local;
-    st(&r.cx, cx, coherent_out);
-    st(&r.cy, cy, coherent_out);
-    st(&r.cz, cz, coherent_out);
-    st(&sums[p].a, sum.a, coherent_out);
-    st(&sums[p].w, sum.w, coherent_out);
-    st(&sums[p].gx, sum.gx, coherent_out);
-    st(&sums[p].gy, sum.gy, coherent_out);
-    st(&sums[p].gz, sum.gz, coherent_out);
+    if (parent_local) {
+      const uint32_t q = p - lo;
+      S.a[q] = sum.a;
+      S.w[q] = sum.w;
+      S.gx[q] = sum.gx;
+      S.gy[q] = sum.gy;
+      S.gz[q] = sum.gz;
+      S.cx[q] = cx;
+      S.cy[q] = cy;
+      S.cz[q] = cz;
+      r.cx = cx;
+      r.cy = cy;
+      r.cz = cz;
+    } else {
+      store_coherent(&r.cx, cx);
+      store_coherent(&r.cy, cy);
+      store_coherent(&r.cz, cz);
+      store_coherent(&sums[p].a, sum.a);
+      store_coherent(&sums[p].w, sum.w);
+      store_coherent(&sums[p].gx, sum.gx);
+      store_coherent(&sums[p].gy, sum.gy);
+      store_coherent(&sums[p].gz, sum.gz);
+    }
     p = parent;
     local = parent_local;
   }
