@@ -48,6 +48,7 @@ struct Pass {
   DevBuf snap;          // link: the snapshot pos + vel (float4 per node)
   uint32_t m = 0;
   float rmax = 0;
+  GridClasses grid_classes{0.0f, 0.0f, 0.0f, 0};
   std::unique_ptr<TreeBuffers> tree;
   std::unique_ptr<GridWork> grid;
   DevBuf center;
@@ -211,7 +212,9 @@ void update_pass(fg_ctx* c, Pass& p, const fg_force_desc& d) {
       if (fresh) {
         upload_f32(c->stream, p.a, d.radii, n);
         p.rmax = 0;
-        for (uint32_t i = 0; d.radii && i < n; ++i) p.rmax = std::max(p.rmax, finite_f32(d.radii[i]));
+        std::vector<float> r(n > 0 ? n : 1, 0.0f);
+        for (uint32_t i = 0; d.radii && i < n; ++i) r[i] = finite_f32(d.radii[i]), p.rmax = std::max(p.rmax, r[i]);
+        p.grid_classes = grid_classes(r.data(), n);
       }
       break;
     case FG_FORCE_RADIAL:
@@ -278,7 +281,7 @@ void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
         if (n >= GRID_MIN) {
           if (!p.grid) p.grid.reset(new GridWork());
           float* vel[3] = {c->V(0), c->V(1), c->V(2)};
-          launch_collide_grid(c->stream, *p.grid, n, nd, pos, vel, p.a.as<float>(), p.rmax,
+          launch_collide_grid(c->stream, *p.grid, n, nd, pos, vel, p.a.as<float>(), p.grid_classes,
                               (float)p.d.strength, c->radix_bits, sp);
         } else {
           const float* vin[3] = {c->V(0), c->V(1), c->V(2)};

@@ -80,7 +80,7 @@ struct alignas(64) BJob {
   uint32_t vcur;     // velocity set current after the link iterations (link_iters & 1)
   // collide
   uint32_t coll_iters;
-  float coll_inv;    // 1 / (2 rmax), 0: no collide
+  GridClasses coll;  // the grid's cell widths and radius classes
   float coll_strength;
   // center
   double center[3];
@@ -155,7 +155,7 @@ struct BatchGrid {
   uint32_t cap = 0, bits = 0;
   uint32_t* keys = nullptr;
   uint32_t* vals = nullptr;
-  uint32_t* starts = nullptr;
+  uint2* runs = nullptr;     // [2^bits] each slot's [first, end) of the sorted points
   float4* sorted = nullptr;
   uint32_t* sseg = nullptr;
   SortWork sort;

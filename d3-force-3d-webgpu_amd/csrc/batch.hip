@@ -197,6 +197,7 @@ struct JobImage {
       off_radius = 0, off_rr = 0, off_rw = 0, off_goal[3] = {0, 0, 0}, off_weight[3] = {0, 0, 0};
   bool mask = false, mb = false, link = false, collide = false, radial = false, pos[3] = {false, false, false};
   float rmax = 0;
+  GridClasses gc{0.0f, 0.0f, 0.0f, 0};
   uint32_t link_iters = 0, coll_iters = 0;
   const fg_force_desc* fd[8] = {};
   uint64_t charge = 0;
@@ -307,6 +308,7 @@ void job_image(const fg_job_desc* j, JobImage& im, bool fill) {
         break;
       case FG_FORCE_COLLIDE:
         for (uint32_t i = 0; i < n; ++i) F(im.off_radius)[i] = d.radii ? finite_f32(d.radii[i]) : 0.0f;
+        im.gc = grid_classes(F(im.off_radius), n);
         break;
       case FG_FORCE_RADIAL:
         for (uint32_t i = 0; i < n; ++i) {
@@ -365,7 +367,7 @@ BJob job_record(const fg_job_desc* j, const JobImage& im, uint8_t* base) {
     r.forces |= BF_COLLIDE;
     r.radius = P(im.off_radius);
     r.coll_iters = im.coll_iters;
-    r.coll_inv = 1.0f / (2.0f * im.rmax);
+    r.coll = im.gc;
     r.coll_strength = (float)d.strength;
   }
   if (im.fd[3]) {

@@ -25,11 +25,22 @@ struct CenterWork {
   uint32_t done;
 };
 
+// The collide grid's two classes of points (one when the radii are even): points up to
+// r0 live in cells 2 r0 wide, larger ones in cells 2 rmax wide (inv_* = 1 / cell width).
+struct GridClasses {
+  float inv_s, inv_b, r0;
+  int32_t two;
+};
+
+// The classes for these radii: two when rmax exceeds four times the 95th percentile radius
+// (then one huge radius cannot make every cell hold every point).
+GridClasses grid_classes(const float* radii, uint32_t n);
+
 struct GridWork {
   uint32_t cap = 0, bits = 0;
   uint32_t* keys = nullptr;
   uint32_t* vals = nullptr;
-  uint32_t* starts = nullptr;
+  uint2* runs = nullptr;     // [2^bits] each slot's [first, end) of the sorted points
   float4* sorted = nullptr;
   SortWork sort;
   void reserve(uint32_t n);
@@ -58,7 +69,7 @@ void launch_collide_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos
                           const float* const* vin, float* const* vout, const float* radius,
                           float strength, StepParams step);
 void launch_collide_grid(hipStream_t s, GridWork& g, uint32_t n, int ndim, float* const* pos,
-                         float* const* vel, const float* radius, float rmax, float strength,
+                         float* const* vel, const float* radius, const GridClasses& gc, float strength,
                          int radix_bits, StepParams step);
 
 }  // namespace fg

@@ -11,11 +11,18 @@
 //   manybody_pairs    exact all-pairs sum with LDS tiles (passes/manyBody.js:23-74)
 //   collide_pairs     exact all-pairs collide for small graphs (passes/collide.js:24-81)
 //   grid_*            collide through a hashed uniform grid (passes/collideGrid.js:6-22):
-//                     cells 2 * rmax wide, a slot per cell from a hashed block offset plus
-//                     the cell's Morton index inside its block, radix sort of (slot, node),
-//                     run starts, then a 3^d neighbourhood query per sorted node.
+//                     cells 2 * rmax wide (and, for heavy-tailed radii, a second class of
+//                     cells 2 * r95 wide for the points up to the 95th percentile radius),
+//                     a slot per cell from a hashed block offset plus the cell's Morton index
+//                     inside its block, radix sort of (slot, node), a table of each slot's
+//                     run, then a neighbourhood query per sorted node.
 #include "forces.h"
 #include "batch.h"
+
+#include <stdlib.h>
+
+#include <algorithm>
+#include <vector>
 
 namespace fg {
 
@@ -444,53 +451,120 @@ __device__ inline int3 grid_cell(float4 q, float inv) {
                    D > 2 ? (int)floorf(q.z * inv) : 0);
 }
 
-// Collide sorted point s (snapshot `me`, node index `node`) with the points of its 3^D
-// neighbour cells (passes/collide.js:34-80 on candidates from passes/collideGrid.js).
-// Cells are visited in a fixed order and, within a cell, points in sorted order; a point
-// found under a cell's slot counts only if it lies in that cell, so slots shared by
-// several cells -- or, in a batch (SEG), by other jobs -- change the work but never the
-// sum.  svals[t] - node_off is the node index of sorted point t.
+// The grid's class of a radius: 0 (fine grid) up to r0, 1 (coarse grid) above.
+__device__ inline uint32_t grid_class(float r, const GridClasses& gc) { return r > gc.r0 ? 1u : 0u; }
+
+// Collide sorted point s (snapshot `me` with its radius, node index `node`) with every
+// point closer than the sum of their radii (passes/collide.js:34-80 on candidates from
+// passes/collideGrid.js).  Class-0 cells are 2 r0 wide, class-1 cells 2 rmax; a partner
+// of class c lies within me.w + (c ? rmax : r0), so the cells to visit in class c are
+// those within ceil((me.w + r_c) / cell_c) of the point's cell -- one ring except for a
+// coarse point among fine cells.  Classes, cells and, within a cell, sorted points are
+// visited in a fixed order; a point found under a cell's slot counts only if it is of that
+// class and lies in that cell, so slots shared by cells -- or, in a batch (SEG), by other
+// jobs -- change the work but never the sum.  runs[slot] is the slot's [first, end) of the
+// sorted points ((0, 0) when empty); svals[t] - node_off is sorted point t's node.
+// The pairs of sorted point s with the points of one cell (class c, coordinates cx, cy, cz)
+// whose run is `run`.
 template <int D, bool SEG>
-__device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node, uint32_t n,
-                                          const uint32_t* __restrict__ skeys,
+__device__ inline void grid_cell_pairs(uint2 run, int cx, int cy, int cz, uint32_t c, float inv,
+                                       uint32_t s, float4 me, float ri2, uint32_t node,
+                                       const uint32_t* __restrict__ svals,
+                                       const float4* __restrict__ sorted,
+                                       const uint32_t* __restrict__ sseg, uint32_t seg,
+                                       uint32_t node_off, const GridClasses& gc, float strength,
+                                       uint32_t sn, uint32_t seed, float& ax, float& ay, float& az) {
+  for (uint32_t t = run.x; t < run.y; ++t) {
+    if (t == s) continue;
+    if (SEG && sseg[t] != seg) continue;
+    const float4 q = sorted[t];
+    if (grid_class(q.w, gc) != c) continue;
+    const int3 qc = grid_cell<D>(q, inv);
+    if (qc.x != cx || qc.y != cy || qc.z != cz) continue;
+    const float rj = q.w, r = me.w + rj;
+    float ex = me.x - q.x, ey = me.y - q.y, ez = me.z - q.z;
+    float l = ex * ex + ey * ey + ez * ez;
+    if (l >= r * r) continue;
+    if (l == 0.f) {
+      coincident(node, svals[t] - node_off, D, sn, seed, ex, ey, ez);
+      l = ex * ex + ey * ey + ez * ez;
+    }
+    const float len = sqrtf(l), rj2 = rj * rj;
+    const float f = (r - len) / len * strength * rj2 / (ri2 + rj2);
+    ax += ex * f;
+    ay += ey * f;
+    az += ez * f;
+  }
+}
+
+// Collide sorted point s (snapshot `me` with its radius, node index `node`) with every
+// point closer than the sum of their radii (passes/collide.js:34-80 on candidates from
+// passes/collideGrid.js).  Class-0 cells are 2 r0 wide, class-1 cells 2 rmax; a partner
+// of class c lies within me.w + (c ? rmax : r0), so the cells to visit in class c are
+// those within ceil((me.w + r_c) / cell_c) of the point's cell -- one ring except for a
+// coarse point among fine cells.  Classes, cells and, within a cell, sorted points are
+// visited in a fixed order; a point found under a cell's slot counts only if it is of that
+// class and lies in that cell, so slots shared by cells -- or, in a batch (SEG), by other
+// jobs -- change the work but never the sum.  runs[slot] is the slot's [first, end) of the
+// sorted points ((0, 0) when empty); svals[t] - node_off is sorted point t's node.  In one
+// ring, the runs of a z-slab of cells are read together so that their latencies overlap.
+template <int D, bool SEG>
+__device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node,
                                           const uint32_t* __restrict__ svals,
-                                          const uint32_t* __restrict__ starts,
+                                          const uint2* __restrict__ runs,
                                           const float4* __restrict__ sorted,
                                           const uint32_t* __restrict__ sseg, uint32_t seg,
-                                          uint32_t node_off, float inv, uint32_t mask,
+                                          uint32_t node_off, const GridClasses& gc, uint32_t mask,
                                           float strength, uint32_t sn, uint32_t seed, float& ax,
                                           float& ay, float& az) {
   const float ri2 = me.w * me.w;
-  const int3 h = grid_cell<D>(me, inv);
+  const uint32_t mine = grid_class(me.w, gc);
+  const uint32_t classes = gc.two ? 2u : 1u;
   constexpr int RY = D > 1 ? 1 : 0, RZ = D > 2 ? 1 : 0;
-  for (int dz = -RZ; dz <= RZ; ++dz)
-    for (int dy = -RY; dy <= RY; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int cx = h.x + dx, cy = h.y + dy, cz = h.z + dz;
-        const uint32_t key = grid_slot<D>(cx, cy, cz, seg, mask);
-        const uint32_t start = starts[key];
-        if (start >= n || skeys[start] != key || (start > 0 && skeys[start - 1] == key)) continue;
-        for (uint32_t t = start; t < n && skeys[t] == key; ++t) {
-          if (t == s) continue;
-          if (SEG && sseg[t] != seg) continue;
-          const float4 q = sorted[t];
-          const int3 c = grid_cell<D>(q, inv);
-          if (c.x != cx || c.y != cy || c.z != cz) continue;
-          const float rj = q.w, r = me.w + rj;
-          float ex = me.x - q.x, ey = me.y - q.y, ez = me.z - q.z;
-          float l = ex * ex + ey * ey + ez * ez;
-          if (l >= r * r) continue;
-          if (l == 0.f) {
-            coincident(node, svals[t] - node_off, D, sn, seed, ex, ey, ez);
-            l = ex * ex + ey * ey + ez * ez;
-          }
-          const float len = sqrtf(l), rj2 = rj * rj;
-          const float f = (r - len) / len * strength * rj2 / (ri2 + rj2);
-          ax += ex * f;
-          ay += ey * f;
-          az += ez * f;
-        }
+  constexpr int SLAB = (2 * RY + 1) * 3;
+  for (uint32_t c = 0; c < classes; ++c) {
+    const float inv = c ? gc.inv_b : gc.inv_s;
+    const int3 h = grid_cell<D>(me, inv);
+    const uint32_t tag = 2u * seg + c;
+    const int R = (c == 0 && mine == 1) ? (int)ceilf((me.w + gc.r0) * inv * 1.0001f) : 1;
+    if (R == 1) {
+      for (int dz = -RZ; dz <= RZ; ++dz) {
+        uint2 rr[SLAB];
+#pragma unroll
+        for (int k = 0; k < SLAB; ++k)
+          rr[k] = runs[grid_slot<D>(h.x + k % 3 - 1, h.y + k / 3 - RY, h.z + dz, tag, mask)];
+#pragma unroll
+        for (int k = 0; k < SLAB; ++k)
+          grid_cell_pairs<D, SEG>(rr[k], h.x + k % 3 - 1, h.y + k / 3 - RY, h.z + dz, c, inv, s, me, ri2,
+                                  node, svals, sorted, sseg, seg, node_off, gc, strength, sn, seed, ax,
+                                  ay, az);
       }
+    } else {
+      const int Ry = D > 1 ? R : 0, Rz = D > 2 ? R : 0;
+      for (int dz = -Rz; dz <= Rz; ++dz)
+        for (int dy = -Ry; dy <= Ry; ++dy)
+          for (int dx = -R; dx <= R; ++dx) {
+            const int cx = h.x + dx, cy = h.y + dy, cz = h.z + dz;
+            grid_cell_pairs<D, SEG>(runs[grid_slot<D>(cx, cy, cz, tag, mask)], cx, cy, cz, c, inv, s, me,
+                                    ri2, node, svals, sorted, sseg, seg, node_off, gc, strength, sn,
+                                    seed, ax, ay, az);
+          }
+    }
+  }
+}
+
+// Empties the run table (a kernel, so that a captured tick can do it).
+__global__ void __launch_bounds__(256) grid_clear(uint32_t slots, uint2* __restrict__ runs) {
+  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < slots; k += gridDim.x * 256)
+    runs[k] = make_uint2(0u, 0u);
+}
+
+// A point's slot: its cell in its own class's grid.
+template <int D>
+__device__ inline uint32_t grid_key(float4 q, uint32_t seg, const GridClasses& gc, uint32_t mask) {
+  const uint32_t c = grid_class(q.w, gc);
+  const int3 cell = grid_cell<D>(q, c ? gc.inv_b : gc.inv_s);
+  return grid_slot<D>(cell.x, cell.y, cell.z, 2u * seg + c, mask);
 }
 
 template <int D>
@@ -499,14 +573,15 @@ __global__ void __launch_bounds__(256) grid_keys(uint32_t n, const float* __rest
                                                  const float* __restrict__ z,
                                                  const float* __restrict__ vx,
                                                  const float* __restrict__ vy,
-                                                 const float* __restrict__ vz, float inv,
+                                                 const float* __restrict__ vz,
+                                                 const float* __restrict__ radius, GridClasses gc,
                                                  uint32_t mask, uint32_t* __restrict__ keys,
                                                  uint32_t* __restrict__ vals) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const float4 q = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f, 0.f);
-  const int3 c = grid_cell<D>(q, inv);
-  keys[i] = grid_slot<D>(c.x, c.y, c.z, 0u, mask);
+  const float4 q = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f,
+                               radius[i]);
+  keys[i] = grid_key<D>(q, 0u, gc, mask);
   vals[i] = i;
 }
 
@@ -520,23 +595,23 @@ __global__ void __launch_bounds__(256) grid_begin(uint32_t n, const uint32_t* __
                                                   const float* __restrict__ vy,
                                                   const float* __restrict__ vz,
                                                   const float* __restrict__ radius,
-                                                  uint32_t* __restrict__ starts,
+                                                  uint2* __restrict__ runs,
                                                   float4* __restrict__ sorted) {
   uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
   uint32_t key = skeys[s], i = svals[s];
-  if (s == 0 || skeys[s - 1] != key) starts[key] = s;
+  if (s == 0 || skeys[s - 1] != key) runs[key].x = s;
+  if (s + 1 == n || skeys[s + 1] != key) runs[key].y = s + 1;
   sorted[s] = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f,
                           radius[i]);
 }
 
 template <int D>
-__global__ void __launch_bounds__(256) grid_query(uint32_t n, const uint32_t* __restrict__ skeys,
-                                                  const uint32_t* __restrict__ svals,
-                                                  const uint32_t* __restrict__ starts,
+__global__ void __launch_bounds__(256) grid_query(uint32_t n, const uint32_t* __restrict__ svals,
+                                                  const uint2* __restrict__ runs,
                                                   const float4* __restrict__ sorted,
                                                   float* __restrict__ vx, float* __restrict__ vy,
-                                                  float* __restrict__ vz, float inv,
+                                                  float* __restrict__ vz, GridClasses gc,
                                                   uint32_t mask, float strength,
                                                   StepParams step) {
   uint32_t s = blockIdx.x * 256 + threadIdx.x;
@@ -544,7 +619,7 @@ __global__ void __launch_bounds__(256) grid_query(uint32_t n, const uint32_t* __
   const float4 me = sorted[s];
   const uint32_t i = svals[s];
   float ax = 0.f, ay = 0.f, az = 0.f;
-  grid_collide_point<D, false>(s, me, i, n, skeys, svals, starts, sorted, nullptr, 0u, 0u, inv, mask,
+  grid_collide_point<D, false>(s, me, i, svals, runs, sorted, nullptr, 0u, 0u, gc, mask,
                                strength, step.step, step.seed, ax, ay, az);
   vx[i] += ax;
   if (D > 1) vy[i] += ay;
@@ -560,8 +635,7 @@ void GridWork::reserve(uint32_t n_) {
   bits = b;
   FG_HIP(hipMalloc(&keys, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
-  FG_HIP(hipMalloc(&starts, sizeof(uint32_t) << bits));
-  FG_HIP(hipMemset(starts, 0xff, sizeof(uint32_t) << bits));
+  FG_HIP(hipMalloc(&runs, sizeof(uint2) << bits));
   FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
   sort.reserve(m, sizeof(uint32_t));
   cap = n_;
@@ -570,9 +644,10 @@ void GridWork::reserve(uint32_t n_) {
 void GridWork::release() {
   if (keys) (void)hipFree(keys);
   if (vals) (void)hipFree(vals);
-  if (starts) (void)hipFree(starts);
+  if (runs) (void)hipFree(runs);
   if (sorted) (void)hipFree(sorted);
-  keys = vals = starts = nullptr;
+  keys = vals = nullptr;
+  runs = nullptr;
   sorted = nullptr;
   sort.release();
   cap = 0;
@@ -580,33 +655,55 @@ void GridWork::release() {
 
 template <int D>
 void grid_run(hipStream_t s, GridWork& g, uint32_t n, float* const* pos, float* const* vel,
-              const float* radius, float inv, uint32_t mask, float strength, int radix_bits,
+              const float* radius, const GridClasses& gc, uint32_t mask, float strength, int radix_bits,
               StepParams step) {
   dim3 gr(ceil_div(n, 256)), b(256);
   hipLaunchKernelGGL(grid_keys<D>, gr, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2],
-                     inv, mask, g.keys, g.vals);
+                     radius, gc, mask, g.keys, g.vals);
   FG_LAUNCH_CHECK();
   uint32_t *sk, *sv;
   radix_sort_u32(s, g.sort, radix_bits, g.keys, g.vals, n, 0, g.bits, &sk, &sv);
+  const uint32_t slots = 1u << g.bits;
+  hipLaunchKernelGGL(grid_clear, dim3(std::min(ceil_div(slots, 256), 2048u)), b, 0, s, slots, g.runs);
   hipLaunchKernelGGL(grid_begin<D>, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv, pos[0],
-                     pos[1], pos[2], vel[0], vel[1], vel[2], radius, g.starts, g.sorted);
+                     pos[1], pos[2], vel[0], vel[1], vel[2], radius, g.runs, g.sorted);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(grid_query<D>, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
-                     (const uint32_t*)g.starts, (const float4*)g.sorted, vel[0], vel[1], vel[2], inv,
-                     mask, strength, step);
+  hipLaunchKernelGGL(grid_query<D>, gr, b, 0, s, n, (const uint32_t*)sv, (const uint2*)g.runs,
+                     (const float4*)g.sorted, vel[0], vel[1], vel[2], gc, mask, strength, step);
   FG_LAUNCH_CHECK();
 }
 
 void launch_collide_grid(hipStream_t s, GridWork& g, uint32_t n, int ndim, float* const* pos,
-                         float* const* vel, const float* radius, float rmax, float strength,
+                         float* const* vel, const float* radius, const GridClasses& gc, float strength,
                          int radix_bits, StepParams step) {
-  if (!n || !(rmax > 0)) return;
+  if (!n || !(gc.inv_b > 0)) return;
   g.reserve(n);
-  float inv = 1.0f / (2.0f * rmax);
   uint32_t mask = (uint32_t)((1ull << g.bits) - 1ull);
-  if (ndim == 1) grid_run<1>(s, g, n, pos, vel, radius, inv, mask, strength, radix_bits, step);
-  else if (ndim == 2) grid_run<2>(s, g, n, pos, vel, radius, inv, mask, strength, radix_bits, step);
-  else grid_run<3>(s, g, n, pos, vel, radius, inv, mask, strength, radix_bits, step);
+  if (ndim == 1) grid_run<1>(s, g, n, pos, vel, radius, gc, mask, strength, radix_bits, step);
+  else if (ndim == 2) grid_run<2>(s, g, n, pos, vel, radius, gc, mask, strength, radix_bits, step);
+  else grid_run<3>(s, g, n, pos, vel, radius, gc, mask, strength, radix_bits, step);
+}
+
+GridClasses grid_classes(const float* radii, uint32_t n) {
+  GridClasses gc{0.0f, 0.0f, 0.0f, 0};
+  float rmax = 0;
+  for (uint32_t i = 0; i < n; ++i) rmax = std::max(rmax, radii[i]);
+  if (!(rmax > 0)) return gc;
+  gc.inv_b = gc.inv_s = 1.0f / (2.0f * rmax);
+  gc.r0 = rmax;
+  static const bool one_class = getenv("D3FGPU_GRID_ONE_CLASS") != nullptr;  // experiment override
+  if (n >= 1024 && !one_class) {
+    std::vector<float> r(radii, radii + n);
+    const size_t k = (size_t)(0.95 * (double)(n - 1));
+    std::nth_element(r.begin(), r.begin() + k, r.end());
+    const float r95 = r[k];
+    if (r95 > 0 && rmax > 4.0f * r95) {
+      gc.two = 1;
+      gc.r0 = r95;
+      gc.inv_s = 1.0f / (2.0f * r95);
+    }
+  }
+  return gc;
 }
 
 }  // namespace fg
@@ -696,9 +793,8 @@ __global__ void __launch_bounds__(256) bgrid_keys(uint32_t total, const uint32_t
   const BJob& B = jobs[seg_job[seg]];
   const uint32_t v = i - B.grid_off, c = B.vcur;
   const float4 q = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
-                               D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, 0.f);
-  const int3 cell = grid_cell<D>(q, B.coll_inv);
-  keys[i] = grid_slot<D>(cell.x, cell.y, cell.z, seg, mask);
+                               D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, B.radius[v]);
+  keys[i] = grid_key<D>(q, seg, B.coll, mask);
   vals[i] = i;
 }
 
@@ -708,13 +804,14 @@ __global__ void __launch_bounds__(256) bgrid_begin(uint32_t total, const uint32_
                                                    const uint32_t* __restrict__ seg_of,
                                                    const uint32_t* __restrict__ seg_job,
                                                    const BJob* __restrict__ jobs,
-                                                   uint32_t* __restrict__ starts,
+                                                   uint2* __restrict__ runs,
                                                    float4* __restrict__ sorted,
                                                    uint32_t* __restrict__ sseg) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= total) return;
   const uint32_t key = skeys[s], i = svals[s];
-  if (s == 0 || skeys[s - 1] != key) starts[key] = s;
+  if (s == 0 || skeys[s - 1] != key) runs[key].x = s;
+  if (s + 1 == total || skeys[s + 1] != key) runs[key].y = s + 1;
   const uint32_t seg = seg_of[i];
   const BJob& B = jobs[seg_job[seg]];
   const uint32_t v = i - B.grid_off, c = B.vcur;
@@ -724,9 +821,8 @@ __global__ void __launch_bounds__(256) bgrid_begin(uint32_t total, const uint32_
 }
 
 template <int D>
-__global__ void __launch_bounds__(256) bgrid_query(uint32_t total, const uint32_t* __restrict__ skeys,
-                                                   const uint32_t* __restrict__ svals,
-                                                   const uint32_t* __restrict__ starts,
+__global__ void __launch_bounds__(256) bgrid_query(uint32_t total, const uint32_t* __restrict__ svals,
+                                                   const uint2* __restrict__ runs,
                                                    const float4* __restrict__ sorted,
                                                    const uint32_t* __restrict__ sseg,
                                                    const uint32_t* __restrict__ seg_job,
@@ -739,8 +835,8 @@ __global__ void __launch_bounds__(256) bgrid_query(uint32_t total, const uint32_
   if (!B.active || iteration >= B.coll_iters) return;
   const uint32_t v = svals[s] - B.grid_off;
   float ax = 0.f, ay = 0.f, az = 0.f;
-  grid_collide_point<D, true>(s, sorted[s], v, total, skeys, svals, starts, sorted, sseg, seg,
-                              B.grid_off, B.coll_inv, mask, B.coll_strength, B.step, B.seed, ax, ay,
+  grid_collide_point<D, true>(s, sorted[s], v, svals, runs, sorted, sseg, seg,
+                              B.grid_off, B.coll, mask, B.coll_strength, B.step, B.seed, ax, ay,
                               az);
   float* const* vel = B.v[B.vcur];
   vel[0][v] += ax;
@@ -871,8 +967,7 @@ void BatchGrid::reserve(uint32_t n_) {
   bits = b;
   FG_HIP(hipMalloc(&keys, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
-  FG_HIP(hipMalloc(&starts, sizeof(uint32_t) << bits));
-  FG_HIP(hipMemset(starts, 0xff, sizeof(uint32_t) << bits));
+  FG_HIP(hipMalloc(&runs, sizeof(uint2) << bits));
   FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
   FG_HIP(hipMalloc(&sseg, sizeof(uint32_t) * m));
   sort.reserve(m, sizeof(uint32_t));
@@ -880,10 +975,11 @@ void BatchGrid::reserve(uint32_t n_) {
 }
 
 void BatchGrid::release() {
-  void* ptrs[] = {keys, vals, starts, sorted, sseg};
+  void* ptrs[] = {keys, vals, runs, sorted, sseg};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  keys = vals = starts = sseg = nullptr;
+  keys = vals = sseg = nullptr;
+  runs = nullptr;
   sorted = nullptr;
   sort.release();
   cap = 0;
@@ -937,11 +1033,13 @@ void batch_collide(hipStream_t s, int ndim, BatchGrid& g, const BSpace& sp, BJob
             (const BJob*)jobs, mask, g.keys, g.vals);
   uint32_t *sk, *sv;
   radix_sort_u32(s, g.sort, radix_bits, g.keys, g.vals, n, 0, g.bits, &sk, &sv);
+  const uint32_t slots = 1u << g.bits;
+  hipLaunchKernelGGL(grid_clear, dim3(std::min(ceil_div(slots, 256), 2048u)), b, 0, s, slots, g.runs);
   FG_BY_DIM(ndim, bgrid_begin, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
-            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, g.starts,
+            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, g.runs,
             g.sorted, g.sseg);
-  FG_BY_DIM(ndim, bgrid_query, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
-            (const uint32_t*)g.starts, (const float4*)g.sorted, (const uint32_t*)g.sseg,
+  FG_BY_DIM(ndim, bgrid_query, gr, b, 0, s, n, (const uint32_t*)sv,
+            (const uint2*)g.runs, (const float4*)g.sorted, (const uint32_t*)g.sseg,
             (const uint32_t*)sp.seg_job, (const BJob*)jobs, mask, iteration);
 }
 

@@ -25,7 +25,8 @@ DECAY = d3fgpu.ALPHA_DECAY
 
 
 def layout_forces(n, ndim, seed, collide=True, center=True, manybody=True, link=True, radius=3.0):
-    """Link (BA m=2, d3's defaults) + manyBody(-30) + collide(radius) + center."""
+    """Link (BA m=2, d3's defaults) + manyBody(-30) + collide(radius) + center; radius None:
+    lognormal radii (mu ln 2, sigma 0.4), the heavy tail that takes the two-class grid."""
     fs = []
     if link and n > 1:
         src, dst = ba_graph(n, 2, seed)
@@ -34,7 +35,11 @@ def layout_forces(n, ndim, seed, collide=True, center=True, manybody=True, link=
     if manybody:
         fs.append(d3fgpu.many_body(np.full(n, -30.0)))
     if collide:
-        fs.append(d3fgpu.collide(np.full(n, radius)))
+        if radius is None:
+            r = np.exp(np.log(2.0) + 0.4 * np.random.default_rng(seed).standard_normal(n))
+        else:
+            r = np.full(n, radius)
+        fs.append(d3fgpu.collide(r))
     if center:
         fs.append(d3fgpu.center())
     return fs
@@ -86,13 +91,14 @@ def test_batch_forces_vs_oracle():
 def test_batch_alone_equals_mix():
     """Each job of a mixed batch ends bit for bit where it ends when run alone."""
     specs = [(2500, 3, 1), (800, 2, 2), (6000, 3, 3), (1200, 2, 4), (400, 3, 5), (3000, 1, 6)]
+    radius = [3.0, None, None, 3.0, None, None]  # None: lognormal radii (two-class grid when n >= 1024)
     ticks = 25
     mix = d3fgpu.Batch(0)
-    ids = [mix.submit(make_job(n, nd, sd)) for n, nd, sd in specs]
+    ids = [mix.submit(make_job(n, nd, sd, radius=r)) for (n, nd, sd), r in zip(specs, radius)]
     mix.step(ticks)
-    for jid, (n, nd, sd) in zip(ids, specs):
+    for jid, (n, nd, sd), r in zip(ids, specs, radius):
         p_mix, v_mix = mix.read(jid)
-        p_one, v_one = run_alone(make_job(n, nd, sd), ticks)
+        p_one, v_one = run_alone(make_job(n, nd, sd, radius=r), ticks)
         assert np.all(np.isfinite(p_mix))
         assert np.array_equal(p_mix, p_one), f"job n={n} {nd}D differs alone vs mix"
         assert np.array_equal(v_mix, v_one)
@@ -144,12 +150,12 @@ def test_batch_equals_single_simulation_link_manybody(n, nd):
     assert np.array_equal(v1, v2)
 
 
-@pytest.mark.parametrize("n,nd", [(9000, 3), (3000, 2)])
-def test_batch_close_to_single_simulation_full_mix(n, nd):
+@pytest.mark.parametrize("n,nd,radius", [(9000, 3, 3.0), (3000, 2, 3.0), (12000, 3, None)])
+def test_batch_close_to_single_simulation_full_mix(n, nd, radius):
     """center sums and collide visits in another order: agreement to rounding, over 10 ticks."""
     ticks = 10
-    p1, _ = plain_run(n, nd, 31, ticks)
-    p2, _ = run_alone(make_job(n, nd, 31), ticks)
+    p1, _ = plain_run(n, nd, 31, ticks, radius=radius)
+    p2, _ = run_alone(make_job(n, nd, 31, radius=radius), ticks)
     err = rel_l2(p2 - phyllotaxis(n, nd), p1 - phyllotaxis(n, nd))
     print(f"[parity] batch vs single, {nd}D n={n}, displacement after {ticks} ticks: rel-L2 {err:.2e}")
     assert err < 1e-4

@@ -249,6 +249,24 @@ def test_collide_grid_two_iterations_lognormal(engine_factory):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_collide_grid_heavy_tail(engine_factory, dims):
+    """1% of the nodes ten times larger than the rest: the two-class grid, where a large node
+    searches several rings of the fine cells."""
+    n = 12000
+    rng = np.random.default_rng(9)
+    pos = make_nodes_fast(n, dims, 6 * n ** (1 / dims), seed=21)
+    radii = 2.0 + rng.random(n)
+    radii[rng.choice(n, n // 100, replace=False)] = 40.0
+    f = d3fgpu.collide(radii, strength=0.7, iterations=1)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [f])
+    ref = oracle_collide(pos, np.zeros_like(pos), dims, radii, 0.7, 1)
+    err = rel_l2(dv, ref)
+    report(f"collide grid heavy tail {dims}D", err, touched=int(np.sum(np.any(ref != 0, 1))))
+    assert err < 1e-4
+    assert close_count(dv, ref, rtol=1e-3) == 0
+
+
 # ---- center / radial / position / integrate, several ticks --------------------------
 
 def run_both(engine, pos, dims, forces_gpu, forces_oracle, ticks, fix=None, fixmask=None):
