@@ -135,6 +135,7 @@ struct BatchTree {
   uint32_t* digit = nullptr;  // segment of each code-sorted entry (the last sort pass)
   uint32_t* order = nullptr;  // tree index at each sorted position
   float4* leaves = nullptr;
+  float4* pts = nullptr;        // per node: x, y, z, strength (what the leaves gather)
   uint32_t* leaf_depth = nullptr;
   uint32_t* leaf_parent = nullptr;
   uint4* links = nullptr;
@@ -156,7 +157,8 @@ struct BatchGrid {
   uint32_t* keys = nullptr;
   uint32_t* vals = nullptr;
   uint2* runs = nullptr;     // [2^bits] each slot's [first, end) of the sorted points
-  float4* sorted = nullptr;
+  float4* sorted = nullptr;   // snapshots (pos + vel, radius) in sorted order
+  float4* snap = nullptr;     // the same per node
   uint32_t* sseg = nullptr;
   SortWork sort;
   void reserve(uint32_t n);

@@ -41,7 +41,8 @@ struct GridWork {
   uint32_t* keys = nullptr;
   uint32_t* vals = nullptr;
   uint2* runs = nullptr;     // [2^bits] each slot's [first, end) of the sorted points
-  float4* sorted = nullptr;
+  float4* sorted = nullptr;   // snapshots (pos + vel, radius) in sorted order
+  float4* snap = nullptr;     // the same per node
   SortWork sort;
   void reserve(uint32_t n);
   void release();

@@ -576,34 +576,28 @@ __global__ void __launch_bounds__(256) grid_keys(uint32_t n, const float* __rest
                                                  const float* __restrict__ vz,
                                                  const float* __restrict__ radius, GridClasses gc,
                                                  uint32_t mask, uint32_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals) {
+                                                 uint32_t* __restrict__ vals, float4* __restrict__ snap) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float4 q = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f,
                                radius[i]);
   keys[i] = grid_key<D>(q, 0u, gc, mask);
   vals[i] = i;
+  snap[i] = q;
 }
 
-template <int D>
+// Run table entries and the snapshots in sorted order (one 16-byte gather per point).
 __global__ void __launch_bounds__(256) grid_begin(uint32_t n, const uint32_t* __restrict__ skeys,
                                                   const uint32_t* __restrict__ svals,
-                                                  const float* __restrict__ x,
-                                                  const float* __restrict__ y,
-                                                  const float* __restrict__ z,
-                                                  const float* __restrict__ vx,
-                                                  const float* __restrict__ vy,
-                                                  const float* __restrict__ vz,
-                                                  const float* __restrict__ radius,
+                                                  const float4* __restrict__ snap,
                                                   uint2* __restrict__ runs,
                                                   float4* __restrict__ sorted) {
   uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
-  uint32_t key = skeys[s], i = svals[s];
+  const uint32_t key = skeys[s];
   if (s == 0 || skeys[s - 1] != key) runs[key].x = s;
   if (s + 1 == n || skeys[s + 1] != key) runs[key].y = s + 1;
-  sorted[s] = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f,
-                          radius[i]);
+  sorted[s] = snap[svals[s]];
 }
 
 template <int D>
@@ -637,6 +631,7 @@ void GridWork::reserve(uint32_t n_) {
   FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&runs, sizeof(uint2) << bits));
   FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
+  FG_HIP(hipMalloc(&snap, sizeof(float4) * m));
   sort.reserve(m, sizeof(uint32_t));
   cap = n_;
 }
@@ -646,8 +641,10 @@ void GridWork::release() {
   if (vals) (void)hipFree(vals);
   if (runs) (void)hipFree(runs);
   if (sorted) (void)hipFree(sorted);
+  if (snap) (void)hipFree(snap);
   keys = vals = nullptr;
   runs = nullptr;
+  sorted = snap = nullptr;
   sorted = nullptr;
   sort.release();
   cap = 0;
@@ -659,14 +656,14 @@ void grid_run(hipStream_t s, GridWork& g, uint32_t n, float* const* pos, float* 
               StepParams step) {
   dim3 gr(ceil_div(n, 256)), b(256);
   hipLaunchKernelGGL(grid_keys<D>, gr, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2],
-                     radius, gc, mask, g.keys, g.vals);
+                     radius, gc, mask, g.keys, g.vals, g.snap);
   FG_LAUNCH_CHECK();
   uint32_t *sk, *sv;
   radix_sort_u32(s, g.sort, radix_bits, g.keys, g.vals, n, 0, g.bits, &sk, &sv);
   const uint32_t slots = 1u << g.bits;
   hipLaunchKernelGGL(grid_clear, dim3(std::min(ceil_div(slots, 256), 2048u)), b, 0, s, slots, g.runs);
-  hipLaunchKernelGGL(grid_begin<D>, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv, pos[0],
-                     pos[1], pos[2], vel[0], vel[1], vel[2], radius, g.runs, g.sorted);
+  hipLaunchKernelGGL(grid_begin, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
+                     (const float4*)g.snap, g.runs, g.sorted);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL(grid_query<D>, gr, b, 0, s, n, (const uint32_t*)sv, (const uint2*)g.runs,
                      (const float4*)g.sorted, vel[0], vel[1], vel[2], gc, mask, strength, step);
@@ -786,7 +783,7 @@ __global__ void __launch_bounds__(256) bgrid_keys(uint32_t total, const uint32_t
                                                   const uint32_t* __restrict__ seg_job,
                                                   const BJob* __restrict__ jobs, uint32_t mask,
                                                   uint32_t* __restrict__ keys,
-                                                  uint32_t* __restrict__ vals) {
+                                                  uint32_t* __restrict__ vals, float4* __restrict__ snap) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
   const uint32_t seg = seg_of[i];
@@ -796,14 +793,13 @@ __global__ void __launch_bounds__(256) bgrid_keys(uint32_t total, const uint32_t
                                D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, B.radius[v]);
   keys[i] = grid_key<D>(q, seg, B.coll, mask);
   vals[i] = i;
+  snap[i] = q;
 }
 
-template <int D>
 __global__ void __launch_bounds__(256) bgrid_begin(uint32_t total, const uint32_t* __restrict__ skeys,
                                                    const uint32_t* __restrict__ svals,
                                                    const uint32_t* __restrict__ seg_of,
-                                                   const uint32_t* __restrict__ seg_job,
-                                                   const BJob* __restrict__ jobs,
+                                                   const float4* __restrict__ snap,
                                                    uint2* __restrict__ runs,
                                                    float4* __restrict__ sorted,
                                                    uint32_t* __restrict__ sseg) {
@@ -812,12 +808,8 @@ __global__ void __launch_bounds__(256) bgrid_begin(uint32_t total, const uint32_
   const uint32_t key = skeys[s], i = svals[s];
   if (s == 0 || skeys[s - 1] != key) runs[key].x = s;
   if (s + 1 == total || skeys[s + 1] != key) runs[key].y = s + 1;
-  const uint32_t seg = seg_of[i];
-  const BJob& B = jobs[seg_job[seg]];
-  const uint32_t v = i - B.grid_off, c = B.vcur;
-  sorted[s] = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
-                          D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, B.radius[v]);
-  sseg[s] = seg;
+  sorted[s] = snap[i];
+  sseg[s] = seg_of[i];
 }
 
 template <int D>
@@ -969,17 +961,19 @@ void BatchGrid::reserve(uint32_t n_) {
   FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&runs, sizeof(uint2) << bits));
   FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
+  FG_HIP(hipMalloc(&snap, sizeof(float4) * m));
   FG_HIP(hipMalloc(&sseg, sizeof(uint32_t) * m));
   sort.reserve(m, sizeof(uint32_t));
   cap = n_;
 }
 
 void BatchGrid::release() {
-  void* ptrs[] = {keys, vals, runs, sorted, sseg};
+  void* ptrs[] = {keys, vals, runs, sorted, snap, sseg};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = vals = sseg = nullptr;
   runs = nullptr;
+  sorted = snap = nullptr;
   sorted = nullptr;
   sort.release();
   cap = 0;
@@ -1030,14 +1024,14 @@ void batch_collide(hipStream_t s, int ndim, BatchGrid& g, const BSpace& sp, BJob
   const uint32_t mask = (uint32_t)((1ull << g.bits) - 1ull);
   dim3 gr(ceil_div(n, 256)), b(256);
   FG_BY_DIM(ndim, bgrid_keys, gr, b, 0, s, n, (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job,
-            (const BJob*)jobs, mask, g.keys, g.vals);
+            (const BJob*)jobs, mask, g.keys, g.vals, g.snap);
   uint32_t *sk, *sv;
   radix_sort_u32(s, g.sort, radix_bits, g.keys, g.vals, n, 0, g.bits, &sk, &sv);
   const uint32_t slots = 1u << g.bits;
   hipLaunchKernelGGL(grid_clear, dim3(std::min(ceil_div(slots, 256), 2048u)), b, 0, s, slots, g.runs);
-  FG_BY_DIM(ndim, bgrid_begin, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
-            (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, g.runs,
-            g.sorted, g.sseg);
+  hipLaunchKernelGGL(bgrid_begin, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
+                     (const uint32_t*)sp.seg_of, (const float4*)g.snap, g.runs, g.sorted, g.sseg);
+  FG_LAUNCH_CHECK();
   FG_BY_DIM(ndim, bgrid_query, gr, b, 0, s, n, (const uint32_t*)sv,
             (const uint2*)g.runs, (const float4*)g.sorted, (const uint32_t*)g.sseg,
             (const uint32_t*)sp.seg_job, (const BJob*)jobs, mask, iteration);

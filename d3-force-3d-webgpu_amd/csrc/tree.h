@@ -51,6 +51,7 @@ struct TreeBuffers {
   uint64_t* skeys = nullptr;    // the sorted codes (points into keys or the sorter's buffer)
   uint32_t* order = nullptr;    // node index at each sorted position
   float4* leaves = nullptr;     // sorted: x, y, z, strength
+  float4* pts = nullptr;        // per node: x, y, z, strength (what the leaves gather)
   uint32_t* leaf_depth = nullptr;
   uint32_t* leaf_parent = nullptr;
   uint4* links = nullptr;       // per binary node: left, right, depth, parent

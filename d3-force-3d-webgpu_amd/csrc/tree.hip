@@ -189,15 +189,18 @@ __device__ inline uint64_t point_key(const float p[3], const float origin[3], in
 template <int D>
 __global__ void __launch_bounds__(256) tree_keys(const float* __restrict__ x,
                                                  const float* __restrict__ y,
-                                                 const float* __restrict__ z, uint32_t n,
+                                                 const float* __restrict__ z,
+                                                 const float* __restrict__ strength, uint32_t n,
                                                  const CubeInfo* __restrict__ cube,
                                                  uint64_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals) {
+                                                 uint32_t* __restrict__ vals,
+                                                 float4* __restrict__ pts) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
   keys[i] = point_key<D>(p, cube->origin, cube->shift);
   vals[i] = i;
+  pts[i] = make_float4(p[0], p[1], p[2], strength[i]);
 }
 
 // Length of the common prefix of the codes at sorted positions a and b, counted as the
@@ -222,17 +225,13 @@ __device__ inline uint32_t depth_of(int prefix) {
 template <int D>
 __global__ void __launch_bounds__(256) tree_leaves(const uint64_t* __restrict__ skeys,
                                                    const uint32_t* __restrict__ order,
-                                                   const float* __restrict__ x,
-                                                   const float* __restrict__ y,
-                                                   const float* __restrict__ z,
-                                                   const float* __restrict__ strength, uint32_t n,
+                                                   const float4* __restrict__ pts, uint32_t n,
                                                    float4* __restrict__ leaves,
                                                    uint32_t* __restrict__ leaf_depth,
                                                    uint32_t* __restrict__ flags) {
   uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
-  uint32_t v = order[s];
-  leaves[s] = make_float4(x[v], D > 1 ? y[v] : 0.0f, D > 2 ? z[v] : 0.0f, strength[v]);
+  leaves[s] = pts[order[s]];
   uint32_t depth = 0;
   if (n > 1) {
     int a = delta<D>(skeys, (int)n, (int)s, (int)s - 1), b = delta<D>(skeys, (int)n, (int)s, (int)s + 1);
@@ -933,7 +932,8 @@ __global__ void __launch_bounds__(256) btree_keys(uint32_t total, const uint32_t
                                                   const BJob* __restrict__ jobs,
                                                   uint64_t* __restrict__ keys,
                                                   uint64_t* __restrict__ codes,
-                                                  uint32_t* __restrict__ vals) {
+                                                  uint32_t* __restrict__ vals,
+                                                  float4* __restrict__ pts) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
   const BJob& B = jobs[seg_job[seg_of[i]]];
@@ -943,6 +943,7 @@ __global__ void __launch_bounds__(256) btree_keys(uint32_t total, const uint32_t
   keys[i] = key;
   codes[i] = key;
   vals[i] = i;
+  pts[i] = make_float4(p[0], p[1], p[2], B.mb_strength[v]);
 }
 
 // The segment of each code-sorted entry: the key of the last, stable, sort pass.
@@ -966,6 +967,7 @@ __global__ void __launch_bounds__(256) btree_leaves(uint32_t total, const uint64
                                                     const uint32_t* __restrict__ seg_of,
                                                     const uint32_t* __restrict__ seg_job,
                                                     const BJob* __restrict__ jobs,
+                                                    const float4* __restrict__ pts,
                                                     float4* __restrict__ leaves,
                                                     uint32_t* __restrict__ leaf_depth,
                                                     uint32_t* __restrict__ flags) {
@@ -973,9 +975,7 @@ __global__ void __launch_bounds__(256) btree_leaves(uint32_t total, const uint64
   if (s >= total) return;
   const BJob& B = jobs[seg_job[seg_of[s]]];
   const uint32_t off = B.tree_off, n = B.n;
-  const uint32_t v = order[s] - off;
-  leaves[s] = make_float4(B.p[0][v], D > 1 ? B.p[1][v] : 0.0f, D > 2 ? B.p[2][v] : 0.0f,
-                          B.mb_strength[v]);
+  leaves[s] = pts[order[s]];
   const int ls = (int)(s - off);
   int a = delta<D>(skeys + off, (int)n, ls, ls - 1), b = delta<D>(skeys + off, (int)n, ls, ls + 1);
   leaf_depth[s] = depth_of<D>(a > b ? a : b) + 1;
@@ -1094,7 +1094,7 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
                      (const uint32_t*)sp.seg_job, jobs);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((btree_keys<D>), dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)sp.seg_of,
-                     (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.keys, t.codes, t.vals);
+                     (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.keys, t.codes, t.vals, t.pts);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(2);
   uint64_t* sk;
@@ -1117,7 +1117,7 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
   t.order = order;
   hipLaunchKernelGGL((btree_leaves<D>), dim3(blocks), dim3(256), 0, st, total, (const uint64_t*)t.skeys,
                      (const uint32_t*)order, (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job,
-                     (const BJob*)jobs, t.leaves, t.leaf_depth, t.flags);
+                     (const BJob*)jobs, (const float4*)t.pts, t.leaves, t.leaf_depth, t.flags);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((btree_build<D>), dim3(blocks), dim3(256), 0, st, total, (const uint64_t*)t.skeys,
                      (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.links,
@@ -1142,8 +1142,8 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   uint32_t bb = blocks < 512 ? blocks : 512;
   hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL((tree_keys<D>), dim3(blocks), dim3(256), 0, st, x, y, z, n,
-                     (const CubeInfo*)t.cube, t.keys, t.vals);
+  hipLaunchKernelGGL((tree_keys<D>), dim3(blocks), dim3(256), 0, st, x, y, z, strength, n,
+                     (const CubeInfo*)t.cube, t.keys, t.vals, t.pts);
   FG_LAUNCH_CHECK();
   uint64_t* sk;
   uint32_t* sv;
@@ -1153,7 +1153,7 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   t.skeys = sk;
   t.order = sv;
   hipLaunchKernelGGL((tree_leaves<D>), dim3(blocks), dim3(256), 0, st, (const uint64_t*)sk,
-                     (const uint32_t*)sv, x, y, z, strength, n, t.leaves, t.leaf_depth, t.flags);
+                     (const uint32_t*)sv, (const float4*)t.pts, n, t.leaves, t.leaf_depth, t.flags);
   FG_LAUNCH_CHECK();
   if (n < 2) {
     if (timer) timer->end(1);
@@ -1196,6 +1196,7 @@ void TreeBuffers::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&keys, sizeof(uint64_t) * m));
   FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&leaves, sizeof(float4) * m));
+  FG_HIP(hipMalloc(&pts, sizeof(float4) * m));
   FG_HIP(hipMalloc(&leaf_depth, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&leaf_parent, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&links, sizeof(uint4) * m));
@@ -1226,12 +1227,12 @@ void TreeBuffers::release() {
   if (spill) (void)hipFree(spill);
   spill = nullptr;
   spill_bytes = 0;
-  void* ptrs[] = {keys, vals, leaves, leaf_depth, leaf_parent, links, span, flags, rec, sums, cube, counters};
+  void* ptrs[] = {keys, vals, leaves, pts, leaf_depth, leaf_parent, links, span, flags, rec, sums, cube, counters};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = nullptr;
   vals = nullptr;
-  leaves = nullptr;
+  leaves = pts = nullptr;
   leaf_depth = leaf_parent = flags = counters = nullptr;
   links = nullptr;
   span = nullptr;
@@ -1271,6 +1272,7 @@ void BatchTree::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&digit, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&leaves, sizeof(float4) * m));
+  FG_HIP(hipMalloc(&pts, sizeof(float4) * m));
   FG_HIP(hipMalloc(&leaf_depth, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&leaf_parent, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&links, sizeof(uint4) * m));
@@ -1290,12 +1292,12 @@ void BatchTree::reserve(uint32_t n) {
 }
 
 void BatchTree::release() {
-  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, leaf_depth, leaf_parent, links, span, flags, rec, sums, counters, spill};
+  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, pts, leaf_depth, leaf_parent, links, span, flags, rec, sums, counters, spill};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = codes = skeys = nullptr;
   vals = digit = order = leaf_depth = leaf_parent = flags = counters = nullptr;
-  leaves = nullptr;
+  leaves = pts = nullptr;
   links = nullptr;
   span = nullptr;
   rec = sums = spill = nullptr;
