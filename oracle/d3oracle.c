@@ -621,6 +621,72 @@ void or_collide_seq(int32_t n, int32_t ndim, const double* x, const double* y, c
   for (int a = 0; a < 3; ++a) free(q[a]);
 }
 
+/* Partners of one node in the Jacobi collide: every j != i closer than ri + rj to the
+   snapshot of i (passes/collide.js:34-80), accumulated in ascending j.  Brute force
+   visits every j; the cell list (cells 2 rmax wide, sorted by cell) visits the 3^d cells
+   around i, collects the partners and sorts them, so both add the same terms in the same
+   order and agree bit for bit. */
+typedef struct {
+  int64_t c[3];
+  int32_t i;
+} CellRef;
+
+static int cellref_cmp(const void* pa, const void* pb) {
+  const CellRef* a = (const CellRef*)pa;
+  const CellRef* b = (const CellRef*)pb;
+  for (int k = 2; k >= 0; --k) {
+    if (a->c[k] < b->c[k]) return -1;
+    if (a->c[k] > b->c[k]) return 1;
+  }
+  return a->i < b->i ? -1 : a->i > b->i;
+}
+
+static int int32_cmp(const void* pa, const void* pb) {
+  int32_t a = *(const int32_t*)pa, b = *(const int32_t*)pb;
+  return a < b ? -1 : a > b;
+}
+
+/* Cell -> first sorted entry, by open addressing over the distinct cells of `refs`. */
+typedef struct {
+  int64_t c[3];
+  int32_t first;  /* -1: empty slot */
+} CellSlot;
+
+static uint64_t cell_hash(const int64_t c[3]) {
+  uint64_t h = (uint64_t)c[0] * 0x9E3779B97F4A7C15ull;
+  h ^= (uint64_t)c[1] * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+  h ^= (uint64_t)c[2] * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
+  return h ^ (h >> 29);
+}
+
+static int32_t cell_find(const CellSlot* tab, uint64_t mask, const int64_t c[3]) {
+  for (uint64_t k = cell_hash(c) & mask;; k = (k + 1) & mask) {
+    if (tab[k].first < 0) return -1;
+    if (tab[k].c[0] == c[0] && tab[k].c[1] == c[1] && tab[k].c[2] == c[2]) return tab[k].first;
+  }
+}
+
+static int collide_hit(const double* snap, const double* radii, int32_t i, int32_t j) {
+  double r = radii[i] + radii[j];
+  double d[3];
+  for (int a = 0; a < 3; ++a) d[a] = snap[3 * i + a] - snap[3 * j + a];
+  double l = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  return !(l >= r * r || l == 0);
+}
+
+static void collide_pair(const double* snap, const double* radii, int32_t i, int32_t j, int32_t ndim,
+                         double strength, double* acc) {
+  double ri = radii[i], ri2 = ri * ri;
+  double rj = radii[j], r = ri + rj;
+  double d[3];
+  for (int a = 0; a < 3; ++a) d[a] = snap[3 * i + a] - snap[3 * j + a];
+  double l = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (l >= r * r || l == 0) return;
+  double sl = sqrt(l), rj2 = rj * rj;
+  double f = (r - sl) / sl * strength * rj2 / (ri2 + rj2);
+  for (int a = 0; a < ndim; ++a) acc[3 * i + a] += d[a] * f;
+}
+
 void or_collide_jacobi(int32_t n, int32_t ndim, const double* x, const double* y, const double* z,
                        double* vx, double* vy, double* vz, const double* radii, double strength,
                        int32_t iterations) {
@@ -628,29 +694,67 @@ void or_collide_jacobi(int32_t n, int32_t ndim, const double* x, const double* y
   double* V[3] = {vx, vy, vz};
   double* snap = (double*)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
   double* acc = (double*)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+  double rmax = 0;
+  for (int32_t i = 0; i < n; ++i)
+    if (radii[i] > rmax) rmax = radii[i];
+  const int grid = n > 4096 && rmax > 0;
+  CellRef* refs = grid ? (CellRef*)malloc(sizeof(CellRef) * (size_t)n) : NULL;
+  int32_t* part = grid ? (int32_t*)malloc(sizeof(int32_t) * (size_t)n) : NULL;
   for (int32_t k = 0; k < iterations; ++k) {
     for (int32_t i = 0; i < n; ++i)
       for (int a = 0; a < 3; ++a) {
         snap[3 * i + a] = a < ndim ? P[a][i] + V[a][i] : 0;
         acc[3 * i + a] = 0;
       }
-    for (int32_t i = 0; i < n; ++i) {
-      double ri = radii[i], ri2 = ri * ri;
-      for (int32_t j = 0; j < n; ++j) {
-        if (j == i) continue;
-        double rj = radii[j], r = ri + rj;
-        double d[3];
-        for (int a = 0; a < 3; ++a) d[a] = snap[3 * i + a] - snap[3 * j + a];
-        double l = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        if (l >= r * r || l == 0) continue;
-        double s = sqrt(l), rj2 = rj * rj;
-        double f = (r - s) / s * strength * rj2 / (ri2 + rj2);
-        for (int a = 0; a < ndim; ++a) acc[3 * i + a] += d[a] * f;
+    if (!grid) {
+      for (int32_t i = 0; i < n; ++i)
+        for (int32_t j = 0; j < n; ++j)
+          if (j != i) collide_pair(snap, radii, i, j, ndim, strength, acc);
+    } else {
+      const double cell = 2 * rmax;
+      for (int32_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) refs[i].c[a] = (int64_t)floor(snap[3 * i + a] / cell);
+        refs[i].i = i;
       }
+      qsort(refs, (size_t)n, sizeof(CellRef), cellref_cmp);
+      uint64_t slots = 1;
+      while (slots < 2ull * (uint64_t)n) slots <<= 1;
+      CellSlot* tab = (CellSlot*)malloc(sizeof(CellSlot) * slots);
+      for (uint64_t k = 0; k < slots; ++k) tab[k].first = -1;
+      for (int32_t t = 0; t < n; ++t) {
+        if (t > 0 && refs[t].c[0] == refs[t - 1].c[0] && refs[t].c[1] == refs[t - 1].c[1] &&
+            refs[t].c[2] == refs[t - 1].c[2])
+          continue;
+        uint64_t k = cell_hash(refs[t].c) & (slots - 1);
+        while (tab[k].first >= 0) k = (k + 1) & (slots - 1);
+        for (int a = 0; a < 3; ++a) tab[k].c[a] = refs[t].c[a];
+        tab[k].first = t;
+      }
+      for (int32_t i = 0; i < n; ++i) {
+        int64_t h[3];
+        for (int a = 0; a < 3; ++a) h[a] = (int64_t)floor(snap[3 * i + a] / cell);
+        int32_t np = 0;
+        for (int dz = -(ndim > 2); dz <= (ndim > 2); ++dz)
+          for (int dy = -(ndim > 1); dy <= (ndim > 1); ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+              int64_t c[3] = {h[0] + dx, h[1] + dy, h[2] + dz};
+              int32_t t = cell_find(tab, slots - 1, c);
+              if (t < 0) continue;
+              for (; t < n; ++t) {
+                if (refs[t].c[0] != c[0] || refs[t].c[1] != c[1] || refs[t].c[2] != c[2]) break;
+                if (refs[t].i != i && collide_hit(snap, radii, i, refs[t].i)) part[np++] = refs[t].i;
+              }
+            }
+        qsort(part, (size_t)np, sizeof(int32_t), int32_cmp);
+        for (int32_t q = 0; q < np; ++q) collide_pair(snap, radii, i, part[q], ndim, strength, acc);
+      }
+      free(tab);
     }
     for (int32_t i = 0; i < n; ++i)
       for (int a = 0; a < ndim; ++a) V[a][i] += acc[3 * i + a];
   }
+  free(refs);
+  free(part);
   free(snap);
   free(acc);
 }

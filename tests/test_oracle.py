@@ -202,3 +202,34 @@ def test_layout_size_matches_published_3d(theta, want):
 def test_layout_size_matches_published_2d():
     """docs/differences.md:37: 2D CPU 552 (c = 1 in 2D, so theta does not shape it)."""
     assert abs(layout_rms(1000, 0.9, 2) - 552) < 1.0
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_collide_jacobi_cell_list_matches_brute_force(dims):
+    """Above 4096 nodes the oracle's Jacobi collide finds partners through a cell list; it
+    must give the all-pairs sum (here against numpy's brute force, summed in another order)."""
+    n = 5000
+    rng = np.random.default_rng(4)
+    pos = np.zeros((n, 3))
+    pos[:, :dims] = (rng.random((n, dims)) - 0.5) * 4 * n ** (1 / dims)
+    vel = np.zeros((n, 3))
+    vel[:, :dims] = rng.standard_normal((n, dims)) * 0.5
+    radii = np.exp(np.log(2) + 0.5 * rng.standard_normal(n))
+    x, y, z = (np.ascontiguousarray(pos[:, a]) for a in range(3))
+    vx, vy, vz = (np.ascontiguousarray(vel[:, a]) for a in range(3))
+    O.lib().or_collide_jacobi(n, dims, O.P(x), O.P(y), O.P(z), O.P(vx), O.P(vy), O.P(vz),
+                              O.P(np.ascontiguousarray(radii)), 0.8, 1)
+    got = np.stack([vx, vy, vz], 1) - vel
+    snap = pos + vel
+    want = np.zeros((n, 3))
+    for lo in range(0, n, 500):
+        d = snap[lo:lo + 500, None, :] - snap[None, :, :]
+        l = (d * d).sum(-1)
+        ri = radii[lo:lo + 500, None]
+        r = ri + radii[None, :]
+        hit = (l < r * r) & (l > 0)
+        s = np.sqrt(np.where(hit, l, 1.0))
+        f = np.where(hit, (r - s) / s * 0.8 * radii[None, :] ** 2 / (ri ** 2 + radii[None, :] ** 2), 0.0)
+        want[lo:lo + 500, :dims] = (d[:, :, :dims] * f[:, :, None]).sum(1)
+    assert np.count_nonzero(np.any(want != 0, 1)) > n // 4
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
