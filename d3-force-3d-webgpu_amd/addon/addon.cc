@@ -186,6 +186,31 @@ napi_value UpdatePins(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// One force descriptor from its JS object (arrays are borrowed for the duration of a call).
+void parse_desc(napi_env env, napi_value o, fg_force_desc& d) {
+  memset(&d, 0, sizeof(d));
+  d.type = (int32_t)num(env, prop(env, o, "type"));
+  d.id = (uint64_t)num(env, prop(env, o, "id"));
+  d.version = (uint64_t)num(env, prop(env, o, "version"));
+  d.iterations = (int32_t)num(env, prop(env, o, "iterations"), 1);
+  d.x = num(env, prop(env, o, "x"));
+  d.y = num(env, prop(env, o, "y"));
+  d.z = num(env, prop(env, o, "z"));
+  d.strength = num(env, prop(env, o, "strength"), 1.0);
+  d.theta2 = num(env, prop(env, o, "theta2"), 0.81);
+  d.distance_min2 = num(env, prop(env, o, "distanceMin2"), 1.0);
+  d.distance_max2 = num(env, prop(env, o, "distanceMax2"), 1e308 * 10);
+  d.axis = (int32_t)num(env, prop(env, o, "axis"));
+  size_t k;
+  d.links = typed<uint32_t>(env, prop(env, o, "links"), napi_uint32_array, &k);
+  d.count = (uint32_t)(k / 2);
+  d.strengths = typed<double>(env, prop(env, o, "strengths"), napi_float64_array, &k);
+  d.radii = typed<double>(env, prop(env, o, "radii"), napi_float64_array, &k);
+  d.distances = typed<double>(env, prop(env, o, "distances"), napi_float64_array, &k);
+  d.bias = typed<double>(env, prop(env, o, "bias"), napi_float64_array, &k);
+  d.targets = typed<double>(env, prop(env, o, "targets"), napi_float64_array, &k);
+}
+
 // setForces(handle, [desc...]): desc = {type, id, version, iterations, x, y, z, strength,
 // theta2, distanceMin2, distanceMax2, axis, links (Uint32Array), strengths, radii,
 // distances, bias, targets (Float64Array)}; arrays may be null when the version is unchanged.
@@ -200,28 +225,7 @@ napi_value SetForces(napi_env env, napi_callback_info info) {
   for (uint32_t i = 0; i < len; ++i) {
     napi_value o;
     napi_get_element(env, argv[1], i, &o);
-    fg_force_desc& d = descs[i];
-    memset(&d, 0, sizeof(d));
-    d.type = (int32_t)num(env, prop(env, o, "type"));
-    d.id = (uint64_t)num(env, prop(env, o, "id"));
-    d.version = (uint64_t)num(env, prop(env, o, "version"));
-    d.iterations = (int32_t)num(env, prop(env, o, "iterations"), 1);
-    d.x = num(env, prop(env, o, "x"));
-    d.y = num(env, prop(env, o, "y"));
-    d.z = num(env, prop(env, o, "z"));
-    d.strength = num(env, prop(env, o, "strength"), 1.0);
-    d.theta2 = num(env, prop(env, o, "theta2"), 0.81);
-    d.distance_min2 = num(env, prop(env, o, "distanceMin2"), 1.0);
-    d.distance_max2 = num(env, prop(env, o, "distanceMax2"), 1e308 * 10);
-    d.axis = (int32_t)num(env, prop(env, o, "axis"));
-    size_t k;
-    d.links = typed<uint32_t>(env, prop(env, o, "links"), napi_uint32_array, &k);
-    d.count = (uint32_t)(k / 2);
-    d.strengths = typed<double>(env, prop(env, o, "strengths"), napi_float64_array, &k);
-    d.radii = typed<double>(env, prop(env, o, "radii"), napi_float64_array, &k);
-    d.distances = typed<double>(env, prop(env, o, "distances"), napi_float64_array, &k);
-    d.bias = typed<double>(env, prop(env, o, "bias"), napi_float64_array, &k);
-    d.targets = typed<double>(env, prop(env, o, "targets"), napi_float64_array, &k);
+    parse_desc(env, o, descs[i]);
   }
   int rc = fg_set_forces(c, len, descs.data());
   if (rc != FG_OK) return throw_status(env, c, rc, "setForces");
@@ -400,6 +404,359 @@ napi_value Stats(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// ---- batches and serving -------------------------------------------------------------
+
+// A job from {n, nDim, pos, vel?, fix?, fixMask?, forces: [desc...], alpha, alphaMin,
+// alphaDecay, alphaTarget, velocityDecay, seed, maxTicks}; `descs` holds the forces.
+void parse_job(napi_env env, napi_value o, fg_job_desc& j, std::vector<fg_force_desc>& descs) {
+  memset(&j, 0, sizeof(j));
+  j.n = (uint32_t)num(env, prop(env, o, "n"));
+  j.ndim = (int32_t)num(env, prop(env, o, "nDim"), 2);
+  size_t k;
+  j.pos = typed<double>(env, prop(env, o, "pos"), napi_float64_array, &k);
+  j.vel = typed<double>(env, prop(env, o, "vel"), napi_float64_array, &k);
+  j.fix = typed<double>(env, prop(env, o, "fix"), napi_float64_array, &k);
+  j.fixmask = typed<uint8_t>(env, prop(env, o, "fixMask"), napi_uint8_array, &k);
+  napi_value fs = prop(env, o, "forces");
+  uint32_t len = 0;
+  napi_get_array_length(env, fs, &len);
+  descs.resize(len);
+  for (uint32_t i = 0; i < len; ++i) {
+    napi_value f;
+    napi_get_element(env, fs, i, &f);
+    parse_desc(env, f, descs[i]);
+  }
+  j.nforces = len;
+  j.forces = descs.data();
+  j.alpha = num(env, prop(env, o, "alpha"), 1.0);
+  j.alpha_min = num(env, prop(env, o, "alphaMin"), 0.001);
+  j.alpha_decay = num(env, prop(env, o, "alphaDecay"), 1.0 - 0.9772372209558107);
+  j.alpha_target = num(env, prop(env, o, "alphaTarget"), 0.0);
+  j.velocity_decay = num(env, prop(env, o, "velocityDecay"), 0.4);
+  j.seed = (uint32_t)num(env, prop(env, o, "seed"));
+  j.max_ticks = (uint32_t)num(env, prop(env, o, "maxTicks"));
+}
+
+struct BatchBox {
+  fg_batch* b;
+};
+struct ServerBox {
+  fg_server* s;
+};
+
+void finalize_batch(napi_env, void* data, void*) {
+  BatchBox* b = (BatchBox*)data;
+  if (b && b->b) fg_batch_destroy(b->b);
+  delete b;
+}
+void finalize_server(napi_env, void* data, void*) {
+  ServerBox* b = (ServerBox*)data;
+  if (b && b->s) fg_serve_destroy(b->s);
+  delete b;
+}
+
+fg_batch* batch_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !((BatchBox*)p)->b) {
+    napi_throw_error(env, nullptr, "d3fgpu: not a live batch handle");
+    return nullptr;
+  }
+  return ((BatchBox*)p)->b;
+}
+fg_server* server_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !((ServerBox*)p)->s) {
+    napi_throw_error(env, nullptr, "d3fgpu: not a live server handle");
+    return nullptr;
+  }
+  return ((ServerBox*)p)->s;
+}
+
+napi_value batch_fail(napi_env env, fg_batch* b, int rc, const char* what) {
+  std::string msg = std::string("d3fgpu ") + what + " failed (" + std::to_string(rc) + ")";
+  if (b) msg += ": " + std::string(fg_batch_last_error(b));
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+// batchCreate(device, budgetBytes) -> handle
+napi_value BatchCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  fg_batch* b = nullptr;
+  int rc = fg_batch_create(&b, (int)num(env, argv[0]), (uint64_t)num(env, argv[1]));
+  if (rc != FG_OK) return batch_fail(env, nullptr, rc, "batchCreate");
+  napi_value r;
+  NAPI_CALL(env, napi_create_external(env, new BatchBox{b}, finalize_batch, nullptr, &r));
+  return r;
+}
+
+napi_value BatchDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  void* p = nullptr;
+  if (napi_get_value_external(env, argv[0], &p) != napi_ok || !p) return nullptr;
+  BatchBox* b = (BatchBox*)p;
+  if (b->b) fg_batch_destroy(b->b);
+  b->b = nullptr;
+  return nullptr;
+}
+
+// batchSubmit(handle, job) -> id (a number); null when the budget has no room now
+napi_value BatchSubmit(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  fg_batch* b = batch_of(env, argv[0]);
+  if (!b) return nullptr;
+  fg_job_desc j;
+  std::vector<fg_force_desc> descs;
+  parse_job(env, argv[1], j, descs);
+  uint64_t id = 0;
+  int rc = fg_batch_submit(b, &j, &id);
+  napi_value r;
+  if (rc == FG_ERR_NOMEM) {
+    napi_get_null(env, &r);
+    return r;
+  }
+  if (rc != FG_OK) return batch_fail(env, b, rc, "batchSubmit");
+  NAPI_CALL(env, napi_create_double(env, (double)id, &r));
+  return r;
+}
+
+napi_value BatchStep(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  fg_batch* b = batch_of(env, argv[0]);
+  if (!b) return nullptr;
+  int rc = fg_batch_step(b, (uint32_t)num(env, argv[1], 1));
+  if (rc != FG_OK) return batch_fail(env, b, rc, "batchStep");
+  return nullptr;
+}
+
+// batchJobs(handle) -> [{id, done, ticks, alpha}]
+napi_value BatchJobs(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  fg_batch* b = batch_of(env, argv[0]);
+  if (!b) return nullptr;
+  std::vector<fg_job_info> jobs(4096);
+  uint32_t count = 0;
+  int rc = fg_batch_jobs(b, jobs.data(), (uint32_t)jobs.size(), &count);
+  if (rc != FG_OK) return batch_fail(env, b, rc, "batchJobs");
+  napi_value arr, o, v;
+  NAPI_CALL(env, napi_create_array_with_length(env, count, &arr));
+  for (uint32_t i = 0; i < count && i < jobs.size(); ++i) {
+    napi_create_object(env, &o);
+    napi_create_double(env, (double)jobs[i].id, &v);
+    napi_set_named_property(env, o, "id", v);
+    napi_get_boolean(env, jobs[i].status == FG_JOB_DONE, &v);
+    napi_set_named_property(env, o, "done", v);
+    napi_create_uint32(env, jobs[i].ticks, &v);
+    napi_set_named_property(env, o, "ticks", v);
+    napi_create_double(env, jobs[i].alpha, &v);
+    napi_set_named_property(env, o, "alpha", v);
+    napi_set_element(env, arr, i, o);
+  }
+  return arr;
+}
+
+// batchRead(handle, id, Float64Array pos, Float64Array vel)
+napi_value BatchRead(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  fg_batch* b = batch_of(env, argv[0]);
+  if (!b) return nullptr;
+  size_t k;
+  int rc = fg_batch_read(b, (uint64_t)num(env, argv[1]), typed<double>(env, argv[2], napi_float64_array, &k),
+                         typed<double>(env, argv[3], napi_float64_array, &k));
+  if (rc != FG_OK) return batch_fail(env, b, rc, "batchRead");
+  return nullptr;
+}
+
+napi_value BatchRelease(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  fg_batch* b = batch_of(env, argv[0]);
+  if (!b) return nullptr;
+  int rc = fg_batch_release(b, (uint64_t)num(env, argv[1]));
+  if (rc != FG_OK) return batch_fail(env, b, rc, "batchRelease");
+  return nullptr;
+}
+
+// batchAutotune(handle, cachePath?) -> {radixBits, walkWaves, walkStack}
+napi_value BatchAutotune(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  fg_batch* b = batch_of(env, argv[0]);
+  if (!b) return nullptr;
+  std::string path;
+  if (!is_nullish(env, argv[1])) {
+    size_t len = 0;
+    napi_get_value_string_utf8(env, argv[1], nullptr, 0, &len);
+    path.resize(len + 1);
+    napi_get_value_string_utf8(env, argv[1], &path[0], len + 1, &len);
+    path.resize(len);
+  }
+  fg_batch_tune t;
+  int rc = fg_batch_autotune(b, path.empty() ? nullptr : path.c_str(), &t);
+  if (rc != FG_OK) return batch_fail(env, b, rc, "batchAutotune");
+  napi_value o, v;
+  NAPI_CALL(env, napi_create_object(env, &o));
+  napi_create_int32(env, t.radix_bits, &v);
+  napi_set_named_property(env, o, "radixBits", v);
+  napi_create_int32(env, t.walk_waves, &v);
+  napi_set_named_property(env, o, "walkWaves", v);
+  napi_create_int32(env, t.walk_stack, &v);
+  napi_set_named_property(env, o, "walkStack", v);
+  return o;
+}
+
+// serveCreate(device, {budget, ticksPerRound}) -> handle
+napi_value ServeCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  fg_serve_config cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  if (!is_nullish(env, argv[1])) {
+    cfg.hbm_budget = (uint64_t)num(env, prop(env, argv[1], "budget"));
+    cfg.ticks_per_round = (uint32_t)num(env, prop(env, argv[1], "ticksPerRound"));
+  }
+  fg_server* s = nullptr;
+  int rc = fg_serve_create(&s, (int)num(env, argv[0]), &cfg);
+  if (rc != FG_OK) return throw_status(env, nullptr, rc, "serveCreate");
+  napi_value r;
+  NAPI_CALL(env, napi_create_external(env, new ServerBox{s}, finalize_server, nullptr, &r));
+  return r;
+}
+
+napi_value ServeDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  void* p = nullptr;
+  if (napi_get_value_external(env, argv[0], &p) != napi_ok || !p) return nullptr;
+  ServerBox* b = (ServerBox*)p;
+  if (b->s) fg_serve_destroy(b->s);
+  b->s = nullptr;
+  return nullptr;
+}
+
+// serveSubmit(handle, job) -> id
+napi_value ServeSubmit(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  fg_server* s = server_of(env, argv[0]);
+  if (!s) return nullptr;
+  fg_job_desc j;
+  std::vector<fg_force_desc> descs;
+  parse_job(env, argv[1], j, descs);
+  uint64_t id = 0;
+  int rc = fg_serve_submit(s, &j, &id);
+  if (rc != FG_OK) {
+    std::string msg = std::string("d3fgpu serveSubmit failed (") + std::to_string(rc) + "): " + fg_serve_last_error(s);
+    napi_throw_error(env, nullptr, msg.c_str());
+    return nullptr;
+  }
+  napi_value r;
+  NAPI_CALL(env, napi_create_double(env, (double)id, &r));
+  return r;
+}
+
+struct ServeWait {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  napi_ref pos_ref = nullptr, vel_ref = nullptr;
+  fg_server* s = nullptr;
+  uint64_t id = 0;
+  double timeout = 0;
+  double *pos = nullptr, *vel = nullptr;
+  fg_serve_result r{};
+  int rc = FG_OK;
+  std::string err;
+};
+
+void serve_execute(napi_env, void* data) {
+  ServeWait* w = (ServeWait*)data;
+  w->rc = fg_serve_wait(w->s, w->id, w->timeout, w->pos, w->vel, &w->r);
+  if (w->rc != FG_OK) w->err = fg_serve_last_error(w->s);
+}
+
+void serve_complete(napi_env env, napi_status status, void* data) {
+  ServeWait* w = (ServeWait*)data;
+  if (status == napi_ok && w->rc == FG_OK) {
+    napi_value o, v;
+    napi_create_object(env, &o);
+    napi_create_uint32(env, w->r.ticks, &v);
+    napi_set_named_property(env, o, "ticks", v);
+    napi_create_double(env, w->r.submitted_s, &v);
+    napi_set_named_property(env, o, "submitted", v);
+    napi_create_double(env, w->r.admitted_s, &v);
+    napi_set_named_property(env, o, "admitted", v);
+    napi_create_double(env, w->r.finished_s, &v);
+    napi_set_named_property(env, o, "finished", v);
+    napi_resolve_deferred(env, w->deferred, o);
+  } else {
+    std::string msg = std::string("d3fgpu layout failed: ") + w->err;
+    napi_value m, e;
+    napi_create_string_utf8(env, msg.c_str(), NAPI_AUTO_LENGTH, &m);
+    napi_create_error(env, nullptr, m, &e);
+    napi_reject_deferred(env, w->deferred, e);
+  }
+  if (w->pos_ref) napi_delete_reference(env, w->pos_ref);
+  if (w->vel_ref) napi_delete_reference(env, w->vel_ref);
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+// serveWaitAsync(handle, id, timeoutSeconds, Float64Array pos, Float64Array vel) ->
+// Promise<{ticks, submitted, admitted, finished}>; the arrays receive the final state
+napi_value ServeWaitAsync(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  get_args(env, info, 5, argv);
+  fg_server* s = server_of(env, argv[0]);
+  if (!s) return nullptr;
+  ServeWait* w = new ServeWait();
+  w->s = s;
+  w->id = (uint64_t)num(env, argv[1]);
+  w->timeout = num(env, argv[2], 600.0);
+  size_t k;
+  w->pos = typed<double>(env, argv[3], napi_float64_array, &k);
+  w->vel = typed<double>(env, argv[4], napi_float64_array, &k);
+  if (w->pos) napi_create_reference(env, argv[3], 1, &w->pos_ref);
+  if (w->vel) napi_create_reference(env, argv[4], 1, &w->vel_ref);
+  napi_value promise, rname;
+  NAPI_CALL(env, napi_create_promise(env, &w->deferred, &promise));
+  NAPI_CALL(env, napi_create_string_utf8(env, "d3fgpu.serveWait", NAPI_AUTO_LENGTH, &rname));
+  NAPI_CALL(env, napi_create_async_work(env, nullptr, rname, serve_execute, serve_complete, w, &w->work));
+  NAPI_CALL(env, napi_queue_async_work(env, w->work));
+  return promise;
+}
+
+// serveStats(handle) -> {submitted, finished, failed, queued, running, p50, p99, layoutsPerSecond}
+napi_value ServeStats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  fg_server* s = server_of(env, argv[0]);
+  if (!s) return nullptr;
+  fg_serve_stats st;
+  int rc = fg_serve_get_stats(s, &st);
+  if (rc != FG_OK) return throw_status(env, nullptr, rc, "serveStats");
+  napi_value o, v;
+  NAPI_CALL(env, napi_create_object(env, &o));
+  const struct {
+    const char* name;
+    double value;
+  } fields[] = {{"submitted", (double)st.submitted}, {"finished", (double)st.finished},
+                {"failed", (double)st.failed},       {"queued", (double)st.queued},
+                {"running", (double)st.running},     {"peakRunning", (double)st.peak_running},
+                {"p50", st.p50_latency_s},          {"p99", st.p99_latency_s},
+                {"layoutsPerSecond", st.layouts_per_s}};
+  for (const auto& f : fields) {
+    napi_create_double(env, f.value, &v);
+    napi_set_named_property(env, o, f.name, v);
+  }
+  return o;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   napi_property_descriptor props[] = {
       {"deviceCount", nullptr, DeviceCount, nullptr, nullptr, nullptr, napi_default, nullptr},
@@ -418,6 +775,19 @@ napi_value Init(napi_env env, napi_value exports) {
       {"idleAsync", nullptr, IdleAsync, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"buffers", nullptr, Buffers, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"stats", nullptr, Stats, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"batchCreate", nullptr, BatchCreate, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"batchDestroy", nullptr, BatchDestroy, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"batchSubmit", nullptr, BatchSubmit, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"batchStep", nullptr, BatchStep, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"batchJobs", nullptr, BatchJobs, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"batchRead", nullptr, BatchRead, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"batchRelease", nullptr, BatchRelease, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"batchAutotune", nullptr, BatchAutotune, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"serveCreate", nullptr, ServeCreate, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"serveDestroy", nullptr, ServeDestroy, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"serveSubmit", nullptr, ServeSubmit, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"serveWaitAsync", nullptr, ServeWaitAsync, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"serveStats", nullptr, ServeStats, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);
   return exports;

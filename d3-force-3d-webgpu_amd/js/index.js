@@ -3,6 +3,8 @@
 export {forceCenter, forceCollide, forceLink, forceManyBody, forceRadial, forceX, forceY, forceZ} from "./forces.js";
 export {default as forceSimulationGPU, isBackendAvailable} from "./simulation.js";
 export {isBackendAvailable as isWebGPUAvailable} from "./simulation.js";
+// Beyond the reference: many layouts per GPU (a batch, and a serving loop).
+export {createLayoutBatch, createLayoutServer} from "./batch.js";
 
 // Support probe with the shape of the reference's checkWebGPUSupport (src/gpu/device.js).
 import {isBackendAvailable} from "./simulation.js";

@@ -88,4 +88,20 @@ it("does not pretend to run forces on the CPU", function() {
   assert.throws(() => d3.forceManyBody()(1), /HIP backend/);
 });
 
+it("prepares batch jobs from d3 forces, in the batch's force order", async function() {
+  assert.strictEqual(typeof d3.createLayoutBatch, "function");
+  assert.strictEqual(typeof d3.createLayoutServer, "function");
+  const {prepareJob} = await import("../../d3-force-3d-webgpu_amd/js/batch.js");
+  const nodes = [{}, {}, {fx: 3}];
+  const links = [{source: 0, target: 1}, {source: 1, target: 2}];
+  const job = prepareJob(nodes, 3, [d3.forceLink(links), d3.forceManyBody(), d3.forceCenter()]);
+  assert.strictEqual(job.n, 3);
+  assert.deepStrictEqual(job.forces.map((f) => f.type), [3, 4, 1]);
+  assert.deepStrictEqual(Array.from(job.forces[0].links), [0, 1, 1, 2]);
+  assert.strictEqual(job.fix[6], 3);                 // the pin travels as fix + mask
+  assert.strictEqual(job.fixMask[2], 1);
+  assert.strictEqual(job.alphaDecay, 1 - Math.pow(0.001, 1 / 300));
+  assert.throws(() => prepareJob([{}, {}], 2, [d3.forceCenter(), d3.forceManyBody()]), /order/);
+});
+
 run();

@@ -168,6 +168,35 @@ it("rejects a force it cannot run", async function() {
   sim.destroy();
 });
 
+it("lays out graphs through a server and a batch, the same either way", async function() {
+  function job(n, seed) {
+    const nodes = [];
+    for (let i = 0; i < n; ++i) nodes.push({});
+    const links = graph(n).map((l) => ({source: l.source, target: (l.target + seed) % l.source}));
+    return [nodes, [d3.forceLink(links), d3.forceManyBody(), d3.forceCollide(3), d3.forceCenter()]];
+  }
+  const server = d3.createLayoutServer({ticksPerRound: 7});
+  const served = await Promise.all([0, 1, 2].map((k) => server.layout(...job(300 + 200 * k, k + 1).slice(0, 1), 3,
+                                                                      job(300 + 200 * k, k + 1)[1])));
+  const stats = server.stats();
+  assert.strictEqual(stats.finished, 3);
+  assert.ok(stats.p99 >= stats.p50 && stats.p50 > 0);
+  server.destroy();
+  for (let k = 0; k < 3; ++k) {
+    const batch = d3.createLayoutBatch();
+    const [nodes, forces] = job(300 + 200 * k, k + 1);
+    const id = batch.add(nodes, 3, forces);
+    while (!batch.done()) batch.step(25);
+    batch.land(id, true);
+    batch.destroy();
+    assert.ok(served[k].ticks >= 300);
+    for (let i = 0; i < nodes.length; ++i) {
+      assert.strictEqual(served[k].nodes[i].x, nodes[i].x);
+      assert.strictEqual(served[k].nodes[i].z, nodes[i].z);
+    }
+  }
+});
+
 it("destroy() stops it for good", async function() {
   const sim = d3.forceSimulationGPU(makeNodes(5, 2), 2).stop().force("charge", d3.forceManyBody());
   await sim.tickAsync(1);
