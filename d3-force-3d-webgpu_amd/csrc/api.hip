@@ -183,15 +183,14 @@ void zero3(hipStream_t s, DevBuf* b, uint32_t n) {
 }
 
 void build_links(fg_ctx* c, Pass& p, const fg_force_desc& d) {
-  std::vector<uint32_t> row;
-  std::vector<LinkEntry> ent;
-  link_csr(c->n, d, row, ent);
-  p.row.alloc(sizeof(uint32_t) * row.size());
-  p.ent.alloc(sizeof(LinkEntry) * ent.size());
-  FG_HIP(hipMemcpyAsync(p.row.p, row.data(), sizeof(uint32_t) * row.size(), hipMemcpyHostToDevice, c->stream));
-  FG_HIP(hipMemcpyAsync(p.ent.p, ent.data(), sizeof(LinkEntry) * ent.size(), hipMemcpyHostToDevice, c->stream));
-  FG_HIP(hipStreamSynchronize(c->stream));
-  p.m = d.count;
+  const uint32_t n = c->n, m = d.count;
+  if (m && !d.links) throw ArgError(FG_ERR_INVALID, "link: links array is null");
+  p.row.alloc(sizeof(uint32_t) * ((size_t)n + 1));
+  p.ent.alloc(sizeof(LinkEntry) * (m ? 2 * (size_t)m : 1));
+  LinkUpload up(c->stream, d);
+  link_csr_device(c->stream, n, m, up.links(), up.strengths(), up.distances(), up.bias(), p.row.as<uint32_t>(),
+                  p.ent.as<LinkEntry>());
+  p.m = m;
 }
 
 void update_pass(fg_ctx* c, Pass& p, const fg_force_desc& d) {
@@ -358,33 +357,6 @@ void land(fg_ctx* c, double* pos, double* vel) {
 }
 
 }  // namespace
-
-namespace fg {
-
-void link_csr(uint32_t n, const fg_force_desc& d, std::vector<uint32_t>& row, std::vector<LinkEntry>& ent) {
-  const uint32_t m = d.count;
-  if (m && !d.links) throw ArgError(FG_ERR_INVALID, "link: links array is null");
-  row.assign((size_t)n + 1, 0);
-  for (uint32_t k = 0; k < m; ++k) {
-    uint32_t s = d.links[2 * k], t = d.links[2 * k + 1];
-    if (s >= n || t >= n) throw ArgError(FG_ERR_INVALID, "link: node index out of range");
-    ++row[s + 1];
-    ++row[t + 1];
-  }
-  for (uint32_t i = 0; i < n; ++i) row[i + 1] += row[i];
-  ent.assign(2 * (size_t)m > 0 ? 2 * (size_t)m : 1, LinkEntry{});
-  std::vector<uint32_t> cursor(row.begin(), row.end() - 1);
-  for (uint32_t k = 0; k < m; ++k) {
-    uint32_t s = d.links[2 * k], t = d.links[2 * k + 1];
-    float dist = d.distances ? finite_f32(d.distances[k]) : 30.0f;
-    float str = d.strengths ? finite_f32(d.strengths[k]) : 0.0f;
-    float bias = d.bias ? finite_f32(d.bias[k]) : 0.5f;
-    ent[cursor[s]++] = LinkEntry{t, 2 * k, dist, str, bias};
-    ent[cursor[t]++] = LinkEntry{s, 2 * k + 1, dist, str, bias};
-  }
-}
-
-}  // namespace fg
 
 extern "C" {
 

@@ -240,8 +240,6 @@ void job_image(const fg_job_desc* j, JobImage& im, bool fill) {
     for (int a = 0; a < 3; ++a) im.off_fix[a] = at, at += fb;
     im.off_mask = at, at += align256(n);
   }
-  std::vector<uint32_t> row;
-  std::vector<LinkEntry> ent;
   for (uint32_t k = 0; k < j->nforces; ++k) {
     const fg_force_desc& d = j->forces[k];
     im.fd[force_rank(d)] = &d;
@@ -253,7 +251,7 @@ void job_image(const fg_job_desc* j, JobImage& im, bool fill) {
       case FG_FORCE_LINK:
         im.link = true;
         im.link_iters = d.iterations > 0 ? (uint32_t)d.iterations : 0u;
-        if (fill) link_csr(n, d, row, ent);
+        if (d.count && !d.links) throw ArgError(FG_ERR_INVALID, "link: links array is null");
         im.off_row = at, at += align256(sizeof(uint32_t) * ((size_t)n + 1));
         im.off_ent = at, at += align256(sizeof(LinkEntry) * (d.count ? 2 * (size_t)d.count : 1));
         break;
@@ -302,9 +300,7 @@ void job_image(const fg_job_desc* j, JobImage& im, bool fill) {
       case FG_FORCE_MANYBODY:
         for (uint32_t i = 0; i < n; ++i) F(im.off_mb)[i] = d.strengths ? finite_f32(d.strengths[i]) : 0.0f;
         break;
-      case FG_FORCE_LINK:
-        memcpy(im.bytes.data() + im.off_row, row.data(), sizeof(uint32_t) * row.size());
-        memcpy(im.bytes.data() + im.off_ent, ent.data(), sizeof(LinkEntry) * ent.size());
+      case FG_FORCE_LINK:  // built on the device once the image is resident (link_csr_device)
         break;
       case FG_FORCE_COLLIDE:
         for (uint32_t i = 0; i < n; ++i) F(im.off_radius)[i] = d.radii ? finite_f32(d.radii[i]) : 0.0f;
@@ -774,6 +770,12 @@ int fg_batch_submit(fg_batch* b, const fg_job_desc* job, uint64_t* id) {
     FG_HIP(hipMalloc(&j->mem, im.bytes.size()));
     try {
       FG_HIP(hipMemcpyAsync(j->mem, im.bytes.data(), im.bytes.size(), hipMemcpyHostToDevice, b->stream));
+      if (im.link) {
+        const fg_force_desc& d = *im.fd[0];
+        LinkUpload up(b->stream, d);
+        link_csr_device(b->stream, job->n, d.count, up.links(), up.strengths(), up.distances(), up.bias(),
+                        (uint32_t*)((uint8_t*)j->mem + im.off_row), (LinkEntry*)((uint8_t*)j->mem + im.off_ent));
+      }
       j->rec = job_record(job, im, (uint8_t*)j->mem);
       FG_HIP(hipMemcpyAsync(b->table + slot, &j->rec, sizeof(BJob), hipMemcpyHostToDevice, b->stream));
       FG_HIP(hipStreamSynchronize(b->stream));

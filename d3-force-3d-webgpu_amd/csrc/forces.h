@@ -52,6 +52,12 @@ struct GridWork {
 // pos/vel/fix: three device arrays each (x, y, z); unused axes may be null.
 void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
                       const uint8_t* mask, const float* const* fix, StepParams step);
+// The CSR of incident links, built on the device: links (2m node indices, device), the
+// per-link FP64 arrays (device, or null for d3's defaults) -> row[n + 1], ent[2m] (device).
+// Synchronizes the stream; throws ArgError on a node index out of range.
+void link_csr_device(hipStream_t s, uint32_t n, uint32_t m, const uint32_t* links, const double* strengths,
+                     const double* distances, const double* bias, uint32_t* row, LinkEntry* ent);
+
 // snap: n float4 of scratch (the snapshot pos + vel the links read)
 void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* const* vin,
                  float* const* vout, const uint32_t* row, const LinkEntry* ent, float4* snap,

@@ -1070,3 +1070,129 @@ void batch_integrate(hipStream_t s, int ndim, const BSpace& sp, BJob* jobs) {
 #undef FG_BY_DIM
 
 }  // namespace fg
+
+// ---- link ingestion on the device -------------------------------------------------------
+//
+// The CSR of incident links built where it is used (passes/link.js:65-101, src/link.js:
+// 56-97): entry e = 2k + side of link k belongs to node links[e]; a stable radix sort of
+// (node, e) groups each node's entries in link order (what a host build in link order
+// gives), row offsets come from a binary search of the sorted nodes, and every entry takes
+// the link's other end and its distance, strength and bias -- d3's defaults (30, 1 / min
+// of the two ends' link counts, count[source] / (count[source] + count[target])) where the
+// descriptor has no array.
+
+namespace fg {
+
+namespace {
+
+__global__ void __launch_bounds__(256) link_check(uint32_t m2, uint32_t n, const uint32_t* __restrict__ links,
+                                                  uint32_t* __restrict__ bad) {
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+  if (e < m2 && links[e] >= n) atomicOr(bad, 1u);
+}
+
+__global__ void __launch_bounds__(256) link_pairs(uint32_t m2, const uint32_t* __restrict__ links,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= m2) return;
+  keys[e] = links[e];
+  vals[e] = e;
+}
+
+// row[i] = the first sorted entry of node i (row[n] = 2m).
+__global__ void __launch_bounds__(256) link_rows(uint32_t n, uint32_t m2, const uint32_t* __restrict__ snodes,
+                                                 uint32_t* __restrict__ row) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i > n) return;
+  uint32_t lo = 0, hi = m2;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (snodes[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  row[i] = lo;
+}
+
+__device__ inline float finite_dev(double v) {
+  if (!(v == v)) return 0.0f;
+  if (v > 3e38) return 3e38f;
+  if (v < -3e38) return -3e38f;
+  return (float)v;
+}
+
+__global__ void __launch_bounds__(256) link_entries(uint32_t m2, const uint32_t* __restrict__ links,
+                                                    const uint32_t* __restrict__ scode,
+                                                    const uint32_t* __restrict__ row,
+                                                    const double* __restrict__ strengths,
+                                                    const double* __restrict__ distances,
+                                                    const double* __restrict__ bias,
+                                                    LinkEntry* __restrict__ ent) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= m2) return;
+  const uint32_t code = scode[s], k = code >> 1;
+  const uint32_t src = links[2 * k], tgt = links[2 * k + 1];
+  const float cs = (float)(row[src + 1] - row[src]), ct = (float)(row[tgt + 1] - row[tgt]);
+  LinkEntry L;
+  L.other = (code & 1u) ? src : tgt;
+  L.code = code;
+  L.distance = distances ? finite_dev(distances[k]) : 30.0f;
+  L.strength = strengths ? finite_dev(strengths[k]) : 1.0f / fminf(cs, ct);
+  L.bias = bias ? finite_dev(bias[k]) : cs / (cs + ct);
+  ent[s] = L;
+}
+
+}  // namespace
+
+void link_csr_device(hipStream_t s, uint32_t n, uint32_t m, const uint32_t* links, const double* strengths,
+                     const double* distances, const double* bias, uint32_t* row, LinkEntry* ent) {
+  const uint32_t m2 = 2 * m;
+  uint32_t* bad = nullptr;
+  uint32_t* keys = nullptr;
+  uint32_t* vals = nullptr;
+  SortWork w;
+  try {
+    FG_HIP(hipMalloc(&bad, sizeof(uint32_t)));
+    FG_HIP(hipMemsetAsync(bad, 0, sizeof(uint32_t), s));
+    if (m2) {
+      hipLaunchKernelGGL(link_check, dim3(ceil_div(m2, 256)), dim3(256), 0, s, m2, n, links, bad);
+      FG_LAUNCH_CHECK();
+    }
+    uint32_t h_bad = 0;
+    FG_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FG_HIP(hipStreamSynchronize(s));
+    if (h_bad) throw ArgError(-1, "link: node index out of range");
+    uint32_t* snodes = nullptr;
+    uint32_t* scode = nullptr;
+    if (m2) {
+      FG_HIP(hipMalloc(&keys, sizeof(uint32_t) * m2));
+      FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m2));
+      hipLaunchKernelGGL(link_pairs, dim3(ceil_div(m2, 256)), dim3(256), 0, s, m2, links, keys, vals);
+      FG_LAUNCH_CHECK();
+      uint32_t bits = 1;
+      while (bits < 32 && (1ull << bits) < (uint64_t)n) ++bits;
+      radix_sort_u32(s, w, 8, keys, vals, m2, 0, bits, &snodes, &scode);
+    }
+    hipLaunchKernelGGL(link_rows, dim3(ceil_div((uint64_t)n + 1, 256)), dim3(256), 0, s, n, m2,
+                       (const uint32_t*)snodes, row);
+    FG_LAUNCH_CHECK();
+    if (m2) {
+      hipLaunchKernelGGL(link_entries, dim3(ceil_div(m2, 256)), dim3(256), 0, s, m2, links,
+                         (const uint32_t*)scode, (const uint32_t*)row, strengths, distances, bias, ent);
+      FG_LAUNCH_CHECK();
+    }
+    FG_HIP(hipStreamSynchronize(s));
+    uint32_t e[4] = {0, 0, 0, 0};
+    if (w.errors) FG_HIP(hipMemcpy(e, w.errors, sizeof(e), hipMemcpyDeviceToHost));
+    if (e[0]) throw HipError(hipErrorUnknown, "link: sort look-back bound reached");
+  } catch (...) {
+    if (bad) (void)hipFree(bad);
+    if (keys) (void)hipFree(keys);
+    if (vals) (void)hipFree(vals);
+    throw;
+  }
+  (void)hipFree(bad);
+  if (keys) (void)hipFree(keys);
+  if (vals) (void)hipFree(vals);
+}
+
+}  // namespace fg

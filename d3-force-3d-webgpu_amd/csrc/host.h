@@ -47,8 +47,28 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
-// The incident links of every node (CSR) from a link descriptor, with d3's defaults for
-// missing arrays (passes/link.js:65-101).  Throws ArgError on a node index out of range.
-void link_csr(uint32_t n, const fg_force_desc& d, std::vector<uint32_t>& row, std::vector<LinkEntry>& ent);
+// A link descriptor's arrays copied to the device for link_csr_device (freed with it).
+struct LinkUpload {
+  DevBuf l, st, di, bi;
+  LinkUpload(hipStream_t s, const fg_force_desc& d) {
+    const size_t m = d.count;
+    if (m) {
+      l.alloc(sizeof(uint32_t) * 2 * m);
+      FG_HIP(hipMemcpyAsync(l.p, d.links, sizeof(uint32_t) * 2 * m, hipMemcpyHostToDevice, s));
+    }
+    copy(s, st, d.strengths, m);
+    copy(s, di, d.distances, m);
+    copy(s, bi, d.bias, m);
+  }
+  static void copy(hipStream_t s, DevBuf& b, const double* src, size_t m) {
+    if (!src || !m) return;
+    b.alloc(sizeof(double) * m);
+    FG_HIP(hipMemcpyAsync(b.p, src, sizeof(double) * m, hipMemcpyHostToDevice, s));
+  }
+  const uint32_t* links() const { return l.as<uint32_t>(); }
+  const double* strengths() const { return st.as<double>(); }
+  const double* distances() const { return di.as<double>(); }
+  const double* bias() const { return bi.as<double>(); }
+};
 
 }  // namespace fg

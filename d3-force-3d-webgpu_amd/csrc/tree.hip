@@ -382,6 +382,26 @@ __device__ inline void collapse_into(uint4 L, const uint4* links, uint32_t* kids
 
 constexpr uint32_t REDUCE_BLOCK = 512;
 
+// d3's children of every binary node (a pass of its own: it depends on the topology only,
+// so it stays off the climb's chain of dependent loads).
+template <int D>
+__global__ void __launch_bounds__(256) tree_collapse(uint32_t n, const uint32_t* __restrict__ seg_of,
+                                                     const uint32_t* __restrict__ seg_job,
+                                                     const BJob* __restrict__ jobs,
+                                                     const uint4* __restrict__ links, NodeRec* rec) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i + 1 >= n) return;
+  if (seg_of) {  // a batch: the last index of each job's segment is not a node
+    const BJob& B = jobs[seg_job[seg_of[i]]];
+    if (i - B.tree_off + 1 >= B.n) return;
+  }
+  uint32_t kids[8];
+  collapse_into<D>(links[i], links, kids);
+  uint4* out = (uint4*)rec[i].kids;
+  out[0] = make_uint4(kids[0], kids[1], kids[2], kids[3]);
+  out[1] = make_uint4(kids[4], kids[5], kids[6], kids[7]);
+}
+
 // One thread per point climbs towards the root; the second of two siblings to reach a
 // node combines them.  Nodes whose points all lie within this workgroup's 512 points are
 // combined through LDS (LDS flag, LDS sums); above them the hand-off is through global
@@ -433,7 +453,6 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
     NodeRec& r = rec[p];
     r.value = (float)value;
     r.depth = L.z;
-    collapse_into<D>(L, links, r.kids);
     const uint32_t parent = L.w;
     bool parent_local = false;
     if (parent != REF_NONE) {
@@ -1127,6 +1146,9 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
                      total, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent, (const uint4*)t.links,
                      (const uint2*)t.span, t.flags, (NodeRec*)t.rec, (NodeSum*)t.sums);
   FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL((tree_collapse<D>), dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)sp.seg_of,
+                     (const uint32_t*)sp.seg_job, (const BJob*)jobs, (const uint4*)t.links, (NodeRec*)t.rec);
+  FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(3);
   launch_bwalk_tuned<D>(st, t, sp, jobs, tune, cus);
   if (timer) timer->end(3);
@@ -1165,6 +1187,9 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   hipLaunchKernelGGL((tree_reduce<D>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st, n,
                      (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
                      (const uint4*)t.links, (const uint2*)t.span, t.flags, t.rec, t.sums);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL((tree_collapse<D>), dim3(blocks), dim3(256), 0, st, n, (const uint32_t*)nullptr,
+                     (const uint32_t*)nullptr, (const BJob*)nullptr, (const uint4*)t.links, t.rec);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(3);
   WalkArgs A;

@@ -212,6 +212,34 @@ def test_link_graph(engine_factory, dims):
     assert close_count(dv, ref, rtol=1e-3) == 0
 
 
+def test_link_defaults_built_on_the_device(engine_factory):
+    """Links given without strengths, distances or bias take d3's defaults (src/link.js:
+    56-97) -- counted on the device from the CSR -- including repeated links; the CSR keeps
+    each node's links in link order."""
+    n, dims = 2500, 3
+    pos = make_nodes_fast(n, dims, 400, seed=3)
+    vel = make_nodes_fast(n, dims, 2, seed=4)
+    src, dst = O.make_graph(n, 7)
+    keep = src != dst
+    src, dst = np.concatenate([src[keep], src[keep][:300]]), np.concatenate([dst[keep], dst[keep][:300]])
+    bias, strengths, distances = O.link_defaults(n, src, dst)
+    links = np.stack([src, dst], 1).astype(np.uint32)
+    f = d3fgpu.Force(d3fgpu.LINK, links=links, iterations=1)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [f], alpha=0.6, vel=vel)
+    ref = oracle_link(pos, vel, dims, src, dst, strengths, np.full(len(src), 30.0), bias, 1, 0.6) - vel
+    err = rel_l2(dv, ref)
+    report("link defaults on the device", err)
+    assert err < 1e-4
+    assert close_count(dv, ref, rtol=1e-3) == 0
+
+
+def test_link_rejects_bad_indices(engine_factory):
+    e = engine_factory()
+    e.set_nodes(make_nodes_fast(10, 2, 10), ndim=2)
+    with pytest.raises(d3fgpu.FgError):
+        e.set_forces([d3fgpu.Force(d3fgpu.LINK, links=np.array([[0, 1], [2, 10]], np.uint32))])
+
+
 # ---- collide ------------------------------------------------------------------------
 
 def oracle_collide(pos, vel, dims, radii, strength, iterations):
