@@ -382,6 +382,58 @@ def bench_serve(jobs, budget_gb=1.0):
             "hbm_budget_gb": budget_gb, "ticks_per_round": 10, "runs": runs}
 
 
+def bench_config_b(n=200_000, steps=20, warmup=5):
+    """200,000 nodes, 3D, link + manyBody + center at theta 0.9 and theta 0 (exact all-pairs):
+    ms/tick, the tree sort's throughput, interactions/s of the walk and of the all-pairs sum."""
+    import d3fgpu
+    src, dst = make_graph(n, 42)
+    bias, strengths, distances = link_defaults(n, src, dst)
+    out = {"nodes": n, "links": len(src)}
+    for theta in (0.9, 0.0):
+        e = d3fgpu.Engine(0)
+        e.set_nodes(phyllotaxis3(n), ndim=3)
+        e.set_forces([d3fgpu.link(src, dst, strengths, distances, bias), d3fgpu.many_body(np.full(n, -30.0), theta=theta),
+                      d3fgpu.center()])
+        a, alpha = alphas(1.0, warmup)
+        e.enqueue(a, 0.6)
+        e.sync()
+        a, alpha = alphas(alpha, steps)
+        t0 = time.perf_counter()
+        e.enqueue(a, 0.6)
+        e.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        k = 5
+        a, alpha = alphas(alpha, k)
+        e.timing_enable(True)
+        e.enqueue(a, 0.6)
+        e.sync()
+        t = e.timing()
+        st = e.stats()
+        e.timing_enable(False)
+        e.close()
+        key = "theta_0.9" if theta else "theta_0"
+        leg = {"ms_per_tick": round(ms, 4), "phases_ms_per_tick": {nm: round(v[0] / k, 4) for nm, v in t.items() if v[1]}}
+        if theta:
+            sort_ms = t["tree_sort"][0] / max(1, t["tree_sort"][1])
+            walk_ms = t["walk"][0] / max(1, t["walk"][1])
+            passes = 8  # 60-bit codes, 8-bit digits
+            leg.update({"sort_passes": passes, "sort_ms": round(sort_ms, 4),
+                        "sort_gb_per_s": round(24 * n * passes / (sort_ms * 1e-3) / 1e9, 1),
+                        "sort_note": "u64 code + u32 index per element, read and written each pass: 24 B/element/pass",
+                        "walk_ms": round(walk_ms, 4), "interactions": int(st.interactions),
+                        "walk_interactions_per_s": round(st.interactions / (walk_ms * 1e-3), 1)})
+        else:
+            pairs_ms = t["pairs"][0] / max(1, t["pairs"][1])
+            inter = n * (n - 1)
+            tf = inter * FLOP_PER_INTERACTION / (pairs_ms * 1e-3) / 1e12
+            leg.update({"pairs_ms": round(pairs_ms, 4), "interactions": inter,
+                        "pairs_interactions_per_s": round(inter / (pairs_ms * 1e-3), 1),
+                        "pairs_tflops": round(tf, 2), "pairs_roofline_frac": round(tf / FP32_PEAK_TFLOPS, 4)})
+        out[key] = leg
+        log(f"config B theta {theta}: {ms:.3f} ms/tick")
+    return out
+
+
 def cpu_baseline(n, ticks=1):
     """The FP64 restatement of d3-force-3d's tick (d3 semantics: sequential link and
     collide, d3-octree Barnes-Hut), one thread, on `ticks` ticks of the same workload."""
@@ -456,6 +508,7 @@ def main():
     ap.add_argument("--no-plain-jobs", action="store_true", help="batched leg without the job-by-job run")
     ap.add_argument("--only-batch", action="store_true", help="run only the batched leg")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs per serving run (0: skip)")
+    ap.add_argument("--no-config-b", action="store_true", help="skip the 200k-node theta 0.9 / theta 0 leg")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and exit")
@@ -516,6 +569,8 @@ def main():
         "walk_lane_utilisation": round(r["lane_visits"] / max(1, 64 * r["opened"]), 4),
         "finite": r["finite"],
     }
+    if not args.no_config_b:
+        out["config_b"] = bench_config_b()
     if args.batch_jobs > 0:
         out["batched"] = bench_batch(args.batch_jobs, not args.no_plain_jobs)
     if args.serve_jobs > 0:

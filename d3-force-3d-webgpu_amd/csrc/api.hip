@@ -52,6 +52,9 @@ struct Pass {
   std::unique_ptr<TreeBuffers> tree;
   std::unique_ptr<GridWork> grid;
   DevBuf center;
+  DevBuf pairs;         // exact many-body: the per-part sums when the sources are split
+  PairsPlan pairs_plan;
+  uint32_t pairs_n = 0;  // node count pairs_plan was made for
 };
 
 // HIP events around the phases of each tick, summed on request.
@@ -251,8 +254,14 @@ void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
       float* vel[3] = {c->V(0), c->V(1), c->V(2)};
       if (!(p.d.theta2 > EXACT_THETA2)) {
         PhaseScope ps(c->T(), FG_PHASE_PAIRS);
+        if (p.pairs_n != n) {
+          p.pairs_plan = pairs_plan(n, device_cus());
+          p.pairs_n = n;
+          const size_t need = pairs_scratch_bytes(p.pairs_plan, n);
+          if (need > p.pairs.bytes) p.pairs.alloc(need);
+        }
         launch_manybody_pairs(c->stream, n, nd, pos, p.a.as<float>(), vel, finite_f32(p.d.distance_min2),
-                              finite_f32(p.d.distance_max2), sp);
+                              finite_f32(p.d.distance_max2), sp, p.pairs_plan, p.pairs.as<float>());
       } else {
         if (!p.tree) p.tree.reset(new TreeBuffers());
         TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2, c->walk_packed_max};

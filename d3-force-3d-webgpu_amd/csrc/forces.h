@@ -69,9 +69,19 @@ void launch_radial(hipStream_t s, uint32_t n, int ndim, float* const* pos, float
                    StepParams step);
 void launch_position(hipStream_t s, uint32_t n, const float* p, float* v, const float* goal,
                      const float* weight, StepParams step);
+// The exact many-body sum (theta 0): `groups` waves of 64 * PAIRS_BODIES bodies, each over
+// `splits` parts of the sources; with more than one part the sums are staged in
+// pairs_scratch_bytes of device memory and added in part order.
+constexpr int PAIRS_BODIES = 4;
+constexpr uint32_t PAIRS_MAX_SPLITS = 16;
+struct PairsPlan {
+  uint32_t groups = 0, splits = 1;
+};
+PairsPlan pairs_plan(uint32_t n, int cus);
+size_t pairs_scratch_bytes(const PairsPlan& p, uint32_t n);
 void launch_manybody_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos,
                            const float* strength, float* const* vel, float dmin2, float dmax2,
-                           StepParams step);
+                           StepParams step, const PairsPlan& plan, float* part);
 void launch_collide_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos,
                           const float* const* vin, float* const* vout, const float* radius,
                           float strength, StepParams step);

@@ -280,67 +280,225 @@ void launch_position(hipStream_t s, uint32_t n, const float* p, float* v, const 
 
 // ---- exact many-body ----------------------------------------------------------------
 
-constexpr int PAIR_TILE = 256;
+constexpr int PAIR_TILE = 256;  // collide's all-pairs block (sources staged in LDS)
 
-template <int D>
-__global__ void __launch_bounds__(PAIR_TILE) manybody_pairs(uint32_t n, const float* __restrict__ x,
-                                                            const float* __restrict__ y,
-                                                            const float* __restrict__ z,
-                                                            const float* __restrict__ strength,
-                                                            float* __restrict__ vx,
-                                                            float* __restrict__ vy,
-                                                            float* __restrict__ vz, float dmin2,
-                                                            float dmax2,
-                                                            StepParams step) {
-  __shared__ float4 tile[PAIR_TILE];
-  const uint32_t i = blockIdx.x * PAIR_TILE + threadIdx.x;
-  const bool live = i < n;
-  const float alpha = step.alpha;
-  const uint32_t sn = step.step, seed = step.seed;
-  float mx = live ? x[i] : 0.f, my = live && D > 1 ? y[i] : 0.f, mz = live && D > 2 ? z[i] : 0.f;
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  for (uint32_t base = 0; base < n; base += PAIR_TILE) {
-    uint32_t j = base + threadIdx.x;
-    tile[threadIdx.x] = j < n ? make_float4(x[j], D > 1 ? y[j] : 0.f, D > 2 ? z[j] : 0.f, strength[j])
-                              : make_float4(0.f, 0.f, 0.f, 0.f);
-    __syncthreads();
-    uint32_t count = n - base < (uint32_t)PAIR_TILE ? n - base : (uint32_t)PAIR_TILE;
-    for (uint32_t k = 0; k < count; ++k) {
-      float4 q = tile[k];
-      float dx = q.x - mx, dy = q.y - my, dz = q.z - mz;
-      float l = dx * dx + dy * dy + dz * dz;
-      if (l >= dmax2 || base + k == i) continue;
-      if (l == 0.f) {
-        coincident(i, base + k, D, sn, seed, dx, dy, dz);
-        l = dx * dx + dy * dy + dz * dz;
-      }
-      if (l < dmin2) l = sqrtf(dmin2 * l);
-      float f = q.w * alpha * __builtin_amdgcn_rcpf(l);
-      ax += dx * f;
-      ay += dy * f;
-      az += dz * f;
+// The exact sum runs one wave per 64*B bodies (B per lane) and streams the sources through
+// registers: each lane loads one source of a 64-source chunk and the wave broadcasts them
+// in turn (readlane), so no LDS and no barrier.  The point force (manyBody.js:99-113) with
+// distanceMin (l -> sqrt(dmin2 l) below it, manyBody.js:95) is
+//     w / l' = w * rsq(l * max(l, dmin2)),
+// one reciprocal square root whatever side of distanceMin the pair falls.  A pair at zero
+// distance (the body itself, or a coincident point) has a zero displacement and adds
+// exactly nothing on the fast path; PAIR_FLOOR keeps its factor finite.  Coincident
+// points take the hash jiggle instead (manyBody.js:100-101 via jiggle, wgsl.js:45-52): the
+// fast path records the smallest distance it saw, and a wave that saw zero outside its own
+// chunks sums again on the careful path, which has the same arithmetic plus the jiggle.
+constexpr float PAIR_FLOOR = 1.17549435e-38f;  // smallest normal float
+constexpr float PAIR_PAD = 3e38f;              // padding source: l = inf, adds exactly zero
+
+template <int D, bool DMAX, bool CAREFUL>
+__device__ inline void pair_accum(float sx, float sy, float sz, float sw, uint32_t j, float mx, float my,
+                                  float mz, uint32_t i, float dmin2, float dmax2, uint32_t sn,
+                                  uint32_t seed, float& ax, float& ay, float& az, float& mn) {
+  float dx = sx - mx, dy = D > 1 ? sy - my : 0.f, dz = D > 2 ? sz - mz : 0.f;
+  float l = dx * dx;
+  if (D > 1) l = __builtin_fmaf(dy, dy, l);
+  if (D > 2) l = __builtin_fmaf(dz, dz, l);
+  const bool near = !DMAX || l < dmax2;  // d3 tests distanceMax before the jiggle
+  if (CAREFUL) {
+    if (l == 0.f && j != i) {
+      coincident(i, j, D, sn, seed, dx, dy, dz);
+      l = dx * dx;
+      if (D > 1) l = __builtin_fmaf(dy, dy, l);
+      if (D > 2) l = __builtin_fmaf(dz, dz, l);
     }
-    __syncthreads();
+  } else {
+    mn = fminf(mn, l);
   }
-  if (live) {
-    vx[i] += ax;
-    if (D > 1) vy[i] += ay;
-    if (D > 2) vz[i] += az;
+  float f = sw * __builtin_amdgcn_rsqf(__builtin_fmaf(l, fmaxf(l, dmin2), PAIR_FLOOR));
+  if (DMAX) f = near ? f : 0.f;
+  ax = __builtin_fmaf(dx, f, ax);
+  if (D > 1) ay = __builtin_fmaf(dy, f, ay);
+  if (D > 2) az = __builtin_fmaf(dz, f, az);
+}
+
+__device__ inline float lane_bcast(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// Chunks [c0, c1) for the lane's B bodies; chunks [own0, own0 + B) hold the wave's own
+// bodies and always take the careful path.  Returns the smallest distance the fast path saw.
+template <int D, int B, bool DMAX, bool CAREFUL>
+__device__ inline float pairs_sum(uint32_t n, uint32_t c0, uint32_t c1, const float* __restrict__ x,
+                                  const float* __restrict__ y, const float* __restrict__ z,
+                                  const float* __restrict__ strength, float alpha, const float (&mx)[B],
+                                  const float (&my)[B], const float (&mz)[B], const uint32_t (&idx)[B],
+                                  uint32_t own0, float dmin2, float dmax2, uint32_t sn, uint32_t seed,
+                                  float (&ax)[B], float (&ay)[B], float (&az)[B]) {
+  const uint32_t lane = threadIdx.x & 63u;
+  float mn = INFINITY;
+  auto load = [&](uint32_t c, float& cx, float& cy, float& cz, float& cw) {
+    const uint32_t j = (c << 6) + lane;
+    const bool live = j < n;
+    cx = live ? x[j] : PAIR_PAD;
+    cy = live && D > 1 ? y[j] : PAIR_PAD;
+    cz = live && D > 2 ? z[j] : PAIR_PAD;
+    cw = live ? strength[j] * alpha : 0.f;  // the product d3's sum forms first
+  };
+  float cx, cy, cz, cw;
+  load(c0, cx, cy, cz, cw);
+  for (uint32_t c = c0; c < c1; ++c) {
+    float nx = 0.f, ny = 0.f, nz = 0.f, nw = 0.f;
+    if (c + 1 < c1) load(c + 1, nx, ny, nz, nw);
+    const uint32_t j0 = c << 6;
+    if (CAREFUL || (c >= own0 && c < own0 + B)) {
+      for (int k = 0; k < 64; ++k) {
+        const float sx = lane_bcast(cx, k), sy = D > 1 ? lane_bcast(cy, k) : 0.f;
+        const float sz = D > 2 ? lane_bcast(cz, k) : 0.f, sw = lane_bcast(cw, k);
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+          pair_accum<D, DMAX, true>(sx, sy, sz, sw, j0 + k, mx[b], my[b], mz[b], idx[b], dmin2, dmax2, sn, seed,
+                                    ax[b], ay[b], az[b], mn);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 64; ++k) {
+        const float sx = lane_bcast(cx, k), sy = D > 1 ? lane_bcast(cy, k) : 0.f;
+        const float sz = D > 2 ? lane_bcast(cz, k) : 0.f, sw = lane_bcast(cw, k);
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+          pair_accum<D, DMAX, false>(sx, sy, sz, sw, j0 + k, mx[b], my[b], mz[b], idx[b], dmin2, dmax2, sn, seed,
+                                     ax[b], ay[b], az[b], mn);
+      }
+    }
+    cx = nx, cy = ny, cz = nz, cw = nw;
+  }
+  return mn;
+}
+
+// One wave per work item: body group g (64*B bodies, B per lane: body b of lane t is
+// (g*B + b)*64 + t) against source part s of `splits` (chunks [s*C/splits, (s+1)*C/splits)
+// of the C = ceil(n/64) chunks); item = s * groups + g.  With one part the sums go straight
+// into the velocities; with several, each part's sums go to part[(s*3 + axis)*n + i] and
+// pairs_combine adds them in part order.  For a given plan every body sums its sources in
+// the same order, so the results do not depend on the launch.
+template <int D, int B, bool DMAX>
+__global__ void __launch_bounds__(64) manybody_pairs(uint32_t n, uint32_t groups, uint32_t splits,
+                                                     const float* __restrict__ x, const float* __restrict__ y,
+                                                     const float* __restrict__ z,
+                                                     const float* __restrict__ strength,
+                                                     float* __restrict__ vx, float* __restrict__ vy,
+                                                     float* __restrict__ vz, float* __restrict__ part,
+                                                     float dmin2, float dmax2, StepParams step) {
+  const uint32_t g = blockIdx.x % groups, sp = blockIdx.x / groups, lane = threadIdx.x;
+  const uint32_t chunks = (n + 63u) >> 6;
+  const uint32_t c0 = (uint32_t)((uint64_t)sp * chunks / splits), c1 = (uint32_t)((uint64_t)(sp + 1) * chunks / splits);
+  const uint32_t own0 = g * B;  // the group's bodies are chunks [own0, own0 + B)
+  float mx[B], my[B], mz[B], ax[B], ay[B], az[B];
+  uint32_t idx[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const uint32_t i = ((own0 + b) << 6) + lane;
+    const bool live = i < n;
+    idx[b] = i;
+    // a padding body sits at NaN: it never reads as coincident and its sums are dropped
+    mx[b] = live ? x[i] : NAN;
+    my[b] = live && D > 1 ? y[i] : NAN;
+    mz[b] = live && D > 2 ? z[i] : NAN;
+    ax[b] = ay[b] = az[b] = 0.f;
+  }
+  const float alpha = step.alpha;
+  const float mn = pairs_sum<D, B, DMAX, false>(n, c0, c1, x, y, z, strength, alpha, mx, my, mz, idx, own0, dmin2,
+                                                dmax2, step.step, step.seed, ax, ay, az);
+  if (__builtin_amdgcn_ballot_w64(mn == 0.f)) {  // a coincident pair outside the own chunks
+#pragma unroll
+    for (int b = 0; b < B; ++b) ax[b] = ay[b] = az[b] = 0.f;
+    pairs_sum<D, B, DMAX, true>(n, c0, c1, x, y, z, strength, alpha, mx, my, mz, idx, own0, dmin2, dmax2, step.step,
+                                step.seed, ax, ay, az);
+  }
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const uint32_t i = idx[b];
+    if (i >= n) continue;
+    if (splits == 1) {
+      vx[i] += ax[b];
+      if (D > 1) vy[i] += ay[b];
+      if (D > 2) vz[i] += az[b];
+    } else {
+      float* out = part + (size_t)sp * 3 * n + i;
+      out[0] = ax[b];
+      if (D > 1) out[n] = ay[b];
+      if (D > 2) out[2 * (size_t)n] = az[b];
+    }
   }
 }
 
-void launch_manybody_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos,
-                           const float* strength, float* const* vel, float dmin2, float dmax2,
-                           StepParams step) {
+template <int D>
+__global__ void pairs_combine(uint32_t n, uint32_t splits, const float* __restrict__ part, float* __restrict__ vx,
+                              float* __restrict__ vy, float* __restrict__ vz) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* v[3] = {vx, vy, vz};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float t = part[(size_t)a * n + i];
+    for (uint32_t s = 1; s < splits; ++s) t += part[((size_t)s * 3 + a) * n + i];
+    v[a][i] += t;
+  }
+}
+
+PairsPlan pairs_plan(uint32_t n, int cus) {
+  PairsPlan p;
+  p.groups = ceil_div(n, 64u * PAIRS_BODIES);
+  p.splits = 1;
+  // each SIMD runs its waves' VALU work in turn: pick the part count whose items spread
+  // most evenly over the SIMDs (items / (SIMDs * ceil(items / SIMDs))), fewest parts on ties
+  const uint64_t simds = 4ull * (uint64_t)(cus > 0 ? cus : 1);
+  const uint32_t chunks = ceil_div(n, 64u);
+  double best = 0;
+  for (uint32_t s = 1; s <= PAIRS_MAX_SPLITS && s <= chunks; ++s) {
+    const uint64_t items = (uint64_t)p.groups * s;
+    const double eff = (double)items / (double)(simds * ((items + simds - 1) / simds));
+    if (eff > best + 0.005) best = eff, p.splits = s;
+  }
+  return p;
+}
+
+size_t pairs_scratch_bytes(const PairsPlan& p, uint32_t n) {
+  return p.splits > 1 ? (size_t)p.splits * 3 * n * sizeof(float) : 0;
+}
+
+void launch_manybody_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* strength,
+                           float* const* vel, float dmin2, float dmax2, StepParams step, const PairsPlan& plan,
+                           float* part) {
   if (n < 2) return;
-  dim3 g(ceil_div(n, PAIR_TILE)), b(PAIR_TILE);
-  if (ndim == 1)
-    hipLaunchKernelGGL(manybody_pairs<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], strength, vel[0], vel[1], vel[2], dmin2, dmax2, step);
-  else if (ndim == 2)
-    hipLaunchKernelGGL(manybody_pairs<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], strength, vel[0], vel[1], vel[2], dmin2, dmax2, step);
-  else
-    hipLaunchKernelGGL(manybody_pairs<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], strength, vel[0], vel[1], vel[2], dmin2, dmax2, step);
+  if (plan.groups != ceil_div(n, 64u * PAIRS_BODIES) || plan.splits < 1 || plan.splits > PAIRS_MAX_SPLITS ||
+      (plan.splits > 1 && !part))
+    throw ArgError(-5, "all-pairs: plan or scratch does not fit the node count");
+  dim3 g(plan.groups * plan.splits), b(64);
+  const bool dmax = dmax2 < 3e38f;  // 3e38 is infinity as the host passes it
+#define FG_PAIRS(D, M)                                                                                  \
+  hipLaunchKernelGGL((manybody_pairs<D, PAIRS_BODIES, M>), g, b, 0, s, n, plan.groups, plan.splits, pos[0], pos[1], \
+                     pos[2], strength, vel[0], vel[1], vel[2], part, dmin2, dmax2, step)
+  if (ndim == 1) {
+    if (dmax) FG_PAIRS(1, true); else FG_PAIRS(1, false);
+  } else if (ndim == 2) {
+    if (dmax) FG_PAIRS(2, true); else FG_PAIRS(2, false);
+  } else {
+    if (dmax) FG_PAIRS(3, true); else FG_PAIRS(3, false);
+  }
+#undef FG_PAIRS
   FG_LAUNCH_CHECK();
+  if (plan.splits > 1) {
+    dim3 cg(ceil_div(n, 256u)), cb(256);
+    if (ndim == 1)
+      hipLaunchKernelGGL(pairs_combine<1>, cg, cb, 0, s, n, plan.splits, part, vel[0], vel[1], vel[2]);
+    else if (ndim == 2)
+      hipLaunchKernelGGL(pairs_combine<2>, cg, cb, 0, s, n, plan.splits, part, vel[0], vel[1], vel[2]);
+    else
+      hipLaunchKernelGGL(pairs_combine<3>, cg, cb, 0, s, n, plan.splits, part, vel[0], vel[1], vel[2]);
+    FG_LAUNCH_CHECK();
+  }
 }
 
 // ---- collide, all pairs -------------------------------------------------------------

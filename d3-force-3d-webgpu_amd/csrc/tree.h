@@ -79,6 +79,9 @@ struct TreeParams {
 // Bytes of walk spill area for `cus` CUs at an LDS stack of `stack` entries.
 size_t spill_bytes_for(int cus, int stack);
 
+// Compute units of the current device.
+int device_cus();
+
 // Build the tree over the current positions and add the many-body force to the velocities.
 void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x,
                    const float* y, const float* z, const float* strength, float* vx, float* vy,

@@ -888,13 +888,6 @@ uint32_t resident_blocks(K kern, int threads, int cus) {
   return (uint32_t)per_cu * (uint32_t)(cus > 0 ? cus : 1);
 }
 
-int device_cus() {
-  int dev = 0, cus = 0;
-  FG_HIP(hipGetDevice(&dev));
-  FG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  return cus;
-}
-
 // Bytes of spill area a persistent walk needs: WALK_DEPTH - STACK entries for each of the
 // at most 32 waves per CU.
 inline size_t spill_bytes(int cus, int stack) {
@@ -1213,6 +1206,13 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
 }
 
 }  // namespace
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  FG_HIP(hipGetDevice(&dev));
+  FG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus;
+}
 
 void TreeBuffers::reserve(uint32_t n) {
   if (n <= cap && keys) return;

@@ -182,6 +182,49 @@ def test_manybody_exact_small(engine_factory, n):
     assert close_count(dv, ref, rtol=1e-3, atol=1e-3) == 0
 
 
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_manybody_exact_many_blocks(engine_factory, dims):
+    """Several blocks of the all-pairs kernel (bodies outside the source tile's block take
+    the fast path), with distanceMin/distanceMax in force."""
+    n = 5000
+    pos = make_nodes_fast(n, dims)
+    s = np.where(np.arange(n) % 5 == 0, 12.0, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [d3fgpu.many_body(s, theta=0, distance_min=2, distance_max=120)],
+                   alpha=0.7)
+    ref = O.manybody_exact(pos, np.zeros_like(pos), s, 0.7, dmin2=4, dmax2=120 ** 2, ndim=dims)
+    report(f"manyBody exact {n} nodes {dims}D", rel_l2(dv, ref))
+    assert rel_l2(dv, ref) < 1e-5
+    # distanceMax is a step: a pair within float rounding of it may fall on either side in
+    # f32 and in f64; nodes with such a pair are left out of the per-node bar
+    edge = np.zeros(n, bool)
+    for lo in range(0, n, 500):
+        l = ((pos[lo:lo + 500, None, :] - pos[None, :, :]) ** 2).sum(-1)
+        edge[lo:lo + 500] = (np.abs(l - 120.0 ** 2) < 2e-6 * 120.0 ** 2).any(1)
+    assert edge.sum() < n // 50
+    assert close_count(dv[~edge], ref[~edge], rtol=1e-4, atol=1e-4) == 0
+
+
+def test_manybody_exact_coincident_across_blocks(engine_factory):
+    """Coincident pairs in different blocks and in one block: each pair adds a push of
+    |strength| * alpha / distanceMin along the hash jiggle, equal and opposite on its two
+    nodes (the FP64 oracle leaves coincident pairs out, so the push is the residual)."""
+    n, alpha = 3000, 0.5
+    pos = make_nodes_fast(n, 3)
+    pairs = [(5, 1705), (100, 2900), (600, 601), (1201, 2410)]
+    for i, j in pairs:
+        pos[j] = pos[i]
+    s = np.full(n, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s, theta=0)], alpha=alpha)
+    ref = O.manybody_exact(pos, np.zeros_like(pos), s, alpha)
+    r = dv - ref
+    hit = np.zeros(n, bool)
+    for i, j in pairs:
+        hit[i] = hit[j] = True
+        np.testing.assert_allclose(np.linalg.norm(r[i]), 30.0 * alpha, rtol=1e-3)
+        np.testing.assert_allclose(r[i], -r[j], rtol=1e-3, atol=1e-3)
+    assert close_count(dv[~hit], ref[~hit]) == 0
+
+
 # ---- link ---------------------------------------------------------------------------
 
 def oracle_link(pos, vel, dims, src, dst, strengths, distances, bias, iterations, alpha):
