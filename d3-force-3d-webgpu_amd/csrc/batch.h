@@ -139,7 +139,6 @@ struct BatchTree {
   uint32_t* leaf_depth = nullptr;
   uint32_t* leaf_parent = nullptr;
   uint4* links = nullptr;
-  uint2* span = nullptr;
   uint32_t* flags = nullptr;
   void* rec = nullptr;   // NodeRec[cap]
   void* sums = nullptr;  // NodeSum[cap]

@@ -55,7 +55,6 @@ struct TreeBuffers {
   uint32_t* leaf_depth = nullptr;
   uint32_t* leaf_parent = nullptr;
   uint4* links = nullptr;       // per binary node: left, right, depth, parent
-  uint2* span = nullptr;        // per binary node: first and last sorted point it covers
   uint32_t* flags = nullptr;
   NodeRec* rec = nullptr;
   NodeSum* sums = nullptr;

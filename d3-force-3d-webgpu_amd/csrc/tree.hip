@@ -269,11 +269,16 @@ __device__ inline void karras(const uint64_t* __restrict__ skeys, int N, int i, 
   last = i < j ? j : i;
 }
 
-// Writes binary node `node` (= off + i) of a tree whose sorted points are [off, off + N).
+// The reduction works on blocks of REDUCE_BLOCK consecutive sorted points (in the index
+// space of all trees); a node is local to a block when all its points lie in it.
+constexpr uint32_t REDUCE_BLOCK = 512;
+constexpr uint32_t PARENT_LOCAL = 0x80000000u;  // in a parent reference: the parent is local
+
+// Writes binary node `node` (= off + i) of a tree whose sorted points are [off, off + N):
+// its links, and itself (with PARENT_LOCAL when local) as the parent of its two children.
 template <int D>
 __device__ inline void karras_write(const uint64_t* __restrict__ skeys, int N, int i, uint32_t off,
-                                    uint4* __restrict__ links, uint2* __restrict__ spans,
-                                    uint32_t* __restrict__ leaf_parent) {
+                                    uint4* __restrict__ links, uint32_t* __restrict__ leaf_parent) {
   int first, last, split, prefix;
   karras<D>(skeys, N, i, first, last, split, prefix);
   const uint32_t node = off + (uint32_t)i, a = off + (uint32_t)split, b = a + 1u;
@@ -282,21 +287,21 @@ __device__ inline void karras_write(const uint64_t* __restrict__ skeys, int N, i
   links[node].x = left;
   links[node].y = right;
   links[node].z = depth_of<D>(prefix);
-  spans[node] = make_uint2(off + (uint32_t)first, off + (uint32_t)last);
-  if (left & REF_POINT) leaf_parent[a] = node;
-  else links[a].w = node;
-  if (right & REF_POINT) leaf_parent[b] = node;
-  else links[b].w = node;
+  const bool local = (off + (uint32_t)first) / REDUCE_BLOCK == (off + (uint32_t)last) / REDUCE_BLOCK;
+  const uint32_t up = node | (local ? PARENT_LOCAL : 0u);
+  if (left & REF_POINT) leaf_parent[a] = up;
+  else links[a].w = up;
+  if (right & REF_POINT) leaf_parent[b] = up;
+  else links[b].w = up;
   if (i == 0) links[node].w = REF_NONE;
 }
 
 template <int D>
 __global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ skeys, uint32_t n,
-                                                  uint4* __restrict__ links, uint2* __restrict__ spans,
-                                                  uint32_t* __restrict__ leaf_parent) {
+                                                  uint4* __restrict__ links, uint32_t* __restrict__ leaf_parent) {
   int i = (int)(blockIdx.x * 256 + threadIdx.x);
   if (i >= (int)n - 1) return;
-  karras_write<D>(skeys, (int)n, i, 0u, links, spans, leaf_parent);
+  karras_write<D>(skeys, (int)n, i, 0u, links, leaf_parent);
 }
 
 // ---- bottom-up reduction + collapse ---------------------------------------------------
@@ -307,22 +312,31 @@ struct NodeAcc {
   double cx, cy, cz;        // centre (0 when the node has no weight)
 };
 
-// Sums of one workgroup's local nodes (node index - first point of the workgroup) in LDS.
+// One workgroup's share of the reduction in LDS: its points and its binary nodes (a local
+// node's index lies within its own points, so both are indexed by index - first point of
+// the workgroup), and the sums of its local nodes.
 struct ReduceLDS {
-  double a[512], w[512], gx[512], gy[512], gz[512], cx[512], cy[512], cz[512];
-  uint32_t flag[512];
+  double a[REDUCE_BLOCK], w[REDUCE_BLOCK], gx[REDUCE_BLOCK], gy[REDUCE_BLOCK], gz[REDUCE_BLOCK];
+  float4 leaf[REDUCE_BLOCK];
+  uint4 link[REDUCE_BLOCK];
+  uint32_t flag[REDUCE_BLOCK];
 };
+
+// A node's |value|-weighted centre from its sums (0 when it has no weight).
+__device__ inline double centre(double g, double w) { return w > 0 ? g / w : 0.0; }
 
 // Adds child k of a node at depth parent_depth to `out`: a point (a d3 child one level
 // below the parent's cell), a deeper node (a d3 child, weighed by |its strength one level
 // below the parent's cell|) or a node in the same cell (its children are the parent's:
-// its sums pass through).  Local children are read from LDS, others coherently.
+// its sums pass through).  A local node's children are local and read from LDS; others
+// are read from global memory, their sums coherently.
 template <int D>
 __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo, bool local,
                                  const ReduceLDS& S, const float4* leaves, const uint4* links,
                                  const NodeRec* rec, const NodeSum* sums, NodeSum& out) {
   if (k & REF_POINT) {
-    float4 q = leaves[k & ~REF_POINT];
+    const uint32_t j = k & ~REF_POINT;
+    const float4 q = local ? S.leaf[j - lo] : leaves[j];
     double sv = (double)q.w;
     out.a += sv * damp<D>((int)parent_depth + 1);
     double w = fabs(sv);
@@ -332,16 +346,23 @@ __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo,
     if (D > 2) out.gz += w * (double)q.z;
     return;
   }
-  const uint32_t kd = links[k].z;
   const uint32_t q = k - lo;
+  const uint32_t kd = local ? S.link[q].z : links[k].z;
   const double ca = local ? S.a[q] : load_coherent(&sums[k].a);
   out.a += ca;
   if (kd > parent_depth) {
     double w = fabs(ca * damp<D>(-(int)(parent_depth + 1)));
     out.w += w;
-    out.gx += w * (local ? S.cx[q] : load_coherent(&rec[k].cx));
-    if (D > 1) out.gy += w * (local ? S.cy[q] : load_coherent(&rec[k].cy));
-    if (D > 2) out.gz += w * (local ? S.cz[q] : load_coherent(&rec[k].cz));
+    if (local) {
+      const double kw = S.w[q];
+      out.gx += w * centre(S.gx[q], kw);
+      if (D > 1) out.gy += w * centre(S.gy[q], kw);
+      if (D > 2) out.gz += w * centre(S.gz[q], kw);
+    } else {
+      out.gx += w * load_coherent(&rec[k].cx);
+      if (D > 1) out.gy += w * load_coherent(&rec[k].cy);
+      if (D > 2) out.gz += w * load_coherent(&rec[k].cz);
+    }
   } else {
     out.w += local ? S.w[q] : load_coherent(&sums[k].w);
     out.gx += local ? S.gx[q] : load_coherent(&sums[k].gx);
@@ -380,8 +401,6 @@ __device__ inline void collapse_into(uint4 L, const uint4* links, uint32_t* kids
   for (; m < 8; ++m) kids[m] = REF_NONE;
 }
 
-constexpr uint32_t REDUCE_BLOCK = 512;
-
 // d3's children of every binary node (a pass of its own: it depends on the topology only,
 // so it stays off the climb's chain of dependent loads).
 template <int D>
@@ -403,28 +422,29 @@ __global__ void __launch_bounds__(256) tree_collapse(uint32_t n, const uint32_t*
 }
 
 // One thread per point climbs towards the root; the second of two siblings to reach a
-// node combines them.  Nodes whose points all lie within this workgroup's 512 points are
-// combined through LDS (LDS flag, LDS sums); above them the hand-off is through global
-// memory, coherent across the device.  Every node's record goes to global memory for the
-// walk; a node whose parent is not local also leaves its sums there, coherently.
+// node combines them.  Nodes whose points all lie within this workgroup's points are
+// combined through LDS (the workgroup's points and nodes are staged there first, and the
+// flags and sums live there); above them the hand-off is through global memory, coherent
+// across the device.  Every node's record goes to global memory for the walk; a node whose
+// parent is not local also leaves its sums there, coherently.
 template <int D>
 __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
                                                             const uint32_t* __restrict__ leaf_parent,
                                                             const uint4* __restrict__ links,
-                                                            const uint2* __restrict__ span,
                                                             uint32_t* flags, NodeRec* rec, NodeSum* sums) {
   __shared__ ReduceLDS S;
   const uint32_t s = blockIdx.x * REDUCE_BLOCK + threadIdx.x;
-  const uint32_t lo = blockIdx.x * REDUCE_BLOCK, hi = lo + REDUCE_BLOCK - 1;
+  const uint32_t lo = blockIdx.x * REDUCE_BLOCK;
   S.flag[threadIdx.x] = 0;
+  if (s < n) {
+    S.leaf[threadIdx.x] = leaves[s];
+    S.link[threadIdx.x] = links[s];  // index s is a node or a tree's last point (never read then)
+  }
   __syncthreads();
   if (s >= n || n < 2) return;
-  uint32_t p = leaf_parent[s];
-  bool local = true;  // whether node p lies within this workgroup's points
-  {
-    uint2 sp = span[p];
-    local = sp.x >= lo && sp.y <= hi;
-  }
+  const uint32_t up = leaf_parent[s];
+  uint32_t p = up & ~PARENT_LOCAL;
+  bool local = (up & PARENT_LOCAL) != 0;  // whether node p lies within this workgroup's points
   for (uint32_t guard = 0; guard < n && p != REF_NONE; ++guard) {
     uint32_t old;
     if (local) {
@@ -435,16 +455,13 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
     if (old == 0) return;  // the sibling is still on its way
     // a local node's children are local; a shared node's children left their sums in
     // global memory (their parent, p, is not local to their writer)
-    const uint4 L = links[p];
+    const uint4 L = local ? S.link[p - lo] : links[p];
     NodeSum sum = {0, 0, 0, 0, 0};
     child_sum<D>(L.x, L.z, lo, local, S, leaves, links, rec, sums, sum);
     child_sum<D>(L.y, L.z, lo, local, S, leaves, links, rec, sums, sum);
-    double cx = 0.0, cy = 0.0, cz = 0.0;
-    if (sum.w > 0) {
-      cx = sum.gx / sum.w;
-      cy = D > 1 ? sum.gy / sum.w : 0.0;
-      cz = D > 2 ? sum.gz / sum.w : 0.0;
-    }
+    const double cx = centre(sum.gx, sum.w);
+    const double cy = D > 1 ? centre(sum.gy, sum.w) : 0.0;
+    const double cz = D > 2 ? centre(sum.gz, sum.w) : 0.0;
     double value = sum.a * damp<D>(-(int)L.z);
     // d3 skips a cell whose strengths add up to exactly 0; sums in another order leave a
     // residue of rounding, so a cancellation to 2^-40 of the cell's weight counts as 0
@@ -453,12 +470,8 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
     NodeRec& r = rec[p];
     r.value = (float)value;
     r.depth = L.z;
-    const uint32_t parent = L.w;
-    bool parent_local = false;
-    if (parent != REF_NONE) {
-      uint2 ps = span[parent];
-      parent_local = ps.x >= lo && ps.y <= hi;
-    }
+    const uint32_t parent = L.w == REF_NONE ? REF_NONE : L.w & ~PARENT_LOCAL;
+    const bool parent_local = L.w != REF_NONE && (L.w & PARENT_LOCAL);
     if (parent_local) {
       const uint32_t q = p - lo;
       S.a[q] = sum.a;
@@ -466,9 +479,6 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
       S.gx[q] = sum.gx;
       S.gy[q] = sum.gy;
       S.gz[q] = sum.gz;
-      S.cx[q] = cx;
-      S.cy[q] = cy;
-      S.cz[q] = cz;
       r.cx = cx;
       r.cy = cy;
       r.cz = cz;
@@ -999,14 +1009,13 @@ __global__ void __launch_bounds__(256) btree_build(uint32_t total, const uint64_
                                                    const uint32_t* __restrict__ seg_of,
                                                    const uint32_t* __restrict__ seg_job,
                                                    const BJob* __restrict__ jobs,
-                                                   uint4* __restrict__ links, uint2* __restrict__ spans,
-                                                   uint32_t* __restrict__ leaf_parent) {
+                                                   uint4* __restrict__ links, uint32_t* __restrict__ leaf_parent) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
   const BJob& B = jobs[seg_job[seg_of[i]]];
   const uint32_t off = B.tree_off, k = i - off;
   if (k + 1 >= B.n) return;
-  karras_write<D>(skeys + off, (int)B.n, (int)k, off, links, spans, leaf_parent);
+  karras_write<D>(skeys + off, (int)B.n, (int)k, off, links, leaf_parent);
 }
 
 // Persistent walk: every wave takes 64-point tiles of any job from one queue until it is
@@ -1133,11 +1142,11 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((btree_build<D>), dim3(blocks), dim3(256), 0, st, total, (const uint64_t*)t.skeys,
                      (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.links,
-                     t.span, t.leaf_parent);
+                     t.leaf_parent);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((tree_reduce<D>), dim3(ceil_div(total, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st,
                      total, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent, (const uint4*)t.links,
-                     (const uint2*)t.span, t.flags, (NodeRec*)t.rec, (NodeSum*)t.sums);
+                     t.flags, (NodeRec*)t.rec, (NodeSum*)t.sums);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((tree_collapse<D>), dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)sp.seg_of,
                      (const uint32_t*)sp.seg_job, (const BJob*)jobs, (const uint4*)t.links, (NodeRec*)t.rec);
@@ -1175,11 +1184,11 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
     return;
   }
   hipLaunchKernelGGL((tree_build<D>), dim3(ceil_div(n - 1, 256)), dim3(256), 0, st,
-                     (const uint64_t*)sk, n, t.links, t.span, t.leaf_parent);
+                     (const uint64_t*)sk, n, t.links, t.leaf_parent);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((tree_reduce<D>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st, n,
                      (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
-                     (const uint4*)t.links, (const uint2*)t.span, t.flags, t.rec, t.sums);
+                     (const uint4*)t.links, t.flags, t.rec, t.sums);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((tree_collapse<D>), dim3(blocks), dim3(256), 0, st, n, (const uint32_t*)nullptr,
                      (const uint32_t*)nullptr, (const BJob*)nullptr, (const uint4*)t.links, t.rec);
@@ -1225,7 +1234,6 @@ void TreeBuffers::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&leaf_depth, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&leaf_parent, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&links, sizeof(uint4) * m));
-  FG_HIP(hipMalloc(&span, sizeof(uint2) * m));
   FG_HIP(hipMalloc(&flags, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
@@ -1252,7 +1260,7 @@ void TreeBuffers::release() {
   if (spill) (void)hipFree(spill);
   spill = nullptr;
   spill_bytes = 0;
-  void* ptrs[] = {keys, vals, leaves, pts, leaf_depth, leaf_parent, links, span, flags, rec, sums, cube, counters};
+  void* ptrs[] = {keys, vals, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, cube, counters};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = nullptr;
@@ -1260,7 +1268,6 @@ void TreeBuffers::release() {
   leaves = pts = nullptr;
   leaf_depth = leaf_parent = flags = counters = nullptr;
   links = nullptr;
-  span = nullptr;
   rec = nullptr;
   sums = nullptr;
   cube = nullptr;
@@ -1301,7 +1308,6 @@ void BatchTree::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&leaf_depth, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&leaf_parent, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&links, sizeof(uint4) * m));
-  FG_HIP(hipMalloc(&span, sizeof(uint2) * m));
   FG_HIP(hipMalloc(&flags, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
@@ -1317,14 +1323,13 @@ void BatchTree::reserve(uint32_t n) {
 }
 
 void BatchTree::release() {
-  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, pts, leaf_depth, leaf_parent, links, span, flags, rec, sums, counters, spill};
+  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, counters, spill};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = codes = skeys = nullptr;
   vals = digit = order = leaf_depth = leaf_parent = flags = counters = nullptr;
   leaves = pts = nullptr;
   links = nullptr;
-  span = nullptr;
   rec = sums = spill = nullptr;
   spill_bytes = 0;
   sort.release();
