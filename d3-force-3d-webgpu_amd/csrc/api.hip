@@ -48,7 +48,7 @@ struct Pass {
   DevBuf snap;          // link: the snapshot pos + vel (float4 per node)
   uint32_t m = 0;
   float rmax = 0;
-  GridClasses grid_classes{0.0f, 0.0f, 0.0f, 0};
+  GridClasses grid_classes{0.0f, 0.0f, 0.0f, 0, 0.0f};
   std::unique_ptr<TreeBuffers> tree;
   std::unique_ptr<GridWork> grid;
   DevBuf center;

@@ -155,7 +155,7 @@ struct BatchGrid {
   uint32_t cap = 0, bits = 0;
   uint32_t* keys = nullptr;
   uint32_t* vals = nullptr;
-  uint2* runs = nullptr;     // [2^bits] each slot's [first, end) of the sorted points
+  GridRun* runs = nullptr;   // [2^bits]
   float4* sorted = nullptr;   // snapshots (pos + vel, radius) in sorted order
   float4* snap = nullptr;     // the same per node
   uint32_t* sseg = nullptr;

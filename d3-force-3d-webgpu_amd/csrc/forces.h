@@ -30,17 +30,23 @@ struct CenterWork {
 struct GridClasses {
   float inv_s, inv_b, r0;
   int32_t two;
+  float rmax;  // the largest |radius| (class 1's bound; class 0's is r0)
 };
 
-// The classes for these radii: two when rmax exceeds four times the 95th percentile radius
-// (then one huge radius cannot make every cell hold every point).
+// The classes for these radii (by |radius|): two when rmax exceeds four times the 95th
+// percentile radius (then one huge radius cannot make every cell hold every point).
 GridClasses grid_classes(const float* radii, uint32_t n);
+
+// A slot of the grid's run table: its [first, end) of the sorted points ((0, 0) when empty).
+struct alignas(8) GridRun {
+  uint32_t first, end;
+};
 
 struct GridWork {
   uint32_t cap = 0, bits = 0;
   uint32_t* keys = nullptr;
   uint32_t* vals = nullptr;
-  uint2* runs = nullptr;     // [2^bits] each slot's [first, end) of the sorted points
+  GridRun* runs = nullptr;   // [2^bits]
   float4* sorted = nullptr;   // snapshots (pos + vel, radius) in sorted order
   float4* snap = nullptr;     // the same per node
   SortWork sort;

@@ -609,8 +609,8 @@ __device__ inline int3 grid_cell(float4 q, float inv) {
                    D > 2 ? (int)floorf(q.z * inv) : 0);
 }
 
-// The grid's class of a radius: 0 (fine grid) up to r0, 1 (coarse grid) above.
-__device__ inline uint32_t grid_class(float r, const GridClasses& gc) { return r > gc.r0 ? 1u : 0u; }
+// The grid's class of a radius: 0 (fine grid) up to |r| = r0, 1 (coarse grid) above.
+__device__ inline uint32_t grid_class(float r, const GridClasses& gc) { return fabsf(r) > gc.r0 ? 1u : 0u; }
 
 // Collide sorted point s (snapshot `me` with its radius, node index `node`) with every
 // point closer than the sum of their radii (passes/collide.js:34-80 on candidates from
@@ -625,14 +625,14 @@ __device__ inline uint32_t grid_class(float r, const GridClasses& gc) { return r
 // The pairs of sorted point s with the points of one cell (class c, coordinates cx, cy, cz)
 // whose run is `run`.
 template <int D, bool SEG>
-__device__ inline void grid_cell_pairs(uint2 run, int cx, int cy, int cz, uint32_t c, float inv,
+__device__ inline void grid_cell_pairs(GridRun run, int cx, int cy, int cz, uint32_t c, float inv,
                                        uint32_t s, float4 me, float ri2, uint32_t node,
                                        const uint32_t* __restrict__ svals,
                                        const float4* __restrict__ sorted,
                                        const uint32_t* __restrict__ sseg, uint32_t seg,
                                        uint32_t node_off, const GridClasses& gc, float strength,
                                        uint32_t sn, uint32_t seed, float& ax, float& ay, float& az) {
-  for (uint32_t t = run.x; t < run.y; ++t) {
+  for (uint32_t t = run.first; t < run.end; ++t) {
     if (t == s) continue;
     if (SEG && sseg[t] != seg) continue;
     const float4 q = sorted[t];
@@ -655,12 +655,28 @@ __device__ inline void grid_cell_pairs(uint2 run, int cx, int cy, int cz, uint32
   }
 }
 
+// Whether cell (cx, cy, cz) of a grid may hold a partner of a point (u: the point in cell
+// units): the distance from the point to the cell's box, in cell units, against `reach`
+// (the point's |radius| plus the largest |radius| of the cell's class, in cell units).  The
+// margin covers the rounding of the cell units and of the pair test, so a cell skipped here
+// holds no point the pair test would take.
+template <int D>
+__device__ inline bool grid_cell_near(float ux, float uy, float uz, int cx, int cy, int cz, float reach,
+                                      float slack) {
+  const float gx = fmaxf(fmaxf((float)cx - ux, ux - (float)(cx + 1)), 0.f);
+  const float gy = D > 1 ? fmaxf(fmaxf((float)cy - uy, uy - (float)(cy + 1)), 0.f) : 0.f;
+  const float gz = D > 2 ? fmaxf(fmaxf((float)cz - uz, uz - (float)(cz + 1)), 0.f) : 0.f;
+  const float t = reach * 1.0001f + slack;
+  return gx * gx + gy * gy + gz * gz <= t * t;
+}
+
 // Collide sorted point s (snapshot `me` with its radius, node index `node`) with every
 // point closer than the sum of their radii (passes/collide.js:34-80 on candidates from
 // passes/collideGrid.js).  Class-0 cells are 2 r0 wide, class-1 cells 2 rmax; a partner
-// of class c lies within me.w + (c ? rmax : r0), so the cells to visit in class c are
-// those within ceil((me.w + r_c) / cell_c) of the point's cell -- one ring except for a
-// coarse point among fine cells.  Classes, cells and, within a cell, sorted points are
+// of class c lies within |me.w| + (c ? rmax : r0), so the cells to visit in class c are
+// those within ceil((|me.w| + r_c) / cell_c) of the point's cell -- one ring except for a
+// coarse point among fine cells -- and of those only the cells whose box comes that close
+// to the point are looked up at all.  Classes, cells and, within a cell, sorted points are
 // visited in a fixed order; a point found under a cell's slot counts only if it is of that
 // class and lies in that cell, so slots shared by cells -- or, in a batch (SEG), by other
 // jobs -- change the work but never the sum.  runs[slot] is the slot's [first, end) of the
@@ -669,13 +685,13 @@ __device__ inline void grid_cell_pairs(uint2 run, int cx, int cy, int cz, uint32
 template <int D, bool SEG>
 __device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node,
                                           const uint32_t* __restrict__ svals,
-                                          const uint2* __restrict__ runs,
+                                          const GridRun* __restrict__ runs,
                                           const float4* __restrict__ sorted,
                                           const uint32_t* __restrict__ sseg, uint32_t seg,
                                           uint32_t node_off, const GridClasses& gc, uint32_t mask,
                                           float strength, uint32_t sn, uint32_t seed, float& ax,
                                           float& ay, float& az) {
-  const float ri2 = me.w * me.w;
+  const float ri2 = me.w * me.w, ri = fabsf(me.w);
   const uint32_t mine = grid_class(me.w, gc);
   const uint32_t classes = gc.two ? 2u : 1u;
   constexpr int RY = D > 1 ? 1 : 0, RZ = D > 2 ? 1 : 0;
@@ -683,19 +699,24 @@ __device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node,
   for (uint32_t c = 0; c < classes; ++c) {
     const float inv = c ? gc.inv_b : gc.inv_s;
     const int3 h = grid_cell<D>(me, inv);
+    const float ux = me.x * inv, uy = D > 1 ? me.y * inv : 0.f, uz = D > 2 ? me.z * inv : 0.f;
+    const float slack = 1e-4f + 1e-6f * (fabsf(ux) + fabsf(uy) + fabsf(uz));
+    const float reach = (ri + (c ? gc.rmax : gc.r0)) * inv;
     const uint32_t tag = 2u * seg + c;
-    const int R = (c == 0 && mine == 1) ? (int)ceilf((me.w + gc.r0) * inv * 1.0001f) : 1;
+    const int R = (c == 0 && mine == 1) ? (int)ceilf((ri + gc.r0) * inv * 1.0001f) : 1;
     if (R == 1) {
       for (int dz = -RZ; dz <= RZ; ++dz) {
-        uint2 rr[SLAB];
+        GridRun rr[SLAB];
+#pragma unroll
+        for (int k = 0; k < SLAB; ++k) {
+          const int cx = h.x + k % 3 - 1, cy = h.y + k / 3 - RY, cz = h.z + dz;
+          rr[k] = grid_cell_near<D>(ux, uy, uz, cx, cy, cz, reach, slack) ? runs[grid_slot<D>(cx, cy, cz, tag, mask)]
+                                                                        : GridRun{0u, 0u};
+        }
 #pragma unroll
         for (int k = 0; k < SLAB; ++k)
-          rr[k] = runs[grid_slot<D>(h.x + k % 3 - 1, h.y + k / 3 - RY, h.z + dz, tag, mask)];
-#pragma unroll
-        for (int k = 0; k < SLAB; ++k)
-          grid_cell_pairs<D, SEG>(rr[k], h.x + k % 3 - 1, h.y + k / 3 - RY, h.z + dz, c, inv, s, me, ri2,
-                                  node, svals, sorted, sseg, seg, node_off, gc, strength, sn, seed, ax,
-                                  ay, az);
+          grid_cell_pairs<D, SEG>(rr[k], h.x + k % 3 - 1, h.y + k / 3 - RY, h.z + dz, c, inv, s, me, ri2, node,
+                                  svals, sorted, sseg, seg, node_off, gc, strength, sn, seed, ax, ay, az);
       }
     } else {
       const int Ry = D > 1 ? R : 0, Rz = D > 2 ? R : 0;
@@ -703,6 +724,7 @@ __device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node,
         for (int dy = -Ry; dy <= Ry; ++dy)
           for (int dx = -R; dx <= R; ++dx) {
             const int cx = h.x + dx, cy = h.y + dy, cz = h.z + dz;
+            if (!grid_cell_near<D>(ux, uy, uz, cx, cy, cz, reach, slack)) continue;
             grid_cell_pairs<D, SEG>(runs[grid_slot<D>(cx, cy, cz, tag, mask)], cx, cy, cz, c, inv, s, me,
                                     ri2, node, svals, sorted, sseg, seg, node_off, gc, strength, sn,
                                     seed, ax, ay, az);
@@ -712,9 +734,9 @@ __device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node,
 }
 
 // Empties the run table (a kernel, so that a captured tick can do it).
-__global__ void __launch_bounds__(256) grid_clear(uint32_t slots, uint2* __restrict__ runs) {
+__global__ void __launch_bounds__(256) grid_clear(uint32_t slots, GridRun* __restrict__ runs) {
   for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < slots; k += gridDim.x * 256)
-    runs[k] = make_uint2(0u, 0u);
+    runs[k] = GridRun{0u, 0u};
 }
 
 // A point's slot: its cell in its own class's grid.
@@ -748,19 +770,19 @@ __global__ void __launch_bounds__(256) grid_keys(uint32_t n, const float* __rest
 __global__ void __launch_bounds__(256) grid_begin(uint32_t n, const uint32_t* __restrict__ skeys,
                                                   const uint32_t* __restrict__ svals,
                                                   const float4* __restrict__ snap,
-                                                  uint2* __restrict__ runs,
+                                                  GridRun* __restrict__ runs,
                                                   float4* __restrict__ sorted) {
   uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
   const uint32_t key = skeys[s];
-  if (s == 0 || skeys[s - 1] != key) runs[key].x = s;
-  if (s + 1 == n || skeys[s + 1] != key) runs[key].y = s + 1;
+  if (s == 0 || skeys[s - 1] != key) runs[key].first = s;
+  if (s + 1 == n || skeys[s + 1] != key) runs[key].end = s + 1;
   sorted[s] = snap[svals[s]];
 }
 
 template <int D>
 __global__ void __launch_bounds__(256) grid_query(uint32_t n, const uint32_t* __restrict__ svals,
-                                                  const uint2* __restrict__ runs,
+                                                  const GridRun* __restrict__ runs,
                                                   const float4* __restrict__ sorted,
                                                   float* __restrict__ vx, float* __restrict__ vy,
                                                   float* __restrict__ vz, GridClasses gc,
@@ -787,7 +809,7 @@ void GridWork::reserve(uint32_t n_) {
   bits = b;
   FG_HIP(hipMalloc(&keys, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
-  FG_HIP(hipMalloc(&runs, sizeof(uint2) << bits));
+  FG_HIP(hipMalloc(&runs, sizeof(GridRun) << bits));
   FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
   FG_HIP(hipMalloc(&snap, sizeof(float4) * m));
   sort.reserve(m, sizeof(uint32_t));
@@ -823,7 +845,7 @@ void grid_run(hipStream_t s, GridWork& g, uint32_t n, float* const* pos, float* 
   hipLaunchKernelGGL(grid_begin, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
                      (const float4*)g.snap, g.runs, g.sorted);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(grid_query<D>, gr, b, 0, s, n, (const uint32_t*)sv, (const uint2*)g.runs,
+  hipLaunchKernelGGL(grid_query<D>, gr, b, 0, s, n, (const uint32_t*)sv, (const GridRun*)g.runs,
                      (const float4*)g.sorted, vel[0], vel[1], vel[2], gc, mask, strength, step);
   FG_LAUNCH_CHECK();
 }
@@ -840,15 +862,17 @@ void launch_collide_grid(hipStream_t s, GridWork& g, uint32_t n, int ndim, float
 }
 
 GridClasses grid_classes(const float* radii, uint32_t n) {
-  GridClasses gc{0.0f, 0.0f, 0.0f, 0};
+  GridClasses gc{0.0f, 0.0f, 0.0f, 0, 0.0f};
   float rmax = 0;
-  for (uint32_t i = 0; i < n; ++i) rmax = std::max(rmax, radii[i]);
+  for (uint32_t i = 0; i < n; ++i) rmax = std::max(rmax, std::fabs(radii[i]));
   if (!(rmax > 0)) return gc;
   gc.inv_b = gc.inv_s = 1.0f / (2.0f * rmax);
   gc.r0 = rmax;
+  gc.rmax = rmax;
   static const bool one_class = getenv("D3FGPU_GRID_ONE_CLASS") != nullptr;  // experiment override
   if (n >= 1024 && !one_class) {
-    std::vector<float> r(radii, radii + n);
+    std::vector<float> r(n);
+    for (uint32_t i = 0; i < n; ++i) r[i] = std::fabs(radii[i]);
     const size_t k = (size_t)(0.95 * (double)(n - 1));
     std::nth_element(r.begin(), r.begin() + k, r.end());
     const float r95 = r[k];
@@ -958,21 +982,21 @@ __global__ void __launch_bounds__(256) bgrid_begin(uint32_t total, const uint32_
                                                    const uint32_t* __restrict__ svals,
                                                    const uint32_t* __restrict__ seg_of,
                                                    const float4* __restrict__ snap,
-                                                   uint2* __restrict__ runs,
+                                                   GridRun* __restrict__ runs,
                                                    float4* __restrict__ sorted,
                                                    uint32_t* __restrict__ sseg) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= total) return;
   const uint32_t key = skeys[s], i = svals[s];
-  if (s == 0 || skeys[s - 1] != key) runs[key].x = s;
-  if (s + 1 == total || skeys[s + 1] != key) runs[key].y = s + 1;
+  if (s == 0 || skeys[s - 1] != key) runs[key].first = s;
+  if (s + 1 == total || skeys[s + 1] != key) runs[key].end = s + 1;
   sorted[s] = snap[i];
   sseg[s] = seg_of[i];
 }
 
 template <int D>
 __global__ void __launch_bounds__(256) bgrid_query(uint32_t total, const uint32_t* __restrict__ svals,
-                                                   const uint2* __restrict__ runs,
+                                                   const GridRun* __restrict__ runs,
                                                    const float4* __restrict__ sorted,
                                                    const uint32_t* __restrict__ sseg,
                                                    const uint32_t* __restrict__ seg_job,
@@ -1117,7 +1141,7 @@ void BatchGrid::reserve(uint32_t n_) {
   bits = b;
   FG_HIP(hipMalloc(&keys, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
-  FG_HIP(hipMalloc(&runs, sizeof(uint2) << bits));
+  FG_HIP(hipMalloc(&runs, sizeof(GridRun) << bits));
   FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
   FG_HIP(hipMalloc(&snap, sizeof(float4) * m));
   FG_HIP(hipMalloc(&sseg, sizeof(uint32_t) * m));
@@ -1191,7 +1215,7 @@ void batch_collide(hipStream_t s, int ndim, BatchGrid& g, const BSpace& sp, BJob
                      (const uint32_t*)sp.seg_of, (const float4*)g.snap, g.runs, g.sorted, g.sseg);
   FG_LAUNCH_CHECK();
   FG_BY_DIM(ndim, bgrid_query, gr, b, 0, s, n, (const uint32_t*)sv,
-            (const uint2*)g.runs, (const float4*)g.sorted, (const uint32_t*)g.sseg,
+            (const GridRun*)g.runs, (const float4*)g.sorted, (const uint32_t*)g.sseg,
             (const uint32_t*)sp.seg_job, (const BJob*)jobs, mask, iteration);
 }
 
