@@ -539,6 +539,12 @@ __device__ inline float dampf(uint32_t k) {
   return ldexpf(odd, D == 3 ? -(int)(k >> 1) : (int)(k >> 1));
 }
 
+// 1 / l' for d3's squared distance l, where below distanceMin l' = sqrt(dmin2 l)
+// (manyBody.js:95): rsq(l * max(l, dmin2)) either way, one reciprocal square root.
+__device__ inline float inv_dist2(float l, float dmin2) {
+  return __builtin_amdgcn_rsqf(l * fmaxf(l, dmin2));
+}
+
 // The force of one node (its cell chain from depth `top`, whose first width squared is
 // w2top) on one body, with d3's rules (barnesHut.js:719-746, manyBody.js:73-99).
 // `cell` is the node's index within its tree (the jiggle's stand-in index is n + cell).
@@ -567,8 +573,7 @@ __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self
     const float4 j = coincident_far(self_node, n + cell, D, step_no, seed);
     ex = j.x, ey = j.y, ez = j.z, fl = j.w;
   }
-  if (fl < dmin2f) fl = sqrtf(dmin2f * fl);
-  const float f = h.value * dampf<D>(h.depth - Dd) * alpha * __builtin_amdgcn_rcpf(fl);
+  const float f = h.value * dampf<D>(h.depth - Dd) * alpha * inv_dist2(fl, dmin2f);
   fx += ex * f;
   if (D > 1) fy += ey * f;
   if (D > 2) fz += ez * f;
@@ -590,8 +595,7 @@ __device__ inline void point_force(float mx, float my, float mz, uint32_t self_n
     const float4 jj = coincident_far(self_node, order[j] - off, D, step_no, seed);
     dx = jj.x, dy = jj.y, dz = jj.z, l = jj.w;
   }
-  if (l < dmin2f) l = sqrtf(dmin2f * l);
-  const float f = q.w * alpha * __builtin_amdgcn_rcpf(l);
+  const float f = q.w * alpha * inv_dist2(l, dmin2f);
   fx += dx * f;
   if (D > 1) fy += dy * f;
   if (D > 2) fz += dz * f;
