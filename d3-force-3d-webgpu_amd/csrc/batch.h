@@ -151,15 +151,17 @@ struct BatchTree {
   ~BatchTree() { release(); }
 };
 
-struct BatchGrid {
+struct BatchGrid {  // as GridWork, over a batch's grid space
   uint32_t cap = 0, bits = 0;
   uint32_t* keys = nullptr;
-  uint32_t* vals = nullptr;
-  GridRun* runs = nullptr;   // [2^bits]
-  float4* sorted = nullptr;   // snapshots (pos + vel, radius) in sorted order
-  float4* snap = nullptr;     // the same per node
-  uint32_t* sseg = nullptr;
-  SortWork sort;
+  uint32_t* rank = nullptr;
+  uint32_t* start = nullptr;    // [2^bits + 1]
+  uint32_t* partial = nullptr;
+  uint32_t* arrived = nullptr;
+  uint32_t* svals = nullptr;
+  float4* sorted = nullptr;
+  float4* snap = nullptr;
+  uint32_t* sseg = nullptr;     // segment of each sorted point
   void reserve(uint32_t n);
   void release();
   ~BatchGrid() { release(); }

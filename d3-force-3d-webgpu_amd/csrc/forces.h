@@ -37,19 +37,26 @@ struct GridClasses {
 // percentile radius (then one huge radius cannot make every cell hold every point).
 GridClasses grid_classes(const float* radii, uint32_t n);
 
-// A slot of the grid's run table: its [first, end) of the sorted points ((0, 0) when empty).
-struct alignas(8) GridRun {
+// A slot's run of sorted points [first, end) as the query reads it from the grid's start
+// table (start[slot], start[slot + 1]).
+struct GridRun {
   uint32_t first, end;
 };
 
+// The grid of one collide iteration, built by counting: the slot of each node, its rank
+// among the nodes that reached the slot first (atomic, so in no fixed order), the slot
+// counts turned by an exclusive scan into each slot's first sorted position, and the
+// nodes placed by rank and then reordered by node index inside each slot.
 struct GridWork {
   uint32_t cap = 0, bits = 0;
-  uint32_t* keys = nullptr;
-  uint32_t* vals = nullptr;
-  GridRun* runs = nullptr;   // [2^bits]
-  float4* sorted = nullptr;   // snapshots (pos + vel, radius) in sorted order
-  float4* snap = nullptr;     // the same per node
-  SortWork sort;
+  uint32_t* keys = nullptr;     // slot of each node
+  uint32_t* rank = nullptr;     // arrival rank of each node in its slot
+  uint32_t* start = nullptr;    // [2^bits + 1]: counts per slot, then first sorted position
+  uint32_t* partial = nullptr;  // scan scratch
+  uint32_t* arrived = nullptr;  // nodes by slot in arrival order
+  uint32_t* svals = nullptr;    // nodes by slot, ascending node index within a slot
+  float4* sorted = nullptr;     // snapshots (pos + vel, radius) in sorted order
+  float4* snap = nullptr;       // the same per node
   void reserve(uint32_t n);
   void release();
   ~GridWork() { release(); }

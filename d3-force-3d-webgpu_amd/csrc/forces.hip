@@ -679,13 +679,13 @@ __device__ inline bool grid_cell_near(float ux, float uy, float uz, int cx, int 
 // to the point are looked up at all.  Classes, cells and, within a cell, sorted points are
 // visited in a fixed order; a point found under a cell's slot counts only if it is of that
 // class and lies in that cell, so slots shared by cells -- or, in a batch (SEG), by other
-// jobs -- change the work but never the sum.  runs[slot] is the slot's [first, end) of the
-// sorted points ((0, 0) when empty); svals[t] - node_off is sorted point t's node.  In one
+// jobs -- change the work but never the sum.  The slot's run of sorted points is
+// [start[slot], start[slot + 1]); svals[t] - node_off is sorted point t's node.  In one
 // ring, the runs of a z-slab of cells are read together so that their latencies overlap.
 template <int D, bool SEG>
 __device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node,
                                           const uint32_t* __restrict__ svals,
-                                          const GridRun* __restrict__ runs,
+                                          const uint32_t* __restrict__ start,
                                           const float4* __restrict__ sorted,
                                           const uint32_t* __restrict__ sseg, uint32_t seg,
                                           uint32_t node_off, const GridClasses& gc, uint32_t mask,
@@ -710,8 +710,11 @@ __device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node,
 #pragma unroll
         for (int k = 0; k < SLAB; ++k) {
           const int cx = h.x + k % 3 - 1, cy = h.y + k / 3 - RY, cz = h.z + dz;
-          rr[k] = grid_cell_near<D>(ux, uy, uz, cx, cy, cz, reach, slack) ? runs[grid_slot<D>(cx, cy, cz, tag, mask)]
-                                                                        : GridRun{0u, 0u};
+          rr[k] = GridRun{0u, 0u};
+          if (grid_cell_near<D>(ux, uy, uz, cx, cy, cz, reach, slack)) {
+            const uint32_t slot = grid_slot<D>(cx, cy, cz, tag, mask);
+            rr[k] = GridRun{start[slot], start[slot + 1]};
+          }
         }
 #pragma unroll
         for (int k = 0; k < SLAB; ++k)
@@ -725,18 +728,17 @@ __device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node,
           for (int dx = -R; dx <= R; ++dx) {
             const int cx = h.x + dx, cy = h.y + dy, cz = h.z + dz;
             if (!grid_cell_near<D>(ux, uy, uz, cx, cy, cz, reach, slack)) continue;
-            grid_cell_pairs<D, SEG>(runs[grid_slot<D>(cx, cy, cz, tag, mask)], cx, cy, cz, c, inv, s, me,
-                                    ri2, node, svals, sorted, sseg, seg, node_off, gc, strength, sn,
-                                    seed, ax, ay, az);
+            const uint32_t slot = grid_slot<D>(cx, cy, cz, tag, mask);
+            grid_cell_pairs<D, SEG>(GridRun{start[slot], start[slot + 1]}, cx, cy, cz, c, inv, s, me, ri2, node,
+                                    svals, sorted, sseg, seg, node_off, gc, strength, sn, seed, ax, ay, az);
           }
     }
   }
 }
 
-// Empties the run table (a kernel, so that a captured tick can do it).
-__global__ void __launch_bounds__(256) grid_clear(uint32_t slots, GridRun* __restrict__ runs) {
-  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < slots; k += gridDim.x * 256)
-    runs[k] = GridRun{0u, 0u};
+// Zeroes the slot counts (a kernel, so that a captured tick can do it).
+__global__ void __launch_bounds__(256) grid_clear(uint32_t words, uint32_t* __restrict__ start) {
+  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < words; k += gridDim.x * 256) start[k] = 0u;
 }
 
 // A point's slot: its cell in its own class's grid.
@@ -747,6 +749,7 @@ __device__ inline uint32_t grid_key(float4 q, uint32_t seg, const GridClasses& g
   return grid_slot<D>(cell.x, cell.y, cell.z, 2u * seg + c, mask);
 }
 
+// Snapshot, slot and arrival rank of every node (counting the slots).
 template <int D>
 __global__ void __launch_bounds__(256) grid_keys(uint32_t n, const float* __restrict__ x,
                                                  const float* __restrict__ y,
@@ -756,33 +759,51 @@ __global__ void __launch_bounds__(256) grid_keys(uint32_t n, const float* __rest
                                                  const float* __restrict__ vz,
                                                  const float* __restrict__ radius, GridClasses gc,
                                                  uint32_t mask, uint32_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals, float4* __restrict__ snap) {
+                                                 uint32_t* __restrict__ rank, float4* __restrict__ snap,
+                                                 uint32_t* __restrict__ count) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float4 q = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f,
                                radius[i]);
-  keys[i] = grid_key<D>(q, 0u, gc, mask);
-  vals[i] = i;
+  const uint32_t key = grid_key<D>(q, 0u, gc, mask);
+  keys[i] = key;
   snap[i] = q;
+  rank[i] = atomicAdd(&count[key], 1u);
 }
 
-// Run table entries and the snapshots in sorted order (one 16-byte gather per point).
-__global__ void __launch_bounds__(256) grid_begin(uint32_t n, const uint32_t* __restrict__ skeys,
-                                                  const uint32_t* __restrict__ svals,
+// Node i into its slot's run at its arrival rank.
+__global__ void __launch_bounds__(256) grid_scatter(uint32_t n, const uint32_t* __restrict__ keys,
+                                                    const uint32_t* __restrict__ rank,
+                                                    const uint32_t* __restrict__ start,
+                                                    uint32_t* __restrict__ arrived) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  arrived[start[keys[i]] + rank[i]] = i;
+}
+
+// Position t of the arrival order: its node moves to its place by node index within the
+// run (the order a stable sort of the slots gives), and its snapshot follows.  A run of
+// L points costs O(L^2) here, as it does in the query.
+__global__ void __launch_bounds__(256) grid_order(uint32_t n, const uint32_t* __restrict__ keys,
+                                                  const uint32_t* __restrict__ start,
+                                                  const uint32_t* __restrict__ arrived,
                                                   const float4* __restrict__ snap,
-                                                  GridRun* __restrict__ runs,
-                                                  float4* __restrict__ sorted) {
-  uint32_t s = blockIdx.x * 256 + threadIdx.x;
-  if (s >= n) return;
-  const uint32_t key = skeys[s];
-  if (s == 0 || skeys[s - 1] != key) runs[key].first = s;
-  if (s + 1 == n || skeys[s + 1] != key) runs[key].end = s + 1;
-  sorted[s] = snap[svals[s]];
+                                                  uint32_t* __restrict__ svals, float4* __restrict__ sorted,
+                                                  const uint32_t* __restrict__ seg_of, uint32_t* __restrict__ sseg) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t i = arrived[t], key = keys[i];
+  const uint32_t first = start[key], end = start[key + 1];
+  uint32_t r = 0;
+  for (uint32_t u = first; u < end; ++u) r += arrived[u] < i ? 1u : 0u;
+  svals[first + r] = i;
+  sorted[first + r] = snap[i];
+  if (sseg) sseg[first + r] = seg_of[i];
 }
 
 template <int D>
 __global__ void __launch_bounds__(256) grid_query(uint32_t n, const uint32_t* __restrict__ svals,
-                                                  const GridRun* __restrict__ runs,
+                                                  const uint32_t* __restrict__ start,
                                                   const float4* __restrict__ sorted,
                                                   float* __restrict__ vx, float* __restrict__ vy,
                                                   float* __restrict__ vz, GridClasses gc,
@@ -793,7 +814,7 @@ __global__ void __launch_bounds__(256) grid_query(uint32_t n, const uint32_t* __
   const float4 me = sorted[s];
   const uint32_t i = svals[s];
   float ax = 0.f, ay = 0.f, az = 0.f;
-  grid_collide_point<D, false>(s, me, i, svals, runs, sorted, nullptr, 0u, 0u, gc, mask,
+  grid_collide_point<D, false>(s, me, i, svals, start, sorted, nullptr, 0u, 0u, gc, mask,
                                strength, step.step, step.seed, ax, ay, az);
   vx[i] += ax;
   if (D > 1) vy[i] += ay;
@@ -807,45 +828,46 @@ void GridWork::reserve(uint32_t n_) {
   release();
   uint32_t m = n_ > 1 ? n_ : 1;
   bits = b;
+  const uint32_t words = (1u << bits) + 1u;
   FG_HIP(hipMalloc(&keys, sizeof(uint32_t) * m));
-  FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
-  FG_HIP(hipMalloc(&runs, sizeof(GridRun) << bits));
+  FG_HIP(hipMalloc(&rank, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&start, sizeof(uint32_t) * words));
+  FG_HIP(hipMalloc(&partial, sizeof(uint32_t) * scan_partials_size(words)));
+  FG_HIP(hipMalloc(&arrived, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&svals, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
   FG_HIP(hipMalloc(&snap, sizeof(float4) * m));
-  sort.reserve(m, sizeof(uint32_t));
   cap = n_;
 }
 
 void GridWork::release() {
-  if (keys) (void)hipFree(keys);
-  if (vals) (void)hipFree(vals);
-  if (runs) (void)hipFree(runs);
-  if (sorted) (void)hipFree(sorted);
-  if (snap) (void)hipFree(snap);
-  keys = vals = nullptr;
-  runs = nullptr;
+  void* ptrs[] = {keys, rank, start, partial, arrived, svals, sorted, snap};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  keys = rank = start = partial = arrived = svals = nullptr;
   sorted = snap = nullptr;
-  sorted = nullptr;
-  sort.release();
   cap = 0;
 }
 
+// The grid by counting: clear the counts, slot + rank per node, scan, scatter by rank,
+// reorder runs by node index (with the snapshots), then the query.
 template <int D>
 void grid_run(hipStream_t s, GridWork& g, uint32_t n, float* const* pos, float* const* vel,
-              const float* radius, const GridClasses& gc, uint32_t mask, float strength, int radix_bits,
-              StepParams step) {
+              const float* radius, const GridClasses& gc, uint32_t mask, float strength, StepParams step) {
   dim3 gr(ceil_div(n, 256)), b(256);
+  const uint32_t words = (1u << g.bits) + 1u;
+  hipLaunchKernelGGL(grid_clear, dim3(std::min(ceil_div(words, 256), 2048u)), b, 0, s, words, g.start);
   hipLaunchKernelGGL(grid_keys<D>, gr, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2],
-                     radius, gc, mask, g.keys, g.vals, g.snap);
+                     radius, gc, mask, g.keys, g.rank, g.snap, g.start);
   FG_LAUNCH_CHECK();
-  uint32_t *sk, *sv;
-  radix_sort_u32(s, g.sort, radix_bits, g.keys, g.vals, n, 0, g.bits, &sk, &sv);
-  const uint32_t slots = 1u << g.bits;
-  hipLaunchKernelGGL(grid_clear, dim3(std::min(ceil_div(slots, 256), 2048u)), b, 0, s, slots, g.runs);
-  hipLaunchKernelGGL(grid_begin, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
-                     (const float4*)g.snap, g.runs, g.sorted);
+  exclusive_scan_u32(s, g.start, words, g.partial);
+  hipLaunchKernelGGL(grid_scatter, gr, b, 0, s, n, (const uint32_t*)g.keys, (const uint32_t*)g.rank,
+                     (const uint32_t*)g.start, g.arrived);
+  hipLaunchKernelGGL(grid_order, gr, b, 0, s, n, (const uint32_t*)g.keys, (const uint32_t*)g.start,
+                     (const uint32_t*)g.arrived, (const float4*)g.snap, g.svals, g.sorted,
+                     (const uint32_t*)nullptr, (uint32_t*)nullptr);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(grid_query<D>, gr, b, 0, s, n, (const uint32_t*)sv, (const GridRun*)g.runs,
+  hipLaunchKernelGGL(grid_query<D>, gr, b, 0, s, n, (const uint32_t*)g.svals, (const uint32_t*)g.start,
                      (const float4*)g.sorted, vel[0], vel[1], vel[2], gc, mask, strength, step);
   FG_LAUNCH_CHECK();
 }
@@ -856,9 +878,10 @@ void launch_collide_grid(hipStream_t s, GridWork& g, uint32_t n, int ndim, float
   if (!n || !(gc.inv_b > 0)) return;
   g.reserve(n);
   uint32_t mask = (uint32_t)((1ull << g.bits) - 1ull);
-  if (ndim == 1) grid_run<1>(s, g, n, pos, vel, radius, gc, mask, strength, radix_bits, step);
-  else if (ndim == 2) grid_run<2>(s, g, n, pos, vel, radius, gc, mask, strength, radix_bits, step);
-  else grid_run<3>(s, g, n, pos, vel, radius, gc, mask, strength, radix_bits, step);
+  (void)radix_bits;  // the grid is built by counting, not sorted
+  if (ndim == 1) grid_run<1>(s, g, n, pos, vel, radius, gc, mask, strength, step);
+  else if (ndim == 2) grid_run<2>(s, g, n, pos, vel, radius, gc, mask, strength, step);
+  else grid_run<3>(s, g, n, pos, vel, radius, gc, mask, strength, step);
 }
 
 GridClasses grid_classes(const float* radii, uint32_t n) {
@@ -964,8 +987,8 @@ template <int D>
 __global__ void __launch_bounds__(256) bgrid_keys(uint32_t total, const uint32_t* __restrict__ seg_of,
                                                   const uint32_t* __restrict__ seg_job,
                                                   const BJob* __restrict__ jobs, uint32_t mask,
-                                                  uint32_t* __restrict__ keys,
-                                                  uint32_t* __restrict__ vals, float4* __restrict__ snap) {
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ rank,
+                                                  float4* __restrict__ snap, uint32_t* __restrict__ count) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
   const uint32_t seg = seg_of[i];
@@ -973,30 +996,15 @@ __global__ void __launch_bounds__(256) bgrid_keys(uint32_t total, const uint32_t
   const uint32_t v = i - B.grid_off, c = B.vcur;
   const float4 q = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
                                D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, B.radius[v]);
-  keys[i] = grid_key<D>(q, seg, B.coll, mask);
-  vals[i] = i;
+  const uint32_t key = grid_key<D>(q, seg, B.coll, mask);
+  keys[i] = key;
   snap[i] = q;
-}
-
-__global__ void __launch_bounds__(256) bgrid_begin(uint32_t total, const uint32_t* __restrict__ skeys,
-                                                   const uint32_t* __restrict__ svals,
-                                                   const uint32_t* __restrict__ seg_of,
-                                                   const float4* __restrict__ snap,
-                                                   GridRun* __restrict__ runs,
-                                                   float4* __restrict__ sorted,
-                                                   uint32_t* __restrict__ sseg) {
-  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
-  if (s >= total) return;
-  const uint32_t key = skeys[s], i = svals[s];
-  if (s == 0 || skeys[s - 1] != key) runs[key].first = s;
-  if (s + 1 == total || skeys[s + 1] != key) runs[key].end = s + 1;
-  sorted[s] = snap[i];
-  sseg[s] = seg_of[i];
+  rank[i] = atomicAdd(&count[key], 1u);
 }
 
 template <int D>
 __global__ void __launch_bounds__(256) bgrid_query(uint32_t total, const uint32_t* __restrict__ svals,
-                                                   const GridRun* __restrict__ runs,
+                                                   const uint32_t* __restrict__ start,
                                                    const float4* __restrict__ sorted,
                                                    const uint32_t* __restrict__ sseg,
                                                    const uint32_t* __restrict__ seg_job,
@@ -1009,7 +1017,7 @@ __global__ void __launch_bounds__(256) bgrid_query(uint32_t total, const uint32_
   if (!B.active || iteration >= B.coll_iters) return;
   const uint32_t v = svals[s] - B.grid_off;
   float ax = 0.f, ay = 0.f, az = 0.f;
-  grid_collide_point<D, true>(s, sorted[s], v, svals, runs, sorted, sseg, seg,
+  grid_collide_point<D, true>(s, sorted[s], v, svals, start, sorted, sseg, seg,
                               B.grid_off, B.coll, mask, B.coll_strength, B.step, B.seed, ax, ay,
                               az);
   float* const* vel = B.v[B.vcur];
@@ -1139,25 +1147,25 @@ void BatchGrid::reserve(uint32_t n_) {
   release();
   uint32_t m = n_ > 1 ? n_ : 1;
   bits = b;
+  const uint32_t words = (1u << bits) + 1u;
   FG_HIP(hipMalloc(&keys, sizeof(uint32_t) * m));
-  FG_HIP(hipMalloc(&vals, sizeof(uint32_t) * m));
-  FG_HIP(hipMalloc(&runs, sizeof(GridRun) << bits));
+  FG_HIP(hipMalloc(&rank, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&start, sizeof(uint32_t) * words));
+  FG_HIP(hipMalloc(&partial, sizeof(uint32_t) * scan_partials_size(words)));
+  FG_HIP(hipMalloc(&arrived, sizeof(uint32_t) * m));
+  FG_HIP(hipMalloc(&svals, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&sorted, sizeof(float4) * m));
   FG_HIP(hipMalloc(&snap, sizeof(float4) * m));
   FG_HIP(hipMalloc(&sseg, sizeof(uint32_t) * m));
-  sort.reserve(m, sizeof(uint32_t));
   cap = n_;
 }
 
 void BatchGrid::release() {
-  void* ptrs[] = {keys, vals, runs, sorted, snap, sseg};
+  void* ptrs[] = {keys, rank, start, partial, arrived, svals, sorted, snap, sseg};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  keys = vals = sseg = nullptr;
-  runs = nullptr;
+  keys = rank = start = partial = arrived = svals = sseg = nullptr;
   sorted = snap = nullptr;
-  sorted = nullptr;
-  sort.release();
   cap = 0;
 }
 
@@ -1200,23 +1208,26 @@ void batch_link(hipStream_t s, int ndim, const BSpace& sp, BJob* jobs, uint32_t 
 
 void batch_collide(hipStream_t s, int ndim, BatchGrid& g, const BSpace& sp, BJob* jobs,
                    uint32_t iteration, int radix_bits) {
+  (void)radix_bits;  // the grid is built by counting, not sorted
   const uint32_t n = sp.total;
   if (!n) return;
   g.reserve(n);
   const uint32_t mask = (uint32_t)((1ull << g.bits) - 1ull);
+  const uint32_t words = (1u << g.bits) + 1u;
   dim3 gr(ceil_div(n, 256)), b(256);
+  hipLaunchKernelGGL(grid_clear, dim3(std::min(ceil_div(words, 256), 2048u)), b, 0, s, words, g.start);
   FG_BY_DIM(ndim, bgrid_keys, gr, b, 0, s, n, (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job,
-            (const BJob*)jobs, mask, g.keys, g.vals, g.snap);
-  uint32_t *sk, *sv;
-  radix_sort_u32(s, g.sort, radix_bits, g.keys, g.vals, n, 0, g.bits, &sk, &sv);
-  const uint32_t slots = 1u << g.bits;
-  hipLaunchKernelGGL(grid_clear, dim3(std::min(ceil_div(slots, 256), 2048u)), b, 0, s, slots, g.runs);
-  hipLaunchKernelGGL(bgrid_begin, gr, b, 0, s, n, (const uint32_t*)sk, (const uint32_t*)sv,
-                     (const uint32_t*)sp.seg_of, (const float4*)g.snap, g.runs, g.sorted, g.sseg);
+            (const BJob*)jobs, mask, g.keys, g.rank, g.snap, g.start);
+  exclusive_scan_u32(s, g.start, words, g.partial);
+  hipLaunchKernelGGL(grid_scatter, gr, b, 0, s, n, (const uint32_t*)g.keys, (const uint32_t*)g.rank,
+                     (const uint32_t*)g.start, g.arrived);
+  hipLaunchKernelGGL(grid_order, gr, b, 0, s, n, (const uint32_t*)g.keys, (const uint32_t*)g.start,
+                     (const uint32_t*)g.arrived, (const float4*)g.snap, g.svals, g.sorted,
+                     (const uint32_t*)sp.seg_of, g.sseg);
   FG_LAUNCH_CHECK();
-  FG_BY_DIM(ndim, bgrid_query, gr, b, 0, s, n, (const uint32_t*)sv,
-            (const GridRun*)g.runs, (const float4*)g.sorted, (const uint32_t*)g.sseg,
-            (const uint32_t*)sp.seg_job, (const BJob*)jobs, mask, iteration);
+  FG_BY_DIM(ndim, bgrid_query, gr, b, 0, s, n, (const uint32_t*)g.svals, (const uint32_t*)g.start,
+            (const float4*)g.sorted, (const uint32_t*)g.sseg, (const uint32_t*)sp.seg_job, (const BJob*)jobs, mask,
+            iteration);
 }
 
 void batch_center(hipStream_t s, int, BatchWork& w, const BSpace& sp, BJob* jobs) {

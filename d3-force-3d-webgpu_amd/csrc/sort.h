@@ -1,4 +1,4 @@
-// sort.h -- device radix sort of (key, u32 value) pairs (see sort.hip).
+// sort.h -- device radix sort of (key, u32 value) pairs and an exclusive scan (see sort.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,5 +33,10 @@ void radix_sort_u32(hipStream_t stream, SortWork& w, int radix_bits, uint32_t* k
 void radix_sort_u64(hipStream_t stream, SortWork& w, int radix_bits, uint64_t* keys,
                     uint32_t* vals, uint32_t n, uint32_t first_bit, uint32_t bits,
                     uint64_t** out_keys, uint32_t** out_vals);
+
+// In-place exclusive prefix sum of n u32 values; `partial` holds scan_partials_size(n)
+// words of scratch.
+uint32_t scan_partials_size(uint32_t n);
+void exclusive_scan_u32(hipStream_t stream, uint32_t* data, uint32_t n, uint32_t* partial);
 
 }  // namespace fg

@@ -283,7 +283,87 @@ void sort_launch(hipStream_t stream, SortWork& w, K* keys, uint32_t* vals, uint3
   *out_vals = av;
 }
 
+// ---- exclusive scan ------------------------------------------------------------------
+
+constexpr uint32_t SCAN_ITEMS = 8, SCAN_TILE = THREADS * SCAN_ITEMS;
+
+// partial[b] = the sum of tile b.
+__global__ void __launch_bounds__(THREADS) scan_tile_sums(const uint32_t* __restrict__ data, uint32_t n,
+                                                          uint32_t* __restrict__ partial) {
+  __shared__ uint32_t s_tmp[WAVES];
+  const uint32_t base = blockIdx.x * SCAN_TILE;
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_ITEMS; ++k) {
+    const uint32_t i = base + k * THREADS + threadIdx.x;
+    if (i < n) sum += data[i];
+  }
+  uint32_t total;
+  block_exclusive_scan(sum, &total, s_tmp);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// One block: the tile sums become each tile's exclusive offset.
+__global__ void __launch_bounds__(THREADS) scan_offsets(uint32_t* __restrict__ partial, uint32_t tiles) {
+  __shared__ uint32_t s_tmp[WAVES];
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < tiles; b += THREADS) {
+    const uint32_t i = b + threadIdx.x;
+    const uint32_t v = i < tiles ? partial[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, &total, s_tmp);
+    if (i < tiles) partial[i] = carry + ex;
+    carry += total;
+  }
+}
+
+// Tile b scanned in LDS (each thread takes SCAN_ITEMS consecutive values) plus its offset.
+__global__ void __launch_bounds__(THREADS) scan_apply(uint32_t* __restrict__ data, uint32_t n,
+                                                      const uint32_t* __restrict__ partial) {
+  __shared__ uint32_t s[SCAN_TILE];
+  __shared__ uint32_t s_tmp[WAVES];
+  const uint32_t base = blockIdx.x * SCAN_TILE;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_ITEMS; ++k) {
+    const uint32_t j = k * THREADS + threadIdx.x;
+    s[j] = base + j < n ? data[base + j] : 0u;
+  }
+  __syncthreads();
+  uint32_t v[SCAN_ITEMS], sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = s[threadIdx.x * SCAN_ITEMS + k];
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t ex = block_exclusive_scan(sum, &total, s_tmp) + partial[blockIdx.x];
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_ITEMS; ++k) {
+    s[threadIdx.x * SCAN_ITEMS + k] = ex;
+    ex += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_ITEMS; ++k) {
+    const uint32_t j = k * THREADS + threadIdx.x;
+    if (base + j < n) data[base + j] = s[j];
+  }
+}
+
 }  // namespace
+
+uint32_t scan_partials_size(uint32_t n) { return ceil_div(n > 0 ? n : 1, SCAN_TILE); }
+
+void exclusive_scan_u32(hipStream_t stream, uint32_t* data, uint32_t n, uint32_t* partial) {
+  if (!n) return;
+  const uint32_t tiles = scan_partials_size(n);
+  hipLaunchKernelGGL(scan_tile_sums, dim3(tiles), dim3(THREADS), 0, stream, (const uint32_t*)data, n, partial);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scan_offsets, dim3(1), dim3(THREADS), 0, stream, partial, tiles);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scan_apply, dim3(tiles), dim3(THREADS), 0, stream, data, n, (const uint32_t*)partial);
+  FG_LAUNCH_CHECK();
+}
 
 void SortWork::reserve(uint32_t n, size_t key_bytes) {
   // tiles at the smallest tile (u32 keys: 256 * 12) and the widest radix (11 bits)

@@ -790,10 +790,10 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
           open_bits |= 1u << c;
       }
     }
-    // push the children some lane must open
+    // push the children some lane must open (open_bits is 0 on lanes not in the mask)
 #pragma unroll 1
     for (uint32_t c = 0; c < nk; ++c) {
-      const uint64_t open = __ballot(in && ((open_bits >> c) & 1u));
+      const uint64_t open = __builtin_amdgcn_ballot_w64(((open_bits >> c) & 1u) != 0u);
       if (!open) continue;
       if (sp >= (uint32_t)WALK_DEPTH) {
         err = 1;
