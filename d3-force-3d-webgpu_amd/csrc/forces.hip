@@ -625,6 +625,32 @@ __device__ inline uint32_t grid_class(float r, const GridClasses& gc) { return f
 // The pairs of sorted point s with the points of one cell (class c, coordinates cx, cy, cz)
 // whose run is `run`.
 template <int D, bool SEG>
+__device__ inline void grid_pair(uint32_t t, float4 q, uint32_t qseg, int cx, int cy, int cz, uint32_t c,
+                                 float inv, uint32_t s, float4 me, float ri2, uint32_t node,
+                                 const uint32_t* __restrict__ svals, uint32_t seg, uint32_t node_off,
+                                 const GridClasses& gc, float strength, uint32_t sn, uint32_t seed, float& ax,
+                                 float& ay, float& az) {
+  if (t == s) return;
+  if (SEG && qseg != seg) return;
+  if (grid_class(q.w, gc) != c) return;
+  const int3 qc = grid_cell<D>(q, inv);
+  if (qc.x != cx || qc.y != cy || qc.z != cz) return;
+  const float rj = q.w, r = me.w + rj;
+  float ex = me.x - q.x, ey = me.y - q.y, ez = me.z - q.z;
+  float l = ex * ex + ey * ey + ez * ez;
+  if (l >= r * r) return;
+  if (l == 0.f) {
+    coincident(node, svals[t] - node_off, D, sn, seed, ex, ey, ez);
+    l = ex * ex + ey * ey + ez * ez;
+  }
+  const float len = sqrtf(l), rj2 = rj * rj;
+  const float f = (r - len) / len * strength * rj2 / (ri2 + rj2);
+  ax += ex * f;
+  ay += ey * f;
+  az += ez * f;
+}
+
+template <int D, bool SEG>
 __device__ inline void grid_cell_pairs(GridRun run, int cx, int cy, int cz, uint32_t c, float inv,
                                        uint32_t s, float4 me, float ri2, uint32_t node,
                                        const uint32_t* __restrict__ svals,
@@ -632,26 +658,62 @@ __device__ inline void grid_cell_pairs(GridRun run, int cx, int cy, int cz, uint
                                        const uint32_t* __restrict__ sseg, uint32_t seg,
                                        uint32_t node_off, const GridClasses& gc, float strength,
                                        uint32_t sn, uint32_t seed, float& ax, float& ay, float& az) {
-  for (uint32_t t = run.first; t < run.end; ++t) {
-    if (t == s) continue;
-    if (SEG && sseg[t] != seg) continue;
-    const float4 q = sorted[t];
-    if (grid_class(q.w, gc) != c) continue;
-    const int3 qc = grid_cell<D>(q, inv);
-    if (qc.x != cx || qc.y != cy || qc.z != cz) continue;
-    const float rj = q.w, r = me.w + rj;
-    float ex = me.x - q.x, ey = me.y - q.y, ez = me.z - q.z;
-    float l = ex * ex + ey * ey + ez * ez;
-    if (l >= r * r) continue;
-    if (l == 0.f) {
-      coincident(node, svals[t] - node_off, D, sn, seed, ex, ey, ez);
-      l = ex * ex + ey * ey + ez * ez;
+  for (uint32_t t = run.first; t < run.end; ++t)
+    grid_pair<D, SEG>(t, sorted[t], SEG ? sseg[t] : 0u, cx, cy, cz, c, inv, s, me, ri2, node, svals, seg, node_off,
+                      gc, strength, sn, seed, ax, ay, az);
+}
+
+// The candidates of the cells of one slab (runs rr[k], cell k at (h.x + k % 3 - 1,
+// h.y + k / 3 - RY, cz)) as one list in cell order, GRID_BATCH candidates loaded at a time
+// so that their latencies overlap; the pairs are taken in the same order as cell by cell.
+constexpr uint32_t GRID_BATCH = 2;
+
+template <int D, bool SEG, int SLAB>
+__device__ inline void grid_slab_pairs(const GridRun (&rr)[SLAB], int3 h, int cz, uint32_t c, float inv, uint32_t s,
+                                       float4 me, float ri2, uint32_t node, const uint32_t* __restrict__ svals,
+                                       const float4* __restrict__ sorted, const uint32_t* __restrict__ sseg,
+                                       uint32_t seg, uint32_t node_off, const GridClasses& gc, float strength,
+                                       uint32_t sn, uint32_t seed, float& ax, float& ay, float& az) {
+  constexpr int RY = D > 1 ? 1 : 0;
+  uint32_t cum[SLAB + 1];
+  cum[0] = 0;
+#pragma unroll
+  for (int k = 0; k < SLAB; ++k) cum[k + 1] = cum[k] + (rr[k].end - rr[k].first);
+  const uint32_t total = cum[SLAB];
+  for (uint32_t j = 0; j < total; j += GRID_BATCH) {
+    float4 q[GRID_BATCH];
+    uint32_t tt[GRID_BATCH], qs[GRID_BATCH];
+    int ox[GRID_BATCH], oy[GRID_BATCH];
+#pragma unroll
+    for (uint32_t u = 0; u < GRID_BATCH; ++u) {
+      const uint32_t jj = j + u;
+      uint32_t first = rr[0].first, base = 0;
+      int dx = -1, dy = -RY;
+#pragma unroll
+      for (int k = 1; k < SLAB; ++k) {
+        if (cum[k] <= jj) {
+          first = rr[k].first;
+          base = cum[k];
+          dx = k % 3 - 1;
+          dy = k / 3 - RY;
+        }
+      }
+      tt[u] = first + (jj - base);
+      ox[u] = dx;
+      oy[u] = dy;
+      q[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      qs[u] = 0u;
+      if (jj < total) {
+        q[u] = sorted[tt[u]];
+        if (SEG) qs[u] = sseg[tt[u]];
+      }
     }
-    const float len = sqrtf(l), rj2 = rj * rj;
-    const float f = (r - len) / len * strength * rj2 / (ri2 + rj2);
-    ax += ex * f;
-    ay += ey * f;
-    az += ez * f;
+#pragma unroll
+    for (uint32_t u = 0; u < GRID_BATCH; ++u) {
+      if (j + u >= total) break;
+      grid_pair<D, SEG>(tt[u], q[u], qs[u], h.x + ox[u], h.y + oy[u], cz, c, inv, s, me, ri2, node, svals, seg,
+                        node_off, gc, strength, sn, seed, ax, ay, az);
+    }
   }
 }
 
@@ -716,10 +778,8 @@ __device__ inline void grid_collide_point(uint32_t s, float4 me, uint32_t node,
             rr[k] = GridRun{start[slot], start[slot + 1]};
           }
         }
-#pragma unroll
-        for (int k = 0; k < SLAB; ++k)
-          grid_cell_pairs<D, SEG>(rr[k], h.x + k % 3 - 1, h.y + k / 3 - RY, h.z + dz, c, inv, s, me, ri2, node,
-                                  svals, sorted, sseg, seg, node_off, gc, strength, sn, seed, ax, ay, az);
+        grid_slab_pairs<D, SEG, SLAB>(rr, h, h.z + dz, c, inv, s, me, ri2, node, svals, sorted, sseg, seg, node_off,
+                                      gc, strength, sn, seed, ax, ay, az);
       }
     } else {
       const int Ry = D > 1 ? R : 0, Rz = D > 2 ? R : 0;
