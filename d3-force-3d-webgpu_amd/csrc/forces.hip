@@ -630,6 +630,10 @@ __device__ inline void grid_pair(uint32_t t, float4 q, uint32_t qseg, int cx, in
                                  const uint32_t* __restrict__ svals, uint32_t seg, uint32_t node_off,
                                  const GridClasses& gc, float strength, uint32_t sn, uint32_t seed, float& ax,
                                  float& ay, float& az) {
+  // the query inlines this at several sites, and which site takes a candidate depends on
+  // its place in the list (so on other jobs' points under shared slots): every operation
+  // is spelled out, no contraction left to the compiler, so every site rounds alike
+#pragma clang fp contract(off)
   if (t == s) return;
   if (SEG && qseg != seg) return;
   if (grid_class(q.w, gc) != c) return;
@@ -637,17 +641,17 @@ __device__ inline void grid_pair(uint32_t t, float4 q, uint32_t qseg, int cx, in
   if (qc.x != cx || qc.y != cy || qc.z != cz) return;
   const float rj = q.w, r = me.w + rj;
   float ex = me.x - q.x, ey = me.y - q.y, ez = me.z - q.z;
-  float l = ex * ex + ey * ey + ez * ez;
+  float l = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
   if (l >= r * r) return;
   if (l == 0.f) {
     coincident(node, svals[t] - node_off, D, sn, seed, ex, ey, ez);
-    l = ex * ex + ey * ey + ez * ez;
+    l = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
   }
   const float len = sqrtf(l), rj2 = rj * rj;
   const float f = (r - len) / len * strength * rj2 / (ri2 + rj2);
-  ax += ex * f;
-  ay += ey * f;
-  az += ez * f;
+  ax = __builtin_fmaf(ex, f, ax);
+  ay = __builtin_fmaf(ey, f, ay);
+  az = __builtin_fmaf(ez, f, az);
 }
 
 template <int D, bool SEG>
