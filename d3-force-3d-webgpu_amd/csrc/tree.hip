@@ -602,6 +602,20 @@ __device__ inline void point_force(float mx, float my, float mz, uint32_t self_n
   ++inter;
 }
 
+// The sum over each aligned group of NK (2, 4 or 8) lanes, in registers (DPP): the two
+// halves of a group of 8 mirrored onto each other, then the pairs of a quad, then
+// neighbours; every lane of a group ends with the group's sum.  All lanes must be active.
+template <int CTRL>
+__device__ inline float dpp_plus(float x) {
+  return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+template <int NK>
+__device__ inline float group_sum(float x) {
+  if (NK >= 8) x = dpp_plus<0x141>(x);  // row_half_mirror: lane i + lane 7 - i
+  if (NK >= 4) x = dpp_plus<0x4e>(x);   // quad_perm(2, 3, 0, 1)
+  return dpp_plus<0xb1>(x);             // quad_perm(1, 0, 3, 2)
+}
+
 // A stack entry beyond the LDS part.
 struct alignas(16) SpillEntry {
   uint64_t mask;
@@ -750,13 +764,10 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
                                        dmax2, dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           }
         }
-        // sum the NK items of each point (aligned groups of NK lanes)
-#pragma unroll
-        for (int o = NK / 2; o > 0; o >>= 1) {
-          fx += __shfl_xor(fx, o, WAVE);
-          if (D > 1) fy += __shfl_xor(fy, o, WAVE);
-          if (D > 2) fz += __shfl_xor(fz, o, WAVE);
-        }
+        // sum the NK items of each point (aligned groups of NK lanes; lane c == 0 gets it)
+        fx = group_sum<NK>(fx);
+        if (D > 1) fy = group_sum<NK>(fy);
+        if (D > 2) fz = group_sum<NK>(fz);
         if (c == 0 && i < k_active) {
           const uint32_t b = L.list[i];
           L.acc[0][b] += fx;
