@@ -59,7 +59,8 @@ struct TreeBuffers {
   NodeRec* rec = nullptr;
   NodeSum* sums = nullptr;
   CubeInfo* cube = nullptr;
-  uint32_t* counters = nullptr; // u64 interactions [0,1], opened [2,3], walk errors [4], tile queue [5], lane visits [6,7]
+  uint32_t* counters = nullptr; // u64 interactions [0,1], opened [2,3], walk errors [4], tile queue [5], lane visits [6,7], deepest stack [8], long runs [10]
+  uint32_t* long_runs = nullptr;  // first positions of runs of equal high halves too long to order in place
   void* spill = nullptr;        // the walk's stack entries beyond LDS, per resident wave
   size_t spill_bytes = 0;
   SortWork sort;

@@ -220,6 +220,152 @@ __device__ inline uint32_t depth_of(int prefix) {
   return d < (uint32_t)Dims<D>::LEVELS ? d : (uint32_t)Dims<D>::LEVELS;
 }
 
+// ---- the order of the codes ---------------------------------------------------------
+//
+// The radix sort orders the points on the high halves of their codes only (half the
+// passes).  Within a run of points sharing a high half they stay in node order; ordering
+// each run on the low halves, node order breaking ties, gives exactly the order of a
+// stable sort on the whole codes (barnesHut.js:299-305 orders its runs the same way).  A run
+// of up to ORDER_RUN points is ordered in place by ranks; a longer one is listed and sorted
+// by order_long_runs, a workgroup per run.
+
+constexpr uint32_t ORDER_RUN = 32;
+constexpr uint32_t LONG_RUNS_COUNTER = 10;  // counters[10]: runs listed for order_long_runs
+constexpr uint32_t LONG_RUN_BLOCKS = 64;    // workgroups of order_long_runs (each takes runs in turn)
+
+template <int D>
+__global__ void __launch_bounds__(256) order_runs(uint32_t n, const uint64_t* __restrict__ keys,
+                                                  const uint32_t* __restrict__ vals, uint64_t* __restrict__ okeys,
+                                                  uint32_t* __restrict__ ovals, uint32_t* __restrict__ counters,
+                                                  uint32_t* __restrict__ long_runs) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  constexpr int HALF = Dims<D>::HALF;
+  const uint64_t k = keys[s];
+  const uint64_t high = k >> HALF;
+  uint32_t first = s, last = s;
+  while (first > 0 && s - first <= ORDER_RUN && (keys[first - 1] >> HALF) == high) --first;
+  while (last + 1 < n && last - s <= ORDER_RUN && (keys[last + 1] >> HALF) == high) ++last;
+  const uint32_t v = vals[s];
+  if (first == last || last - first + 1 > ORDER_RUN) {
+    // alone (in place) or in a long run (copied as it is for order_long_runs)
+    okeys[s] = k;
+    ovals[s] = v;
+    if (first == s && first != last) {
+      const uint32_t slot = atomicAdd(&counters[LONG_RUNS_COUNTER], 1u);
+      long_runs[slot] = s;
+    }
+    return;
+  }
+  // the run is in node order: this point goes after the points with a smaller low half and
+  // after those with the same one that come before it
+  const uint64_t lo = k & ((1ull << HALF) - 1ull);
+  uint32_t place = 0;
+  for (uint32_t j = first; j <= last; ++j) {
+    const uint64_t other = keys[j] & ((1ull << HALF) - 1ull);
+    place += (other < lo || (other == lo && j < s)) ? 1u : 0u;
+  }
+  okeys[first + place] = k;
+  ovals[first + place] = v;
+}
+
+// Each workgroup sorts listed long runs of (okeys, ovals) on the low halves: stable 8-bit
+// passes from the least significant digit, through the same positions of (tkeys, tvals)
+// and back; ranks within a chunk of 256 come from wave ballots and per-wave digit counts.
+template <int D>
+__global__ void __launch_bounds__(256) order_long_runs(uint32_t n, const uint32_t* __restrict__ counters,
+                                                       const uint32_t* __restrict__ long_runs,
+                                                       uint64_t* __restrict__ okeys, uint32_t* __restrict__ ovals,
+                                                       uint64_t* __restrict__ tkeys, uint32_t* __restrict__ tvals) {
+  constexpr int HALF = Dims<D>::HALF;
+  constexpr int PASSES = (HALF + 7) / 8;
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcount[4][256];
+  __shared__ uint32_t s_end;
+  const uint32_t li = threadIdx.x, lane = li & 63u, wave = li >> 6;
+  const uint32_t runs = counters[LONG_RUNS_COUNTER];
+  for (uint32_t r = blockIdx.x; r < runs; r += gridDim.x) {
+    const uint32_t first = long_runs[r];
+    const uint64_t high = okeys[first] >> HALF;
+    // the run's end: the first position after it whose high half differs
+    if (li == 0) s_end = n;
+    __syncthreads();
+    uint32_t end = n;
+    for (uint32_t b = first; b < n; b += 256) {
+      const uint32_t t = b + li;
+      if (t < n && (okeys[t] >> HALF) != high) atomicMin(&s_end, t);
+      __syncthreads();
+      end = s_end;  // read by every thread between two barriers: the same value, the same exit
+      __syncthreads();
+      if (end < n) break;
+    }
+    const uint32_t count = end - first;
+    for (int pass = 0; pass < PASSES; ++pass) {
+      const uint32_t shift = 8u * (uint32_t)pass;
+      uint64_t* sk = (pass & 1) ? tkeys : okeys;
+      uint32_t* sv = (pass & 1) ? tvals : ovals;
+      uint64_t* dk = (pass & 1) ? okeys : tkeys;
+      uint32_t* dv = (pass & 1) ? ovals : tvals;
+      // digit counts of the run, then their exclusive sum as each digit's base
+      base[li] = 0;
+      __syncthreads();
+      for (uint32_t i = li; i < count; i += 256) atomicAdd(&base[(uint32_t)(sk[first + i] >> shift) & 255u], 1u);
+      __syncthreads();
+      if (li == 0) {
+        uint32_t sum = 0;
+        for (int d = 0; d < 256; ++d) {
+          const uint32_t c = base[d];
+          base[d] = sum;
+          sum += c;
+        }
+      }
+      __syncthreads();
+      for (uint32_t b = 0; b < count; b += 256) {
+        const uint32_t i = b + li;
+        const bool valid = i < count;
+        uint64_t k = 0;
+        uint32_t v = 0, d = 256u;
+        if (valid) {
+          k = sk[first + i];
+          v = sv[first + i];
+          d = (uint32_t)(k >> shift) & 255u;
+        }
+        // lanes of this wave with the same digit
+        uint64_t peers = __builtin_amdgcn_ballot_w64(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+          const uint64_t on = __builtin_amdgcn_ballot_w64(valid && ((d >> bit) & 1u));
+          peers &= ((d >> bit) & 1u) ? on : ~on;
+        }
+        const uint32_t before = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+        wcount[0][li] = 0;
+        wcount[1][li] = 0;
+        wcount[2][li] = 0;
+        wcount[3][li] = 0;
+        __syncthreads();
+        if (valid && before == 0) wcount[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+          uint32_t rank = base[d] + before;
+          for (uint32_t w = 0; w < wave; ++w) rank += wcount[w][d];
+          dk[first + rank] = k;
+          dv[first + rank] = v;
+        }
+        __syncthreads();
+        base[li] += wcount[0][li] + wcount[1][li] + wcount[2][li] + wcount[3][li];
+        __syncthreads();
+      }
+    }
+    if (PASSES & 1) {  // the last pass wrote the scratch: back into the output
+      for (uint32_t i = li; i < count; i += 256) {
+        okeys[first + i] = tkeys[first + i];
+        ovals[first + i] = tvals[first + i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---- leaves -------------------------------------------------------------------------
 
 template <int D>
@@ -1184,10 +1330,19 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   hipLaunchKernelGGL((tree_keys<D>), dim3(blocks), dim3(256), 0, st, x, y, z, strength, n,
                      (const CubeInfo*)t.cube, t.keys, t.vals, t.pts);
   FG_LAUNCH_CHECK();
-  uint64_t* sk;
-  uint32_t* sv;
+  uint64_t* hk;
+  uint32_t* hv;
   if (timer) timer->end(1), timer->begin(2);
-  radix_sort_u64(st, t.sort, radix_bits, t.keys, t.vals, n, 0, Dims<D>::BITS, &sk, &sv);
+  // the high halves, then every run of equal high halves in order (into the other buffers)
+  radix_sort_u64(st, t.sort, radix_bits, t.keys, t.vals, n, Dims<D>::HALF, Dims<D>::HALF, &hk, &hv);
+  uint64_t* sk = hk == t.keys ? (uint64_t*)t.sort.keys_alt : t.keys;
+  uint32_t* sv = hv == t.vals ? t.sort.vals_alt : t.vals;
+  hipLaunchKernelGGL((order_runs<D>), dim3(blocks), dim3(256), 0, st, n, (const uint64_t*)hk, (const uint32_t*)hv, sk,
+                     sv, t.counters, t.long_runs);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL((order_long_runs<D>), dim3(LONG_RUN_BLOCKS), dim3(256), 0, st, n, (const uint32_t*)t.counters,
+                     (const uint32_t*)t.long_runs, sk, sv, hk, hv);
+  FG_LAUNCH_CHECK();
   if (timer) timer->end(2), timer->begin(1);
   t.skeys = sk;
   t.order = sv;
@@ -1254,6 +1409,7 @@ void TreeBuffers::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
   FG_HIP(hipMalloc(&cube, sizeof(CubeInfo)));
   FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 16));
+  FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   CubeInfo c0 = {};
   for (int a = 0; a < 3; ++a) c0.box[a] = 0xffffffffu, c0.box[3 + a] = 0u;
   FG_HIP(hipMemcpy(cube, &c0, sizeof(c0), hipMemcpyHostToDevice));
@@ -1275,13 +1431,13 @@ void TreeBuffers::release() {
   if (spill) (void)hipFree(spill);
   spill = nullptr;
   spill_bytes = 0;
-  void* ptrs[] = {keys, vals, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, cube, counters};
+  void* ptrs[] = {keys, vals, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, cube, counters, long_runs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = nullptr;
   vals = nullptr;
   leaves = pts = nullptr;
-  leaf_depth = leaf_parent = flags = counters = nullptr;
+  leaf_depth = leaf_parent = flags = counters = long_runs = nullptr;
   links = nullptr;
   rec = nullptr;
   sums = nullptr;

@@ -92,6 +92,21 @@ def test_coincident_points(engine_factory):
     check_tree(engine_factory(), pos, 3)
 
 
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_long_runs_of_equal_high_halves(engine_factory, dims):
+    """Clumps far smaller than a cell of the codes' high half (2,500, 150 and 40 points with
+    distinct low halves, plus 40 coincident points): runs too long to be ordered in place,
+    so their order comes from the long-run sort; order and topology stay bit-exact."""
+    rng = np.random.default_rng(77 + dims)
+    n = 6000
+    pos = make_nodes_fast(n, dims)
+    for start, count, centre, width in ((0, 2500, 10.0, 1e-3), (2500, 150, -40.0, 2e-3), (2650, 40, 60.0, 0.0),
+                                        (2690, 40, -70.0, 5e-4)):
+        blob = centre + width * rng.random((count, dims))
+        pos[start:start + count, :dims] = blob.astype(np.float32).astype(np.float64)
+    check_tree(engine_factory(), pos, dims)
+
+
 @pytest.mark.parametrize("n", [200_000, 1_000_000])
 def test_phyllotaxis_config_sizes(engine_factory, n):
     x, y, z = O.phyllotaxis(n, 3)
