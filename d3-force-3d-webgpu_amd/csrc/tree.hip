@@ -233,9 +233,17 @@ constexpr uint32_t ORDER_RUN = 32;
 constexpr uint32_t LONG_RUNS_COUNTER = 10;  // counters[10]: runs listed for order_long_runs
 constexpr uint32_t LONG_RUN_BLOCKS = 64;    // workgroups of order_long_runs (each takes runs in turn)
 
+// Positions a and b in one run: the same high half (and, in a batch, the same job: seg).
+template <int D>
+__device__ inline bool same_run(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ seg, uint32_t a,
+                                uint64_t high, uint32_t sg) {
+  return (keys[a] >> Dims<D>::HALF) == high && (!seg || seg[a] == sg);
+}
+
 template <int D>
 __global__ void __launch_bounds__(256) order_runs(uint32_t n, const uint64_t* __restrict__ keys,
-                                                  const uint32_t* __restrict__ vals, uint64_t* __restrict__ okeys,
+                                                  const uint32_t* __restrict__ vals,
+                                                  const uint32_t* __restrict__ seg, uint64_t* __restrict__ okeys,
                                                   uint32_t* __restrict__ ovals, uint32_t* __restrict__ counters,
                                                   uint32_t* __restrict__ long_runs) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
@@ -243,9 +251,10 @@ __global__ void __launch_bounds__(256) order_runs(uint32_t n, const uint64_t* __
   constexpr int HALF = Dims<D>::HALF;
   const uint64_t k = keys[s];
   const uint64_t high = k >> HALF;
+  const uint32_t sg = seg ? seg[s] : 0u;
   uint32_t first = s, last = s;
-  while (first > 0 && s - first <= ORDER_RUN && (keys[first - 1] >> HALF) == high) --first;
-  while (last + 1 < n && last - s <= ORDER_RUN && (keys[last + 1] >> HALF) == high) ++last;
+  while (first > 0 && s - first <= ORDER_RUN && same_run<D>(keys, seg, first - 1, high, sg)) --first;
+  while (last + 1 < n && last - s <= ORDER_RUN && same_run<D>(keys, seg, last + 1, high, sg)) ++last;
   const uint32_t v = vals[s];
   if (first == last || last - first + 1 > ORDER_RUN) {
     // alone (in place) or in a long run (copied as it is for order_long_runs)
@@ -269,12 +278,14 @@ __global__ void __launch_bounds__(256) order_runs(uint32_t n, const uint64_t* __
   ovals[first + place] = v;
 }
 
-// Each workgroup sorts listed long runs of (okeys, ovals) on the low halves: stable 8-bit
+// Each workgroup sorts listed long runs of (okeys, ovals) on the low halves (seg: the job of
+// each position in a batch, runs never crossing jobs, or null): stable 8-bit
 // passes from the least significant digit, through the same positions of (tkeys, tvals)
 // and back; ranks within a chunk of 256 come from wave ballots and per-wave digit counts.
 template <int D>
 __global__ void __launch_bounds__(256) order_long_runs(uint32_t n, const uint32_t* __restrict__ counters,
                                                        const uint32_t* __restrict__ long_runs,
+                                                       const uint32_t* __restrict__ seg,
                                                        uint64_t* __restrict__ okeys, uint32_t* __restrict__ ovals,
                                                        uint64_t* __restrict__ tkeys, uint32_t* __restrict__ tvals) {
   constexpr int HALF = Dims<D>::HALF;
@@ -287,13 +298,14 @@ __global__ void __launch_bounds__(256) order_long_runs(uint32_t n, const uint32_
   for (uint32_t r = blockIdx.x; r < runs; r += gridDim.x) {
     const uint32_t first = long_runs[r];
     const uint64_t high = okeys[first] >> HALF;
+    const uint32_t sg = seg ? seg[first] : 0u;
     // the run's end: the first position after it whose high half differs
     if (li == 0) s_end = n;
     __syncthreads();
     uint32_t end = n;
     for (uint32_t b = first; b < n; b += 256) {
       const uint32_t t = b + li;
-      if (t < n && (okeys[t] >> HALF) != high) atomicMin(&s_end, t);
+      if (t < n && !same_run<D>(okeys, seg, t, high, sg)) atomicMin(&s_end, t);
       __syncthreads();
       end = s_end;  // read by every thread between two barriers: the same value, the same exit
       __syncthreads();
@@ -1279,23 +1291,34 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
                      (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.keys, t.codes, t.vals, t.pts);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(2);
-  uint64_t* sk;
-  uint32_t* sv;
-  radix_sort_u64(st, t.sort, tune.radix_bits, t.keys, t.vals, total, 0, Dims<D>::BITS, &sk, &sv);
-  uint32_t* order = sv;
+  // high halves, then jobs (stable: within a job the high halves stay in order), then the
+  // codes gathered in that order and each run of equal (job, high half) ordered on the low
+  // halves: the order of (job, code, tree index)
+  uint64_t* hk;
+  uint32_t* hv;
+  radix_sort_u64(st, t.sort, tune.radix_bits, t.keys, t.vals, total, Dims<D>::HALF, Dims<D>::HALF, &hk, &hv);
+  uint32_t* order1 = hv;
+  uint32_t* sd = nullptr;
   if (sp.segs > 1) {
-    hipLaunchKernelGGL(btree_digit, dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)sv,
+    hipLaunchKernelGGL(btree_digit, dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)hv,
                        (const uint32_t*)sp.seg_of, t.digit);
     FG_LAUNCH_CHECK();
     uint32_t bits = 1;
     while ((1u << bits) < sp.segs) ++bits;
-    uint32_t* sd;
-    radix_sort_u32(st, t.sort_seg, tune.radix_bits, t.digit, sv, total, 0, bits, &sd, &order);
+    radix_sort_u32(st, t.sort_seg, tune.radix_bits, t.digit, hv, total, 0, bits, &sd, &order1);
   }
-  if (timer) timer->end(2), timer->begin(1);
-  hipLaunchKernelGGL(btree_gather, dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)order,
-                     (const uint64_t*)t.codes, t.skeys);
+  hipLaunchKernelGGL(btree_gather, dim3(blocks), dim3(256), 0, st, total, (const uint32_t*)order1,
+                     (const uint64_t*)t.codes, hk);
   FG_LAUNCH_CHECK();
+  uint32_t* order = hv == t.vals ? t.sort.vals_alt : t.vals;  // aliases none of hv, order1, sd
+  hipLaunchKernelGGL((order_runs<D>), dim3(blocks), dim3(256), 0, st, total, (const uint64_t*)hk,
+                     (const uint32_t*)order1, (const uint32_t*)sd, t.skeys, order, t.counters, t.long_runs);
+  FG_LAUNCH_CHECK();
+  hipLaunchKernelGGL((order_long_runs<D>), dim3(LONG_RUN_BLOCKS), dim3(256), 0, st, total,
+                     (const uint32_t*)t.counters, (const uint32_t*)t.long_runs, (const uint32_t*)sd, t.skeys, order,
+                     hk, order1);
+  FG_LAUNCH_CHECK();
+  if (timer) timer->end(2), timer->begin(1);
   t.order = order;
   hipLaunchKernelGGL((btree_leaves<D>), dim3(blocks), dim3(256), 0, st, total, (const uint64_t*)t.skeys,
                      (const uint32_t*)order, (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job,
@@ -1337,11 +1360,11 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   radix_sort_u64(st, t.sort, radix_bits, t.keys, t.vals, n, Dims<D>::HALF, Dims<D>::HALF, &hk, &hv);
   uint64_t* sk = hk == t.keys ? (uint64_t*)t.sort.keys_alt : t.keys;
   uint32_t* sv = hv == t.vals ? t.sort.vals_alt : t.vals;
-  hipLaunchKernelGGL((order_runs<D>), dim3(blocks), dim3(256), 0, st, n, (const uint64_t*)hk, (const uint32_t*)hv, sk,
-                     sv, t.counters, t.long_runs);
+  hipLaunchKernelGGL((order_runs<D>), dim3(blocks), dim3(256), 0, st, n, (const uint64_t*)hk, (const uint32_t*)hv,
+                     (const uint32_t*)nullptr, sk, sv, t.counters, t.long_runs);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((order_long_runs<D>), dim3(LONG_RUN_BLOCKS), dim3(256), 0, st, n, (const uint32_t*)t.counters,
-                     (const uint32_t*)t.long_runs, sk, sv, hk, hv);
+                     (const uint32_t*)t.long_runs, (const uint32_t*)nullptr, sk, sv, hk, hv);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(2), timer->begin(1);
   t.skeys = sk;
@@ -1484,6 +1507,7 @@ void BatchTree::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
   FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 16));
   FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * 16));
+  FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   sort.reserve(m, sizeof(uint64_t));
   sort_seg.reserve(m, sizeof(uint32_t));
   if (!spill) {
@@ -1494,11 +1518,12 @@ void BatchTree::reserve(uint32_t n) {
 }
 
 void BatchTree::release() {
-  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, counters, spill};
+  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, counters, spill,
+                  long_runs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = codes = skeys = nullptr;
-  vals = digit = order = leaf_depth = leaf_parent = flags = counters = nullptr;
+  vals = digit = order = leaf_depth = leaf_parent = flags = counters = long_runs = nullptr;
   leaves = pts = nullptr;
   links = nullptr;
   rec = sums = spill = nullptr;
