@@ -416,10 +416,11 @@ def bench_config_b(n=200_000, steps=20, warmup=5):
         if theta:
             sort_ms = t["tree_sort"][0] / max(1, t["tree_sort"][1])
             walk_ms = t["walk"][0] / max(1, t["walk"][1])
-            passes = 8  # 60-bit codes, 8-bit digits
+            passes = 4  # the 30-bit high halves of the 60-bit codes, 8-bit digits (then the runs are ordered)
             leg.update({"sort_passes": passes, "sort_ms": round(sort_ms, 4),
                         "sort_gb_per_s": round(24 * n * passes / (sort_ms * 1e-3) / 1e9, 1),
-                        "sort_note": "u64 code + u32 index per element, read and written each pass: 24 B/element/pass",
+                        "sort_note": "u64 code + u32 index per element, read and written each pass: 24 B/element/pass; "
+                                     "sort_ms also holds the run ordering",
                         "walk_ms": round(walk_ms, 4), "interactions": int(st.interactions),
                         "walk_interactions_per_s": round(st.interactions / (walk_ms * 1e-3), 1)})
         else:
@@ -431,6 +432,40 @@ def bench_config_b(n=200_000, steps=20, warmup=5):
                         "pairs_tflops": round(tf, 2), "pairs_roofline_frac": round(tf / FP32_PEAK_TFLOPS, 4)})
         out[key] = leg
         log(f"config B theta {theta}: {ms:.3f} ms/tick")
+    return out
+
+
+# The reference's own size table (README.md:86-92): link + charge + center, 3D, ms/tick on an
+# RTX 4090 (WebGPU).
+REF_SIZES_MS = {1000: 0.067, 5000: 0.10, 20000: 0.18, 50000: 0.27, 100000: 0.37, 200000: 0.71, 1000000: 2.6}
+
+
+def bench_ref_sizes(sizes=(1000, 5000, 20000, 50000, 100000, 200000)):
+    """The reference's benchmark sizes: link + manyBody(theta 0.9) + center on its benchmark
+    graph (examples/benchmark.html:86-93) from d3's phyllotaxis start, ms/tick after 10
+    warmup ticks (no readback in the timed region)."""
+    import d3fgpu
+    out = {}
+    for n in sizes:
+        src, dst = make_graph(n, 42)
+        bias, strengths, distances = link_defaults(n, src, dst)
+        e = d3fgpu.Engine(0)
+        e.set_nodes(phyllotaxis3(n), ndim=3)
+        e.set_forces([d3fgpu.link(src, dst, strengths, distances, bias), d3fgpu.many_body(np.full(n, -30.0)),
+                      d3fgpu.center()])
+        a, alpha = alphas(1.0, 10)
+        e.enqueue(a, 0.6)
+        e.sync()
+        steps = 300 if n <= 50_000 else (150 if n <= 100_000 else 100)
+        a, alpha = alphas(alpha, steps)
+        t0 = time.perf_counter()
+        e.enqueue(a, 0.6)
+        e.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        e.close()
+        out[str(n)] = {"ms_per_tick": round(ms, 4), "reference_ms": REF_SIZES_MS[n],
+                       "vs_reference": round(REF_SIZES_MS[n] / ms, 3)}
+        log(f"size {n}: {ms:.4f} ms/tick (reference {REF_SIZES_MS[n]})")
     return out
 
 
@@ -509,6 +544,7 @@ def main():
     ap.add_argument("--only-batch", action="store_true", help="run only the batched leg")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs per serving run (0: skip)")
     ap.add_argument("--no-config-b", action="store_true", help="skip the 200k-node theta 0.9 / theta 0 leg")
+    ap.add_argument("--no-ref-sizes", action="store_true", help="skip the reference's size table (1k..200k)")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and exit")
@@ -571,6 +607,8 @@ def main():
     }
     if not args.no_config_b:
         out["config_b"] = bench_config_b()
+    if not args.no_ref_sizes:
+        out["reference_sizes"] = bench_ref_sizes()
     if args.batch_jobs > 0:
         out["batched"] = bench_batch(args.batch_jobs, not args.no_plain_jobs)
     if args.serve_jobs > 0:
