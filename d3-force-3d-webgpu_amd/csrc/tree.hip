@@ -486,15 +486,15 @@ __device__ inline double centre(double g, double w) { return w > 0 ? g / w : 0.0
 // Adds child k of a node at depth parent_depth to `out`: a point (a d3 child one level
 // below the parent's cell), a deeper node (a d3 child, weighed by |its strength one level
 // below the parent's cell|) or a node in the same cell (its children are the parent's:
-// its sums pass through).  A local node's children are local and read from LDS; others
-// are read from global memory, their sums coherently.
-template <int D>
-__device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo, bool local,
+// its sums pass through).  LOCAL: the node and so its children lie within this
+// workgroup's points and are read from LDS; otherwise from global memory, sums coherently.
+template <int D, bool LOCAL>
+__device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo,
                                  const ReduceLDS& S, const float4* leaves, const uint4* links,
                                  const NodeRec* rec, const NodeSum* sums, NodeSum& out) {
   if (k & REF_POINT) {
     const uint32_t j = k & ~REF_POINT;
-    const float4 q = local ? S.leaf[j - lo] : leaves[j];
+    const float4 q = LOCAL ? S.leaf[j - lo] : leaves[j];
     double sv = (double)q.w;
     out.a += sv * damp<D>((int)parent_depth + 1);
     double w = fabs(sv);
@@ -505,13 +505,13 @@ __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo,
     return;
   }
   const uint32_t q = k - lo;
-  const uint32_t kd = local ? S.link[q].z : links[k].z;
-  const double ca = local ? S.a[q] : load_coherent(&sums[k].a);
+  const uint32_t kd = LOCAL ? S.link[q].z : links[k].z;
+  const double ca = LOCAL ? S.a[q] : load_coherent(&sums[k].a);
   out.a += ca;
   if (kd > parent_depth) {
     double w = fabs(ca * damp<D>(-(int)(parent_depth + 1)));
     out.w += w;
-    if (local) {
+    if (LOCAL) {
       const double kw = S.w[q];
       out.gx += w * centre(S.gx[q], kw);
       if (D > 1) out.gy += w * centre(S.gy[q], kw);
@@ -522,10 +522,10 @@ __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo,
       if (D > 2) out.gz += w * load_coherent(&rec[k].cz);
     }
   } else {
-    out.w += local ? S.w[q] : load_coherent(&sums[k].w);
-    out.gx += local ? S.gx[q] : load_coherent(&sums[k].gx);
-    if (D > 1) out.gy += local ? S.gy[q] : load_coherent(&sums[k].gy);
-    if (D > 2) out.gz += local ? S.gz[q] : load_coherent(&sums[k].gz);
+    out.w += LOCAL ? S.w[q] : load_coherent(&sums[k].w);
+    out.gx += LOCAL ? S.gx[q] : load_coherent(&sums[k].gx);
+    if (D > 1) out.gy += LOCAL ? S.gy[q] : load_coherent(&sums[k].gy);
+    if (D > 2) out.gz += LOCAL ? S.gz[q] : load_coherent(&sums[k].gz);
   }
 }
 
@@ -585,6 +585,57 @@ __global__ void __launch_bounds__(256) tree_collapse(uint32_t n, const uint32_t*
 // flags and sums live there); above them the hand-off is through global memory, coherent
 // across the device.  Every node's record goes to global memory for the walk; a node whose
 // parent is not local also leaves its sums there, coherently.
+//
+// A node covers its children's points, so once a climb leaves the workgroup's own nodes
+// it never returns: the climb is two loops.  The first touches only LDS for its hand-off
+// (a relaxed LDS atomic between LDS-only fences), so it never waits for the node records
+// it writes to global memory on the way; the second publishes through the device.
+
+// Combines node p's children (a node whose sibling has arrived) and writes its record;
+// returns its parent reference (with the locality bit).
+template <int D, bool LOCAL>
+__device__ inline uint32_t reduce_node(uint32_t p, uint32_t lo, ReduceLDS& S, const float4* leaves,
+                                       const uint4* links, NodeRec* rec, NodeSum* sums) {
+  const uint4 L = LOCAL ? S.link[p - lo] : links[p];
+  NodeSum sum = {0, 0, 0, 0, 0};
+  child_sum<D, LOCAL>(L.x, L.z, lo, S, leaves, links, rec, sums, sum);
+  child_sum<D, LOCAL>(L.y, L.z, lo, S, leaves, links, rec, sums, sum);
+  const double cx = centre(sum.gx, sum.w);
+  const double cy = D > 1 ? centre(sum.gy, sum.w) : 0.0;
+  const double cz = D > 2 ? centre(sum.gz, sum.w) : 0.0;
+  double value = sum.a * damp<D>(-(int)L.z);
+  // d3 skips a cell whose strengths add up to exactly 0; sums in another order leave a
+  // residue of rounding, so a cancellation to 2^-40 of the cell's weight counts as 0
+  double scale = sum.w * damp<D>((int)L.z + 1);
+  if (fabs(sum.a) <= fabs(scale) * 0x1p-40 || !(sum.w > 0)) value = 0.0;
+  NodeRec& r = rec[p];
+  r.value = (float)value;
+  r.depth = L.z;
+  // a node's parent is local only if the node is (the parent covers the node's points)
+  const bool parent_local = LOCAL && L.w != REF_NONE && (L.w & PARENT_LOCAL);
+  if (parent_local) {
+    const uint32_t q = p - lo;
+    S.a[q] = sum.a;
+    S.w[q] = sum.w;
+    S.gx[q] = sum.gx;
+    S.gy[q] = sum.gy;
+    S.gz[q] = sum.gz;
+    r.cx = cx;
+    r.cy = cy;
+    r.cz = cz;
+  } else {
+    store_coherent(&r.cx, cx);
+    store_coherent(&r.cy, cy);
+    store_coherent(&r.cz, cz);
+    store_coherent(&sums[p].a, sum.a);
+    store_coherent(&sums[p].w, sum.w);
+    store_coherent(&sums[p].gx, sum.gx);
+    store_coherent(&sums[p].gy, sum.gy);
+    store_coherent(&sums[p].gz, sum.gz);
+  }
+  return L.w;
+}
+
 template <int D>
 __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
                                                             const uint32_t* __restrict__ leaf_parent,
@@ -600,58 +651,25 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
   }
   __syncthreads();
   if (s >= n || n < 2) return;
-  const uint32_t up = leaf_parent[s];
-  uint32_t p = up & ~PARENT_LOCAL;
-  bool local = (up & PARENT_LOCAL) != 0;  // whether node p lies within this workgroup's points
-  for (uint32_t guard = 0; guard < n && p != REF_NONE; ++guard) {
-    uint32_t old;
-    if (local) {
-      old = __hip_atomic_fetch_add(&S.flag[p - lo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    } else {
-      old = publish_count(&flags[p]);
-    }
-    if (old == 0) return;  // the sibling is still on its way
-    // a local node's children are local; a shared node's children left their sums in
-    // global memory (their parent, p, is not local to their writer)
-    const uint4 L = local ? S.link[p - lo] : links[p];
-    NodeSum sum = {0, 0, 0, 0, 0};
-    child_sum<D>(L.x, L.z, lo, local, S, leaves, links, rec, sums, sum);
-    child_sum<D>(L.y, L.z, lo, local, S, leaves, links, rec, sums, sum);
-    const double cx = centre(sum.gx, sum.w);
-    const double cy = D > 1 ? centre(sum.gy, sum.w) : 0.0;
-    const double cz = D > 2 ? centre(sum.gz, sum.w) : 0.0;
-    double value = sum.a * damp<D>(-(int)L.z);
-    // d3 skips a cell whose strengths add up to exactly 0; sums in another order leave a
-    // residue of rounding, so a cancellation to 2^-40 of the cell's weight counts as 0
-    double scale = sum.w * damp<D>((int)L.z + 1);
-    if (fabs(sum.a) <= fabs(scale) * 0x1p-40 || !(sum.w > 0)) value = 0.0;
-    NodeRec& r = rec[p];
-    r.value = (float)value;
-    r.depth = L.z;
-    const uint32_t parent = L.w == REF_NONE ? REF_NONE : L.w & ~PARENT_LOCAL;
-    const bool parent_local = L.w != REF_NONE && (L.w & PARENT_LOCAL);
-    if (parent_local) {
-      const uint32_t q = p - lo;
-      S.a[q] = sum.a;
-      S.w[q] = sum.w;
-      S.gx[q] = sum.gx;
-      S.gy[q] = sum.gy;
-      S.gz[q] = sum.gz;
-      r.cx = cx;
-      r.cy = cy;
-      r.cz = cz;
-    } else {
-      store_coherent(&r.cx, cx);
-      store_coherent(&r.cy, cy);
-      store_coherent(&r.cz, cz);
-      store_coherent(&sums[p].a, sum.a);
-      store_coherent(&sums[p].w, sum.w);
-      store_coherent(&sums[p].gx, sum.gx);
-      store_coherent(&sums[p].gy, sum.gy);
-      store_coherent(&sums[p].gz, sum.gz);
-    }
-    p = parent;
-    local = parent_local;
+  uint32_t up = leaf_parent[s];  // every point has a parent when n >= 2
+  uint32_t guard = 0;
+  // this workgroup's own nodes: flags, sums, points and links in LDS (REF_NONE, above the
+  // root, has the locality bit set too)
+  while (up != REF_NONE && (up & PARENT_LOCAL)) {
+    const uint32_t p = up & ~PARENT_LOCAL;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    const uint32_t old =
+        __hip_atomic_fetch_add(&S.flag[p - lo], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    if (old == 0 || ++guard >= n) return;  // the sibling is still on its way
+    up = reduce_node<D, true>(p, lo, S, leaves, links, rec, sums);
+  }
+  // shared nodes: their children left their sums in global memory (their parent, p, is
+  // not local to their writer)
+  while (up != REF_NONE) {
+    const uint32_t p = up;
+    if (publish_count(&flags[p]) == 0 || ++guard >= n) return;
+    up = reduce_node<D, false>(p, lo, S, leaves, links, rec, sums);
   }
 }
 
