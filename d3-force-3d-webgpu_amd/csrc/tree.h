@@ -74,6 +74,7 @@ struct TreeBuffers {
 struct TreeParams {
   double theta2, dmin2, dmax2;
   uint32_t walk_packed_max = 24;  // openings by at most this many points run packed
+  uint32_t walk_tile = 0;         // points per tile of a small (exact-mode) walk; 0: by size
 };
 
 // Bytes of walk spill area for `cus` CUs at an LDS stack of `stack` entries.

@@ -28,6 +28,7 @@
 #include "batch.h"
 
 #include <string>
+#include <type_traits>
 
 namespace fg {
 
@@ -676,6 +677,10 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
 // ---- walk ---------------------------------------------------------------------------
 
 constexpr int WALK_WAVES = 4;
+// Trees of at most this many points are walked in the exact mode: forces that do not
+// depend on how the points are tiled (so small trees can take small tiles, more waves);
+// larger trees fill the device with 64-point tiles and take the cheaper mode.
+constexpr uint32_t WALK_EXACT_MAX = 65536;
 constexpr int WALK_STACK = 48;   // stack entries per wave in LDS (a walk at 1M points peaks near 30)
 constexpr int WALK_DEPTH = 192;  // stack entries per wave in all; 7 * 21 + 8 = 155 in 3D for distinct codes
 
@@ -715,6 +720,25 @@ __device__ inline float dampf(uint32_t k) {
   return ldexpf(odd, D == 3 ? -(int)(k >> 1) : (int)(k >> 1));
 }
 
+// How a force term a * b reaches its accumulator: ACC_FUSED acc += a * b (a fused
+// multiply-add where the compiler likes), ACC_ADD acc + a * b with the product rounded on
+// its own (so a running sum of several terms rounds like a sum of the rounded terms),
+// ACC_SET acc = a * b.
+enum : int { ACC_FUSED = 0, ACC_ADD = 1, ACC_SET = 2 };
+template <int ACC>
+__device__ inline void accumulate(float& acc, float a, float b) {
+#pragma clang fp contract(off)
+  if (ACC == ACC_SET) {
+    acc = a * b;
+  } else if (ACC == ACC_ADD) {
+    acc += a * b;
+  }
+}
+template <>
+__device__ inline void accumulate<ACC_FUSED>(float& acc, float a, float b) {
+  acc += a * b;
+}
+
 // 1 / l' for d3's squared distance l, where below distanceMin l' = sqrt(dmin2 l)
 // (manyBody.js:95): rsq(l * max(l, dmin2)) either way, one reciprocal square root.
 __device__ inline float inv_dist2(float l, float dmin2) {
@@ -724,8 +748,9 @@ __device__ inline float inv_dist2(float l, float dmin2) {
 // The force of one node (its cell chain from depth `top`, whose first width squared is
 // w2top) on one body, with d3's rules (barnesHut.js:719-746, manyBody.js:73-99).
 // `cell` is the node's index within its tree (the jiggle's stand-in index is n + cell).
-// Returns whether the body is done with the node (false: it must open it).
-template <int D>
+// Returns whether the body is done with the node (false: it must open it).  ACC: how the
+// force reaches fx, fy, fz (accumulate).
+template <int D, int ACC>
 __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self_node, uint32_t n,
                                   uint32_t cell, uint32_t top, double w2top, const NodeHdr& h,
                                   double theta2, double dmax2, float dmin2f, float alpha,
@@ -750,16 +775,16 @@ __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self
     ex = j.x, ey = j.y, ez = j.z, fl = j.w;
   }
   const float f = h.value * dampf<D>(h.depth - Dd) * alpha * inv_dist2(fl, dmin2f);
-  fx += ex * f;
-  if (D > 1) fy += ey * f;
-  if (D > 2) fz += ez * f;
+  accumulate<ACC>(fx, ex, f);
+  if (D > 1) accumulate<ACC>(fy, ey, f);
+  if (D > 2) accumulate<ACC>(fz, ez, f);
   ++inter;
   return true;
 }
 
 // The force of one point (node index `other`) on one body (barnesHut.js:705-717,
 // manyBody.js:99-113).
-template <int D>
+template <int D, int ACC>
 __device__ inline void point_force(float mx, float my, float mz, uint32_t self_node, float4 q,
                                    const uint32_t* order, uint32_t j, uint32_t off, float dmax2f,
                                    float dmin2f, float alpha, uint32_t step_no, uint32_t seed,
@@ -772,9 +797,9 @@ __device__ inline void point_force(float mx, float my, float mz, uint32_t self_n
     dx = jj.x, dy = jj.y, dz = jj.z, l = jj.w;
   }
   const float f = q.w * alpha * inv_dist2(l, dmin2f);
-  fx += dx * f;
-  if (D > 1) fy += dy * f;
-  if (D > 2) fz += dz * f;
+  accumulate<ACC>(fx, dx, f);
+  if (D > 1) accumulate<ACC>(fy, dy, f);
+  if (D > 2) accumulate<ACC>(fz, dz, f);
   ++inter;
 }
 
@@ -816,7 +841,7 @@ struct WalkCount {
 
 // One wave walks the union of the traversals of sorted points [base, base + count) of one
 // tree (count <= 64) and adds their forces to the velocities.
-template <int D, int STACK>
+template <int D, int STACK, bool EXACT>
 __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __restrict__ leaves,
                                           const uint32_t* __restrict__ order,
                                           const NodeRec* __restrict__ rec, const WalkTree& T,
@@ -834,9 +859,10 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
   const float4 me4 = live ? leaves[s] : make_float4(0.f, 0.f, 0.f, 0.f);
   const uint32_t self_node = live ? order[s] - off : 0u;
   L.body[lane] = make_float4(me4.x, me4.y, me4.z, __uint_as_float(self_node));
-  L.acc[0][lane] = 0.f;
-  L.acc[1][lane] = 0.f;
-  L.acc[2][lane] = 0.f;
+  // EXACT: each point's force accumulates in L.acc, one term per opening (its children's
+  // forces summed in a fixed tree), so the sum depends neither on the other points of the
+  // tile nor on which mode ran an opening.  Otherwise the one-lane mode accumulates in
+  // registers (ax, ay, az, fused multiply-adds) and the packed mode in L.acc.
   float ax = 0.f, ay = 0.f, az = 0.f;
   uint32_t inter = 0;
 
@@ -846,9 +872,19 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
   {
     const NodeRec& r = rec[off];
     const NodeHdr h = *(const NodeHdr*)&r;
-    bool done = live ? cell_force<D>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, size * size, h,
+    bool done = live ? cell_force<D, ACC_FUSED>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, size * size, h,
                                      theta2, dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter)
                      : true;
+    if (EXACT) {
+      L.acc[0][lane] = ax;
+      L.acc[1][lane] = ay;
+      L.acc[2][lane] = az;
+      ax = ay = az = 0.f;
+    } else {
+      L.acc[0][lane] = 0.f;
+      L.acc[1][lane] = 0.f;
+      L.acc[2][lane] = 0.f;
+    }
     uint64_t open = __ballot(!done);
     if (open) {
       if (lane == 0) {
@@ -932,11 +968,11 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
           if (k & REF_POINT) {
             const uint32_t j = k & ~REF_POINT;
             if (j != base + b)
-              point_force<D>(bd.x, bd.y, bd.z, bnode, *(const float4*)&L.stage[c][0], order, j, off,
+              point_force<D, EXACT ? ACC_SET : ACC_FUSED>(bd.x, bd.y, bd.z, bnode, *(const float4*)&L.stage[c][0], order, j, off,
                              dmax2f, dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           } else {
             const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
-            open_item = !cell_force<D>(bd.x, bd.y, bd.z, bnode, n, k - off, top, w2top, h, theta2,
+            open_item = !cell_force<D, EXACT ? ACC_SET : ACC_FUSED>(bd.x, bd.y, bd.z, bnode, n, k - off, top, w2top, h, theta2,
                                        dmax2, dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           }
         }
@@ -958,7 +994,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
         const uint64_t bal = L.ballots[(i * NK) / WAVE];
         open_bits = (uint32_t)(bal >> ((i * NK) % WAVE)) & ((1u << NK) - 1u);
       }
-    } else {
+    } else if (!EXACT) {
       // one lane per point, children in turn
 #pragma unroll 1
       for (uint32_t c = 0; c < nk; ++c) {
@@ -967,14 +1003,62 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
         if (k & REF_POINT) {
           const uint32_t j = k & ~REF_POINT;
           if (j != s)
-            point_force<D>(me4.x, me4.y, me4.z, self_node, *(const float4*)&L.stage[c][0], order, j,
-                           off, dmax2f, dmin2f, alpha, step_no, seed, ax, ay, az, inter);
+            point_force<D, ACC_FUSED>(me4.x, me4.y, me4.z, self_node, *(const float4*)&L.stage[c][0], order, j,
+                                  off, dmax2f, dmin2f, alpha, step_no, seed, ax, ay, az, inter);
           continue;
         }
         const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
-        if (!cell_force<D>(me4.x, me4.y, me4.z, self_node, n, k - off, top, w2top, h, theta2, dmax2,
-                           dmin2f, alpha, step_no, seed, ax, ay, az, inter))
+        if (!cell_force<D, ACC_FUSED>(me4.x, me4.y, me4.z, self_node, n, k - off, top, w2top, h, theta2,
+                                  dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter))
           open_bits |= 1u << c;
+      }
+    } else {
+      // one lane per point, children in turn -- by pairs, in the order in which group_sum
+      // adds them: child c with NK - 1 - c (NK = 8) or c + 2 (NK = 4); the pairs of an
+      // 8-child node as (0, 2), then (1, 3)
+      float qx = 0.f, qy = 0.f, qz = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;
+      float sx = 0.f, sy = 0.f, sz = 0.f;
+      constexpr uint32_t PAIRS = NK / 2;
+#pragma unroll 1
+      for (uint32_t t = 0; t < PAIRS; ++t) {
+        const uint32_t ca = NK == 8 ? ((t & 1u) << 1 | t >> 1) : t;
+        const uint32_t cb = NK == 8 ? 7u - ca : ca + PAIRS;
+        // the pair's sum: the first child's force, plus the second's
+        float px = 0.f, py = 0.f, pz = 0.f;
+        auto child = [&](uint32_t c, auto acc) {
+          constexpr int ACC = decltype(acc)::value;
+          if (c >= nk || !in) return;
+          const uint32_t k = L.cur[c];
+          if (k & REF_POINT) {
+            const uint32_t j = k & ~REF_POINT;
+            if (j != s)
+              point_force<D, ACC>(me4.x, me4.y, me4.z, self_node, *(const float4*)&L.stage[c][0], order, j,
+                                  off, dmax2f, dmin2f, alpha, step_no, seed, px, py, pz, inter);
+          } else {
+            const NodeHdr hd = *(const NodeHdr*)&L.stage[c][0];
+            if (!cell_force<D, ACC>(me4.x, me4.y, me4.z, self_node, n, k - off, top, w2top, hd, theta2,
+                                    dmax2, dmin2f, alpha, step_no, seed, px, py, pz, inter))
+              open_bits |= 1u << c;
+          }
+        };
+        child(ca, std::integral_constant<int, ACC_SET>{});
+        child(cb, std::integral_constant<int, ACC_ADD>{});
+        if (PAIRS == 1 || ((t & 1u) == 0u && PAIRS == 4)) {
+          qx = px, qy = py, qz = pz;  // NK = 2: the sum; NK = 8: the first pair of a half
+        } else if (PAIRS == 2) {
+          if (t == 0) qx = px, qy = py, qz = pz;
+          else qx = qx + px, qy = qy + py, qz = qz + pz;
+        } else if (t == 1) {
+          hx = qx + px, hy = qy + py, hz = qz + pz;
+        } else {
+          qx = hx + (qx + px), qy = hy + (qy + py), qz = hz + (qz + pz);
+        }
+      }
+      sx = qx, sy = qy, sz = qz;
+      if (in) {
+        L.acc[0][lane] += sx;
+        if (D > 1) L.acc[1][lane] += sy;
+        if (D > 2) L.acc[2][lane] += sz;
       }
     }
     // push the children some lane must open (open_bits is 0 on lanes not in the mask)
@@ -1007,9 +1091,15 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
   }
   if (live) {
     const uint32_t o = self_node;
-    T.vx[o] += ax + L.acc[0][lane];
-    if (D > 1) T.vy[o] += ay + L.acc[1][lane];
-    if (D > 2) T.vz[o] += az + L.acc[2][lane];
+    if (EXACT) {
+      T.vx[o] += L.acc[0][lane];
+      if (D > 1) T.vy[o] += L.acc[1][lane];
+      if (D > 2) T.vz[o] += L.acc[2][lane];
+    } else {
+      T.vx[o] += ax + L.acc[0][lane];
+      if (D > 1) T.vy[o] += ay + L.acc[1][lane];
+      if (D > 2) T.vz[o] += az + L.acc[2][lane];
+    }
   }
   C.opened += opened;
   C.lane_visits += lane_visits;
@@ -1036,12 +1126,24 @@ struct WalkArgs {
   double theta2, dmin2, dmax2;
   uint32_t n;
   uint32_t packed_max;  // openings by at most this many points use the packed mode
+  uint32_t tile;        // points per tile (1..64)
 };
 
-// One simulation: every wave takes 64-point tiles [64 t, 64 t + 64) of the sorted points
-// from a queue (counters[5]) until none is left; `spill` holds WALK_DEPTH - STACK entries
-// per resident wave.
-template <int D, int STACK, int WAVES>
+// Points per tile of an exact walk over n points: the smallest of 16 and 32 that leaves
+// at most two tiles per SIMD (a small walk is bound by the latency of each wave's chain of
+// openings, and a smaller tile has a shorter chain), else 64.  `want` (1..64) overrides.
+inline uint32_t walk_tile_for(uint32_t n, int cus, uint32_t want) {
+  if (want >= 1u && want <= (uint32_t)WAVE) return want;
+  const uint64_t room = 2ull * 4ull * (uint64_t)(cus > 0 ? cus : 1);
+  for (uint32_t t = 16; t < (uint32_t)WAVE; t *= 2)
+    if ((n + t - 1) / t <= room) return t;
+  return (uint32_t)WAVE;
+}
+
+// One simulation: every wave takes tiles [tile t, tile (t + 1)) of the sorted points from a
+// queue (counters[5]) until none is left; `spill` holds WALK_DEPTH - STACK entries per
+// resident wave.  Tiles other than 64 points need EXACT.
+template <int D, int STACK, int WAVES, bool EXACT>
 __global__ void __launch_bounds__(WAVES * WAVE)
     tree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
               const NodeRec* __restrict__ rec, const CubeInfo* __restrict__ cube,
@@ -1065,17 +1167,18 @@ __global__ void __launch_bounds__(WAVES * WAVE)
   T.vx = vx;
   T.vy = vy;
   T.vz = vz;
-  const uint32_t tiles = (A.n + WAVE - 1) / WAVE;
+  const uint32_t tiles = (A.n + A.tile - 1) / A.tile;
   WalkCount C;
   for (;;) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(&counters[5], 1u);
     t = (uint32_t)__builtin_amdgcn_readfirstlane(t);
     if (t >= tiles) break;
-    const uint32_t base = t * WAVE;
-    const uint32_t count = A.n - base < (uint32_t)WAVE ? A.n - base : (uint32_t)WAVE;
+    const uint32_t base = t * A.tile;
+    const uint32_t count = A.n - base < A.tile ? A.n - base : A.tile;
     __builtin_amdgcn_wave_barrier();
-    walk_tile<D, STACK>(lds[wave], leaves, order, rec, T, base, count, A.packed_max, lane, my_spill, C);
+    walk_tile<D, STACK, EXACT>(lds[wave], leaves, order, rec, T, base, count, A.packed_max, lane, my_spill,
+                               C);
   }
   walk_flush(C, lane, counters);
 }
@@ -1210,7 +1313,8 @@ __global__ void __launch_bounds__(256) btree_build(uint32_t total, const uint64_
 }
 
 // Persistent walk: every wave takes 64-point tiles of any job from one queue until it is
-// empty; the per-tile work is the single-simulation walk's.
+// empty; the per-tile work is the single-simulation walk's, in the same mode for the
+// job's size (so a job's forces are the single simulation's whatever its tiles).
 template <int D, int STACK, int WAVES>
 __global__ void __launch_bounds__(WAVES * WAVE)
     btree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
@@ -1246,7 +1350,10 @@ __global__ void __launch_bounds__(WAVES * WAVE)
     const uint32_t end = T.off + T.n;
     const uint32_t count = end - tile.y < (uint32_t)WAVE ? end - tile.y : (uint32_t)WAVE;
     __builtin_amdgcn_wave_barrier();
-    walk_tile<D, STACK>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
+    if (T.n <= WALK_EXACT_MAX)
+      walk_tile<D, STACK, true>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
+    else
+      walk_tile<D, STACK, false>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
   }
   walk_flush(C, lane, counters);
 }
@@ -1411,11 +1518,13 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   A.dmax2 = p.dmax2;
   A.n = n;
   A.packed_max = p.walk_packed_max;
-  auto kern = tree_walk<D, WALK_STACK, WALK_WAVES>;
   const int cus = device_cus();
+  const bool exact = n <= WALK_EXACT_MAX;
+  A.tile = exact ? walk_tile_for(n, cus, p.walk_tile) : (uint32_t)WAVE;
+  auto kern = exact ? tree_walk<D, WALK_STACK, WALK_WAVES, true> : tree_walk<D, WALK_STACK, WALK_WAVES, false>;
   uint32_t wblocks = resident_blocks(kern, WALK_WAVES * WAVE, cus);
   if (wblocks > (uint32_t)cus * (32 / WALK_WAVES)) wblocks = (uint32_t)cus * (32 / WALK_WAVES);
-  const uint32_t need = ceil_div(ceil_div(n, WAVE), WALK_WAVES);
+  const uint32_t need = ceil_div(ceil_div(n, A.tile), WALK_WAVES);
   if (wblocks > need) wblocks = need;
   if (t.spill_bytes < spill_bytes(cus, WALK_STACK)) t.reserve_spill(spill_bytes(cus, WALK_STACK));
   hipLaunchKernelGGL(kern, dim3(wblocks), dim3(WALK_WAVES * WAVE), 0, st, (const float4*)t.leaves,

@@ -150,15 +150,21 @@ def test_batch_equals_single_simulation_link_manybody(n, nd):
     assert np.array_equal(v1, v2)
 
 
-@pytest.mark.parametrize("n,nd,radius", [(9000, 3, 3.0), (3000, 2, 3.0), (12000, 3, None)])
-def test_batch_close_to_single_simulation_full_mix(n, nd, radius):
-    """center sums and collide visits in another order: agreement to rounding, over 10 ticks."""
-    ticks = 10
+@pytest.mark.parametrize("n,nd,radius,ticks,tol", [(9000, 3, 3.0, 10, 1e-4), (3000, 2, 3.0, 5, 1e-5),
+                                                   (12000, 3, None, 10, 1e-4)])
+def test_batch_close_to_single_simulation_full_mix(n, nd, radius, ticks, tol):
+    """center sums and collide visits in another order: agreement to rounding.
+
+    Below 8192 nodes the single simulation collides all pairs and the batch through its
+    grid, so the sums round differently from the first tick (rel-L2 ~5e-9 after one tick
+    at 3000 nodes in 2D); the overlap-resolving collide dynamics then multiply such a
+    difference several times a tick (to ~1e-4 after 10 ticks, a value set by chance
+    rounding), so that case is held to 1e-5 after 5 ticks."""
     p1, _ = plain_run(n, nd, 31, ticks, radius=radius)
     p2, _ = run_alone(make_job(n, nd, 31, radius=radius), ticks)
     err = rel_l2(p2 - phyllotaxis(n, nd), p1 - phyllotaxis(n, nd))
     print(f"[parity] batch vs single, {nd}D n={n}, displacement after {ticks} ticks: rel-L2 {err:.2e}")
-    assert err < 1e-4
+    assert err < tol
 
 
 def test_batch_finish_at_alpha_min():

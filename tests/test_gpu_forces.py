@@ -128,6 +128,60 @@ def test_manybody_tree_sizes(engine_factory, n):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("dims", [1, 2, 3])
+@pytest.mark.parametrize("coincident", [False, True])
+def test_manybody_tree_forces_do_not_depend_on_tiling(monkeypatch, dims, coincident):
+    """Trees of up to 65536 points are walked in the exact mode: every point's force comes
+    out bit for bit the same whatever the tile size (points per wave) and whichever mode
+    (one lane per point, or packed (point, child) items) ran each opening -- packed limit 0
+    runs every opening one lane per point, 64 every opening packed.  A clump makes deep
+    cell chains; 40 copies of one point make a deep run of equal codes.
+
+    Against the FP64 oracle only without the copies: the tree puts a group of coincident
+    points at the deepest level, as the reference's does (a point's leaf depth comes from
+    the codes around it, barnesHut.js:267-268), where d3 keeps the group at the depth at
+    which it parts from other points, so in 3D (and 1D) the group's strength reaches the
+    cells above it with another damping than d3's."""
+    n = 6000
+    pos = make_nodes_fast(n, dims, 20 * np.cbrt(n))
+    pos[: n // 10] = make_nodes_fast(n // 10, dims, 3.0)
+    if coincident:
+        pos[n // 10: n // 10 + 40] = pos[0]
+    s = np.full(n, -30.0) * (1 + 0.5 * np.sin(np.arange(n)))
+    out = []
+    for tile, packed in [(64, 24), (64, 0), (64, 64), (32, 0), (16, 24), (7, 64), (1, 24)]:
+        monkeypatch.setenv("D3FGPU_WALK_TILE", str(tile))
+        monkeypatch.setenv("D3FGPU_WALK_PACKED_MAX", str(packed))
+        e = d3fgpu.Engine(0)
+        try:
+            _, dv = gpu_dv(e, pos, dims, [d3fgpu.many_body(s)])
+            assert e.stats().walk_errors == 0
+        finally:
+            e.close()
+        out.append(((tile, packed), dv))
+    for key, dv in out[1:]:
+        assert np.array_equal(dv, out[0][1]), f"tile/packed {key} differs from {out[0][0]}"
+    if not coincident:
+        ref, _ = oracle_manybody(pos, dims, s)
+        err = rel_l2(out[0][1], ref)
+        report(f"manyBody tree {dims}D n={n} clump, any tiling", err)
+        if dims > 1:
+            assert err < 1e-4
+
+
+def test_manybody_tree_above_exact_mode(engine_factory):
+    """Past 65536 points the walk takes 64-point tiles and the cheaper accumulation."""
+    n = 70000
+    pos = make_nodes_fast(n, 3, 20 * np.cbrt(n))
+    s = np.full(n, -30.0)
+    _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s)])
+    ref, _ = oracle_manybody(pos, 3, s)
+    err = rel_l2(dv, ref)
+    report(f"manyBody tree n={n}", err)
+    assert close_count(dv, ref, rtol=1e-3, atol=1e-3) <= tree_outliers(n)
+    assert err < 1e-4
+
+
 @pytest.mark.parametrize("dims,outlier", [(2, 120), (3, 120), (2, 2000), (3, 2000)])
 def test_manybody_tree_clumps(engine_factory, dims, outlier):
     pos = make_nodes_fast(9000, dims, 2)
