@@ -246,7 +246,10 @@ void update_pass(fg_ctx* c, Pass& p, const fg_force_desc& d) {
   p.id = d.id;
 }
 
-void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
+// Enqueues one force pass.  defer_center: a center pass computes its shift only and
+// returns its work area, for the integrator to apply the shift (it must be the tick's last
+// pass); otherwise null.
+const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center = false) {
   const uint32_t n = c->n;
   const int nd = c->ndim;
   float* pos[3] = {c->P(0), c->P(1), c->P(2)};
@@ -303,7 +306,9 @@ void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
     }
     case FG_FORCE_CENTER: {
       PhaseScope ps(c->T(), FG_PHASE_CENTER);
-      launch_center(c->stream, n, nd, pos, p.center.as<CenterWork>(), p.d.x, p.d.y, p.d.z, p.d.strength);
+      launch_center(c->stream, n, nd, pos, p.center.as<CenterWork>(), p.d.x, p.d.y, p.d.z, p.d.strength,
+                    !defer_center);
+      if (defer_center) return p.center.as<CenterWork>();
       break;
     }
     case FG_FORCE_RADIAL: {
@@ -321,6 +326,7 @@ void run_pass(fg_ctx* c, Pass& p, StepParams sp) {
       break;
     default: break;
   }
+  return nullptr;
 }
 
 void refresh_mirror(fg_ctx* c) {
@@ -588,13 +594,16 @@ int fg_enqueue(fg_ctx* c, uint32_t steps, const double* alphas, double decay) {
     for (uint32_t s = 0; s < steps; ++s) {
       StepParams sp{(float)alphas[s], (float)decay, (uint32_t)c->steps, c->seed};
       PhaseScope tick(c->T(), FG_PHASE_TICK);
-      for (auto& p : c->passes) run_pass(c, *p, sp);
+      // a center pass last in the list leaves its shift to the integrator (one launch fewer)
+      const CenterWork* shift = nullptr;
+      for (size_t i = 0; i < c->passes.size(); ++i)
+        shift = run_pass(c, *c->passes[i], sp, i + 1 == c->passes.size());
       float* pos[3] = {c->P(0), c->P(1), c->P(2)};
       float* vel[3] = {c->V(0), c->V(1), c->V(2)};
       float* fix[3] = {c->fix[0].as<float>(), c->fix[1].as<float>(), c->fix[2].as<float>()};
       {
         PhaseScope ps(c->T(), FG_PHASE_INTEGRATE);
-        launch_integrate(c->stream, c->n, c->ndim, pos, vel, mask, fix, sp);
+        launch_integrate(c->stream, c->n, c->ndim, pos, vel, mask, fix, sp, shift);
       }
       c->steps++;
     }

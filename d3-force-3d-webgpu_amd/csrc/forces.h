@@ -62,9 +62,11 @@ struct GridWork {
   ~GridWork() { release(); }
 };
 
-// pos/vel/fix: three device arrays each (x, y, z); unused axes may be null.
+// pos/vel/fix: three device arrays each (x, y, z); unused axes may be null.  center (or
+// null): a center pass whose shift was computed but not yet applied.
 void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
-                      const uint8_t* mask, const float* const* fix, StepParams step);
+                      const uint8_t* mask, const float* const* fix, StepParams step,
+                      const CenterWork* center = nullptr);
 // The CSR of incident links, built on the device: links (2m node indices, device), the
 // per-link FP64 arrays (device, or null for d3's defaults) -> row[n + 1], ent[2m] (device).
 // Synchronizes the stream; throws ArgError on a node index out of range.
@@ -75,8 +77,9 @@ void link_csr_device(hipStream_t s, uint32_t n, uint32_t m, const uint32_t* link
 void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* const* vin,
                  float* const* vout, const uint32_t* row, const LinkEntry* ent, float4* snap,
                  StepParams step);
+// apply false: the shift is only computed (w->shift), for the integrator to apply.
 void launch_center(hipStream_t s, uint32_t n, int ndim, float* const* pos, CenterWork* w, double cx,
-                   double cy, double cz, double strength);
+                   double cy, double cz, double strength, bool apply = true);
 void launch_radial(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
                    const float* radius, const float* weight, float cx, float cy, float cz,
                    StepParams step);

@@ -36,23 +36,33 @@ __global__ void __launch_bounds__(256) integrate_kernel(uint32_t n, float* __res
                                                         const float* __restrict__ fx,
                                                         const float* __restrict__ fy,
                                                         const float* __restrict__ fz,
+                                                        const CenterWork* __restrict__ center,
                                                         StepParams step) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float decay = step.decay;
   uint32_t m = mask ? mask[i] : 0u;
-  float v = vx[i] * decay, p = x[i] + v;
+  // a center force that is the tick's last applies its shift here, as center_apply would
+  // just before: the position minus the shift, rounded, then plus the velocity
+  float px = x[i], py = D > 1 ? y[i] : 0.f, pz = D > 2 ? z[i] : 0.f;
+  if (center) {
+    const float sx = center->shift[0], sy = center->shift[1], sz = center->shift[2];
+    if (sx != 0.0f) px -= sx;
+    if (D > 1 && sy != 0.0f) py -= sy;
+    if (D > 2 && sz != 0.0f) pz -= sz;
+  }
+  float v = vx[i] * decay, p = px + v;
   if (m & 1u) p = fx[i], v = 0.0f;
   x[i] = p;
   vx[i] = v;
   if (D > 1) {
-    v = vy[i] * decay, p = y[i] + v;
+    v = vy[i] * decay, p = py + v;
     if (m & 2u) p = fy[i], v = 0.0f;
     y[i] = p;
     vy[i] = v;
   }
   if (D > 2) {
-    v = vz[i] * decay, p = z[i] + v;
+    v = vz[i] * decay, p = pz + v;
     if (m & 4u) p = fz[i], v = 0.0f;
     z[i] = p;
     vz[i] = v;
@@ -60,15 +70,19 @@ __global__ void __launch_bounds__(256) integrate_kernel(uint32_t n, float* __res
 }
 
 void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
-                      const uint8_t* mask, const float* const* fix, StepParams step) {
+                      const uint8_t* mask, const float* const* fix, StepParams step,
+                      const CenterWork* center) {
   if (!n) return;
   dim3 g(ceil_div(n, 256)), b(256);
   if (ndim == 1)
-    hipLaunchKernelGGL(integrate_kernel<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask, fix[0], fix[1], fix[2], step);
+    hipLaunchKernelGGL(integrate_kernel<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask,
+                       fix[0], fix[1], fix[2], center, step);
   else if (ndim == 2)
-    hipLaunchKernelGGL(integrate_kernel<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask, fix[0], fix[1], fix[2], step);
+    hipLaunchKernelGGL(integrate_kernel<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask,
+                       fix[0], fix[1], fix[2], center, step);
   else
-    hipLaunchKernelGGL(integrate_kernel<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask, fix[0], fix[1], fix[2], step);
+    hipLaunchKernelGGL(integrate_kernel<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask,
+                       fix[0], fix[1], fix[2], center, step);
   FG_LAUNCH_CHECK();
 }
 
@@ -89,21 +103,35 @@ __global__ void __launch_bounds__(256) snapshot_kernel(uint32_t n, const float* 
   snap[i] = make_float4(x[i] + vx[i], D > 1 ? y[i] + vy[i] : 0.f, D > 2 ? z[i] + vz[i] : 0.f, 0.f);
 }
 
+// The snapshot p + v of node j: read from the snapshot array, or summed where it is read.
+struct SnapArray {
+  const float4* __restrict__ snap;
+  __device__ float4 operator()(uint32_t j) const { return snap[j]; }
+};
+template <int D>
+struct SnapDirect {
+  const float *__restrict__ x, *__restrict__ y, *__restrict__ z;
+  const float *__restrict__ vx, *__restrict__ vy, *__restrict__ vz;
+  __device__ float4 operator()(uint32_t j) const {
+    return make_float4(x[j] + vx[j], D > 1 ? y[j] + vy[j] : 0.f, D > 2 ? z[j] + vz[j] : 0.f, 0.f);
+  }
+};
+
 // One node's incident links: d3's spring along each link between the snapshots of its
 // ends, and the node's share of it (bias for the target, 1 - bias for the source).
-template <int D>
-__device__ inline void link_node(uint32_t i, const float4* __restrict__ snap,
+template <int D, class Snap>
+__device__ inline void link_node(uint32_t i, Snap snap,
                                  const float* __restrict__ vx, const float* __restrict__ vy,
                                  const float* __restrict__ vz, float* __restrict__ ox,
                                  float* __restrict__ oy, float* __restrict__ oz,
                                  const uint32_t* __restrict__ row, const LinkEntry* __restrict__ ent,
                                  float alpha, uint32_t sn, uint32_t seed) {
-  const float4 me = snap[i];
+  const float4 me = snap(i);
   float ax = 0.f, ay = 0.f, az = 0.f;
   const uint32_t e0 = row[i], e1 = row[i + 1];
   for (uint32_t e = e0; e < e1; ++e) {
     const LinkEntry L = ent[e];
-    const float4 t = snap[L.other];
+    const float4 t = snap(L.other);
     const bool target = L.code & 1u;
     const uint32_t id = L.code >> 1;
     // d = snap[target] - snap[source], the same vector at both ends
@@ -137,14 +165,50 @@ __global__ void __launch_bounds__(256) link_kernel(uint32_t n, const float4* __r
                                                    StepParams step) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  link_node<D>(i, snap, vx, vy, vz, ox, oy, oz, row, ent, step.alpha, step.step, step.seed);
+  link_node<D>(i, SnapArray{snap}, vx, vy, vz, ox, oy, oz, row, ent, step.alpha, step.step, step.seed);
 }
+
+// The same with the snapshots summed where they are read: one launch instead of two, for
+// graphs small enough that the launch costs more than the extra gathers.
+template <int D>
+__global__ void __launch_bounds__(256) link_direct_kernel(uint32_t n, const float* __restrict__ x,
+                                                          const float* __restrict__ y,
+                                                          const float* __restrict__ z,
+                                                          const float* __restrict__ vx,
+                                                          const float* __restrict__ vy,
+                                                          const float* __restrict__ vz,
+                                                          float* __restrict__ ox, float* __restrict__ oy,
+                                                          float* __restrict__ oz,
+                                                          const uint32_t* __restrict__ row,
+                                                          const LinkEntry* __restrict__ ent,
+                                                          StepParams step) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  link_node<D>(i, SnapDirect<D>{x, y, z, vx, vy, vz}, vx, vy, vz, ox, oy, oz, row, ent, step.alpha,
+               step.step, step.seed);
+}
+
+// Graphs of at most this many nodes take the one-launch link pass.
+constexpr uint32_t LINK_DIRECT_MAX = 32768;
 
 void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* const* vin,
                  float* const* vout, const uint32_t* row, const LinkEntry* ent, float4* snap,
                  StepParams step) {
   if (!n) return;
   dim3 g(ceil_div(n, 256)), b(256);
+  if (n <= LINK_DIRECT_MAX) {
+    if (ndim == 1)
+      hipLaunchKernelGGL(link_direct_kernel<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2],
+                         vout[0], vout[1], vout[2], row, ent, step);
+    else if (ndim == 2)
+      hipLaunchKernelGGL(link_direct_kernel<2>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2],
+                         vout[0], vout[1], vout[2], row, ent, step);
+    else
+      hipLaunchKernelGGL(link_direct_kernel<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2],
+                         vout[0], vout[1], vout[2], row, ent, step);
+    FG_LAUNCH_CHECK();
+    return;
+  }
   if (ndim == 1) {
     hipLaunchKernelGGL(snapshot_kernel<1>, g, b, 0, s, n, pos[0], pos[1], pos[2], vin[0], vin[1], vin[2], snap);
     hipLaunchKernelGGL(link_kernel<1>, g, b, 0, s, n, (const float4*)snap, vin[0], vin[1], vin[2], vout[0], vout[1], vout[2], row, ent, step);
@@ -215,13 +279,14 @@ __global__ void __launch_bounds__(256) center_apply(uint32_t n, int ndim, float*
 }
 
 void launch_center(hipStream_t s, uint32_t n, int ndim, float* const* pos, CenterWork* w, double cx,
-                   double cy, double cz, double strength) {
+                   double cy, double cz, double strength, bool apply) {
   if (!n) return;
   uint32_t blocks = ceil_div(n, 256);
   uint32_t rb = blocks < CenterWork::MAX_BLOCKS ? blocks : CenterWork::MAX_BLOCKS;
   hipLaunchKernelGGL(center_reduce, dim3(rb), dim3(256), 0, s, n, ndim, pos[0], pos[1], pos[2], w,
                      cx, cy, cz, strength);
   FG_LAUNCH_CHECK();
+  if (!apply) return;  // the integrator applies the shift
   hipLaunchKernelGGL(center_apply, dim3(blocks), dim3(256), 0, s, n, ndim, pos[0], pos[1], pos[2],
                      (const CenterWork*)w);
   FG_LAUNCH_CHECK();
@@ -1043,7 +1108,7 @@ __global__ void __launch_bounds__(256) blink(uint32_t total, const uint32_t* __r
   const BJob& B = *r.B;
   if (!B.active || !(B.forces & BF_LINK) || iteration >= B.link_iters) return;
   const uint32_t c = iteration & 1u;
-  link_node<D>(r.v, snap + B.node_off, B.v[c][0], B.v[c][1], B.v[c][2], B.v[c ^ 1][0], B.v[c ^ 1][1],
+  link_node<D>(r.v, SnapArray{snap + B.node_off}, B.v[c][0], B.v[c][1], B.v[c][2], B.v[c ^ 1][0], B.v[c ^ 1][1],
                B.v[c ^ 1][2], B.link_row, B.link_ent, B.alpha_f, B.step, B.seed);
 }
 

@@ -27,12 +27,31 @@ struct SortWork {
 
 // Sorts bits [first_bit, first_bit + bits) of the keys, 8 or 11 bits per pass.  The sorted
 // pairs end up in (keys, vals) or in the work's alternate buffers: *out_* says which.
+// prepared: the caller's own kernel has already done the sort's clear (sort_clear_block
+// with sort_prepare's arguments), so the sort launches one kernel fewer.
 void radix_sort_u32(hipStream_t stream, SortWork& w, int radix_bits, uint32_t* keys,
                     uint32_t* vals, uint32_t n, uint32_t first_bit, uint32_t bits,
                     uint32_t** out_keys, uint32_t** out_vals);
 void radix_sort_u64(hipStream_t stream, SortWork& w, int radix_bits, uint64_t* keys,
                     uint32_t* vals, uint32_t n, uint32_t first_bit, uint32_t bits,
-                    uint64_t** out_keys, uint32_t** out_vals);
+                    uint64_t** out_keys, uint32_t** out_vals, bool prepared = false);
+
+// What a sort clears before its passes: the digit histograms, the tile tickets, and the
+// look-back epoch to bump.
+struct SortClear {
+  uint32_t* ghist;
+  uint32_t hist_words;
+  uint32_t* tickets;
+  uint32_t* epoch;
+};
+// Reserves the work for a sort of n u64 keys over `bits` bits and returns its clear.
+SortClear sort_prepare(SortWork& w, int radix_bits, uint32_t n, uint32_t bits);
+// The clear, by one workgroup of any size (every thread calls it).
+__device__ inline void sort_clear_block(const SortClear& c) {
+  for (uint32_t i = threadIdx.x; i < c.hist_words; i += blockDim.x) c.ghist[i] = 0;
+  if (threadIdx.x < 16) c.tickets[threadIdx.x] = 0;
+  if (threadIdx.x == 0) c.epoch[1] = (c.epoch[0] += 16u);
+}
 
 // In-place exclusive prefix sum of n u32 values; `partial` holds scan_partials_size(n)
 // words of scratch.

@@ -55,12 +55,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uin
   return before + x - v;
 }
 
-__global__ void sort_clear(uint32_t* ghist, uint32_t hist_words, uint32_t* tickets,
-                           uint32_t* epoch) {
-  for (uint32_t i = threadIdx.x; i < hist_words; i += blockDim.x) ghist[i] = 0;
-  if (threadIdx.x < 16) tickets[threadIdx.x] = 0;
-  if (threadIdx.x == 0) epoch[1] = (epoch[0] += 16u);
-}
+__global__ void sort_clear(SortClear c) { sort_clear_block(c); }
 
 template <typename K, int RB>
 __global__ void __launch_bounds__(THREADS) sort_histogram(const K* __restrict__ keys, uint32_t n,
@@ -242,7 +237,8 @@ __global__ void __launch_bounds__(THREADS)
 
 template <typename K, int RB>
 void sort_launch(hipStream_t stream, SortWork& w, K* keys, uint32_t* vals, uint32_t n,
-                 uint32_t first_bit, uint32_t bits, K** out_keys, uint32_t** out_vals) {
+                 uint32_t first_bit, uint32_t bits, K** out_keys, uint32_t** out_vals,
+                 bool prepared = false) {
   constexpr uint32_t R = 1u << RB;
   constexpr int ITEMS = sizeof(K) == 8 ? 8 : 12;
   constexpr uint32_t TILE = THREADS * ITEMS;
@@ -258,9 +254,10 @@ void sort_launch(hipStream_t stream, SortWork& w, K* keys, uint32_t* vals, uint3
   }
   uint32_t tiles = ceil_div(n, TILE);
   if ((uint64_t)tiles * R > w.lookback_words) throw ArgError(-1, "sort: look-back buffer too small");
-  hipLaunchKernelGGL(sort_clear, dim3(1), dim3(256), 0, stream, w.ghist, passes * R, w.tickets,
-                     w.epoch);
-  FG_LAUNCH_CHECK();
+  if (!prepared) {
+    hipLaunchKernelGGL(sort_clear, dim3(1), dim3(256), 0, stream, SortClear{w.ghist, passes * R, w.tickets, w.epoch});
+    FG_LAUNCH_CHECK();
+  }
   uint32_t hblocks = tiles < 1024 ? tiles : 1024;
   hipLaunchKernelGGL((sort_histogram<K, RB>), dim3(hblocks), dim3(THREADS), 0, stream,
                      (const K*)keys, n, first_bit, bits, passes, w.ghist);
@@ -414,10 +411,18 @@ void radix_sort_u32(hipStream_t stream, SortWork& w, int radix_bits, uint32_t* k
 
 void radix_sort_u64(hipStream_t stream, SortWork& w, int radix_bits, uint64_t* keys,
                     uint32_t* vals, uint32_t n, uint32_t first_bit, uint32_t bits,
-                    uint64_t** out_keys, uint32_t** out_vals) {
+                    uint64_t** out_keys, uint32_t** out_vals, bool prepared) {
   w.reserve(n, sizeof(uint64_t));
-  if (radix_bits == 11) sort_launch<uint64_t, 11>(stream, w, keys, vals, n, first_bit, bits, out_keys, out_vals);
-  else sort_launch<uint64_t, 8>(stream, w, keys, vals, n, first_bit, bits, out_keys, out_vals);
+  if (radix_bits == 11)
+    sort_launch<uint64_t, 11>(stream, w, keys, vals, n, first_bit, bits, out_keys, out_vals, prepared);
+  else
+    sort_launch<uint64_t, 8>(stream, w, keys, vals, n, first_bit, bits, out_keys, out_vals, prepared);
+}
+
+SortClear sort_prepare(SortWork& w, int radix_bits, uint32_t n, uint32_t bits) {
+  w.reserve(n, sizeof(uint64_t));
+  const uint32_t rb = radix_bits == 11 ? 11u : 8u;
+  return SortClear{w.ghist, ((bits + rb - 1) / rb) << rb, w.tickets, w.epoch};
 }
 
 }  // namespace fg

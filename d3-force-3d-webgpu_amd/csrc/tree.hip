@@ -139,11 +139,19 @@ __device__ inline void cover_box(uint32_t* box, float origin[3], float& size, in
   }
 }
 
+// Also clears, in its first workgroup, the tree's counters and the high-half sort's
+// histograms (work the tick's later kernels need cleared, here without launches of their
+// own).
 template <int D>
 __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
                                                  const float* __restrict__ y,
                                                  const float* __restrict__ z, uint32_t n,
-                                                 CubeInfo* __restrict__ cube) {
+                                                 CubeInfo* __restrict__ cube, uint32_t* __restrict__ counters,
+                                                 SortClear sort_clear) {
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < 16) counters[threadIdx.x] = 0;
+    sort_clear_block(sort_clear);
+  }
   float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
@@ -429,15 +437,19 @@ __device__ inline void karras(const uint64_t* __restrict__ skeys, int N, int i, 
 }
 
 // The reduction works on blocks of REDUCE_BLOCK consecutive sorted points (in the index
-// space of all trees); a node is local to a block when all its points lie in it.
+// space of all trees); a node is local to a block when all its points lie in it.  A tree
+// of at most REDUCE_ONE points is reduced by one workgroup, all of it in LDS.
 constexpr uint32_t REDUCE_BLOCK = 512;
+constexpr uint32_t REDUCE_ONE = 1024;
 constexpr uint32_t PARENT_LOCAL = 0x80000000u;  // in a parent reference: the parent is local
 
 // Writes binary node `node` (= off + i) of a tree whose sorted points are [off, off + N):
-// its links, and itself (with PARENT_LOCAL when local) as the parent of its two children.
+// its links, and itself (with PARENT_LOCAL when local to a reduction block of `block`
+// points) as the parent of its two children.
 template <int D>
 __device__ inline void karras_write(const uint64_t* __restrict__ skeys, int N, int i, uint32_t off,
-                                    uint4* __restrict__ links, uint32_t* __restrict__ leaf_parent) {
+                                    uint32_t block, uint4* __restrict__ links,
+                                    uint32_t* __restrict__ leaf_parent) {
   int first, last, split, prefix;
   karras<D>(skeys, N, i, first, last, split, prefix);
   const uint32_t node = off + (uint32_t)i, a = off + (uint32_t)split, b = a + 1u;
@@ -446,7 +458,7 @@ __device__ inline void karras_write(const uint64_t* __restrict__ skeys, int N, i
   links[node].x = left;
   links[node].y = right;
   links[node].z = depth_of<D>(prefix);
-  const bool local = (off + (uint32_t)first) / REDUCE_BLOCK == (off + (uint32_t)last) / REDUCE_BLOCK;
+  const bool local = (off + (uint32_t)first) / block == (off + (uint32_t)last) / block;
   const uint32_t up = node | (local ? PARENT_LOCAL : 0u);
   if (left & REF_POINT) leaf_parent[a] = up;
   else links[a].w = up;
@@ -457,10 +469,11 @@ __device__ inline void karras_write(const uint64_t* __restrict__ skeys, int N, i
 
 template <int D>
 __global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ skeys, uint32_t n,
-                                                  uint4* __restrict__ links, uint32_t* __restrict__ leaf_parent) {
+                                                  uint32_t block, uint4* __restrict__ links,
+                                                  uint32_t* __restrict__ leaf_parent) {
   int i = (int)(blockIdx.x * 256 + threadIdx.x);
   if (i >= (int)n - 1) return;
-  karras_write<D>(skeys, (int)n, i, 0u, links, leaf_parent);
+  karras_write<D>(skeys, (int)n, i, 0u, block, links, leaf_parent);
 }
 
 // ---- bottom-up reduction + collapse ---------------------------------------------------
@@ -474,11 +487,12 @@ struct NodeAcc {
 // One workgroup's share of the reduction in LDS: its points and its binary nodes (a local
 // node's index lies within its own points, so both are indexed by index - first point of
 // the workgroup), and the sums of its local nodes.
+template <uint32_t B>
 struct ReduceLDS {
-  double a[REDUCE_BLOCK], w[REDUCE_BLOCK], gx[REDUCE_BLOCK], gy[REDUCE_BLOCK], gz[REDUCE_BLOCK];
-  float4 leaf[REDUCE_BLOCK];
-  uint4 link[REDUCE_BLOCK];
-  uint32_t flag[REDUCE_BLOCK];
+  double a[B], w[B], gx[B], gy[B], gz[B];
+  float4 leaf[B];
+  uint4 link[B];
+  uint32_t flag[B];
 };
 
 // A node's |value|-weighted centre from its sums (0 when it has no weight).
@@ -489,9 +503,9 @@ __device__ inline double centre(double g, double w) { return w > 0 ? g / w : 0.0
 // below the parent's cell|) or a node in the same cell (its children are the parent's:
 // its sums pass through).  LOCAL: the node and so its children lie within this
 // workgroup's points and are read from LDS; otherwise from global memory, sums coherently.
-template <int D, bool LOCAL>
+template <int D, bool LOCAL, uint32_t B>
 __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo,
-                                 const ReduceLDS& S, const float4* leaves, const uint4* links,
+                                 const ReduceLDS<B>& S, const float4* leaves, const uint4* links,
                                  const NodeRec* rec, const NodeSum* sums, NodeSum& out) {
   if (k & REF_POINT) {
     const uint32_t j = k & ~REF_POINT;
@@ -594,13 +608,13 @@ __global__ void __launch_bounds__(256) tree_collapse(uint32_t n, const uint32_t*
 
 // Combines node p's children (a node whose sibling has arrived) and writes its record;
 // returns its parent reference (with the locality bit).
-template <int D, bool LOCAL>
-__device__ inline uint32_t reduce_node(uint32_t p, uint32_t lo, ReduceLDS& S, const float4* leaves,
+template <int D, bool LOCAL, uint32_t B>
+__device__ inline uint32_t reduce_node(uint32_t p, uint32_t lo, ReduceLDS<B>& S, const float4* leaves,
                                        const uint4* links, NodeRec* rec, NodeSum* sums) {
   const uint4 L = LOCAL ? S.link[p - lo] : links[p];
   NodeSum sum = {0, 0, 0, 0, 0};
-  child_sum<D, LOCAL>(L.x, L.z, lo, S, leaves, links, rec, sums, sum);
-  child_sum<D, LOCAL>(L.y, L.z, lo, S, leaves, links, rec, sums, sum);
+  child_sum<D, LOCAL, B>(L.x, L.z, lo, S, leaves, links, rec, sums, sum);
+  child_sum<D, LOCAL, B>(L.y, L.z, lo, S, leaves, links, rec, sums, sum);
   const double cx = centre(sum.gx, sum.w);
   const double cy = D > 1 ? centre(sum.gy, sum.w) : 0.0;
   const double cz = D > 2 ? centre(sum.gz, sum.w) : 0.0;
@@ -637,14 +651,14 @@ __device__ inline uint32_t reduce_node(uint32_t p, uint32_t lo, ReduceLDS& S, co
   return L.w;
 }
 
-template <int D>
-__global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
+template <int D, uint32_t B>
+__global__ void __launch_bounds__(B) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
                                                             const uint32_t* __restrict__ leaf_parent,
                                                             const uint4* __restrict__ links,
                                                             uint32_t* flags, NodeRec* rec, NodeSum* sums) {
-  __shared__ ReduceLDS S;
-  const uint32_t s = blockIdx.x * REDUCE_BLOCK + threadIdx.x;
-  const uint32_t lo = blockIdx.x * REDUCE_BLOCK;
+  __shared__ ReduceLDS<B> S;
+  const uint32_t s = blockIdx.x * B + threadIdx.x;
+  const uint32_t lo = blockIdx.x * B;
   S.flag[threadIdx.x] = 0;
   if (s < n) {
     S.leaf[threadIdx.x] = leaves[s];
@@ -663,14 +677,14 @@ __global__ void __launch_bounds__(REDUCE_BLOCK) tree_reduce(uint32_t n, const fl
         __hip_atomic_fetch_add(&S.flag[p - lo], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     if (old == 0 || ++guard >= n) return;  // the sibling is still on its way
-    up = reduce_node<D, true>(p, lo, S, leaves, links, rec, sums);
+    up = reduce_node<D, true, B>(p, lo, S, leaves, links, rec, sums);
   }
   // shared nodes: their children left their sums in global memory (their parent, p, is
   // not local to their writer)
   while (up != REF_NONE) {
     const uint32_t p = up;
     if (publish_count(&flags[p]) == 0 || ++guard >= n) return;
-    up = reduce_node<D, false>(p, lo, S, leaves, links, rec, sums);
+    up = reduce_node<D, false, B>(p, lo, S, leaves, links, rec, sums);
   }
 }
 
@@ -1129,14 +1143,17 @@ struct WalkArgs {
   uint32_t tile;        // points per tile (1..64)
 };
 
-// Points per tile of an exact walk over n points: the smallest of 16 and 32 that leaves
-// at most two tiles per SIMD (a small walk is bound by the latency of each wave's chain of
-// openings, and a smaller tile has a shorter chain), else 64.  `want` (1..64) overrides.
+// Points per tile of an exact walk over n points.  A small walk is bound by the latency of
+// each wave's chain of openings, and a smaller tile has a shorter chain but adds waves and
+// total work; measured on 256 CUs (link + manyBody + center ticks, 1k..65k points), the
+// best tile keeps about 2 tiles per SIMD or fewer, and 4-point tiles pay only for the
+// smallest trees (about 0.6 tiles per SIMD).  `want` (1..64) overrides.
 inline uint32_t walk_tile_for(uint32_t n, int cus, uint32_t want) {
   if (want >= 1u && want <= (uint32_t)WAVE) return want;
-  const uint64_t room = 2ull * 4ull * (uint64_t)(cus > 0 ? cus : 1);
-  for (uint32_t t = 16; t < (uint32_t)WAVE; t *= 2)
-    if ((n + t - 1) / t <= room) return t;
+  const uint64_t simds = 4ull * (uint64_t)(cus > 0 ? cus : 1);
+  if ((uint64_t)((n + 3) / 4) * 5 <= simds * 3) return 4;
+  for (uint32_t t = 8; t < (uint32_t)WAVE; t *= 2)
+    if ((uint64_t)((n + t - 1) / t) * 5 <= simds * 11) return t;
   return (uint32_t)WAVE;
 }
 
@@ -1309,7 +1326,7 @@ __global__ void __launch_bounds__(256) btree_build(uint32_t total, const uint64_
   const BJob& B = jobs[seg_job[seg_of[i]]];
   const uint32_t off = B.tree_off, k = i - off;
   if (k + 1 >= B.n) return;
-  karras_write<D>(skeys + off, (int)B.n, (int)k, off, links, leaf_parent);
+  karras_write<D>(skeys + off, (int)B.n, (int)k, off, REDUCE_BLOCK, links, leaf_parent);
 }
 
 // Persistent walk: every wave takes 64-point tiles of any job from one queue until it is
@@ -1453,7 +1470,7 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
                      (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.links,
                      t.leaf_parent);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL((tree_reduce<D>), dim3(ceil_div(total, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st,
+  hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK>), dim3(ceil_div(total, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st,
                      total, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent, (const uint4*)t.links,
                      t.flags, (NodeRec*)t.rec, (NodeSum*)t.sums);
   FG_LAUNCH_CHECK();
@@ -1471,9 +1488,9 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
          StepParams step, int radix_bits, PhaseTimer* timer) {
   const uint32_t blocks = ceil_div(n, 256);
   if (timer) timer->begin(1);
-  hipLaunchKernelGGL(tree_counters_clear, dim3(1), dim3(64), 0, st, t.counters);
+  const SortClear sc = sort_prepare(t.sort, radix_bits, n, Dims<D>::HALF);
   uint32_t bb = blocks < 512 ? blocks : 512;
-  hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube);
+  hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube, t.counters, sc);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((tree_keys<D>), dim3(blocks), dim3(256), 0, st, x, y, z, strength, n,
                      (const CubeInfo*)t.cube, t.keys, t.vals, t.pts);
@@ -1482,7 +1499,7 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   uint32_t* hv;
   if (timer) timer->end(1), timer->begin(2);
   // the high halves, then every run of equal high halves in order (into the other buffers)
-  radix_sort_u64(st, t.sort, radix_bits, t.keys, t.vals, n, Dims<D>::HALF, Dims<D>::HALF, &hk, &hv);
+  radix_sort_u64(st, t.sort, radix_bits, t.keys, t.vals, n, Dims<D>::HALF, Dims<D>::HALF, &hk, &hv, true);
   uint64_t* sk = hk == t.keys ? (uint64_t*)t.sort.keys_alt : t.keys;
   uint32_t* sv = hv == t.vals ? t.sort.vals_alt : t.vals;
   hipLaunchKernelGGL((order_runs<D>), dim3(blocks), dim3(256), 0, st, n, (const uint64_t*)hk, (const uint32_t*)hv,
@@ -1501,12 +1518,19 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
     if (timer) timer->end(1);
     return;
   }
+  // a small tree is reduced by one workgroup: every node local, no device-wide hand-off
+  const uint32_t rblock = n <= REDUCE_ONE ? REDUCE_ONE : REDUCE_BLOCK;
   hipLaunchKernelGGL((tree_build<D>), dim3(ceil_div(n - 1, 256)), dim3(256), 0, st,
-                     (const uint64_t*)sk, n, t.links, t.leaf_parent);
+                     (const uint64_t*)sk, n, rblock, t.links, t.leaf_parent);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL((tree_reduce<D>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st, n,
-                     (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
-                     (const uint4*)t.links, t.flags, t.rec, t.sums);
+  if (rblock == REDUCE_ONE)
+    hipLaunchKernelGGL((tree_reduce<D, REDUCE_ONE>), dim3(1), dim3(REDUCE_ONE), 0, st, n,
+                       (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
+                       (const uint4*)t.links, t.flags, t.rec, t.sums);
+  else
+    hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st,
+                       n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
+                       (const uint4*)t.links, t.flags, t.rec, t.sums);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((tree_collapse<D>), dim3(blocks), dim3(256), 0, st, n, (const uint32_t*)nullptr,
                      (const uint32_t*)nullptr, (const BJob*)nullptr, (const uint4*)t.links, t.rec);
