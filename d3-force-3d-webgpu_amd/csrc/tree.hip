@@ -489,7 +489,13 @@ struct NodeAcc {
 // the workgroup), and the sums of its local nodes.
 template <uint32_t B>
 struct ReduceLDS {
+  // the one-workgroup reduction also keeps each node's centre (as its record holds it), so
+  // a parent need not divide again: LDS to spare there, and its climb is the whole tick's
+  // chain; with many workgroups per CU the room is worth more
+  static constexpr bool CACHE = B >= REDUCE_ONE;
+  static constexpr uint32_t CB = CACHE ? B : 1;
   double a[B], w[B], gx[B], gy[B], gz[B];
+  double cx[CB], cy[CB], cz[CB];
   float4 leaf[B];
   uint4 link[B];
   uint32_t flag[B];
@@ -526,7 +532,11 @@ __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo,
   if (kd > parent_depth) {
     double w = fabs(ca * damp<D>(-(int)(parent_depth + 1)));
     out.w += w;
-    if (LOCAL) {
+    if (LOCAL && ReduceLDS<B>::CACHE) {
+      out.gx += w * S.cx[q];
+      if (D > 1) out.gy += w * S.cy[q];
+      if (D > 2) out.gz += w * S.cz[q];
+    } else if (LOCAL) {
       const double kw = S.w[q];
       out.gx += w * centre(S.gx[q], kw);
       if (D > 1) out.gy += w * centre(S.gy[q], kw);
@@ -635,6 +645,11 @@ __device__ inline uint32_t reduce_node(uint32_t p, uint32_t lo, ReduceLDS<B>& S,
     S.gx[q] = sum.gx;
     S.gy[q] = sum.gy;
     S.gz[q] = sum.gz;
+    if (ReduceLDS<B>::CACHE) {
+      S.cx[q] = cx;
+      if (D > 1) S.cy[q] = cy;
+      if (D > 2) S.cz[q] = cz;
+    }
     r.cx = cx;
     r.cy = cy;
     r.cz = cz;
