@@ -958,14 +958,11 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
     if (lane < (uint32_t)NK) my_kid = L.kids[e][lane];
     {
       const uint32_t k = my_c < (uint32_t)NK ? L.kids[e][my_c] : REF_NONE;
+      // one load for point and node children alike (two would wait for each other)
+      const bool point = (k & REF_POINT) != 0;
+      const uint2* src = point ? (const uint2*)(leaves + (k & ~REF_POINT)) : (const uint2*)(rec + k);
       uint2 v = make_uint2(0u, 0u);
-      if (k != REF_NONE) {
-        if (k & REF_POINT) {
-          if (my_w < 2) v = ((const uint2*)(leaves + (k & ~REF_POINT)))[my_w];
-        } else {
-          v = ((const uint2*)(rec + k))[my_w];
-        }
-      }
+      if (k != REF_NONE && (!point || my_w < 2)) v = src[my_w];
       if (my_c < (uint32_t)NK) {
         L.stage[my_c][2 * my_w] = v.x;
         L.stage[my_c][2 * my_w + 1] = v.y;
