@@ -224,17 +224,28 @@ void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const f
 
 // ---- center -------------------------------------------------------------------------
 
+constexpr uint32_t CENTER_BLOCKS = 256;
+
 __global__ void __launch_bounds__(256) center_reduce(uint32_t n, int ndim, const float* __restrict__ x,
                                                      const float* __restrict__ y,
                                                      const float* __restrict__ z,
                                                      CenterWork* __restrict__ w, double cx, double cy,
                                                      double cz, double strength) {
   double s[3] = {0, 0, 0};
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+  auto take = [&](uint32_t i) {
     s[0] += (double)x[i];
     if (ndim > 1) s[1] += (double)y[i];
     if (ndim > 2) s[2] += (double)z[i];
+  };
+  const uint32_t stride = gridDim.x * 256;
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {  // four independent loads in flight
+    take(i);
+    take(i + stride);
+    take(i + 2 * stride);
+    take(i + 3 * stride);
   }
+  for (; i < n; i += stride) take(i);
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -282,7 +293,8 @@ void launch_center(hipStream_t s, uint32_t n, int ndim, float* const* pos, Cente
                    double cy, double cz, double strength, bool apply) {
   if (!n) return;
   uint32_t blocks = ceil_div(n, 256);
-  uint32_t rb = blocks < CenterWork::MAX_BLOCKS ? blocks : CenterWork::MAX_BLOCKS;
+  // few workgroups: each ends on one atomic on a shared counter, and those serialise
+  uint32_t rb = blocks < CENTER_BLOCKS ? blocks : CENTER_BLOCKS;
   hipLaunchKernelGGL(center_reduce, dim3(rb), dim3(256), 0, s, n, ndim, pos[0], pos[1], pos[2], w,
                      cx, cy, cz, strength);
   FG_LAUNCH_CHECK();

@@ -36,12 +36,15 @@ struct NodeSum {
 };
 
 // Cube of the tree (d3's cover), and the running box used to find it.
+constexpr uint32_t BBOX_BLOCKS = 256;  // workgroups of the single-simulation bbox kernel, at most
+
 struct CubeInfo {
   float origin[3];
   float size;
   int32_t shift;
   uint32_t done;        // blocks of the bbox kernel that have finished
   uint32_t box[6];      // order-preserving min x, y, z, max x, y, z
+  uint32_t part[BBOX_BLOCKS][6];  // the same per workgroup of the bbox kernel
 };
 
 struct TreeBuffers {

@@ -88,7 +88,8 @@ __device__ inline float cell_of(float x, float origin, int k) {
 
 // Reduces the 256 threads' boxes and folds the block's box into the u32 box words with
 // order-preserving atomics (barnesHut.js:140-167).
-__device__ inline void block_box(float lo[3], float hi[3], uint32_t* box) {
+// The workgroup's box (256 threads): threads 0..2 get axis a's order-preserving min and max.
+__device__ inline void block_box_of(float lo[3], float hi[3], uint32_t& omin, uint32_t& omax) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
 #pragma unroll
@@ -104,10 +105,19 @@ __device__ inline void block_box(float lo[3], float hi[3], uint32_t* box) {
   __syncthreads();
   if (threadIdx.x < 3) {
     const int a = threadIdx.x;
-    float l = fminf(fminf(s_lo[0][a], s_lo[1][a]), fminf(s_lo[2][a], s_lo[3][a]));
-    float h = fmaxf(fmaxf(s_hi[0][a], s_hi[1][a]), fmaxf(s_hi[2][a], s_hi[3][a]));
-    atomicMin(&box[a], float_ordered(l));
-    atomicMax(&box[3 + a], float_ordered(h));
+    omin = float_ordered(fminf(fminf(s_lo[0][a], s_lo[1][a]), fminf(s_lo[2][a], s_lo[3][a])));
+    omax = float_ordered(fmaxf(fmaxf(s_hi[0][a], s_hi[1][a]), fmaxf(s_hi[2][a], s_hi[3][a])));
+  }
+  __syncthreads();
+}
+
+// ... folded into a box in global memory with atomics (the batch's per-job boxes).
+__device__ inline void block_box(float lo[3], float hi[3], uint32_t* box) {
+  uint32_t omin = 0, omax = 0;
+  block_box_of(lo, hi, omin, omax);
+  if (threadIdx.x < 3) {
+    atomicMin(&box[threadIdx.x], omin);
+    atomicMax(&box[3 + threadIdx.x], omax);
   }
   __syncthreads();
 }
@@ -153,17 +163,58 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
     sort_clear_block(sort_clear);
   }
   float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
+  auto take = [&](uint32_t i) {
+    const float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
 #pragma unroll
     for (int a = 0; a < 3; ++a) lo[a] = fminf(lo[a], p[a]), hi[a] = fmaxf(hi[a], p[a]);
+  };
+  const uint32_t stride = gridDim.x * 256;
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {  // four independent loads in flight
+    take(i);
+    take(i + stride);
+    take(i + 2 * stride);
+    take(i + 3 * stride);
   }
-  block_box(lo, hi, cube->box);
+  for (; i < n; i += stride) take(i);
+  // each workgroup leaves its box in its own slot (no atomics on shared words: on this
+  // device those serialise across the chip), the last one to finish folds them
+  uint32_t omin = 0, omax = 0;
+  block_box_of(lo, hi, omin, omax);
+  if (threadIdx.x < 3) {
+    store_coherent(&cube->part[blockIdx.x][threadIdx.x], omin);
+    store_coherent(&cube->part[blockIdx.x][3 + threadIdx.x], omax);
+  }
   __shared__ uint32_t s_last;
+  __syncthreads();
   if (threadIdx.x == 0) s_last = publish_count(&cube->done) + 1 == gridDim.x;
   __syncthreads();
-  if (!s_last || threadIdx.x != 0) return;
+  if (!s_last) return;
+  uint32_t m[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      m[a] = min(m[a], load_coherent(&cube->part[b][a]));
+      m[3 + a] = max(m[3 + a], load_coherent(&cube->part[b][3 + a]));
+    }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      m[a] = min(m[a], (uint32_t)__shfl_xor((int)m[a], o, WAVE));
+      m[3 + a] = max(m[3 + a], (uint32_t)__shfl_xor((int)m[3 + a], o, WAVE));
+    }
+  __shared__ uint32_t s_m[4][6];
+  const uint32_t wave = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0)
+    for (int a = 0; a < 6; ++a) s_m[wave][a] = m[a];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   // the last block turns the box into the cube
+  for (int a = 0; a < 3; ++a) {
+    cube->box[a] = min(min(s_m[0][a], s_m[1][a]), min(s_m[2][a], s_m[3][a]));
+    cube->box[3 + a] = max(max(s_m[0][3 + a], s_m[1][3 + a]), max(s_m[2][3 + a], s_m[3][3 + a]));
+  }
   float origin[3], size;
   int32_t shift;
   cover_box<D>(cube->box, origin, size, shift);
@@ -1501,7 +1552,7 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   const uint32_t blocks = ceil_div(n, 256);
   if (timer) timer->begin(1);
   const SortClear sc = sort_prepare(t.sort, radix_bits, n, Dims<D>::HALF);
-  uint32_t bb = blocks < 512 ? blocks : 512;
+  const uint32_t bb = blocks < BBOX_BLOCKS ? blocks : BBOX_BLOCKS;
   hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube, t.counters, sc);
   FG_LAUNCH_CHECK();
   hipLaunchKernelGGL((tree_keys<D>), dim3(blocks), dim3(256), 0, st, x, y, z, strength, n,
