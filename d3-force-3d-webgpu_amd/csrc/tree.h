@@ -10,6 +10,7 @@
 namespace fg {
 
 constexpr uint32_t REF_POINT = 0x80000000u;  // a child reference that is a sorted point
+constexpr uint32_t TREE_COUNTERS = 16;       // words of TreeBuffers::counters
 constexpr uint32_t REF_NONE = 0xffffffffu;
 
 // What the walk reads of a binary node: its centre of charge (FP64), its strength at the
@@ -62,7 +63,7 @@ struct TreeBuffers {
   NodeRec* rec = nullptr;
   NodeSum* sums = nullptr;
   CubeInfo* cube = nullptr;
-  uint32_t* counters = nullptr; // u64 interactions [0,1], opened [2,3], walk errors [4], tile queue [5], lane visits [6,7], deepest stack [8], long runs [10]
+  uint32_t* counters = nullptr; // TREE_COUNTERS words: u64 interactions [0,1], opened [2,3], walk errors [4], tile queue [5], lane visits [6,7], deepest stack [8], long runs [10]
   uint32_t* long_runs = nullptr;  // first positions of runs of equal high halves too long to order in place
   void* spill = nullptr;        // the walk's stack entries beyond LDS, per resident wave
   size_t spill_bytes = 0;

@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
                                                  CubeInfo* __restrict__ cube, uint32_t* __restrict__ counters,
                                                  SortClear sort_clear) {
   if (blockIdx.x == 0) {
-    if (threadIdx.x < 16) counters[threadIdx.x] = 0;
+    if (threadIdx.x < TREE_COUNTERS) counters[threadIdx.x] = 0;
     sort_clear_block(sort_clear);
   }
   float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
@@ -1185,19 +1185,34 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
   C.max_sp = max_sp > C.max_sp ? max_sp : C.max_sp;
 }
 
-// The wave's counters: one atomic each.
+// The workgroup's counters: its waves add theirs in LDS, then one atomic per counter (the
+// waves of a walk finish close together, and atomics on shared words serialise).  Every
+// wave of the workgroup must call it.
 __device__ inline void walk_flush(const WalkCount& C, uint32_t lane, uint32_t* counters) {
+  __shared__ unsigned long long s_sum[3];
+  __shared__ uint32_t s_err, s_max;
+  if (threadIdx.x == 0) s_sum[0] = s_sum[1] = s_sum[2] = 0, s_err = 0, s_max = 0;
+  __syncthreads();
   uint32_t tot = C.inter;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, WAVE);
   if (lane == 0) {
-    atomicAdd((unsigned long long*)&counters[0], (unsigned long long)tot);
-    atomicAdd((unsigned long long*)&counters[2], (unsigned long long)C.opened);
-    atomicAdd((unsigned long long*)&counters[6], (unsigned long long)C.lane_visits);
-    if (C.err) atomicAdd(&counters[4], 1u);
-    atomicMax(&counters[8], C.max_sp);
+    atomicAdd(&s_sum[0], (unsigned long long)tot);
+    atomicAdd(&s_sum[1], (unsigned long long)C.opened);
+    atomicAdd(&s_sum[2], (unsigned long long)C.lane_visits);
+    if (C.err) atomicAdd(&s_err, 1u);
+    atomicMax(&s_max, C.max_sp);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd((unsigned long long*)&counters[0], s_sum[0]);
+    atomicAdd((unsigned long long*)&counters[2], s_sum[1]);
+    atomicAdd((unsigned long long*)&counters[6], s_sum[2]);
+    if (s_err) atomicAdd(&counters[4], s_err);
+    atomicMax(&counters[8], s_max);
   }
 }
+
 
 struct WalkArgs {
   double theta2, dmin2, dmax2;
@@ -1249,10 +1264,19 @@ __global__ void __launch_bounds__(WAVES * WAVE)
   T.vz = vz;
   const uint32_t tiles = (A.n + A.tile - 1) / A.tile;
   WalkCount C;
-  for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(&counters[5], 1u);
-    t = (uint32_t)__builtin_amdgcn_readfirstlane(t);
+  // with a wave for every tile, wave w walks tile w and no queue is touched (atomics on one
+  // word serialise across the device: with thousands of waves, taking tiles from a queue
+  // cost more than the slowest tile of a 200k-point walk); otherwise the waves that finish
+  // first take more tiles from the queue.  Which wave walks which tile changes no result.
+  const bool statics = gridDim.x * WAVES >= tiles;
+  for (uint32_t round = 0;; ++round) {
+    uint32_t t = blockIdx.x * WAVES + wave;
+    if (statics) {
+      if (round > 0) break;
+    } else {
+      if (lane == 0) t = atomicAdd(&counters[5], 1u);
+      t = (uint32_t)__builtin_amdgcn_readfirstlane(t);
+    }
     if (t >= tiles) break;
     const uint32_t base = t * A.tile;
     const uint32_t count = A.n - base < A.tile ? A.n - base : A.tile;
@@ -1279,7 +1303,7 @@ inline size_t spill_bytes(int cus, int stack) {
 }
 
 __global__ void tree_counters_clear(uint32_t* c) {
-  if (threadIdx.x < 16) c[threadIdx.x] = 0;
+  if (threadIdx.x < TREE_COUNTERS) c[threadIdx.x] = 0;
 }
 
 // ---- batched trees ------------------------------------------------------------------
@@ -1645,12 +1669,12 @@ void TreeBuffers::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
   FG_HIP(hipMalloc(&cube, sizeof(CubeInfo)));
-  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 16));
+  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * TREE_COUNTERS));
   FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   CubeInfo c0 = {};
   for (int a = 0; a < 3; ++a) c0.box[a] = 0xffffffffu, c0.box[3 + a] = 0u;
   FG_HIP(hipMemcpy(cube, &c0, sizeof(c0), hipMemcpyHostToDevice));
-  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * 16));
+  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * TREE_COUNTERS));
   sort.reserve(m, sizeof(uint64_t));
   cap = n;
 }
@@ -1719,8 +1743,8 @@ void BatchTree::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&flags, sizeof(uint32_t) * m));
   FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
-  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * 16));
-  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * 16));
+  FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * TREE_COUNTERS));
+  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * TREE_COUNTERS));
   FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   sort.reserve(m, sizeof(uint64_t));
   sort_seg.reserve(m, sizeof(uint32_t));
