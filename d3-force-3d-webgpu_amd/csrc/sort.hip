@@ -240,8 +240,12 @@ void sort_launch(hipStream_t stream, SortWork& w, K* keys, uint32_t* vals, uint3
                  uint32_t first_bit, uint32_t bits, K** out_keys, uint32_t** out_vals,
                  bool prepared = false) {
   constexpr uint32_t R = 1u << RB;
+  // u64 keys: 8 per thread, or 4 for a short sort (twice the tiles: more waves to hide
+  // each wave's latency where the device is mostly idle, against a longer look-back)
+  constexpr uint32_t SHORT_SORT = 131072;
   constexpr int ITEMS = sizeof(K) == 8 ? 8 : 12;
-  constexpr uint32_t TILE = THREADS * ITEMS;
+  const bool short_sort = sizeof(K) == 8 && n <= SHORT_SORT;
+  const uint32_t TILE = THREADS * (short_sort ? 4 : ITEMS);
   uint32_t passes = (bits + RB - 1) / RB;
   K* a = keys;
   uint32_t* av = vals;
@@ -264,10 +268,16 @@ void sort_launch(hipStream_t stream, SortWork& w, K* keys, uint32_t* vals, uint3
   FG_LAUNCH_CHECK();
   for (uint32_t p = 0; p < passes; ++p) {
     uint32_t pb = bits - p * RB < (uint32_t)RB ? bits - p * RB : (uint32_t)RB;
-    hipLaunchKernelGGL((sort_pass<K, RB, ITEMS>), dim3(tiles), dim3(THREADS), 0, stream,
-                       (const K*)a, (const uint32_t*)av, b, bv, n, first_bit + p * RB, pb,
-                       (const uint32_t*)(w.ghist + p * R), w.lookback, w.tickets + p,
-                       (const uint32_t*)w.epoch, p, w.errors);
+    if (short_sort)
+      hipLaunchKernelGGL((sort_pass<K, RB, 4>), dim3(tiles), dim3(THREADS), 0, stream,
+                         (const K*)a, (const uint32_t*)av, b, bv, n, first_bit + p * RB, pb,
+                         (const uint32_t*)(w.ghist + p * R), w.lookback, w.tickets + p,
+                         (const uint32_t*)w.epoch, p, w.errors);
+    else
+      hipLaunchKernelGGL((sort_pass<K, RB, ITEMS>), dim3(tiles), dim3(THREADS), 0, stream,
+                         (const K*)a, (const uint32_t*)av, b, bv, n, first_bit + p * RB, pb,
+                         (const uint32_t*)(w.ghist + p * R), w.lookback, w.tickets + p,
+                         (const uint32_t*)w.epoch, p, w.errors);
     FG_LAUNCH_CHECK();
     K* t = a;
     a = b;
@@ -363,8 +373,8 @@ void exclusive_scan_u32(hipStream_t stream, uint32_t* data, uint32_t n, uint32_t
 }
 
 void SortWork::reserve(uint32_t n, size_t key_bytes) {
-  // tiles at the smallest tile (u32 keys: 256 * 12) and the widest radix (11 bits)
-  uint64_t tiles = ceil_div(n > 0 ? n : 1, THREADS * 8);
+  // tiles at the smallest tile (a short u64 sort: 256 * 4) and the widest radix (11 bits)
+  uint64_t tiles = ceil_div(n > 0 ? n : 1, THREADS * 4);
   uint64_t need_lb = tiles * 2048;
   size_t need_keys = (size_t)(n > 0 ? n : 1) * key_bytes;
   if (cap_n >= n && cap_key_bytes >= key_bytes && lookback_words >= need_lb) return;
