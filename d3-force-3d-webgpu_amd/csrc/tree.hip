@@ -7,21 +7,23 @@
 // (collapse).  Codes, order, topology and children come out bit-identical to that
 // algorithm.  The kernels are this backend's own:
 //
-//   tree_bbox     grid-stride min/max with order-preserving u32 atomics; the last block
-//                 to finish turns the box into the cube (one launch instead of two)
+//   tree_bbox     grid-stride min/max into per-workgroup slots; the last workgroup to
+//                 finish folds them and turns the box into the cube (one launch)
 //   tree_keys     exact cellOf -> compact code (high half << HALF | low half) per node
-//   (sort)        LSD radix sort of the 2*HALF-bit codes with the node index as value
-//   tree_leaves   sorted points' (x, y, z, strength) and leaf depths
-//   tree_build    Karras ranges and splits; parent links for the climb
+//   (sort)        LSD radix sort of the high halves with the node index as value, then
+//                 each run of equal high halves ordered on the low halves (order_runs)
+//   tree_leaves_build  sorted points' (x, y, z, strength) and leaf depths, and the Karras
+//                 ranges and splits of the binary nodes with parent links for the climb
 //   tree_reduce   one thread per point climbs to the root; the second of two siblings to
 //                 arrive at a node (atomic flag) combines them -- left, then right, so the
 //                 sums do not depend on timing -- in FP64, with d3's |value| weighting
-//                 (manyBody.js:42-71), and lists the node's d3 children (collapse)
-//   tree_walk     one wave per 64 consecutive sorted points: the wave walks the union of
-//                 its points' traversals from a stack in LDS of (node, lane mask); every
-//                 lane takes d3's own decision at every cell (barnesHut.js:719-746), so
-//                 each point gets exactly its d3 traversal; node records are read with
-//                 wave-uniform loads.
+//                 (manyBody.js:42-71); then lists its node's d3 children (collapse)
+//   tree_walk     one wave per tile of up to 64 consecutive sorted points: the wave walks
+//                 the union of its points' traversals from a stack in LDS of (node, lane
+//                 mask); every lane takes d3's own decision at every cell
+//                 (barnesHut.js:719-746), so each point gets exactly its d3 traversal.
+//
+// A batch of trees (run_batch) uses the same steps with per-job segments (btree_*).
 #include <math.h>
 
 #include "tree.h"
@@ -438,28 +440,7 @@ __global__ void __launch_bounds__(256) order_long_runs(uint32_t n, const uint32_
   }
 }
 
-// ---- leaves -------------------------------------------------------------------------
-
-template <int D>
-__global__ void __launch_bounds__(256) tree_leaves(const uint64_t* __restrict__ skeys,
-                                                   const uint32_t* __restrict__ order,
-                                                   const float4* __restrict__ pts, uint32_t n,
-                                                   float4* __restrict__ leaves,
-                                                   uint32_t* __restrict__ leaf_depth,
-                                                   uint32_t* __restrict__ flags) {
-  uint32_t s = blockIdx.x * 256 + threadIdx.x;
-  if (s >= n) return;
-  leaves[s] = pts[order[s]];
-  uint32_t depth = 0;
-  if (n > 1) {
-    int a = delta<D>(skeys, (int)n, (int)s, (int)s - 1), b = delta<D>(skeys, (int)n, (int)s, (int)s + 1);
-    depth = depth_of<D>(a > b ? a : b) + 1;
-  }
-  leaf_depth[s] = depth;
-  if (s + 1 < n) flags[s] = 0;
-}
-
-// ---- Karras build -------------------------------------------------------------------
+// ---- leaves and Karras build ---------------------------------------------------------
 
 // Karras 2012 node i of the sorted codes k[0, N): the range [first, last] it covers, its
 // split and the common prefix of its range (barnesHut.js:587-624).
@@ -518,13 +499,32 @@ __device__ inline void karras_write(const uint64_t* __restrict__ skeys, int N, i
   if (i == 0) links[node].w = REF_NONE;
 }
 
+
+// One tree's leaves (the sorted points' records and leaf depths) and binary nodes (Karras
+// ranges and splits, parent links for the climb) in one launch: thread s does sorted
+// point s and node s; neither part reads what the other writes.
 template <int D>
-__global__ void __launch_bounds__(256) tree_build(const uint64_t* __restrict__ skeys, uint32_t n,
-                                                  uint32_t block, uint4* __restrict__ links,
-                                                  uint32_t* __restrict__ leaf_parent) {
-  int i = (int)(blockIdx.x * 256 + threadIdx.x);
-  if (i >= (int)n - 1) return;
-  karras_write<D>(skeys, (int)n, i, 0u, block, links, leaf_parent);
+__global__ void __launch_bounds__(256) tree_leaves_build(const uint64_t* __restrict__ skeys,
+                                                         const uint32_t* __restrict__ order,
+                                                         const float4* __restrict__ pts, uint32_t n,
+                                                         uint32_t block, float4* __restrict__ leaves,
+                                                         uint32_t* __restrict__ leaf_depth,
+                                                         uint32_t* __restrict__ flags,
+                                                         uint4* __restrict__ links,
+                                                         uint32_t* __restrict__ leaf_parent) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n) return;
+  leaves[s] = pts[order[s]];
+  uint32_t depth = 0;
+  if (n > 1) {
+    int a = delta<D>(skeys, (int)n, (int)s, (int)s - 1), b = delta<D>(skeys, (int)n, (int)s, (int)s + 1);
+    depth = depth_of<D>(a > b ? a : b) + 1;
+  }
+  leaf_depth[s] = depth;
+  if (s + 1 < n) {
+    flags[s] = 0;
+    karras_write<D>(skeys, (int)n, (int)s, 0u, block, links, leaf_parent);
+  }
 }
 
 // ---- bottom-up reduction + collapse ---------------------------------------------------
@@ -717,21 +717,13 @@ __device__ inline uint32_t reduce_node(uint32_t p, uint32_t lo, ReduceLDS<B>& S,
   return L.w;
 }
 
+// The climb of point s (n >= 2).
 template <int D, uint32_t B>
-__global__ void __launch_bounds__(B) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
-                                                            const uint32_t* __restrict__ leaf_parent,
-                                                            const uint4* __restrict__ links,
-                                                            uint32_t* flags, NodeRec* rec, NodeSum* sums) {
-  __shared__ ReduceLDS<B> S;
-  const uint32_t s = blockIdx.x * B + threadIdx.x;
-  const uint32_t lo = blockIdx.x * B;
-  S.flag[threadIdx.x] = 0;
-  if (s < n) {
-    S.leaf[threadIdx.x] = leaves[s];
-    S.link[threadIdx.x] = links[s];  // index s is a node or a tree's last point (never read then)
-  }
-  __syncthreads();
-  if (s >= n || n < 2) return;
+__device__ inline void reduce_climb(uint32_t s, uint32_t n, uint32_t lo, ReduceLDS<B>& S,
+                                    const float4* __restrict__ leaves,
+                                    const uint32_t* __restrict__ leaf_parent,
+                                    const uint4* __restrict__ links, uint32_t* flags, NodeRec* rec,
+                                    NodeSum* sums) {
   uint32_t up = leaf_parent[s];  // every point has a parent when n >= 2
   uint32_t guard = 0;
   // this workgroup's own nodes: flags, sums, points and links in LDS (REF_NONE, above the
@@ -751,6 +743,34 @@ __global__ void __launch_bounds__(B) tree_reduce(uint32_t n, const float4* __res
     const uint32_t p = up;
     if (publish_count(&flags[p]) == 0 || ++guard >= n) return;
     up = reduce_node<D, false, B>(p, lo, S, leaves, links, rec, sums);
+  }
+}
+
+// COLLAPSE (one tree): thread s also lists d3's children of binary node s once its climb
+// is over (the topology is complete before the kernel starts; the records' children are
+// not what the climb writes), which saves the collapse pass its launch.
+template <int D, uint32_t B, bool COLLAPSE>
+__global__ void __launch_bounds__(B) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
+                                                            const uint32_t* __restrict__ leaf_parent,
+                                                            const uint4* __restrict__ links,
+                                                            uint32_t* flags, NodeRec* rec, NodeSum* sums) {
+  __shared__ ReduceLDS<B> S;
+  const uint32_t s = blockIdx.x * B + threadIdx.x;
+  const uint32_t lo = blockIdx.x * B;
+  S.flag[threadIdx.x] = 0;
+  if (s < n) {
+    S.leaf[threadIdx.x] = leaves[s];
+    S.link[threadIdx.x] = links[s];  // index s is a node or a tree's last point (never read then)
+  }
+  __syncthreads();
+  if (s >= n || n < 2) return;
+  reduce_climb<D, B>(s, n, lo, S, leaves, leaf_parent, links, flags, rec, sums);
+  if (COLLAPSE && s + 1 < n) {
+    uint32_t kids[8];
+    collapse_into<D>(links[s], links, kids);
+    uint4* out = (uint4*)rec[s].kids;
+    out[0] = make_uint4(kids[0], kids[1], kids[2], kids[3]);
+    out[1] = make_uint4(kids[4], kids[5], kids[6], kids[7]);
   }
 }
 
@@ -1557,7 +1577,7 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
                      (const uint32_t*)sp.seg_of, (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.links,
                      t.leaf_parent);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK>), dim3(ceil_div(total, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st,
+  hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK, false>), dim3(ceil_div(total, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st,
                      total, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent, (const uint4*)t.links,
                      t.flags, (NodeRec*)t.rec, (NodeSum*)t.sums);
   FG_LAUNCH_CHECK();
@@ -1598,29 +1618,24 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   if (timer) timer->end(2), timer->begin(1);
   t.skeys = sk;
   t.order = sv;
-  hipLaunchKernelGGL((tree_leaves<D>), dim3(blocks), dim3(256), 0, st, (const uint64_t*)sk,
-                     (const uint32_t*)sv, (const float4*)t.pts, n, t.leaves, t.leaf_depth, t.flags);
+  // a small tree is reduced by one workgroup: every node local, no device-wide hand-off
+  const uint32_t rblock = n <= REDUCE_ONE ? REDUCE_ONE : REDUCE_BLOCK;
+  hipLaunchKernelGGL((tree_leaves_build<D>), dim3(blocks), dim3(256), 0, st, (const uint64_t*)sk,
+                     (const uint32_t*)sv, (const float4*)t.pts, n, rblock, t.leaves, t.leaf_depth, t.flags,
+                     t.links, t.leaf_parent);
   FG_LAUNCH_CHECK();
   if (n < 2) {
     if (timer) timer->end(1);
     return;
   }
-  // a small tree is reduced by one workgroup: every node local, no device-wide hand-off
-  const uint32_t rblock = n <= REDUCE_ONE ? REDUCE_ONE : REDUCE_BLOCK;
-  hipLaunchKernelGGL((tree_build<D>), dim3(ceil_div(n - 1, 256)), dim3(256), 0, st,
-                     (const uint64_t*)sk, n, rblock, t.links, t.leaf_parent);
-  FG_LAUNCH_CHECK();
   if (rblock == REDUCE_ONE)
-    hipLaunchKernelGGL((tree_reduce<D, REDUCE_ONE>), dim3(1), dim3(REDUCE_ONE), 0, st, n,
+    hipLaunchKernelGGL((tree_reduce<D, REDUCE_ONE, true>), dim3(1), dim3(REDUCE_ONE), 0, st, n,
                        (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
                        (const uint4*)t.links, t.flags, t.rec, t.sums);
   else
-    hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0, st,
-                       n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
+    hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK, true>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0,
+                       st, n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
                        (const uint4*)t.links, t.flags, t.rec, t.sums);
-  FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL((tree_collapse<D>), dim3(blocks), dim3(256), 0, st, n, (const uint32_t*)nullptr,
-                     (const uint32_t*)nullptr, (const BJob*)nullptr, (const uint4*)t.links, t.rec);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(3);
   WalkArgs A;
