@@ -440,6 +440,42 @@ __global__ void __launch_bounds__(256) order_long_runs(uint32_t n, const uint32_
   }
 }
 
+// ---- small trees: the whole codes in order by counting ------------------------------
+
+// Up to RANK_MAX points, one launch puts the codes in order instead of the radix sort and
+// the run ordering (six launches): the point with code k and index i goes to the number
+// of points whose (code, index) is smaller, which is the stable order on the whole codes
+// that those give.  A workgroup takes 64 points; its RANK_PARTS waves count over one part
+// of all the points each (n^2 comparisons in all, read from LDS as broadcasts).
+constexpr uint32_t RANK_MAX = 4096, RANK_PARTS = 8;
+
+__global__ void __launch_bounds__(RANK_PARTS * 64) tree_rank_sort(const uint64_t* __restrict__ keys,
+                                                                  const uint32_t* __restrict__ vals,
+                                                                  uint32_t n, uint64_t* __restrict__ okeys,
+                                                                  uint32_t* __restrict__ ovals) {
+  __shared__ uint64_t sk[RANK_MAX];
+  __shared__ uint32_t part[RANK_PARTS][64];
+  for (uint32_t j = threadIdx.x; j < n; j += RANK_PARTS * 64) sk[j] = keys[j];
+  __syncthreads();
+  const uint32_t q = threadIdx.x / 64, li = threadIdx.x & 63;
+  const uint32_t i = blockIdx.x * 64 + li;
+  const uint64_t my = i < n ? sk[i] : 0ull;
+  const uint32_t j0 = (uint32_t)((uint64_t)n * q / RANK_PARTS), j1 = (uint32_t)((uint64_t)n * (q + 1) / RANK_PARTS);
+  uint32_t rank = 0;
+  for (uint32_t j = j0; j < j1; ++j) {
+    const uint64_t k = sk[j];
+    rank += (k < my || (k == my && j < i)) ? 1u : 0u;
+  }
+  part[q][li] = rank;
+  __syncthreads();
+  if (q != 0 || i >= n) return;
+  uint32_t r = 0;
+#pragma unroll
+  for (uint32_t p = 0; p < RANK_PARTS; ++p) r += part[p][li];
+  okeys[r] = my;
+  ovals[r] = vals[i];
+}
+
 // ---- leaves and Karras build ---------------------------------------------------------
 
 // Karras 2012 node i of the sorted codes k[0, N): the range [first, last] it covers, its
@@ -1602,19 +1638,29 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   hipLaunchKernelGGL((tree_keys<D>), dim3(blocks), dim3(256), 0, st, x, y, z, strength, n,
                      (const CubeInfo*)t.cube, t.keys, t.vals, t.pts);
   FG_LAUNCH_CHECK();
-  uint64_t* hk;
-  uint32_t* hv;
   if (timer) timer->end(1), timer->begin(2);
-  // the high halves, then every run of equal high halves in order (into the other buffers)
-  radix_sort_u64(st, t.sort, radix_bits, t.keys, t.vals, n, Dims<D>::HALF, Dims<D>::HALF, &hk, &hv, true);
-  uint64_t* sk = hk == t.keys ? (uint64_t*)t.sort.keys_alt : t.keys;
-  uint32_t* sv = hv == t.vals ? t.sort.vals_alt : t.vals;
-  hipLaunchKernelGGL((order_runs<D>), dim3(blocks), dim3(256), 0, st, n, (const uint64_t*)hk, (const uint32_t*)hv,
-                     (const uint32_t*)nullptr, sk, sv, t.counters, t.long_runs);
-  FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL((order_long_runs<D>), dim3(LONG_RUN_BLOCKS), dim3(256), 0, st, n, (const uint32_t*)t.counters,
-                     (const uint32_t*)t.long_runs, (const uint32_t*)nullptr, sk, sv, hk, hv);
-  FG_LAUNCH_CHECK();
+  uint64_t* sk;
+  uint32_t* sv;
+  if (n <= RANK_MAX) {
+    sk = (uint64_t*)t.sort.keys_alt;
+    sv = t.sort.vals_alt;
+    hipLaunchKernelGGL(tree_rank_sort, dim3(ceil_div(n, 64)), dim3(RANK_PARTS * 64), 0, st,
+                       (const uint64_t*)t.keys, (const uint32_t*)t.vals, n, sk, sv);
+    FG_LAUNCH_CHECK();
+  } else {
+    uint64_t* hk;
+    uint32_t* hv;
+    // the high halves, then every run of equal high halves in order (into the other buffers)
+    radix_sort_u64(st, t.sort, radix_bits, t.keys, t.vals, n, Dims<D>::HALF, Dims<D>::HALF, &hk, &hv, true);
+    sk = hk == t.keys ? (uint64_t*)t.sort.keys_alt : t.keys;
+    sv = hv == t.vals ? t.sort.vals_alt : t.vals;
+    hipLaunchKernelGGL((order_runs<D>), dim3(blocks), dim3(256), 0, st, n, (const uint64_t*)hk, (const uint32_t*)hv,
+                       (const uint32_t*)nullptr, sk, sv, t.counters, t.long_runs);
+    FG_LAUNCH_CHECK();
+    hipLaunchKernelGGL((order_long_runs<D>), dim3(LONG_RUN_BLOCKS), dim3(256), 0, st, n, (const uint32_t*)t.counters,
+                       (const uint32_t*)t.long_runs, (const uint32_t*)nullptr, sk, sv, hk, hv);
+    FG_LAUNCH_CHECK();
+  }
   if (timer) timer->end(2), timer->begin(1);
   t.skeys = sk;
   t.order = sv;
