@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(256) order_long_runs(uint32_t n, const uint32_
 // of points whose (code, index) is smaller, which is the stable order on the whole codes
 // that those give.  A workgroup takes 64 points; its RANK_PARTS waves count over one part
 // of all the points each (n^2 comparisons in all, read from LDS as broadcasts).
-constexpr uint32_t RANK_MAX = 4096, RANK_PARTS = 8;
+constexpr uint32_t RANK_MAX = 8192, RANK_PARTS = 16;
 
 __global__ void __launch_bounds__(RANK_PARTS * 64) tree_rank_sort(const uint64_t* __restrict__ keys,
                                                                   const uint32_t* __restrict__ vals,
