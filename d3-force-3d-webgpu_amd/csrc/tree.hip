@@ -671,8 +671,8 @@ __device__ inline void collapse_into(uint4 L, const uint4* links, uint32_t* kids
   for (; m < 8; ++m) kids[m] = REF_NONE;
 }
 
-// d3's children of every binary node (a pass of its own: it depends on the topology only,
-// so it stays off the climb's chain of dependent loads).
+// d3's children of every binary node of a batch's trees (a pass of its own: it depends on
+// the topology only; a single tree's reduce threads do it after their climb instead).
 template <int D>
 __global__ void __launch_bounds__(256) tree_collapse(uint32_t n, const uint32_t* __restrict__ seg_of,
                                                      const uint32_t* __restrict__ seg_job,
