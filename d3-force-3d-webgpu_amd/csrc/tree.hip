@@ -816,7 +816,7 @@ constexpr int WALK_WAVES = 4;
 // Trees of at most this many points are walked in the exact mode: forces that do not
 // depend on how the points are tiled (so small trees can take small tiles, more waves);
 // larger trees fill the device with 64-point tiles and take the cheaper mode.
-constexpr uint32_t WALK_EXACT_MAX = 65536;
+constexpr uint32_t WALK_EXACT_MAX = 131072;
 constexpr int WALK_STACK = 48;   // stack entries per wave in LDS (a walk at 1M points peaks near 30)
 constexpr int WALK_DEPTH = 192;  // stack entries per wave in all; 7 * 21 + 8 = 155 in 3D for distinct codes
 
@@ -1279,15 +1279,14 @@ struct WalkArgs {
 
 // Points per tile of an exact walk over n points.  A small walk is bound by the latency of
 // each wave's chain of openings, and a smaller tile has a shorter chain but adds waves and
-// total work; measured on 256 CUs (link + manyBody + center ticks, 1k..65k points), the
-// best tile keeps about 2 tiles per SIMD or fewer, and 4-point tiles pay only for the
-// smallest trees (about 0.6 tiles per SIMD).  `want` (1..64) overrides.
+// total work; measured on 256 CUs (link + manyBody + center ticks, 1k..65k points, tiles of
+// 1..32), the best is the smallest tile that leaves at most about 4.5 tiles per SIMD (all
+// of them resident, each wave on its own tile).  `want` (1..64) overrides.
 inline uint32_t walk_tile_for(uint32_t n, int cus, uint32_t want) {
   if (want >= 1u && want <= (uint32_t)WAVE) return want;
   const uint64_t simds = 4ull * (uint64_t)(cus > 0 ? cus : 1);
-  if ((uint64_t)((n + 3) / 4) * 5 <= simds * 3) return 4;
-  for (uint32_t t = 8; t < (uint32_t)WAVE; t *= 2)
-    if ((uint64_t)((n + t - 1) / t) * 5 <= simds * 11) return t;
+  for (uint32_t t = 1; t < (uint32_t)WAVE; t *= 2)
+    if ((uint64_t)((n + t - 1) / t) * 2 <= simds * 9) return t;
   return (uint32_t)WAVE;
 }
 
