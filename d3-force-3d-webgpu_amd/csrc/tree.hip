@@ -1277,16 +1277,20 @@ struct WalkArgs {
   uint32_t tile;        // points per tile (1..64)
 };
 
-// Points per tile of an exact walk over n points.  A small walk is bound by the latency of
-// each wave's chain of openings, and a smaller tile has a shorter chain but adds waves and
-// total work; measured on 256 CUs (link + manyBody + center ticks, 1k..65k points, tiles of
-// 1..32), the best is the smallest tile that leaves at most about 4.5 tiles per SIMD (all
-// of them resident, each wave on its own tile).  `want` (1..64) overrides.
-inline uint32_t walk_tile_for(uint32_t n, int cus, uint32_t want) {
+// Points per tile of an exact walk over n points, on a device of `simds` SIMDs where the
+// walk keeps `waves` waves resident.  A small walk is bound by the latency of each wave's
+// chain of openings: a smaller tile has a shorter chain but adds waves and total work.
+// Measured on 256 CUs (link + manyBody + center ticks, 1k..120k points, tiles of 1..32):
+// the tiles of 1, 2 and 4 points pay up to about 2.5 tiles per SIMD, 8-point tiles up to
+// about 5, larger ones as long as every tile has its own resident wave (no queue).  `want`
+// (1..64) overrides.
+inline uint32_t walk_tile_for(uint32_t n, uint32_t simds, uint32_t waves, uint32_t want) {
   if (want >= 1u && want <= (uint32_t)WAVE) return want;
-  const uint64_t simds = 4ull * (uint64_t)(cus > 0 ? cus : 1);
-  for (uint32_t t = 1; t < (uint32_t)WAVE; t *= 2)
-    if ((uint64_t)((n + t - 1) / t) * 2 <= simds * 9) return t;
+  for (uint32_t t = 1; t < (uint32_t)WAVE; t *= 2) {
+    const uint64_t tiles = (n + t - 1) / t;
+    const uint64_t most = t <= 4u ? (uint64_t)simds * 5 / 2 : t == 8u ? (uint64_t)simds * 5 : (uint64_t)waves;
+    if (tiles <= most && tiles <= waves) return t;
+  }
   return (uint32_t)WAVE;
 }
 
@@ -1691,10 +1695,10 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   A.packed_max = p.walk_packed_max;
   const int cus = device_cus();
   const bool exact = n <= WALK_EXACT_MAX;
-  A.tile = exact ? walk_tile_for(n, cus, p.walk_tile) : (uint32_t)WAVE;
   auto kern = exact ? tree_walk<D, WALK_STACK, WALK_WAVES, true> : tree_walk<D, WALK_STACK, WALK_WAVES, false>;
   uint32_t wblocks = resident_blocks(kern, WALK_WAVES * WAVE, cus);
   if (wblocks > (uint32_t)cus * (32 / WALK_WAVES)) wblocks = (uint32_t)cus * (32 / WALK_WAVES);
+  A.tile = exact ? walk_tile_for(n, 4u * (uint32_t)cus, wblocks * WALK_WAVES, p.walk_tile) : (uint32_t)WAVE;
   const uint32_t need = ceil_div(ceil_div(n, A.tile), WALK_WAVES);
   if (wblocks > need) wblocks = need;
   if (t.spill_bytes < spill_bytes(cus, WALK_STACK)) t.reserve_spill(spill_bytes(cus, WALK_STACK));
