@@ -816,7 +816,7 @@ constexpr int WALK_WAVES = 4;
 // Trees of at most this many points are walked in the exact mode: forces that do not
 // depend on how the points are tiled (so small trees can take small tiles, more waves);
 // larger trees fill the device with 64-point tiles and take the cheaper mode.
-constexpr uint32_t WALK_EXACT_MAX = 131072;
+constexpr uint32_t WALK_EXACT_MAX = 163840;
 constexpr int WALK_STACK = 48;   // stack entries per wave in LDS (a walk at 1M points peaks near 30)
 constexpr int WALK_DEPTH = 192;  // stack entries per wave in all; 7 * 21 + 8 = 155 in 3D for distinct codes
 
@@ -1280,15 +1280,16 @@ struct WalkArgs {
 // Points per tile of an exact walk over n points, on a device of `simds` SIMDs where the
 // walk keeps `waves` waves resident.  A small walk is bound by the latency of each wave's
 // chain of openings: a smaller tile has a shorter chain but adds waves and total work.
-// Measured on 256 CUs (link + manyBody + center ticks, 1k..120k points, tiles of 1..32):
-// the tiles of 1, 2 and 4 points pay up to about 2.5 tiles per SIMD, 8-point tiles up to
-// about 5, larger ones as long as every tile has its own resident wave (no queue).  `want`
-// (1..64) overrides.
+// Measured on 256 CUs (link + manyBody + center ticks, 1k..200k points, tiles of 1..64):
+// the tiles of 1, 2 and 4 points pay up to about 2.5 tiles per SIMD, 8- and 32-point ones
+// up to about 5, 16-point ones as long as every tile has its own resident wave (no
+// queue).  `want` (1..64) overrides.
 inline uint32_t walk_tile_for(uint32_t n, uint32_t simds, uint32_t waves, uint32_t want) {
   if (want >= 1u && want <= (uint32_t)WAVE) return want;
   for (uint32_t t = 1; t < (uint32_t)WAVE; t *= 2) {
     const uint64_t tiles = (n + t - 1) / t;
-    const uint64_t most = t <= 4u ? (uint64_t)simds * 5 / 2 : t == 8u ? (uint64_t)simds * 5 : (uint64_t)waves;
+    const uint64_t most = t <= 4u ? (uint64_t)simds * 5 / 2
+                          : t == 16u ? (uint64_t)waves : (uint64_t)simds * 5;
     if (tiles <= most && tiles <= waves) return t;
   }
   return (uint32_t)WAVE;

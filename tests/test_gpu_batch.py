@@ -141,9 +141,9 @@ def plain_run(n, nd, seed, ticks, **kw):
     return p, v
 
 
-@pytest.mark.parametrize("n,nd,ticks", [(3000, 3, 30), (2000, 2, 30), (140000, 3, 3)])
+@pytest.mark.parametrize("n,nd,ticks", [(3000, 3, 30), (2000, 2, 30), (170000, 3, 3)])
 def test_batch_equals_single_simulation_link_manybody(n, nd, ticks):
-    """Bit for bit; 140000 nodes is past the exact-mode walk (64-point tiles, the cheaper
+    """Bit for bit; 170000 nodes is past the exact-mode walk (64-point tiles, the cheaper
     accumulation)."""
     p1, v1 = plain_run(n, nd, 21, ticks, collide=False, center=False)
     p2, v2 = run_alone(make_job(n, nd, 21, collide=False, center=False), ticks)

@@ -131,7 +131,7 @@ def test_manybody_tree_sizes(engine_factory, n):
 @pytest.mark.parametrize("dims", [1, 2, 3])
 @pytest.mark.parametrize("coincident", [False, True])
 def test_manybody_tree_forces_do_not_depend_on_tiling(monkeypatch, dims, coincident):
-    """Trees of up to 131072 points are walked in the exact mode: every point's force comes
+    """Trees of up to 163840 points are walked in the exact mode: every point's force comes
     out bit for bit the same whatever the tile size (points per wave) and whichever mode
     (one lane per point, or packed (point, child) items) ran each opening -- packed limit 0
     runs every opening one lane per point, 64 every opening packed.  A clump makes deep
@@ -170,8 +170,8 @@ def test_manybody_tree_forces_do_not_depend_on_tiling(monkeypatch, dims, coincid
 
 
 def test_manybody_tree_above_exact_mode(engine_factory):
-    """Past 131072 points the walk takes 64-point tiles and the cheaper accumulation."""
-    n = 140000
+    """Past 163840 points the walk takes 64-point tiles and the cheaper accumulation."""
+    n = 170000
     pos = make_nodes_fast(n, 3, 20 * np.cbrt(n))
     s = np.full(n, -30.0)
     _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s)])
