@@ -125,15 +125,10 @@ __device__ inline void block_box(float lo[3], float hi[3], uint32_t* box) {
 }
 
 // d3's cover() of the box (barnesHut.js:169-181): origin = floor(min), size = the smallest
-// power of two with max < origin + size; shift = LEVELS - log2(size).  Resets the box.
+// power of two with max < origin + size; shift = LEVELS - log2(size).
 template <int D>
-__device__ inline void cover_box(uint32_t* box, float origin[3], float& size, int32_t& shift) {
-  float blo[3], bhi[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    blo[a] = float_unordered(__hip_atomic_load(&box[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    bhi[a] = float_unordered(__hip_atomic_load(&box[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  }
+__device__ inline void cover_of(const float blo[3], const float bhi[3], float origin[3], float& size,
+                                int32_t& shift) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) origin[a] = floorf(blo[a]);
   size = 1.0f;
@@ -144,6 +139,18 @@ __device__ inline void cover_box(uint32_t* box, float origin[3], float& size, in
     log2size += 1;
   }
   shift = Dims<D>::LEVELS - log2size;
+}
+
+// The same of an order-preserving box in global memory, which it resets.
+template <int D>
+__device__ inline void cover_box(uint32_t* box, float origin[3], float& size, int32_t& shift) {
+  float blo[3], bhi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    blo[a] = float_unordered(__hip_atomic_load(&box[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    bhi[a] = float_unordered(__hip_atomic_load(&box[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  cover_of<D>(blo, bhi, origin, size, shift);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     box[a] = 0xffffffffu;
@@ -256,9 +263,14 @@ __global__ void __launch_bounds__(256) tree_keys(const float* __restrict__ x,
                                                  const CubeInfo* __restrict__ cube,
                                                  uint64_t* __restrict__ keys,
                                                  uint32_t* __restrict__ vals,
-                                                 float4* __restrict__ pts) {
+                                                 float4* __restrict__ pts, uint32_t* __restrict__ rank,
+                                                 uint32_t* __restrict__ groups_done) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
+  if (rank) {  // the counting sort's ranks and its groups' counters
+    rank[i] = 0;
+    if (i < (n + 63u) / 64u) groups_done[i] = 0;
+  }
   const float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
   keys[i] = point_key<D>(p, cube->origin, cube->shift);
   vals[i] = i;
@@ -442,38 +454,57 @@ __global__ void __launch_bounds__(256) order_long_runs(uint32_t n, const uint32_
 
 // ---- small trees: the whole codes in order by counting ------------------------------
 
-// Up to RANK_MAX points, one launch puts the codes in order instead of the radix sort and
-// the run ordering (six launches): the point with code k and index i goes to the number
-// of points whose (code, index) is smaller, which is the stable order on the whole codes
-// that those give.  A workgroup takes 64 points; its RANK_PARTS waves count over one part
-// of all the points each (n^2 comparisons in all, read from LDS as broadcasts).
-constexpr uint32_t RANK_MAX = 8192, RANK_PARTS = 16;
+// Up to RANK_MAX points, the codes go in order by counting instead of by the radix sort
+// and the run ordering (six launches): the point with code k and index i goes to the
+// number of points whose (code, index) is smaller, which is the stable order on the whole
+// codes that those give.  Workgroup (g, q) counts for the 64 points of group g over part q
+// of the codes (gridDim.y parts, so that the n^2 comparisons spread over the whole device;
+// its RANK_WAVES waves split the part again, each code read from LDS as a broadcast) and
+// adds its counts to the points' ranks; the last of a group's workgroups to finish places
+// the group's points.  The ranks and the groups' counters start at zero (tree_keys).
+constexpr uint32_t RANK_MAX = 8192, RANK_WAVES = 16;
 
-__global__ void __launch_bounds__(RANK_PARTS * 64) tree_rank_sort(const uint64_t* __restrict__ keys,
+__global__ void __launch_bounds__(RANK_WAVES * 64) tree_rank_sort(const uint64_t* __restrict__ keys,
                                                                   const uint32_t* __restrict__ vals,
-                                                                  uint32_t n, uint64_t* __restrict__ okeys,
+                                                                  uint32_t n, uint32_t* rank, uint32_t* done,
+                                                                  uint64_t* __restrict__ okeys,
                                                                   uint32_t* __restrict__ ovals) {
-  __shared__ uint64_t sk[RANK_MAX];
-  __shared__ uint32_t part[RANK_PARTS][64];
-  for (uint32_t j = threadIdx.x; j < n; j += RANK_PARTS * 64) sk[j] = keys[j];
+  extern __shared__ uint64_t sk[];  // this workgroup's part of the codes
+  __shared__ uint32_t part[RANK_WAVES][64];
+  const uint32_t parts = gridDim.y, q = blockIdx.y, g = blockIdx.x;
+  const uint32_t k0 = (uint32_t)((uint64_t)n * q / parts), k1 = (uint32_t)((uint64_t)n * (q + 1) / parts);
+  for (uint32_t j = k0 + threadIdx.x; j < k1; j += RANK_WAVES * 64) sk[j - k0] = keys[j];
+  const uint32_t w = threadIdx.x / 64, li = threadIdx.x & 63;
+  const uint32_t i = g * 64 + li, first = g * 64, last = min(first + 64, n);
+  const uint64_t my = i < n ? keys[i] : 0ull;
   __syncthreads();
-  const uint32_t q = threadIdx.x / 64, li = threadIdx.x & 63;
-  const uint32_t i = blockIdx.x * 64 + li;
-  const uint64_t my = i < n ? sk[i] : 0ull;
-  const uint32_t j0 = (uint32_t)((uint64_t)n * q / RANK_PARTS), j1 = (uint32_t)((uint64_t)n * (q + 1) / RANK_PARTS);
-  uint32_t rank = 0;
-  for (uint32_t j = j0; j < j1; ++j) {
-    const uint64_t k = sk[j];
-    rank += (k < my || (k == my && j < i)) ? 1u : 0u;
-  }
-  part[q][li] = rank;
-  __syncthreads();
-  if (q != 0 || i >= n) return;
+  const uint32_t cnt = k1 - k0;
+  const uint32_t a = k0 + cnt * w / RANK_WAVES, b = k0 + cnt * (w + 1) / RANK_WAVES;
   uint32_t r = 0;
+  if (b <= first) {  // every code here has a smaller index: equal codes count
+    for (uint32_t j = a; j < b; ++j) r += sk[j - k0] <= my ? 1u : 0u;
+  } else if (a >= last) {  // every code here has a larger index
+    for (uint32_t j = a; j < b; ++j) r += sk[j - k0] < my ? 1u : 0u;
+  } else {
+    for (uint32_t j = a; j < b; ++j) {
+      const uint64_t k = sk[j - k0];
+      r += (k < my || (k == my && j < i)) ? 1u : 0u;
+    }
+  }
+  part[w][li] = r;
+  __syncthreads();
+  if (w != 0) return;
+  uint32_t sum = 0;
 #pragma unroll
-  for (uint32_t p = 0; p < RANK_PARTS; ++p) r += part[p][li];
-  okeys[r] = my;
-  ovals[r] = vals[i];
+  for (uint32_t p = 0; p < RANK_WAVES; ++p) sum += part[p][li];
+  if (i < n) __hip_atomic_fetch_add(&rank[i], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t before = 0;
+  if (li == 0) before = publish_count(&done[g]);  // after this wave's additions completed
+  before = __shfl(before, 0, WAVE);
+  if (before + 1 != parts || i >= n) return;
+  const uint32_t at = load_coherent(&rank[i]);
+  okeys[at] = my;
+  ovals[at] = vals[i];
 }
 
 // ---- leaves and Karras build ---------------------------------------------------------
@@ -780,6 +811,134 @@ __device__ inline void reduce_climb(uint32_t s, uint32_t n, uint32_t lo, ReduceL
     if (publish_count(&flags[p]) == 0 || ++guard >= n) return;
     up = reduce_node<D, false, B>(p, lo, S, leaves, links, rec, sums);
   }
+}
+
+// A tree of at most REDUCE_ONE points built and reduced by one workgroup, one point per
+// thread, in one launch instead of five (tree_bbox, tree_keys, tree_rank_sort,
+// tree_leaves_build, tree_reduce), with the same arithmetic: the box over the waves and
+// the cube; the codes; their stable order; the leaves, leaf depths and Karras nodes, the
+// nodes' links and parents kept in LDS for the climb; the reduction and d3's children.
+//
+// The order: (code, index) pairs merged in runs of 1, 2, 4, ... between two LDS buffers.
+// A pair's place in the merge of its run with the sibling run is its place in its own run
+// plus the number of the sibling's pairs below it (a binary search); the pairs are all
+// distinct, so this is the order on (code, index), the stable order that the rank sort and
+// the radix sort give.  log2(n) levels of at most log2(n) + 1 LDS reads each, where the
+// counting of the rank sort would take n reads per point on the one CU.
+template <int D>
+__global__ void __launch_bounds__(REDUCE_ONE) tree_small(const float* __restrict__ x, const float* __restrict__ y,
+                                                         const float* __restrict__ z,
+                                                         const float* __restrict__ strength, uint32_t n,
+                                                         CubeInfo* __restrict__ cube, uint32_t* __restrict__ counters,
+                                                         uint64_t* __restrict__ skeys, uint32_t* __restrict__ order,
+                                                         float4* __restrict__ leaves, uint32_t* __restrict__ leaf_depth,
+                                                         uint4* __restrict__ links, uint32_t* flags, NodeRec* rec,
+                                                         NodeSum* sums) {
+  constexpr uint32_t B = REDUCE_ONE, W = B / WAVE;
+  __shared__ ReduceLDS<B> S;
+  __shared__ uint64_t s_key[2][B];
+  __shared__ uint32_t s_idx[2][B];
+  __shared__ uint32_t s_parent[B];
+  __shared__ uint32_t s_m[W][6];
+  __shared__ float s_origin[3];
+  __shared__ int32_t s_shift;
+  const uint32_t i = threadIdx.x, lane = i & (WAVE - 1), wave = i / WAVE;
+  if (i < TREE_COUNTERS) counters[i] = 0;
+  // the box: per wave, then over the waves in order-preserving words (as tree_bbox)
+  float p[3] = {0.f, 0.f, 0.f};
+  float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  if (i < n) {
+    p[0] = x[i];
+    if (D > 1) p[1] = y[i];
+    if (D > 2) p[2] = z[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) lo[a] = p[a], hi[a] = p[a];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], o, WAVE));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], o, WAVE));
+    }
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) s_m[wave][a] = float_ordered(lo[a]), s_m[wave][3 + a] = float_ordered(hi[a]);
+  __syncthreads();
+  if (i == 0) {
+    uint32_t m[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    for (uint32_t w = 0; w < W; ++w)
+      for (int a = 0; a < 3; ++a) m[a] = min(m[a], s_m[w][a]), m[3 + a] = max(m[3 + a], s_m[w][3 + a]);
+    float blo[3], bhi[3], origin[3], size;
+    int32_t shift;
+    for (int a = 0; a < 3; ++a) blo[a] = float_unordered(m[a]), bhi[a] = float_unordered(m[3 + a]);
+    cover_of<D>(blo, bhi, origin, size, shift);
+    for (int a = 0; a < 3; ++a) s_origin[a] = origin[a], cube->origin[a] = origin[a];
+    s_shift = shift;
+    cube->size = size;
+    cube->shift = shift;
+  }
+  __syncthreads();
+  // the codes, merged into order; the result lands in buffer 0
+  uint32_t levels = 0;
+  while ((1u << levels) < n) ++levels;
+  uint32_t cur = levels & 1u, pos = i;
+  uint64_t code = 0;
+  if (i < n) {
+    const float origin[3] = {s_origin[0], s_origin[1], s_origin[2]};
+    code = point_key<D>(p, origin, s_shift);
+    s_key[cur][i] = code;
+    s_idx[cur][i] = i;
+  }
+  __syncthreads();
+  for (uint32_t r = 1; r < n; r <<= 1, cur ^= 1u) {
+    if (i < n) {
+      const uint32_t sib = (pos ^ r) & ~(r - 1u);  // first place of the sibling run
+      const uint32_t len = sib < n ? min(r, n - sib) : 0u;
+      uint32_t below = 0;  // the sibling's pairs below (code, i): branch-free lower bound
+      for (uint32_t step = r; step > 0; step >>= 1) {
+        if (below + step <= len) {
+          const uint32_t j = sib + below + step - 1u;
+          const uint64_t k = s_key[cur][j];
+          if (k < code || (k == code && s_idx[cur][j] < i)) below += step;
+        }
+      }
+      pos = (pos & ~(2u * r - 1u)) + (pos & (r - 1u)) + below;
+      s_key[cur ^ 1u][pos] = code;
+      s_idx[cur ^ 1u][pos] = i;
+    }
+    __syncthreads();
+  }
+  // sorted place pos: the leaf, its code and point
+  if (i < n) {
+    const float4 leaf = make_float4(p[0], p[1], p[2], strength[i]);
+    S.leaf[pos] = leaf;
+    leaves[pos] = leaf;
+    skeys[pos] = code;
+    order[pos] = i;
+  }
+  S.flag[i] = 0;
+  // sorted point i: its depth; binary node i: its links, and its parent links (in LDS)
+  const uint64_t* sorted = s_key[0];
+  if (i < n) {
+    uint32_t depth = 0;
+    if (n > 1) {
+      const int a = delta<D>(sorted, (int)n, (int)i, (int)i - 1), b = delta<D>(sorted, (int)n, (int)i, (int)i + 1);
+      depth = depth_of<D>(a > b ? a : b) + 1;
+    }
+    leaf_depth[i] = depth;
+    if (i + 1 < n) karras_write<D>(sorted, (int)n, (int)i, 0u, B, S.link, s_parent);
+  }
+  __syncthreads();
+  if (i >= n || n < 2) return;
+  // every point climbs; binary nodes are 0 .. n - 2
+  if (i + 1 < n) links[i] = S.link[i];
+  reduce_climb<D, B>(i, n, 0u, S, leaves, s_parent, S.link, flags, rec, sums);
+  if (i + 1 >= n) return;
+  uint32_t kids[8];
+  collapse_into<D>(S.link[i], S.link, kids);
+  uint4* out = (uint4*)rec[i].kids;
+  out[0] = make_uint4(kids[0], kids[1], kids[2], kids[3]);
+  out[1] = make_uint4(kids[4], kids[5], kids[6], kids[7]);
 }
 
 // COLLAPSE (one tree): thread s also lists d3's children of binary node s once its climb
@@ -1629,27 +1788,70 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
   if (timer) timer->end(3);
 }
 
+// The walk over a built tree: forces into v (barnesHut.js:434-470).
+template <int D>
+void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, float* vz, const TreeParams& p,
+          StepParams step, PhaseTimer* timer) {
+  if (timer) timer->begin(3);
+  WalkArgs A;
+  A.theta2 = p.theta2;
+  A.dmin2 = p.dmin2;
+  A.dmax2 = p.dmax2;
+  A.n = n;
+  A.packed_max = p.walk_packed_max;
+  const int cus = device_cus();
+  const bool exact = n <= WALK_EXACT_MAX;
+  auto kern = exact ? tree_walk<D, WALK_STACK, WALK_WAVES, true> : tree_walk<D, WALK_STACK, WALK_WAVES, false>;
+  uint32_t wblocks = resident_blocks(kern, WALK_WAVES * WAVE, cus);
+  if (wblocks > (uint32_t)cus * (32 / WALK_WAVES)) wblocks = (uint32_t)cus * (32 / WALK_WAVES);
+  A.tile = exact ? walk_tile_for(n, 4u * (uint32_t)cus, wblocks * WALK_WAVES, p.walk_tile) : (uint32_t)WAVE;
+  const uint32_t need = ceil_div(ceil_div(n, A.tile), WALK_WAVES);
+  if (wblocks > need) wblocks = need;
+  if (t.spill_bytes < spill_bytes(cus, WALK_STACK)) t.reserve_spill(spill_bytes(cus, WALK_STACK));
+  hipLaunchKernelGGL(kern, dim3(wblocks), dim3(WALK_WAVES * WAVE), 0, st, (const float4*)t.leaves,
+                     (const uint32_t*)t.order, (const NodeRec*)t.rec, (const CubeInfo*)t.cube, vx, vy, vz,
+                     t.counters, (SpillEntry*)t.spill, step, A);
+  FG_LAUNCH_CHECK();
+  if (timer) timer->end(3);
+}
+
 template <int D>
 void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float* y, const float* z,
          const float* strength, float* vx, float* vy, float* vz, const TreeParams& p,
          StepParams step, int radix_bits, PhaseTimer* timer) {
-  const uint32_t blocks = ceil_div(n, 256);
   if (timer) timer->begin(1);
+  if (n <= REDUCE_ONE) {  // one workgroup builds and reduces the whole tree
+    hipLaunchKernelGGL((tree_small<D>), dim3(1), dim3(REDUCE_ONE), 0, st, x, y, z, strength, n, t.cube,
+                       t.counters, t.keys, t.vals, t.leaves, t.leaf_depth, t.links, t.flags, t.rec, t.sums);
+    FG_LAUNCH_CHECK();
+    t.skeys = t.keys;
+    t.order = t.vals;
+    if (timer) timer->end(1);
+    if (n >= 2) walk<D>(st, t, n, vx, vy, vz, p, step, timer);
+    return;
+  }
+  const uint32_t blocks = ceil_div(n, 256);
   const SortClear sc = sort_prepare(t.sort, radix_bits, n, Dims<D>::HALF);
   const uint32_t bb = blocks < BBOX_BLOCKS ? blocks : BBOX_BLOCKS;
   hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube, t.counters, sc);
   FG_LAUNCH_CHECK();
+  // the counting sort's ranks and counters live in leaf_parent and flags until the build
+  const bool counting = n <= RANK_MAX;
   hipLaunchKernelGGL((tree_keys<D>), dim3(blocks), dim3(256), 0, st, x, y, z, strength, n,
-                     (const CubeInfo*)t.cube, t.keys, t.vals, t.pts);
+                     (const CubeInfo*)t.cube, t.keys, t.vals, t.pts, counting ? t.leaf_parent : nullptr,
+                     t.flags);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(2);
   uint64_t* sk;
   uint32_t* sv;
-  if (n <= RANK_MAX) {
+  if (counting) {
     sk = (uint64_t*)t.sort.keys_alt;
     sv = t.sort.vals_alt;
-    hipLaunchKernelGGL(tree_rank_sort, dim3(ceil_div(n, 64)), dim3(RANK_PARTS * 64), 0, st,
-                       (const uint64_t*)t.keys, (const uint32_t*)t.vals, n, sk, sv);
+    const uint32_t groups = ceil_div(n, 64), want = ceil_div(2u * (uint32_t)device_cus(), groups);
+    const uint32_t parts = want < 1 ? 1 : want > RANK_WAVES ? RANK_WAVES : want;
+    hipLaunchKernelGGL(tree_rank_sort, dim3(groups, parts), dim3(RANK_WAVES * 64),
+                       sizeof(uint64_t) * ceil_div(n, parts), st, (const uint64_t*)t.keys, (const uint32_t*)t.vals,
+                       n, t.leaf_parent, t.flags, sk, sv);
     FG_LAUNCH_CHECK();
   } else {
     uint64_t* hk;
@@ -1668,46 +1870,16 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   if (timer) timer->end(2), timer->begin(1);
   t.skeys = sk;
   t.order = sv;
-  // a small tree is reduced by one workgroup: every node local, no device-wide hand-off
-  const uint32_t rblock = n <= REDUCE_ONE ? REDUCE_ONE : REDUCE_BLOCK;
   hipLaunchKernelGGL((tree_leaves_build<D>), dim3(blocks), dim3(256), 0, st, (const uint64_t*)sk,
-                     (const uint32_t*)sv, (const float4*)t.pts, n, rblock, t.leaves, t.leaf_depth, t.flags,
+                     (const uint32_t*)sv, (const float4*)t.pts, n, REDUCE_BLOCK, t.leaves, t.leaf_depth, t.flags,
                      t.links, t.leaf_parent);
   FG_LAUNCH_CHECK();
-  if (n < 2) {
-    if (timer) timer->end(1);
-    return;
-  }
-  if (rblock == REDUCE_ONE)
-    hipLaunchKernelGGL((tree_reduce<D, REDUCE_ONE, true>), dim3(1), dim3(REDUCE_ONE), 0, st, n,
-                       (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
-                       (const uint4*)t.links, t.flags, t.rec, t.sums);
-  else
-    hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK, true>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0,
-                       st, n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
-                       (const uint4*)t.links, t.flags, t.rec, t.sums);
+  hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK, true>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0,
+                     st, n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
+                     (const uint4*)t.links, t.flags, t.rec, t.sums);
   FG_LAUNCH_CHECK();
-  if (timer) timer->end(1), timer->begin(3);
-  WalkArgs A;
-  A.theta2 = p.theta2;
-  A.dmin2 = p.dmin2;
-  A.dmax2 = p.dmax2;
-  A.n = n;
-  A.packed_max = p.walk_packed_max;
-  const int cus = device_cus();
-  const bool exact = n <= WALK_EXACT_MAX;
-  auto kern = exact ? tree_walk<D, WALK_STACK, WALK_WAVES, true> : tree_walk<D, WALK_STACK, WALK_WAVES, false>;
-  uint32_t wblocks = resident_blocks(kern, WALK_WAVES * WAVE, cus);
-  if (wblocks > (uint32_t)cus * (32 / WALK_WAVES)) wblocks = (uint32_t)cus * (32 / WALK_WAVES);
-  A.tile = exact ? walk_tile_for(n, 4u * (uint32_t)cus, wblocks * WALK_WAVES, p.walk_tile) : (uint32_t)WAVE;
-  const uint32_t need = ceil_div(ceil_div(n, A.tile), WALK_WAVES);
-  if (wblocks > need) wblocks = need;
-  if (t.spill_bytes < spill_bytes(cus, WALK_STACK)) t.reserve_spill(spill_bytes(cus, WALK_STACK));
-  hipLaunchKernelGGL(kern, dim3(wblocks), dim3(WALK_WAVES * WAVE), 0, st, (const float4*)t.leaves,
-                     (const uint32_t*)sv, (const NodeRec*)t.rec, (const CubeInfo*)t.cube, vx, vy, vz,
-                     t.counters, (SpillEntry*)t.spill, step, A);
-  FG_LAUNCH_CHECK();
-  if (timer) timer->end(3);
+  if (timer) timer->end(1);
+  walk<D>(st, t, n, vx, vy, vz, p, step, timer);
 }
 
 }  // namespace
