@@ -52,9 +52,19 @@ def test_500_nodes(engine_factory, dims):
     check_tree(engine_factory(), make_nodes(500, dims), dims)
 
 
-@pytest.mark.parametrize("n", [2, 3, 1025, 8192, 8193, 32769])
+@pytest.mark.parametrize("n", [1, 2, 3, 1023, 1024, 1025, 4097, 8192, 8193, 32769])
 def test_sizes(engine_factory, n):
+    """Either side of the one-workgroup build (1,024 points) and of the counting sort
+    (8,192 points; 4,097 splits its codes over the most parts)."""
     check_tree(engine_factory(), make_nodes_fast(n, 3, 20 * np.cbrt(n)), 3)
+
+
+@pytest.mark.parametrize("dims", [1, 2])
+def test_one_workgroup_build_low_dims(engine_factory, dims):
+    """1,024 points with 60 coincident: the one-workgroup build in 1D and 2D."""
+    pos = make_nodes(1024, dims, 40)
+    pos[100:160] = pos[7]
+    check_tree(engine_factory(), pos, dims)
 
 
 def test_mixed_signs(engine_factory):
