@@ -130,6 +130,7 @@ struct fg_ctx {
   int radix_bits = 8;
   uint32_t walk_packed_max = 24;
   uint32_t walk_tile = 0;
+  uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
   std::string err;
   // readback
   float* staging = nullptr;
@@ -268,7 +269,8 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
                               finite_f32(p.d.distance_max2), sp, p.pairs_plan, p.pairs.as<float>());
       } else {
         if (!p.tree) p.tree.reset(new TreeBuffers());
-        TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2, c->walk_packed_max, c->walk_tile};
+        TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2, c->walk_packed_max, c->walk_tile,
+                      c->walk_solo_max};
         tree_manybody(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), vel[0],
                       vel[1], vel[2], tp, sp, c->radix_bits, c->T());
       }
@@ -398,6 +400,7 @@ int fg_create(fg_ctx** out, int device) {
   // experiment overrides (the tuned defaults apply otherwise)
   if (const char* v = getenv("D3FGPU_WALK_PACKED_MAX")) c->walk_packed_max = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_WALK_TILE")) c->walk_tile = (uint32_t)atoi(v);
+  if (const char* v = getenv("D3FGPU_WALK_SOLO")) c->walk_solo_max = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_RADIX_BITS")) c->radix_bits = atoi(v) == 11 ? 11 : 8;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;

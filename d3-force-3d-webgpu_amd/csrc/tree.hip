@@ -1034,6 +1034,18 @@ __device__ inline void accumulate<ACC_FUSED>(float& acc, float a, float b) {
   acc += a * b;
 }
 
+// dx^2 + dy^2 + dz^2 as one fixed chain of fused multiply-adds, for the exact-mode terms
+// (ACC_SET, ACC_ADD): left to the compiler, the sum is contracted differently in different
+// kernels, and the exact walks (tiled, one point per lane group, the batch's) must give the
+// same terms bit for bit.  The cheaper mode (ACC_FUSED) leaves it to the compiler.
+template <int D, typename T>
+__device__ inline T sq_len(T dx, T dy, T dz) {
+  T l = dx * dx;
+  if (D > 1) l = __builtin_fma(dy, dy, l);
+  if (D > 2) l = __builtin_fma(dz, dz, l);
+  return l;
+}
+
 // 1 / l' for d3's squared distance l, where below distanceMin l' = sqrt(dmin2 l)
 // (manyBody.js:95): rsq(l * max(l, dmin2)) either way, one reciprocal square root.
 __device__ inline float inv_dist2(float l, float dmin2) {
@@ -1054,7 +1066,7 @@ __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self
   if (top > h.depth) return false;  // points sharing the smallest cell: open
   if (h.value == 0.0f) return true;  // d3 skips cells with no strength
   const double dx = h.cx - mx, dy = D > 1 ? h.cy - my : 0.0, dz = D > 2 ? h.cz - mz : 0.0;
-  const double l = dx * dx + dy * dy + dz * dz;
+  const double l = ACC == ACC_FUSED ? dx * dx + dy * dy + dz * dz : sq_len<D>(dx, dy, dz);
   const double reach = theta2 * l;
   double w2 = w2top;
   uint32_t Dd = top;
@@ -1085,7 +1097,7 @@ __device__ inline void point_force(float mx, float my, float mz, uint32_t self_n
                                    float dmin2f, float alpha, uint32_t step_no, uint32_t seed,
                                    float& fx, float& fy, float& fz, uint32_t& inter) {
   float dx = q.x - mx, dy = D > 1 ? q.y - my : 0.f, dz = D > 2 ? q.z - mz : 0.f;
-  float l = dx * dx + dy * dy + dz * dz;
+  float l = ACC == ACC_FUSED ? dx * dx + dy * dy + dz * dz : sq_len<D>(dx, dy, dz);
   if (!(l < dmax2f)) return;
   if (l == 0.0f) {
     const float4 jj = coincident_far(self_node, order[j] - off, D, step_no, seed);
@@ -1167,7 +1179,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
   {
     const NodeRec& r = rec[off];
     const NodeHdr h = *(const NodeHdr*)&r;
-    bool done = live ? cell_force<D, ACC_FUSED>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, size * size, h,
+    bool done = live ? cell_force<D, EXACT ? ACC_SET : ACC_FUSED>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, size * size, h,
                                      theta2, dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter)
                      : true;
     if (EXACT) {
@@ -1506,6 +1518,149 @@ __global__ void __launch_bounds__(WAVES * WAVE)
   walk_flush(C, lane, counters);
 }
 
+// ---- walk of single points ---------------------------------------------------------
+//
+// A small tree's walk is a chain of openings per wave, each waiting on its loads and LDS
+// round trips with no other wave to hide behind.  Here every point has a group of NK = 2^D
+// lanes of its own (64 / NK points per wave, no union of traversals): lane c of a group
+// takes child c of each node the point opens, straight from its record into registers, and
+// the point's stack (children and depth of each entry to open) lives in LDS.  The terms
+// and their order are those of the exact-mode tiled walk: the root's term, then one term
+// per opening (the children's forces summed by group_sum), openings in the order of the
+// stack, which takes a node's opened children in order and opens the last first.  So the
+// forces are bit for bit those of any tiling.
+template <int D>
+struct SoloLDS {
+  static constexpr uint32_t NK = 1u << D, G = WAVE / NK;
+  alignas(16) uint32_t kids[G][WALK_DEPTH][NK];  // an entry's children go in as 16-byte stores
+  uint32_t depth[G][WALK_DEPTH];
+};
+
+template <int D>
+__global__ void __launch_bounds__(WAVE) tree_walk_solo(const float4* __restrict__ leaves,
+                                                       const uint32_t* __restrict__ order,
+                                                       const NodeRec* __restrict__ rec,
+                                                       const CubeInfo* __restrict__ cube, float* __restrict__ vx,
+                                                       float* __restrict__ vy, float* __restrict__ vz,
+                                                       uint32_t* __restrict__ counters, StepParams step, WalkArgs A) {
+  constexpr uint32_t NK = 1u << D, G = WAVE / NK;
+  __shared__ SoloLDS<D> L;
+  const uint32_t lane = threadIdx.x, g = lane / NK, c = lane % NK;
+  const uint32_t n = A.n, s = blockIdx.x * G + g;
+  const bool live = s < n;
+  const double size = (double)cube->size;
+  const double theta2 = A.theta2, dmax2 = A.dmax2;
+  const float dmin2f = (float)A.dmin2, dmax2f = (float)fmin(A.dmax2, 3.0e38);
+  const float alpha = step.alpha;
+  const uint32_t step_no = step.step, seed = step.seed;
+  const float4 me = live ? leaves[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t self_node = live ? order[s] : 0u;
+  uint32_t inter = 0, sp = 0;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  {  // the root: one term (every lane of the group the same; lane 0 counts it)
+    const NodeRec& r = rec[0];
+    const NodeHdr h = *(const NodeHdr*)&r;
+    const bool done = live ? cell_force<D, ACC_SET>(me.x, me.y, me.z, self_node, n, 0u, 0u, size * size, h,
+                                                      theta2, dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter)
+                           : true;
+    if (c != 0) inter = 0;
+    if (!done) {
+      if (c == 0) L.depth[g][0] = r.depth;
+      L.kids[g][0][c] = r.kids[c];
+      sp = 1;
+    }
+  }
+  uint32_t max_sp = sp, opened = 0, guard = 0, err = 0;
+  const uint32_t guard_max = 2u * n + 8u;
+  while (__builtin_amdgcn_ballot_w64(sp > 0u)) {
+    if (sp == 0u) continue;  // the group's walk is over (every lane of a group agrees)
+    if (++guard > guard_max) {
+      err = 1;
+      sp = 0;
+      continue;
+    }
+    --sp;
+    const uint32_t k = L.kids[g][sp][c];
+    const uint32_t top = L.depth[g][sp] + 1;
+    opened += c == 0 ? 1u : 0u;
+    const double wt = ldexp(size, -(int)top);
+    const double w2top = wt * wt;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    bool open = false;
+    uint4 k0 = make_uint4(0u, 0u, 0u, 0u), k1 = k0;
+    uint32_t cdepth = 0;
+    if (k != REF_NONE) {
+      if (k & REF_POINT) {
+        const uint32_t j = k & ~REF_POINT;
+        const float4 q = leaves[j];
+        if (j != s)
+          point_force<D, ACC_SET>(me.x, me.y, me.z, self_node, q, order, j, 0u, dmax2f, dmin2f, alpha, step_no, seed,
+                                  fx, fy, fz, inter);
+      } else {
+        const uint4* R = (const uint4*)(rec + k);
+        const uint4 q0 = R[0], q1 = R[1];
+        k0 = R[2];
+        k1 = R[3];
+        NodeHdr h;
+        __builtin_memcpy(&h, &q0, 16);
+        __builtin_memcpy((char*)&h + 16, &q1, 16);
+        open = !cell_force<D, ACC_SET>(me.x, me.y, me.z, self_node, n, k, top, w2top, h, theta2, dmax2, dmin2f,
+                                       alpha, step_no, seed, fx, fy, fz, inter);
+        cdepth = h.depth;
+      }
+    }
+    fx = group_sum<NK>(fx);
+    if (D > 1) fy = group_sum<NK>(fy);
+    if (D > 2) fz = group_sum<NK>(fz);
+    ax += fx;
+    if (D > 1) ay += fy;
+    if (D > 2) az += fz;
+    // the opened children in order, from the entry just popped upwards
+    const uint32_t bits = (uint32_t)(__builtin_amdgcn_ballot_w64(open) >> (g * NK)) & ((1u << NK) - 1u);
+    if (open) {
+      const uint32_t slot = sp + (uint32_t)__popc(bits & ((1u << c) - 1u));
+      if (slot < (uint32_t)WALK_DEPTH) {
+        L.depth[g][slot] = cdepth;
+        uint32_t* e = L.kids[g][slot];
+        if (NK == 8) {
+          *(uint4*)e = k0;
+          *(uint4*)(e + 4) = k1;
+        } else if (NK == 4) {
+          *(uint4*)e = k0;
+        } else {
+          *(uint2*)e = make_uint2(k0.x, k0.y);
+        }
+      } else {
+        err = 1;
+      }
+    }
+    sp += (uint32_t)__popc(bits);
+    if (sp > (uint32_t)WALK_DEPTH) sp = WALK_DEPTH;
+    max_sp = sp > max_sp ? sp : max_sp;
+  }
+  if (live && c == 0) {
+    vx[self_node] += ax;
+    if (D > 1) vy[self_node] += ay;
+    if (D > 2) vz[self_node] += az;
+  }
+  // the wave's counts into lane 0 for the workgroup's flush
+  WalkCount C;
+  uint32_t op = opened, e = err, m = max_sp;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    op += __shfl_xor(op, o, WAVE);
+    e |= __shfl_xor(e, o, WAVE);
+    const uint32_t mo = __shfl_xor(m, o, WAVE);
+    m = mo > m ? mo : m;
+  }
+  C.opened = op;
+  C.lane_visits = op;
+  C.inter = inter;
+  C.err = e;
+  C.max_sp = m;
+  walk_flush(C, lane, counters);
+}
+
 // Resident workgroups of a persistent walk kernel on this device (all CUs).
 template <typename K>
 uint32_t resident_blocks(K kern, int threads, int cus) {
@@ -1801,6 +1956,19 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
   A.packed_max = p.walk_packed_max;
   const int cus = device_cus();
   const bool exact = n <= WALK_EXACT_MAX;
+  // measured on 256 CUs (link + manyBody + center ticks): a group of lanes per point is the
+  // faster walk up to 8,000 points in 3D (0.208 against 0.223 ms/tick) but not at 12,000
+  // (0.269 against 0.265), and still at 16,000 in 1D and 2D
+  const uint32_t solo_max = p.walk_solo_max != WALK_SOLO_BY_DIMS ? p.walk_solo_max : D == 3 ? 10240u : 16384u;
+  if (exact && n <= solo_max) {  // a group of lanes per point
+    constexpr uint32_t G = WAVE >> D;
+    hipLaunchKernelGGL((tree_walk_solo<D>), dim3(ceil_div(n, G)), dim3(WAVE), 0, st, (const float4*)t.leaves,
+                       (const uint32_t*)t.order, (const NodeRec*)t.rec, (const CubeInfo*)t.cube, vx, vy, vz,
+                       t.counters, step, A);
+    FG_LAUNCH_CHECK();
+    if (timer) timer->end(3);
+    return;
+  }
   auto kern = exact ? tree_walk<D, WALK_STACK, WALK_WAVES, true> : tree_walk<D, WALK_STACK, WALK_WAVES, false>;
   uint32_t wblocks = resident_blocks(kern, WALK_WAVES * WAVE, cus);
   if (wblocks > (uint32_t)cus * (32 / WALK_WAVES)) wblocks = (uint32_t)cus * (32 / WALK_WAVES);

@@ -134,8 +134,9 @@ def test_manybody_tree_forces_do_not_depend_on_tiling(monkeypatch, dims, coincid
     """Trees of up to 163840 points are walked in the exact mode: every point's force comes
     out bit for bit the same whatever the tile size (points per wave) and whichever mode
     (one lane per point, or packed (point, child) items) ran each opening -- packed limit 0
-    runs every opening one lane per point, 64 every opening packed.  A clump makes deep
-    cell chains; 40 copies of one point make a deep run of equal codes.
+    runs every opening one lane per point, 64 every opening packed -- and in the walk that
+    gives each point a group of lanes of its own (solo, the default for small trees).  A
+    clump makes deep cell chains; 40 copies of one point make a deep run of equal codes.
 
     Against the FP64 oracle only without the copies: the tree puts a group of coincident
     points at the deepest level, as the reference's does (a point's leaf depth comes from
@@ -149,18 +150,20 @@ def test_manybody_tree_forces_do_not_depend_on_tiling(monkeypatch, dims, coincid
         pos[n // 10: n // 10 + 40] = pos[0]
     s = np.full(n, -30.0) * (1 + 0.5 * np.sin(np.arange(n)))
     out = []
-    for tile, packed in [(64, 24), (64, 0), (64, 64), (32, 0), (16, 24), (7, 64), (1, 24)]:
+    for tile, packed, solo in [(64, 24, 0), (64, 0, 0), (64, 64, 0), (32, 0, 0), (16, 24, 0), (7, 64, 0),
+                               (1, 24, 0), (0, 24, n)]:
         monkeypatch.setenv("D3FGPU_WALK_TILE", str(tile))
         monkeypatch.setenv("D3FGPU_WALK_PACKED_MAX", str(packed))
+        monkeypatch.setenv("D3FGPU_WALK_SOLO", str(solo))
         e = d3fgpu.Engine(0)
         try:
             _, dv = gpu_dv(e, pos, dims, [d3fgpu.many_body(s)])
             assert e.stats().walk_errors == 0
         finally:
             e.close()
-        out.append(((tile, packed), dv))
+        out.append(((tile, packed, solo), dv))
     for key, dv in out[1:]:
-        assert np.array_equal(dv, out[0][1]), f"tile/packed {key} differs from {out[0][0]}"
+        assert np.array_equal(dv, out[0][1]), f"tile/packed/solo {key} differs from {out[0][0]}"
     if not coincident:
         ref, _ = oracle_manybody(pos, dims, s)
         err = rel_l2(out[0][1], ref)
