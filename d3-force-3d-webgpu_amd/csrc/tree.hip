@@ -1365,31 +1365,70 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
         if (D > 2) L.acc[2][lane] += sz;
       }
     }
-    // push the children some lane must open (open_bits is 0 on lanes not in the mask)
+    if (EXACT) {
+      // push the children some lane must open (open_bits is 0 on lanes not in the mask), in
+      // order: the k-th opened child goes to entry sp + k (none beyond WALK_DEPTH).  Lane 0
+      // writes each entry's lane mask as the ballots come; then lane (c, w) moves word w of
+      // child c's children, and lane (c, 0) its depth, all children at once.
+      uint32_t pushed = 0, ranks = 0;  // bit c: child c is pushed; the count so far
+#pragma unroll
+      for (uint32_t c = 0; c < (uint32_t)NK; ++c) {
+        const uint64_t open = __builtin_amdgcn_ballot_w64(((open_bits >> c) & 1u) != 0u);
+        if (!open) continue;
+        const uint32_t slot = sp + ranks;
+        ++ranks;
+        if (slot >= (uint32_t)WALK_DEPTH) {
+          err = 1;
+          continue;
+        }
+        pushed |= 1u << c;
+        if (lane == 0) {
+          if (slot < (uint32_t)STACK) L.mask[slot] = open;
+          else store_coherent(&spill[slot - STACK].mask, open);
+        }
+      }
+      if ((pushed >> my_c) & 1u) {
+        const uint32_t slot = sp + (uint32_t)__popc(pushed & ((1u << my_c) - 1u));
+        const uint32_t word = L.stage[my_c][8 + my_w];
+        const uint32_t cdepth = ((const NodeHdr*)&L.stage[my_c][0])->depth;
+        if (slot < (uint32_t)STACK) {
+          L.kids[slot][my_w] = word;
+          if (my_w == 0) L.depth[slot] = cdepth;
+        } else {
+          SpillEntry* E = spill + (slot - STACK);
+          store_coherent(&E->kids[my_w], word);
+          if (my_w == 0) store_coherent(&E->depth, cdepth);
+        }
+      }
+      sp += (uint32_t)__popc(pushed);
+    } else {
+      // one child at a time (measured cheaper than the above where the walk is bound by
+      // issue, 1M points)
 #pragma unroll 1
-    for (uint32_t c = 0; c < nk; ++c) {
-      const uint64_t open = __builtin_amdgcn_ballot_w64(((open_bits >> c) & 1u) != 0u);
-      if (!open) continue;
-      if (sp >= (uint32_t)WALK_DEPTH) {
-        err = 1;
-        continue;
-      }
-      const uint32_t cdepth = ((const NodeHdr*)&L.stage[c][0])->depth;
-      if (sp < (uint32_t)STACK) {
-        if (lane == 0) {
-          L.depth[sp] = cdepth;
-          L.mask[sp] = open;
+      for (uint32_t c = 0; c < nk; ++c) {
+        const uint64_t open = __builtin_amdgcn_ballot_w64(((open_bits >> c) & 1u) != 0u);
+        if (!open) continue;
+        if (sp >= (uint32_t)WALK_DEPTH) {
+          err = 1;
+          continue;
         }
-        if (lane < 8) L.kids[sp][lane] = L.stage[c][8 + lane];
-      } else {
-        SpillEntry* E = spill + (sp - STACK);
-        if (lane == 0) {
-          store_coherent(&E->mask, open);
-          store_coherent(&E->depth, cdepth);
+        const uint32_t cdepth = ((const NodeHdr*)&L.stage[c][0])->depth;
+        if (sp < (uint32_t)STACK) {
+          if (lane == 0) {
+            L.depth[sp] = cdepth;
+            L.mask[sp] = open;
+          }
+          if (lane < 8) L.kids[sp][lane] = L.stage[c][8 + lane];
+        } else {
+          SpillEntry* E = spill + (sp - STACK);
+          if (lane == 0) {
+            store_coherent(&E->mask, open);
+            store_coherent(&E->depth, cdepth);
+          }
+          if (lane < 8) store_coherent(&E->kids[lane], L.stage[c][8 + lane]);
         }
-        if (lane < 8) store_coherent(&E->kids[lane], L.stage[c][8 + lane]);
+        ++sp;
       }
-      ++sp;
     }
     max_sp = sp > max_sp ? sp : max_sp;
   }
