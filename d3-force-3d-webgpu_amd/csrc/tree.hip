@@ -819,12 +819,12 @@ __device__ inline void reduce_climb(uint32_t s, uint32_t n, uint32_t lo, ReduceL
 // the cube; the codes; their stable order; the leaves, leaf depths and Karras nodes, the
 // nodes' links and parents kept in LDS for the climb; the reduction and d3's children.
 //
-// The order: (code, index) pairs merged in runs of 1, 2, 4, ... between two LDS buffers.
-// A pair's place in the merge of its run with the sibling run is its place in its own run
-// plus the number of the sibling's pairs below it (a binary search); the pairs are all
-// distinct, so this is the order on (code, index), the stable order that the rank sort and
-// the radix sort give.  log2(n) levels of at most log2(n) + 1 LDS reads each, where the
-// counting of the rank sort would take n reads per point on the one CU.
+// The order: each wave sorts its 64 (code, index) pairs in registers (bitonic), then runs of
+// 64, 128, ... are merged between two LDS buffers.  A pair's place in the merge of its run
+// with the sibling run is its place in its own run plus the number of the sibling's pairs
+// below it (a binary search); the pairs are all distinct, so this is the order on (code,
+// index), the stable order that the rank sort and the radix sort give.  The counting of the
+// rank sort would take n reads per point on the one CU.
 template <int D>
 __global__ void __launch_bounds__(REDUCE_ONE) tree_small(const float* __restrict__ x, const float* __restrict__ y,
                                                          const float* __restrict__ z,
@@ -839,6 +839,7 @@ __global__ void __launch_bounds__(REDUCE_ONE) tree_small(const float* __restrict
   __shared__ uint64_t s_key[2][B];
   __shared__ uint32_t s_idx[2][B];
   __shared__ uint32_t s_parent[B];
+  __shared__ float4 s_pts[B];
   __shared__ uint32_t s_m[W][6];
   __shared__ float s_origin[3];
   __shared__ int32_t s_shift;
@@ -878,43 +879,60 @@ __global__ void __launch_bounds__(REDUCE_ONE) tree_small(const float* __restrict
     cube->shift = shift;
   }
   __syncthreads();
-  // the codes, merged into order; the result lands in buffer 0
-  uint32_t levels = 0;
-  while ((1u << levels) < n) ++levels;
-  uint32_t cur = levels & 1u, pos = i;
-  uint64_t code = 0;
+  // the codes in order: each wave sorts its 64 (code, index) pairs in registers (a bitonic
+  // network over the lanes, padding lanes last), then the waves' runs are merged in LDS;
+  // the result lands in buffer 0
+  uint32_t levels = 0;  // merge levels after the waves' own sorts
+  while ((64u << levels) < n) ++levels;
+  uint32_t cur = levels & 1u;
+  uint64_t code = ~0ull;
+  uint32_t idx = i;
   if (i < n) {
     const float origin[3] = {s_origin[0], s_origin[1], s_origin[2]};
     code = point_key<D>(p, origin, s_shift);
-    s_key[cur][i] = code;
-    s_idx[cur][i] = i;
+    s_pts[i] = make_float4(p[0], p[1], p[2], strength[i]);
+  }
+#pragma unroll
+  for (uint32_t k = 2; k <= WAVE; k <<= 1)
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      const uint64_t pc = __shfl_xor(code, (int)j, WAVE);
+      const uint32_t pi = (uint32_t)__shfl_xor((int)idx, (int)j, WAVE);
+      const bool p_less = pc < code || (pc == code && pi < idx);
+      const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);  // lower of an ascending pair
+      if (keep_min ? p_less : !p_less) code = pc, idx = pi;
+    }
+  uint32_t pos = i;  // lane l of wave w holds the wave's l-th pair
+  if (code != ~0ull) {
+    s_key[cur][pos] = code;
+    s_idx[cur][pos] = idx;
   }
   __syncthreads();
-  for (uint32_t r = 1; r < n; r <<= 1, cur ^= 1u) {
-    if (i < n) {
+  for (uint32_t r = WAVE; r < n; r <<= 1, cur ^= 1u) {
+    if (code != ~0ull) {
       const uint32_t sib = (pos ^ r) & ~(r - 1u);  // first place of the sibling run
       const uint32_t len = sib < n ? min(r, n - sib) : 0u;
-      uint32_t below = 0;  // the sibling's pairs below (code, i): branch-free lower bound
+      uint32_t below = 0;  // the sibling's pairs below (code, idx): branch-free lower bound
       for (uint32_t step = r; step > 0; step >>= 1) {
         if (below + step <= len) {
           const uint32_t j = sib + below + step - 1u;
           const uint64_t k = s_key[cur][j];
-          if (k < code || (k == code && s_idx[cur][j] < i)) below += step;
+          if (k < code || (k == code && s_idx[cur][j] < idx)) below += step;
         }
       }
       pos = (pos & ~(2u * r - 1u)) + (pos & (r - 1u)) + below;
       s_key[cur ^ 1u][pos] = code;
-      s_idx[cur ^ 1u][pos] = i;
+      s_idx[cur ^ 1u][pos] = idx;
     }
     __syncthreads();
   }
   // sorted place pos: the leaf, its code and point
-  if (i < n) {
-    const float4 leaf = make_float4(p[0], p[1], p[2], strength[i]);
+  if (code != ~0ull) {
+    const float4 leaf = s_pts[idx];
     S.leaf[pos] = leaf;
     leaves[pos] = leaf;
     skeys[pos] = code;
-    order[pos] = i;
+    order[pos] = idx;
   }
   S.flag[i] = 0;
   // sorted point i: its depth; binary node i: its links, and its parent links (in LDS)
