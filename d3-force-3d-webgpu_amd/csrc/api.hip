@@ -131,6 +131,7 @@ struct fg_ctx {
   uint32_t walk_packed_max = 24;
   uint32_t walk_tile = 0;
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
+  uint32_t reduce_block = 0;
   std::string err;
   // readback
   float* staging = nullptr;
@@ -270,7 +271,7 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
       } else {
         if (!p.tree) p.tree.reset(new TreeBuffers());
         TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2, c->walk_packed_max, c->walk_tile,
-                      c->walk_solo_max};
+                      c->walk_solo_max, c->reduce_block};
         tree_manybody(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), vel[0],
                       vel[1], vel[2], tp, sp, c->radix_bits, c->T());
       }
@@ -401,6 +402,7 @@ int fg_create(fg_ctx** out, int device) {
   if (const char* v = getenv("D3FGPU_WALK_PACKED_MAX")) c->walk_packed_max = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_WALK_TILE")) c->walk_tile = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_WALK_SOLO")) c->walk_solo_max = (uint32_t)atoi(v);
+  if (const char* v = getenv("D3FGPU_REDUCE_BLOCK")) c->reduce_block = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_RADIX_BITS")) c->radix_bits = atoi(v) == 11 ? 11 : 8;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;

@@ -83,6 +83,7 @@ struct TreeParams {
   uint32_t walk_tile = 0;         // points per tile of a small (exact-mode) walk; 0: by size
   // trees of at most this many points walk one point per group of lanes (0: never)
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
+  uint32_t reduce_block = 0;  // points per workgroup of the reduction (512 or 1024; 0: by size)
 };
 
 // Bytes of walk spill area for `cus` CUs at an LDS stack of `stack` entries.

@@ -2095,13 +2095,24 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   if (timer) timer->end(2), timer->begin(1);
   t.skeys = sk;
   t.order = sv;
+  // points per reduction workgroup: 1,024 (one fewer device-coherent level on the climb,
+  // centres cached in LDS, one workgroup per CU) measured faster up to 20,000 points (3,000:
+  // 0.119 -> 0.114 ms/tick, 8,000 0.208 -> 0.205), even at 50,000-100,000, slower at 1M
+  // (1.884 -> 1.920)
+  const uint32_t rb = p.reduce_block == 0 ? (n <= 32768u ? REDUCE_ONE : REDUCE_BLOCK)
+                      : p.reduce_block == REDUCE_ONE ? REDUCE_ONE : REDUCE_BLOCK;
   hipLaunchKernelGGL((tree_leaves_build<D>), dim3(blocks), dim3(256), 0, st, (const uint64_t*)sk,
-                     (const uint32_t*)sv, (const float4*)t.pts, n, REDUCE_BLOCK, t.leaves, t.leaf_depth, t.flags,
+                     (const uint32_t*)sv, (const float4*)t.pts, n, rb, t.leaves, t.leaf_depth, t.flags,
                      t.links, t.leaf_parent);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK, true>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0,
-                     st, n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
-                     (const uint4*)t.links, t.flags, t.rec, t.sums);
+  if (rb == REDUCE_ONE)
+    hipLaunchKernelGGL((tree_reduce<D, REDUCE_ONE, true>), dim3(ceil_div(n, REDUCE_ONE)), dim3(REDUCE_ONE), 0, st,
+                       n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent, (const uint4*)t.links, t.flags,
+                       t.rec, t.sums);
+  else
+    hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK, true>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0,
+                       st, n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
+                       (const uint4*)t.links, t.flags, t.rec, t.sums);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1);
   walk<D>(st, t, n, vx, vy, vz, p, step, timer);
