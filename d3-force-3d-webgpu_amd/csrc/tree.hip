@@ -1593,17 +1593,22 @@ struct SoloLDS {
   uint32_t depth[G][WALK_DEPTH];
 };
 
-template <int D>
-__global__ void __launch_bounds__(WAVE) tree_walk_solo(const float4* __restrict__ leaves,
+// WAVES (1 or 2) waves per workgroup, their stacks 55-74 KB of LDS each: one wave per
+// workgroup while every wave can have a CU of its own (2 shared a CU's cache and LDS:
+// 1,000 points 0.093 -> 0.096 ms/tick), two beyond that (half the workgroups and their
+// counter atomics: 5,000 points 0.181 -> 0.173, 8,000 0.207 -> 0.195).
+template <int D, uint32_t WAVES>
+__global__ void __launch_bounds__(WAVES * WAVE) tree_walk_solo(const float4* __restrict__ leaves,
                                                        const uint32_t* __restrict__ order,
                                                        const NodeRec* __restrict__ rec,
                                                        const CubeInfo* __restrict__ cube, float* __restrict__ vx,
                                                        float* __restrict__ vy, float* __restrict__ vz,
                                                        uint32_t* __restrict__ counters, StepParams step, WalkArgs A) {
   constexpr uint32_t NK = 1u << D, G = WAVE / NK;
-  __shared__ SoloLDS<D> L;
-  const uint32_t lane = threadIdx.x, g = lane / NK, c = lane % NK;
-  const uint32_t n = A.n, s = blockIdx.x * G + g;
+  __shared__ SoloLDS<D> lds[WAVES];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE, g = lane / NK, c = lane % NK;
+  SoloLDS<D>& L = lds[wave];
+  const uint32_t n = A.n, s = (blockIdx.x * WAVES + wave) * G + g;
   const bool live = s < n;
   const double size = (double)cube->size;
   const double theta2 = A.theta2, dmax2 = A.dmax2;
@@ -2019,9 +2024,14 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
   const uint32_t solo_max = p.walk_solo_max != WALK_SOLO_BY_DIMS ? p.walk_solo_max : D == 3 ? 10240u : 16384u;
   if (exact && n <= solo_max) {  // a group of lanes per point
     constexpr uint32_t G = WAVE >> D;
-    hipLaunchKernelGGL((tree_walk_solo<D>), dim3(ceil_div(n, G)), dim3(WAVE), 0, st, (const float4*)t.leaves,
-                       (const uint32_t*)t.order, (const NodeRec*)t.rec, (const CubeInfo*)t.cube, vx, vy, vz,
-                       t.counters, step, A);
+    if (ceil_div(n, G) <= (uint32_t)cus)
+      hipLaunchKernelGGL((tree_walk_solo<D, 1>), dim3(ceil_div(n, G)), dim3(WAVE), 0, st, (const float4*)t.leaves,
+                         (const uint32_t*)t.order, (const NodeRec*)t.rec, (const CubeInfo*)t.cube, vx, vy, vz,
+                         t.counters, step, A);
+    else
+      hipLaunchKernelGGL((tree_walk_solo<D, 2>), dim3(ceil_div(n, 2 * G)), dim3(2 * WAVE), 0, st,
+                         (const float4*)t.leaves, (const uint32_t*)t.order, (const NodeRec*)t.rec,
+                         (const CubeInfo*)t.cube, vx, vy, vz, t.counters, step, A);
     FG_LAUNCH_CHECK();
     if (timer) timer->end(3);
     return;
