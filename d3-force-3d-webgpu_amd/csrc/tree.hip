@@ -2019,9 +2019,11 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
   const int cus = device_cus();
   const bool exact = n <= WALK_EXACT_MAX;
   // measured on 256 CUs (link + manyBody + center ticks): a group of lanes per point is the
-  // faster walk up to 8,000 points in 3D (0.208 against 0.223 ms/tick) but not at 12,000
-  // (0.269 against 0.265), and still at 16,000 in 1D and 2D
-  const uint32_t solo_max = p.walk_solo_max != WALK_SOLO_BY_DIMS ? p.walk_solo_max : D == 3 ? 10240u : 16384u;
+  // faster walk up to 12,000 points in 3D (0.254 against 0.258 ms/tick) but not at 16,000
+  // (0.296 against 0.264), and still at 24,000 in 2D (0.234 against 0.237) and 32,000 in 1D
+  // (0.259 against 0.291)
+  const uint32_t solo_max = p.walk_solo_max != WALK_SOLO_BY_DIMS ? p.walk_solo_max
+                            : D == 3 ? 12288u : D == 2 ? 24576u : 32768u;
   if (exact && n <= solo_max) {  // a group of lanes per point
     constexpr uint32_t G = WAVE >> D;
     if (ceil_div(n, G) <= (uint32_t)cus)
