@@ -132,6 +132,7 @@ struct fg_ctx {
   uint32_t walk_tile = 0;
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
   uint32_t reduce_block = 0;
+  const Pass* center_fused = nullptr;  // this tick's last pass, a center left to the integrator's launch
   std::string err;
   // readback
   float* staging = nullptr;
@@ -309,6 +310,10 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
     }
     case FG_FORCE_CENTER: {
       PhaseScope ps(c->T(), FG_PHASE_CENTER);
+      if (defer_center && n <= CENTER_FUSE_MAX) {  // the integrator's launch does it all
+        c->center_fused = &p;
+        return nullptr;
+      }
       launch_center(c->stream, n, nd, pos, p.center.as<CenterWork>(), p.d.x, p.d.y, p.d.z, p.d.strength,
                     !defer_center);
       if (defer_center) return p.center.as<CenterWork>();
@@ -601,6 +606,7 @@ int fg_enqueue(fg_ctx* c, uint32_t steps, const double* alphas, double decay) {
       PhaseScope tick(c->T(), FG_PHASE_TICK);
       // a center pass last in the list leaves its shift to the integrator (one launch fewer)
       const CenterWork* shift = nullptr;
+      c->center_fused = nullptr;
       for (size_t i = 0; i < c->passes.size(); ++i)
         shift = run_pass(c, *c->passes[i], sp, i + 1 == c->passes.size());
       float* pos[3] = {c->P(0), c->P(1), c->P(2)};
@@ -608,7 +614,11 @@ int fg_enqueue(fg_ctx* c, uint32_t steps, const double* alphas, double decay) {
       float* fix[3] = {c->fix[0].as<float>(), c->fix[1].as<float>(), c->fix[2].as<float>()};
       {
         PhaseScope ps(c->T(), FG_PHASE_INTEGRATE);
-        launch_integrate(c->stream, c->n, c->ndim, pos, vel, mask, fix, sp, shift);
+        if (const Pass* cp = c->center_fused)
+          launch_center_integrate(c->stream, c->n, c->ndim, pos, vel, mask, fix, sp, cp->d.x, cp->d.y, cp->d.z,
+                                  cp->d.strength);
+        else
+          launch_integrate(c->stream, c->n, c->ndim, pos, vel, mask, fix, sp, shift);
       }
       c->steps++;
     }

@@ -78,6 +78,12 @@ void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const f
                  float* const* vout, const uint32_t* row, const LinkEntry* ent, float4* snap,
                  StepParams step);
 // apply false: the shift is only computed (w->shift), for the integrator to apply.
+// A tick whose last force is center over at most this many nodes: center and integration in
+// one launch (launch_center_integrate), with the shift center_reduce would give.
+constexpr uint32_t CENTER_FUSE_MAX = 1024;
+void launch_center_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
+                             const uint8_t* mask, const float* const* fix, StepParams step, double cx, double cy,
+                             double cz, double strength);
 void launch_center(hipStream_t s, uint32_t n, int ndim, float* const* pos, CenterWork* w, double cx,
                    double cy, double cz, double strength, bool apply = true);
 void launch_radial(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,

@@ -22,31 +22,26 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <stdexcept>
 #include <vector>
 
 namespace fg {
 
 // ---- integrate ----------------------------------------------------------------------
 
+// Node i's step: the position minus a center shift (when a center force is the tick's
+// last, applied here as center_apply would just before: rounded, then plus the velocity),
+// the velocity decayed and added; a pinned axis takes its fixed value and no velocity.
 template <int D>
-__global__ void __launch_bounds__(256) integrate_kernel(uint32_t n, float* __restrict__ x, float* __restrict__ y,
-                                                        float* __restrict__ z, float* __restrict__ vx,
-                                                        float* __restrict__ vy, float* __restrict__ vz,
-                                                        const uint8_t* __restrict__ mask,
-                                                        const float* __restrict__ fx,
-                                                        const float* __restrict__ fy,
-                                                        const float* __restrict__ fz,
-                                                        const CenterWork* __restrict__ center,
-                                                        StepParams step) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const float decay = step.decay;
+__device__ inline void integrate_node(uint32_t i, float* __restrict__ x, float* __restrict__ y,
+                                      float* __restrict__ z, float* __restrict__ vx, float* __restrict__ vy,
+                                      float* __restrict__ vz, const uint8_t* __restrict__ mask,
+                                      const float* __restrict__ fx, const float* __restrict__ fy,
+                                      const float* __restrict__ fz, bool shifted, float sx, float sy, float sz,
+                                      float decay) {
   uint32_t m = mask ? mask[i] : 0u;
-  // a center force that is the tick's last applies its shift here, as center_apply would
-  // just before: the position minus the shift, rounded, then plus the velocity
   float px = x[i], py = D > 1 ? y[i] : 0.f, pz = D > 2 ? z[i] : 0.f;
-  if (center) {
-    const float sx = center->shift[0], sy = center->shift[1], sz = center->shift[2];
+  if (shifted) {
     if (sx != 0.0f) px -= sx;
     if (D > 1 && sy != 0.0f) py -= sy;
     if (D > 2 && sz != 0.0f) pz -= sz;
@@ -69,6 +64,65 @@ __global__ void __launch_bounds__(256) integrate_kernel(uint32_t n, float* __res
   }
 }
 
+template <int D>
+__global__ void __launch_bounds__(256) integrate_kernel(uint32_t n, float* __restrict__ x, float* __restrict__ y,
+                                                        float* __restrict__ z, float* __restrict__ vx,
+                                                        float* __restrict__ vy, float* __restrict__ vz,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const float* __restrict__ fx,
+                                                        const float* __restrict__ fy,
+                                                        const float* __restrict__ fz,
+                                                        const CenterWork* __restrict__ center,
+                                                        StepParams step) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  if (center) sx = center->shift[0], sy = center->shift[1], sz = center->shift[2];
+  integrate_node<D>(i, x, y, z, vx, vy, vz, mask, fx, fy, fz, center != nullptr, sx, sy, sz, step.decay);
+}
+
+// A tick whose last force is center, over at most CENTER_FUSE_MAX nodes, in one launch: one
+// workgroup sums the positions with center_reduce's arithmetic (node i on thread i, sums
+// over each wave, then over each 256 threads as one of center_reduce's workgroups, those
+// folded in order), so the shift is the same bit for bit, and then integrates every node.
+template <int D>
+__global__ void __launch_bounds__(CENTER_FUSE_MAX) center_integrate(
+    uint32_t n, float* __restrict__ x, float* __restrict__ y, float* __restrict__ z, float* __restrict__ vx,
+    float* __restrict__ vy, float* __restrict__ vz, const uint8_t* __restrict__ mask, const float* __restrict__ fx,
+    const float* __restrict__ fy, const float* __restrict__ fz, double cx, double cy, double cz, double strength,
+    StepParams step) {
+  const uint32_t i = threadIdx.x, lane = i & 63, wave = i >> 6;
+  double s[3] = {0, 0, 0};
+  if (i < n) {
+    s[0] += (double)x[i];
+    if (D > 1) s[1] += (double)y[i];
+    if (D > 2) s[2] += (double)z[i];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s[a] += __shfl_xor(s[a], o, WAVE);
+  __shared__ double sh[CENTER_FUSE_MAX / 64][3];
+  __shared__ float shift[3];
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) sh[wave][a] = s[a];
+  __syncthreads();
+  if (i == 0) {
+    const uint32_t groups = (n + 255) / 256;
+    double t[3] = {0, 0, 0};
+    for (uint32_t b = 0; b < groups; ++b)
+      for (int a = 0; a < 3; ++a) {
+        double part = 0;
+        part += sh[4 * b][a] + sh[4 * b + 1][a] + sh[4 * b + 2][a] + sh[4 * b + 3][a];
+        t[a] += part;
+      }
+    const double c[3] = {cx, cy, cz};
+    for (int a = 0; a < 3; ++a) shift[a] = a < D ? (float)((t[a] / n - c[a]) * strength) : 0.0f;
+  }
+  __syncthreads();
+  if (i < n) integrate_node<D>(i, x, y, z, vx, vy, vz, mask, fx, fy, fz, true, shift[0], shift[1], shift[2], step.decay);
+}
+
 void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
                       const uint8_t* mask, const float* const* fix, StepParams step,
                       const CenterWork* center) {
@@ -83,6 +137,23 @@ void launch_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, fl
   else
     hipLaunchKernelGGL(integrate_kernel<3>, g, b, 0, s, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mask,
                        fix[0], fix[1], fix[2], center, step);
+  FG_LAUNCH_CHECK();
+}
+
+void launch_center_integrate(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
+                             const uint8_t* mask, const float* const* fix, StepParams step, double cx, double cy,
+                             double cz, double strength) {
+  if (!n) return;
+  if (n > CENTER_FUSE_MAX) throw std::logic_error("center_integrate: more nodes than one workgroup takes");
+  if (ndim == 1)
+    hipLaunchKernelGGL(center_integrate<1>, dim3(1), dim3(CENTER_FUSE_MAX), 0, s, n, pos[0], pos[1], pos[2], vel[0],
+                       vel[1], vel[2], mask, fix[0], fix[1], fix[2], cx, cy, cz, strength, step);
+  else if (ndim == 2)
+    hipLaunchKernelGGL(center_integrate<2>, dim3(1), dim3(CENTER_FUSE_MAX), 0, s, n, pos[0], pos[1], pos[2], vel[0],
+                       vel[1], vel[2], mask, fix[0], fix[1], fix[2], cx, cy, cz, strength, step);
+  else
+    hipLaunchKernelGGL(center_integrate<3>, dim3(1), dim3(CENTER_FUSE_MAX), 0, s, n, pos[0], pos[1], pos[2], vel[0],
+                       vel[1], vel[2], mask, fix[0], fix[1], fix[2], cx, cy, cz, strength, step);
   FG_LAUNCH_CHECK();
 }
 
