@@ -513,3 +513,32 @@ def test_coincident_nodes_separate(engine_factory):
     assert np.all(np.isfinite(p))
     d = np.linalg.norm(p[:, None] - p[None], axis=2) + np.eye(10) * 1e9
     assert d.min() > 0.5
+
+
+@pytest.mark.parametrize("dims,n", [(1, 700), (2, 1024), (3, 1), (3, 513), (3, 1025)])
+def test_center_last_in_one_launch(dims, n):
+    """A tick whose last force is center over at most 1,024 nodes computes the shift and
+    integrates in one launch; it must leave the state center followed by a force that adds
+    nothing leaves (center then runs on its own kernels, its shift applied before the
+    integration), bit for bit, with velocities and pinned axes.  1,025 nodes: the usual path
+    both ways."""
+    rng = np.random.default_rng(dims * 10007 + n)
+    pos = np.zeros((n, 3))
+    pos[:, :dims] = rng.normal(0.0, 50.0, (n, dims)).astype(np.float32)
+    vel = np.zeros((n, 3))
+    vel[:, :dims] = rng.normal(0.0, 2.0, (n, dims)).astype(np.float32)
+    fix = np.full((n, 3), np.nan)
+    fix[::37, :dims] = 7.0
+    out = []
+    for forces in ([d3fgpu.center(1.0, 2.0, 3.0)],
+                   [d3fgpu.center(1.0, 2.0, 3.0), d3fgpu.radial(np.full(n, 10.0), np.zeros(n))]):
+        e = d3fgpu.Engine(0)
+        try:
+            e.set_nodes(pos, vel=vel, fix=fix, ndim=dims)
+            e.set_forces(forces)
+            e.enqueue([0.5, 0.49, 0.48], 0.6)
+            out.append(e.readback())
+        finally:
+            e.close()
+    assert np.all(np.isfinite(out[0][0]))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
