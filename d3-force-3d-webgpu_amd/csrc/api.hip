@@ -128,7 +128,7 @@ struct fg_ctx {
   uint64_t steps = 0;
   uint64_t epoch = 1;
   int radix_bits = 8;
-  uint32_t walk_packed_max = 24;
+  uint32_t walk_packed_max = WALK_PACKED_BY_MODE;
   uint32_t walk_tile = 0;
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
   uint32_t reduce_block = 0;

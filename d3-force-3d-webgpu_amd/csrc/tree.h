@@ -76,10 +76,11 @@ struct TreeBuffers {
 };
 
 constexpr uint32_t WALK_SOLO_BY_DIMS = 0xffffffffu;  // walk_solo_max: the measured default per dims
+constexpr uint32_t WALK_PACKED_BY_MODE = 0xffffffffu;  // walk_packed_max: the measured default per mode
 
 struct TreeParams {
   double theta2, dmin2, dmax2;
-  uint32_t walk_packed_max = 24;  // openings by at most this many points run packed
+  uint32_t walk_packed_max = WALK_PACKED_BY_MODE;  // openings by at most this many points run packed
   uint32_t walk_tile = 0;         // points per tile of a small (exact-mode) walk; 0: by size
   // trees of at most this many points walk one point per group of lanes (0: never)
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;

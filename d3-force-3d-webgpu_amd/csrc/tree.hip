@@ -2015,9 +2015,12 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
   A.dmin2 = p.dmin2;
   A.dmax2 = p.dmax2;
   A.n = n;
-  A.packed_max = p.walk_packed_max;
   const int cus = device_cus();
   const bool exact = n <= WALK_EXACT_MAX;
+  // measured (link + manyBody + center ticks): 32 for the exact mode (100,000 points 0.511
+  // -> 0.495 ms/tick against 24, 20,000-50,000 flat); 24 for the cheaper mode (1M: 32 within
+  // the noise, 40 and 16 slower)
+  A.packed_max = p.walk_packed_max != WALK_PACKED_BY_MODE ? p.walk_packed_max : exact ? 32u : 24u;
   // measured on 256 CUs (link + manyBody + center ticks): a group of lanes per point is the
   // faster walk up to 12,000 points in 3D (0.254 against 0.258 ms/tick) but not at 16,000
   // (0.296 against 0.264), and still at 24,000 in 2D (0.234 against 0.237) and 32,000 in 1D
