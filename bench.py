@@ -38,7 +38,6 @@ def log(*a):
 def make_workload(n, with_collide=True, with_radial=True, with_pins=True):
     """Node positions, the force list and pins of the 1M-node configuration."""
     import d3fgpu
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     pos = phyllotaxis3(n)
     src, dst = make_graph(n, 42)
     bias, strengths, distances = link_defaults(n, src, dst)

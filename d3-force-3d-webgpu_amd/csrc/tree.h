@@ -34,6 +34,8 @@ struct NodeSum {
   double a;       // sum of strength * c^leafDepth
   double w;       // sum of the d3-children weights inside this node's cell group
   double gx, gy, gz;  // weight * centre, same children
+  uint32_t same;  // all of the node's points lie at one position (nodes in equal-code runs)
+  uint32_t pad;
 };
 
 // Cube of the tree (d3's cover), and the running box used to find it.

@@ -617,6 +617,7 @@ struct ReduceLDS {
   float4 leaf[B];
   uint4 link[B];
   uint32_t flag[B];
+  uint8_t same[B];  // NodeSum::same of the local nodes
 };
 
 // A node's |value|-weighted centre from its sums (0 when it has no weight).
@@ -736,28 +737,73 @@ __global__ void __launch_bounds__(256) tree_collapse(uint32_t n, const uint32_t*
 
 // Combines node p's children (a node whose sibling has arrived) and writes its record;
 // returns its parent reference (with the locality bit).
+// Whether child k of an equal-code node has all its points at one position (a point has).
+template <bool LOCAL, uint32_t B>
+__device__ inline bool child_same(uint32_t k, uint32_t lo, const ReduceLDS<B>& S, const NodeSum* sums) {
+  if (k & REF_POINT) return true;
+  return LOCAL ? S.same[k - lo] != 0 : load_coherent(&sums[k].same) != 0u;
+}
+
+// Whether sorted points a and a + 1 lie at one position.
+template <int D, bool LOCAL, uint32_t B>
+__device__ inline bool same_position(uint32_t a, uint32_t lo, const ReduceLDS<B>& S, const float4* leaves) {
+  const float4 u = LOCAL ? S.leaf[a - lo] : leaves[a];
+  const float4 v = LOCAL ? S.leaf[a + 1 - lo] : leaves[a + 1];
+  return u.x == v.x && (D < 2 || u.y == v.y) && (D < 3 || u.z == v.z);
+}
+
 template <int D, bool LOCAL, uint32_t B>
 __device__ inline uint32_t reduce_node(uint32_t p, uint32_t lo, ReduceLDS<B>& S, const float4* leaves,
                                        const uint4* links, NodeRec* rec, NodeSum* sums) {
+  constexpr uint32_t LEVELS = (uint32_t)Dims<D>::LEVELS;
   const uint4 L = LOCAL ? S.link[p - lo] : links[p];
-  NodeSum sum = {0, 0, 0, 0, 0};
+  NodeSum sum = {};
   child_sum<D, LOCAL, B>(L.x, L.z, lo, S, leaves, links, rec, sums, sum);
   child_sum<D, LOCAL, B>(L.y, L.z, lo, S, leaves, links, rec, sums, sum);
+  // A node at the deepest level is a run of equal codes (unequal codes part above it).  The
+  // codes put points at one position there, one level below the run; d3 keeps them as one
+  // leaf, their chain, in the cell where they part from the other points: one level below
+  // the run's parent, its strength the chain's undamped sum (src/manyBody.js:61-68).  So
+  // the root of a run whose points all coincide takes that depth, and the nodes inside a
+  // run record depth 0: below any parent's cell, so a walk always opens them.  Codes and
+  // topology stay as they are.
+  uint32_t vdepth = L.z, rdepth = L.z;  // the depth of the value, and of the record
+  int lead = (int)L.z + 1;              // the depth of the node's own points
+  uint32_t same = 0;
+  if (L.z == LEVELS) {
+    same = child_same<LOCAL, B>(L.x, lo, S, sums) && child_same<LOCAL, B>(L.y, lo, S, sums) &&
+           same_position<D, LOCAL, B>(L.x & ~REF_POINT, lo, S, leaves);
+    const bool has_parent = L.w != REF_NONE;
+    uint32_t pd = 0;
+    if (has_parent) {
+      const uint32_t q = L.w & ~PARENT_LOCAL;
+      pd = LOCAL && (L.w & PARENT_LOCAL) ? S.link[q - lo].z : links[q].z;
+    }
+    if (has_parent && pd == LEVELS) {
+      rdepth = 0;
+    } else if (same) {
+      const uint32_t dl = has_parent ? pd + 1 : 0u;
+      sum.a *= damp<D>((int)dl - (int)(LEVELS + 1));
+      vdepth = rdepth = dl;
+      lead = (int)dl;
+    }
+  }
   const double cx = centre(sum.gx, sum.w);
   const double cy = D > 1 ? centre(sum.gy, sum.w) : 0.0;
   const double cz = D > 2 ? centre(sum.gz, sum.w) : 0.0;
-  double value = sum.a * damp<D>(-(int)L.z);
+  double value = sum.a * damp<D>(-(int)vdepth);
   // d3 skips a cell whose strengths add up to exactly 0; sums in another order leave a
   // residue of rounding, so a cancellation to 2^-40 of the cell's weight counts as 0
-  double scale = sum.w * damp<D>((int)L.z + 1);
+  double scale = sum.w * damp<D>(lead);
   if (fabs(sum.a) <= fabs(scale) * 0x1p-40 || !(sum.w > 0)) value = 0.0;
   NodeRec& r = rec[p];
   r.value = (float)value;
-  r.depth = L.z;
+  r.depth = rdepth;
   // a node's parent is local only if the node is (the parent covers the node's points)
   const bool parent_local = LOCAL && L.w != REF_NONE && (L.w & PARENT_LOCAL);
   if (parent_local) {
     const uint32_t q = p - lo;
+    S.same[q] = (uint8_t)same;
     S.a[q] = sum.a;
     S.w[q] = sum.w;
     S.gx[q] = sum.gx;
@@ -780,6 +826,7 @@ __device__ inline uint32_t reduce_node(uint32_t p, uint32_t lo, ReduceLDS<B>& S,
     store_coherent(&sums[p].gx, sum.gx);
     store_coherent(&sums[p].gy, sum.gy);
     store_coherent(&sums[p].gz, sum.gz);
+    store_coherent(&sums[p].same, same);
   }
   return L.w;
 }

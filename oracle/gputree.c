@@ -214,9 +214,18 @@ void or_gt_cells(int32_t n, int32_t ndim, const uint32_t* order, const double* x
   if (n < 2) return;
   const double* P[3] = {x, y, z};
   int32_t inner = n - 1;
+  const int levels = LEVELS_OF[ndim];
   double* A = (double*)malloc(sizeof(double) * (size_t)inner);
   double* G = (double*)malloc(sizeof(double) * 3 * (size_t)inner);
   double* W = (double*)malloc(sizeof(double) * (size_t)inner);
+  /* same[i]: all points of node i lie at one position; up[i]: the depth of its parent */
+  uint8_t* same = (uint8_t*)calloc((size_t)inner, 1);
+  int32_t* up = (int32_t*)malloc(sizeof(int32_t) * (size_t)inner);
+  up[0] = -1;
+  for (int32_t i = 0; i < inner; ++i) {
+    if (!(left[i] & LEAF)) up[left[i]] = (int32_t)depth[i];
+    if (!(right[i] & LEAF)) up[right[i]] = (int32_t)depth[i];
+  }
   /* iterative post-order from the root (node 0) */
   int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)n + 16);
   int32_t sp = 0;
@@ -258,14 +267,35 @@ void or_gt_cells(int32_t n, int32_t ndim, const uint32_t* order, const double* x
         w += W[k];
       }
     }
+    /* d3 keeps points at one position as one leaf in the cell where they part from the
+     * other points, its value their undamped sum (src/manyBody.js:61-68); on the GPU tree
+     * that is the root of a run of equal codes (depth LEVELS) whose points coincide, one
+     * level below its parent's cell */
+    int vd = dp;
+    if (dp == levels) {
+      uint32_t split = left[node] & ~LEAF;
+      int sl = (left[node] & LEAF) || same[left[node]];
+      int sr = (right[node] & LEAF) || same[right[node]];
+      uint32_t u = order[split], v = order[split + 1];
+      int eq = 1;
+      for (int q = 0; q < ndim; ++q) eq = eq && P[q][u] == P[q][v];
+      same[node] = (uint8_t)(sl && sr && eq);
+      if (up[node] != levels && same[node]) {
+        int dl = up[node] < 0 ? 0 : up[node] + 1;
+        a *= damp(ndim, dl - (levels + 1));
+        vd = dl;
+      }
+    }
     A[node] = a;
     W[node] = w;
     for (int q = 0; q < 3; ++q) {
       G[3 * node + q] = g[q];
       com[3 * node + q] = (q < ndim && w > 0) ? g[q] / w : 0;
     }
-    value[node] = a * damp(ndim, -dp);
+    value[node] = a * damp(ndim, -vd);
   }
+  free(same);
+  free(up);
   free(stack);
   free(A);
   free(G);
