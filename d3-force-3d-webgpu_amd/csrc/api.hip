@@ -239,7 +239,7 @@ void update_pass(fg_ctx* c, Pass& p, const fg_force_desc& d) {
     case FG_FORCE_CENTER:
       if (!p.center.p) {
         p.center.alloc(sizeof(CenterWork));
-        FG_HIP(hipMemset(p.center.p, 0, sizeof(CenterWork)));
+        clear_device(p.center.p, sizeof(CenterWork));
       }
       break;
     default:

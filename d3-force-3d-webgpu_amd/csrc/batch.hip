@@ -712,7 +712,7 @@ int fg_batch_create(fg_batch** out, int device, uint64_t hbm_budget) {
     FG_HIP(hipMemGetInfo(&free_b, &total_b));
     b->budget = hbm_budget ? hbm_budget : (uint64_t)(0.8 * (double)free_b);
     FG_HIP(hipMalloc(&b->table, sizeof(BJob) * MAX_SLOTS));
-    FG_HIP(hipMemset(b->table, 0, sizeof(BJob) * MAX_SLOTS));
+    clear_device(b->table, sizeof(BJob) * MAX_SLOTS);
     b->slot_job.assign(MAX_SLOTS, nullptr);
     if (const char* v = getenv("D3FGPU_RADIX_BITS")) b->tune.radix_bits = atoi(v) == 11 ? 11 : 8;
     return FG_OK;

@@ -32,6 +32,20 @@ struct ArgError : std::runtime_error {
 
 #define FG_LAUNCH_CHECK() FG_HIP(hipGetLastError())
 
+// Set-up writes to freshly allocated device memory (zeroed counters, initial records).  A
+// plain hipMemset or hipMemcpy goes to the null stream, which the engines' non-blocking
+// streams are not ordered after, and may return before it lands; memory reused from an
+// earlier allocation would then still hold the old contents when a kernel first reads it.
+// So these wait for the write before returning (they run at allocation time only).
+inline void clear_device(void* p, size_t bytes) {
+  FG_HIP(hipMemset(p, 0, bytes));
+  FG_HIP(hipStreamSynchronize(nullptr));
+}
+inline void upload_device(void* dst, const void* src, size_t bytes) {
+  FG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  FG_HIP(hipStreamSynchronize(nullptr));
+}
+
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // ---- device helpers ------------------------------------------------------------------

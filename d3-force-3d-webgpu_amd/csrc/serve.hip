@@ -175,9 +175,12 @@ void worker_loop(fg_server* s) {
     if (fg_batch_step(s->batch, k) != FG_OK || fg_batch_get_stats(s->batch, &bst) != FG_OK) {
       std::lock_guard<std::mutex> lk(s->mu);
       s->err = fg_batch_last_error(s->batch);
+      // the failed jobs leave the batch, so their memory no longer counts against the budget
       for (auto& kv : s->running) {
+        (void)fg_batch_release(s->batch, kv.first);
         kv.second->status = FG_SERVE_FAILED;
         kv.second->error = s->err;
+        kv.second->t_finish = s->now();
         s->failed++;
       }
       s->running.clear();
@@ -337,7 +340,9 @@ int fg_serve_wait(fg_server* s, uint64_t id, double timeout_s, double* pos, doub
   }
   if (r->status != FG_SERVE_FINISHED) {
     s->err = r->error.empty() ? "the server stopped" : r->error;
-    s->all.erase(it);
+    // a job still queued or running when the server stops may be in the worker's hands (an
+    // upload outside the lock): the server frees it once the worker has ended
+    if (r->status == FG_SERVE_FAILED) s->all.erase(it);
     return FG_ERR_STATE;
   }
   if (pos) memcpy(pos, r->out_pos.data(), sizeof(double) * r->out_pos.size());

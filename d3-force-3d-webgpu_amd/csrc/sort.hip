@@ -383,12 +383,12 @@ void SortWork::reserve(uint32_t n, size_t key_bytes) {
   FG_HIP(hipMalloc(&vals_alt, sizeof(uint32_t) * (size_t)(n > 0 ? n : 1)));
   FG_HIP(hipMalloc(&ghist, sizeof(uint32_t) * 8 * 2048));
   FG_HIP(hipMalloc(&lookback, sizeof(uint64_t) * need_lb));
-  FG_HIP(hipMemset(lookback, 0, sizeof(uint64_t) * need_lb));
+  clear_device(lookback, sizeof(uint64_t) * need_lb);
   FG_HIP(hipMalloc(&tickets, sizeof(uint32_t) * 16));
   FG_HIP(hipMalloc(&epoch, sizeof(uint32_t) * 4));
-  FG_HIP(hipMemset(epoch, 0, sizeof(uint32_t) * 4));
+  clear_device(epoch, sizeof(uint32_t) * 4);
   FG_HIP(hipMalloc(&errors, sizeof(uint32_t) * 4));
-  FG_HIP(hipMemset(errors, 0, sizeof(uint32_t) * 4));
+  clear_device(errors, sizeof(uint32_t) * 4);
   cap_n = n;
   cap_key_bytes = key_bytes;
   lookback_words = need_lb;

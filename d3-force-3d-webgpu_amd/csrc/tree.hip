@@ -2208,8 +2208,8 @@ void TreeBuffers::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   CubeInfo c0 = {};
   for (int a = 0; a < 3; ++a) c0.box[a] = 0xffffffffu, c0.box[3 + a] = 0u;
-  FG_HIP(hipMemcpy(cube, &c0, sizeof(c0), hipMemcpyHostToDevice));
-  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * TREE_COUNTERS));
+  upload_device(cube, &c0, sizeof(c0));
+  clear_device(counters, sizeof(uint32_t) * TREE_COUNTERS);
   sort.reserve(m, sizeof(uint64_t));
   cap = n;
 }
@@ -2279,7 +2279,7 @@ void BatchTree::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&rec, sizeof(NodeRec) * m));
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
   FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * TREE_COUNTERS));
-  FG_HIP(hipMemset(counters, 0, sizeof(uint32_t) * TREE_COUNTERS));
+  clear_device(counters, sizeof(uint32_t) * TREE_COUNTERS);
   FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   sort.reserve(m, sizeof(uint64_t));
   sort_seg.reserve(m, sizeof(uint32_t));

@@ -190,10 +190,11 @@ enum fg_phase {
   FG_PHASE_PAIRS = 4,      /* the exact all-pairs many-body kernel */
   FG_PHASE_LINK = 5,
   FG_PHASE_COLLIDE = 6,
-  FG_PHASE_CENTER = 7,
+  FG_PHASE_CENTER = 7,     /* empty up to 1,024 nodes when center is the last force: see INTEGRATE */
   FG_PHASE_RADIAL = 8,
   FG_PHASE_POSITION = 9,
-  FG_PHASE_INTEGRATE = 10,
+  FG_PHASE_INTEGRATE = 10, /* the integrator; up to 1,024 nodes with center the last force, the
+                              same launch finds and applies the center shift */
   FG_PHASE_COUNT = 16
 };
 

@@ -2,6 +2,8 @@
 phyllotaxis start, and the state after GPU ticks), one application of each force on the GPU
 against the FP64 oracle.
 
+  config A: 1,000 nodes, 3D, link + manyBody(theta 0.9) + center, at ticks 0, 1, 150 and
+            299 of the GPU's own run; and the layout's size after 300 ticks
   config B: 200,000 nodes, 3D, link + manyBody (theta 0.9, and theta 0 on a sample of
             2,000 nodes against the exact all-pairs sum) + center
   config C: 1,000,000 nodes, 3D, link + manyBody(theta 0.9) + center + collide (2 iterations,
@@ -184,3 +186,64 @@ def test_config_c_forces(tick):
         record("C 1M", name, tick, err, touched=int(np.count_nonzero(np.any(ref != 0, 1))))
         assert err < (1e-4 if name.startswith("collide") else 1e-5)
     check_center("C 1M", tick, pos, vel)
+
+
+# ---- config A: the reference's 1,000-node benchmark graph ------------------------------
+
+def config_a():
+    n = 1000
+    src, dst = make_graph(n, 42)
+    bias, strengths, distances = link_defaults(n, src, dst)
+    s = np.full(n, -30.0)
+    forces = [d3fgpu.link(src, dst, strengths, distances, bias), d3fgpu.many_body(s, theta=0.9), d3fgpu.center()]
+    return n, src, dst, bias, strengths, distances, s, forces
+
+
+@pytest.mark.parametrize("tick", [0, 1, 150, 299])
+def test_config_a_forces(tick):
+    """Config A (examples/benchmark.html:86-93: 1,000 nodes, 1,499 links, link + manyBody +
+    center from d3's start): the state the GPU reaches after `tick` ticks, then one
+    application of each force on the GPU and in the FP64 oracle."""
+    n, src, dst, bias, strengths, distances, s, forces = config_a()
+    pos = phyllotaxis3(n).astype(np.float32).astype(np.float64)
+    vel = np.zeros_like(pos)
+    if tick:
+        pos, vel = gpu_state_after(pos, forces, tick)
+    check_manybody("A 1k", tick, pos, vel, s)
+    store = []
+    links = np.ascontiguousarray(np.stack([src, dst], 1).reshape(-1), np.uint32)
+    store.append(links)
+    ld = O.ForceDesc(type=3, iterations=1, count=len(src), links=O.P(links, O.C.c_uint32),
+                     strengths=keep(strengths, store), distances=keep(distances, store), bias=keep(bias, store))
+    _, dv = gpu_increment(pos, vel, [forces[0]])
+    _, ref = oracle_increment(pos, vel, [ld])
+    err = rel_l2(dv, ref)
+    record("A 1k", "link", tick, err)
+    assert err < 1e-5
+    check_center("A 1k", tick, pos, vel)
+
+
+def test_config_a_layout_size():
+    """300 ticks of config A on the GPU: the layout's RMS radius against the FP64 oracle's
+    with the GPU's Jacobi link (374.19) and the reference's published GPU figure, 374
+    (docs/differences.md:35-40; d3's sequential CPU layout is 372)."""
+    n, src, dst, bias, strengths, distances, s, forces = config_a()
+    pos = phyllotaxis3(n)
+    p, _ = gpu_state_after(pos, forces, 300)
+    rms = float(np.sqrt(np.mean(np.sum(p ** 2, 1))))
+    store = []
+    links = np.ascontiguousarray(np.stack([src, dst], 1).reshape(-1), np.uint32)
+    store.append(links)
+    descs = [O.ForceDesc(type=3, iterations=1, count=len(src), links=O.P(links, O.C.c_uint32),
+                         strengths=keep(strengths, store), distances=keep(distances, store), bias=keep(bias, store)),
+             O.ForceDesc(type=4, strengths=keep(s, store), theta2=0.81, distance_min2=1.0, distance_max2=np.inf),
+             O.ForceDesc(type=1, strength=1.0)]
+    sim = O.Sim(pos, ndim=3)
+    sim.set_forces(descs, jacobi=True)
+    sim.tick(300)
+    op, _ = sim.state()
+    want = float(np.sqrt(np.mean(np.sum(op ** 2, 1))))
+    PARITY.append({"case": "A 1k RMS radius after 300 ticks", "gpu": rms, "oracle_jacobi": want, "published_gpu": 374})
+    print(f"[parity] A 1k RMS radius after 300 ticks: GPU {rms:.2f}, FP64 Jacobi {want:.2f}, published 374", flush=True)
+    assert abs(rms - want) < 1.0
+    assert abs(rms - 374) < 1.5

@@ -235,6 +235,36 @@ def test_gpu_jiggle_known_answer(theta, engine_factory):
             np.testing.assert_allclose(dv[1], -u * s[0] * 0.5, rtol=2e-6, atol=1e-12)
 
 
+@pytest.mark.parametrize("dims,n", [(3, 12288), (3, 12289), (2, 24576), (2, 24577), (1, 20000), (1, 32768),
+                                    (1, 32769)])
+def test_manybody_lane_group_walk_switch_points(monkeypatch, dims, n):
+    """Either side of the sizes up to which a tree is walked with a group of 2^D lanes per
+    point (12,288 points in 3D, 24,576 in 2D, 32,768 in 1D, with 1D's largest workgroups in
+    LDS at 20,000): the default walk equals the tiled exact-mode walk bit for bit, has no
+    walk errors, and matches the FP64 oracle."""
+    pos = make_nodes_fast(n, dims, 20 * n ** (1 / dims), seed=n)
+    s = np.full(n, -30.0) * (1 + 0.25 * np.sin(np.arange(n)))
+    out = []
+    for solo in (None, "0"):
+        if solo is None:
+            monkeypatch.delenv("D3FGPU_WALK_SOLO", raising=False)
+        else:
+            monkeypatch.setenv("D3FGPU_WALK_SOLO", solo)
+        e = d3fgpu.Engine(0)
+        try:
+            _, dv = gpu_dv(e, pos, dims, [d3fgpu.many_body(s)])
+            assert e.stats().walk_errors == 0
+        finally:
+            e.close()
+        out.append(dv)
+    assert np.array_equal(out[0], out[1])
+    ref, _ = oracle_manybody(pos, dims, s)
+    err = rel_l2(out[0], ref)
+    report(f"manyBody tree {dims}D n={n} (lane-group walk switch)", err)
+    assert err < 1e-4
+    assert close_count(out[0], ref, rtol=1e-3) <= tree_outliers(n)
+
+
 def test_manybody_tree_above_exact_mode(engine_factory):
     """Past 163840 points the walk takes 64-point tiles and the cheaper accumulation."""
     n = 170000
@@ -605,3 +635,32 @@ def test_center_last_in_one_launch(dims, n):
             e.close()
     assert np.all(np.isfinite(out[0][0]))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_center_in_fresh_engines_over_reused_memory():
+    """Engines made and freed in turn get memory back that held other data: a center pass's
+    counters must be cleared before its first launch reads them (set-up clears are waited
+    for), so every fresh engine moves every node by the FP64 shift."""
+    rng = np.random.default_rng(8)
+    for k in range(12):
+        n = int(rng.integers(300, 5000))
+        pos = make_nodes_fast(n, 3, 300, seed=k)
+        e = d3fgpu.Engine(0)
+        try:  # dirty the allocator with a tree, links and velocities
+            src = rng.integers(0, n, n).astype(np.uint32)
+            dst = ((src + 1) % n).astype(np.uint32)
+            b, st, di = O.link_defaults(n, src, dst)
+            gpu_dv(e, pos, 3, [d3fgpu.link(src, dst, st, di, b), d3fgpu.many_body(np.full(n, -30.0))],
+                   vel=rng.standard_normal((n, 3)))
+        finally:
+            e.close()
+        shifted = pos + np.array([1e-3, -2.5e-4, 4e-5]) * (k + 1)
+        e = d3fgpu.Engine(0)
+        try:
+            p, _ = gpu_dv(e, shifted, 3, [d3fgpu.center()])
+        finally:
+            e.close()
+        p32 = shifted.astype(np.float32)
+        mean = p32.astype(np.float64).mean(0)
+        want = (p32 - mean.astype(np.float32)).astype(np.float64)
+        assert np.array_equal(p, want), f"engine {k}: center did not move the nodes by the mean"

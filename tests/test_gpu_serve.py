@@ -80,3 +80,27 @@ def test_serve_concurrent_submitters():
         pos, _, r = srv.wait(i, timeout=10.0)
         assert r["status"] == d3fgpu.SERVE_FINISHED and np.all(np.isfinite(pos))
     srv.close()
+
+
+def test_serve_config_e_large_job_alone_and_in_mix():
+    """Config E's size range reaches 500,000 nodes: a 300,000-node 3D job (BA m=2, link +
+    manyBody + collide + center + radial) served among smaller jobs arriving while it runs
+    ends bit for bit where it ends alone, after the same number of ticks."""
+    big = dict(n=300_000, ndim=3, m=2, mix=[True, True, True, True, True], seed=77)
+    specs = [big] + serve_specs(6, seed=5, lo=1000, hi=20000)
+    jobs = [serve_job(s) for s in specs]
+    srv = d3fgpu.Server(0, ticks_per_round=10)
+    ids = [srv.submit(jobs[0])]
+    time.sleep(0.05)
+    for j in jobs[1:]:
+        ids.append(srv.submit(j))
+        time.sleep(0.02)
+    results = [srv.wait(i, timeout=300.0) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert st["finished"] == len(jobs) and st["failed"] == 0 and st["peak_running"] >= 2
+    pos, vel, r = results[0]
+    assert r["status"] == d3fgpu.SERVE_FINISHED and np.all(np.isfinite(pos))
+    (p1, v1), ticks = alone(jobs[0])
+    assert r["ticks"] == ticks
+    assert np.array_equal(pos, p1) and np.array_equal(vel, v1)
