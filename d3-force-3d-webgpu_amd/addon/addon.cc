@@ -1,11 +1,16 @@
 // addon.cc -- N-API binding of the C ABI (include/d3fgpu.h) for the Node.js host.
 //
-// A thin layer: every JS call maps to one fg_* call.  The JS engine (js/engine.js) keeps the
-// reference engine's logic (edit diffing, readback landing, busy/idle, src/gpu/engine.js);
-// this file only converts arguments.  Waiting happens off the JS thread: readbackAsync and
-// idleAsync run fg_readback_wait / fg_sync in napi_async_work and settle a Promise.
+// Most JS calls map to one fg_* call.  The exception is the node mirror: the JS engine
+// (js/engine.js) packs the simulation's node objects into one Float64Array of STATE doubles
+// per node and hands it over whole; this file keeps, per engine, the shadow of what the
+// device was last told (positions and pins), uploads only the runs of nodes that differ
+// from it (load / push), and merges a readback into the packed state without overwriting
+// nodes the caller moved since the last exchange (land).  Waiting happens off the JS
+// thread: readbackAsync and idleAsync run fg_readback_wait / fg_sync in napi_async_work
+// and settle a Promise.
 #include <node_api.h>
 #include <stdint.h>
+#include <math.h>
 #include <string.h>
 
 #include <string>
@@ -38,11 +43,29 @@ bool get_args(napi_env env, napi_callback_info info, size_t want, napi_value* ar
   return true;
 }
 
+// The packed node state: x, y, z, vx, vy, vz, fx, fy, fz per node (fx..fz NaN: not pinned).
+constexpr size_t STATE = 9;
+// The shadow keeps x, y, z, fx, fy, fz per node: what the device holds (or will once the
+// uploads land), compared bit for bit.
+constexpr size_t SHADOW = 6;
+
 // The JS handle owns a box, so that destroy() can free the context before the handle is
 // collected and the finalizer then finds nothing to free.
 struct Box {
   fg_ctx* c;
+  std::vector<double> shadow;
+  uint32_t n = 0;
+  int nd = 2;
 };
+
+Box* box_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !((Box*)p)->c) {
+    napi_throw_error(env, nullptr, "d3fgpu: not a live engine handle");
+    return nullptr;
+  }
+  return (Box*)p;
+}
 
 fg_ctx* ctx_of(napi_env env, napi_value v) {
   void* p = nullptr;
@@ -183,6 +206,145 @@ napi_value UpdatePins(napi_env env, napi_callback_info info) {
   uint8_t* mask = typed<uint8_t>(env, argv[3], napi_uint8_array, &km);
   int rc = fg_update_pins(c, (uint32_t)num(env, argv[1]), (uint32_t)km, fix, mask);
   if (rc != FG_OK) return throw_status(env, c, rc, "updatePins");
+  return nullptr;
+}
+
+// ---- the node mirror -------------------------------------------------------------------
+
+bool same_bits(const double* a, const double* b, int k) { return memcmp(a, b, sizeof(double) * (size_t)k) == 0; }
+
+// load(handle, state, nDim): every node goes up (a new node set, or after the host moved the
+// nodes itself); the shadow becomes the state.
+napi_value Load(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  Box* b = box_of(env, argv[0]);
+  if (!b) return nullptr;
+  size_t k;
+  const double* st = typed<double>(env, argv[1], napi_float64_array, &k);
+  const int nd = (int)num(env, argv[2], 2);
+  const uint32_t n = (uint32_t)(k / STATE);
+  std::vector<double> pos(3 * (size_t)n, 0.0), vel(3 * (size_t)n, 0.0), fix(3 * (size_t)n, 0.0);
+  std::vector<uint8_t> mask(n, 0);
+  b->shadow.assign(SHADOW * (size_t)n, 0.0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const double* s = st + STATE * i;
+    double* sh = b->shadow.data() + SHADOW * i;
+    for (int a = 0; a < 3; ++a) {
+      const bool used = a < nd;
+      pos[3 * i + a] = used ? s[a] : 0.0;
+      vel[3 * i + a] = used ? s[3 + a] : 0.0;
+      const bool pinned = used && s[6 + a] == s[6 + a];
+      fix[3 * i + a] = pinned ? s[6 + a] : 0.0;
+      if (pinned) mask[i] |= (uint8_t)(1u << a);
+      sh[a] = pos[3 * i + a];
+      sh[3 + a] = used ? s[6 + a] : NAN;
+    }
+  }
+  b->n = n;
+  b->nd = nd;
+  int rc = fg_set_nodes(b->c, n, nd, pos.data(), vel.data(), fix.data(), mask.data());
+  if (rc != FG_OK) return throw_status(env, b->c, rc, "load");
+  return nullptr;
+}
+
+// push(handle, state) -> nodes uploaded: positions and pins that differ from the shadow go
+// up, one fg_update_positions / fg_update_pins call per run of consecutive changed nodes.
+napi_value Push(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  Box* b = box_of(env, argv[0]);
+  if (!b) return nullptr;
+  size_t k;
+  const double* st = typed<double>(env, argv[1], napi_float64_array, &k);
+  if (k / STATE != b->n) {
+    napi_throw_error(env, nullptr, "d3fgpu push: the state has another node count than the last load");
+    return nullptr;
+  }
+  const int nd = b->nd;
+  uint32_t moved = 0;
+  std::vector<double> run_pos, run_fix;
+  std::vector<uint8_t> run_mask;
+  uint32_t pos_first = 0, pin_first = 0;
+  bool in_pos = false, in_pin = false;
+  auto flush_pos = [&](uint32_t end) {
+    if (!in_pos) return 0;
+    in_pos = false;
+    moved += end - pos_first;
+    return fg_update_positions(b->c, pos_first, end - pos_first, run_pos.data());
+  };
+  auto flush_pin = [&](uint32_t end) {
+    if (!in_pin) return 0;
+    in_pin = false;
+    return fg_update_pins(b->c, pin_first, end - pin_first, run_fix.data(), run_mask.data());
+  };
+  int rc = FG_OK;
+  for (uint32_t i = 0; i <= b->n && rc == FG_OK; ++i) {
+    bool dpos = false, dpin = false;
+    double p[3] = {0, 0, 0}, f[3] = {NAN, NAN, NAN};
+    if (i < b->n) {
+      const double* s = st + STATE * i;
+      for (int a = 0; a < nd; ++a) p[a] = s[a], f[a] = s[6 + a];
+      double* sh = b->shadow.data() + SHADOW * i;
+      dpos = !same_bits(p, sh, 3);
+      dpin = !same_bits(f, sh + 3, 3);
+      if (dpos) memcpy(sh, p, sizeof(p));
+      if (dpin) memcpy(sh + 3, f, sizeof(f));
+    }
+    if (dpos) {
+      if (!in_pos) in_pos = true, pos_first = i, run_pos.clear();
+      run_pos.insert(run_pos.end(), p, p + 3);
+    } else {
+      rc = flush_pos(i);
+    }
+    if (rc != FG_OK) break;
+    if (dpin) {
+      if (!in_pin) in_pin = true, pin_first = i, run_fix.clear(), run_mask.clear();
+      uint8_t m = 0;
+      for (int a = 0; a < 3; ++a) {
+        const bool pinned = f[a] == f[a];
+        run_fix.push_back(pinned ? f[a] : 0.0);
+        if (pinned) m |= (uint8_t)(1u << a);
+      }
+      run_mask.push_back(m);
+    } else {
+      rc = flush_pin(i);
+    }
+  }
+  if (rc != FG_OK) return throw_status(env, b->c, rc, "push");
+  napi_value r;
+  NAPI_CALL(env, napi_create_uint32(env, moved, &r));
+  return r;
+}
+
+// land(handle, state, pos, vel): a readback's positions and velocities into the packed
+// state.  A node whose position in the state still equals the shadow has not been moved by
+// the caller since the last exchange: it takes the device's position (and so does the
+// shadow); a node the caller moved keeps its position, which the next push sends up.
+// Velocities always come from the device.
+napi_value Land(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  Box* b = box_of(env, argv[0]);
+  if (!b) return nullptr;
+  size_t k, kp, kv;
+  double* st = typed<double>(env, argv[1], napi_float64_array, &k);
+  const double* pos = typed<double>(env, argv[2], napi_float64_array, &kp);
+  const double* vel = typed<double>(env, argv[3], napi_float64_array, &kv);
+  if (k / STATE != b->n || kp < 3 * (size_t)b->n || kv < 3 * (size_t)b->n) {
+    napi_throw_error(env, nullptr, "d3fgpu land: arrays do not match the node count");
+    return nullptr;
+  }
+  const int nd = b->nd;
+  for (uint32_t i = 0; i < b->n; ++i) {
+    double* s = st + STATE * i;
+    double* sh = b->shadow.data() + SHADOW * i;
+    double p[3] = {0, 0, 0};
+    for (int a = 0; a < nd; ++a) p[a] = s[a];
+    if (same_bits(p, sh, 3))
+      for (int a = 0; a < nd; ++a) s[a] = sh[a] = pos[3 * i + a];
+    for (int a = 0; a < nd; ++a) s[3 + a] = vel[3 * i + a];
+  }
   return nullptr;
 }
 
@@ -383,6 +545,26 @@ napi_value Buffers(napi_env env, napi_callback_info info) {
   napi_create_double(env, (double)b.epoch, &v);
   napi_set_named_property(env, o, "epoch", v);
   return o;
+}
+
+// readDevice(handle, pointer (BigInt), floats) -> Float32Array: a copy of device memory,
+// for tests of what gpuBuffers() points to (fg_debug_copy_from_device).
+napi_value ReadDevice(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  fg_ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  uint64_t addr = 0;
+  bool lossless = false;
+  NAPI_CALL(env, napi_get_value_bigint_uint64(env, argv[1], &addr, &lossless));
+  const size_t count = (size_t)num(env, argv[2]);
+  napi_value buf, arr;
+  void* data = nullptr;
+  NAPI_CALL(env, napi_create_arraybuffer(env, sizeof(float) * count, &data, &buf));
+  int rc = fg_debug_copy_from_device(c, (const void*)(uintptr_t)addr, sizeof(float) * count, data);
+  if (rc != FG_OK) return throw_status(env, c, rc, "readDevice");
+  NAPI_CALL(env, napi_create_typedarray(env, napi_float32_array, count, buf, 0, &arr));
+  return arr;
 }
 
 napi_value Stats(napi_env env, napi_callback_info info) {
@@ -764,6 +946,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"destroy", nullptr, Destroy, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fits", nullptr, Fits, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"setNodes", nullptr, SetNodes, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"load", nullptr, Load, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"push", nullptr, Push, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"land", nullptr, Land, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"updatePositions", nullptr, UpdatePositions, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"updatePins", nullptr, UpdatePins, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"setForces", nullptr, SetForces, nullptr, nullptr, nullptr, napi_default, nullptr},
@@ -775,6 +960,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"idleAsync", nullptr, IdleAsync, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"buffers", nullptr, Buffers, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"stats", nullptr, Stats, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"readDevice", nullptr, ReadDevice, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"batchCreate", nullptr, BatchCreate, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"batchDestroy", nullptr, BatchDestroy, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"batchSubmit", nullptr, BatchSubmit, nullptr, nullptr, nullptr, napi_default, nullptr},

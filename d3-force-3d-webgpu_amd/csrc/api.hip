@@ -133,6 +133,13 @@ struct fg_ctx {
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
   uint32_t reduce_block = 0;
   const Pass* center_fused = nullptr;  // this tick's last pass, a center left to the integrator's launch
+  // test hooks, read at creation, so that a host's fallbacks can be driven without a real
+  // failure: D3FGPU_CAPACITY=N makes fg_fits refuse more than N nodes; D3FGPU_FAIL=enqueue:K
+  // or wait:K makes the K-th fg_enqueue, or the K-th fg_sync / fg_readback_wait, report a
+  // HIP failure (the device is not touched)
+  uint64_t capacity = 0;
+  int fail_kind = 0;  // 1: enqueue, 2: wait
+  uint64_t fail_at = 0, fail_calls = 0;
   std::string err;
   // readback
   float* staging = nullptr;
@@ -409,6 +416,14 @@ int fg_create(fg_ctx** out, int device) {
   if (const char* v = getenv("D3FGPU_WALK_SOLO")) c->walk_solo_max = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_REDUCE_BLOCK")) c->reduce_block = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_RADIX_BITS")) c->radix_bits = atoi(v) == 11 ? 11 : 8;
+  if (const char* v = getenv("D3FGPU_CAPACITY")) c->capacity = strtoull(v, nullptr, 10);
+  if (const char* v = getenv("D3FGPU_FAIL")) {
+    const char* colon = strchr(v, ':');
+    if (colon) {
+      c->fail_kind = strncmp(v, "enqueue", (size_t)(colon - v)) == 0 ? 1 : strncmp(v, "wait", (size_t)(colon - v)) == 0 ? 2 : 0;
+      c->fail_at = strtoull(colon + 1, nullptr, 10);
+    }
+  }
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return FG_ERR_HIP;
@@ -455,6 +470,7 @@ int fg_fits(fg_ctx* c, uint32_t n, int32_t ndim, int32_t* fits) {
     // state, pins, tree (keys, sort scratch, leaves, links, records, sums), grid, link CSR
     double per_node = 4.0 * 9 + 1 + 8 * 2 + 4 * 2 + 16 + 4 * 3 + 16 + 64 + 40 + 16 + 8 + 64;
     *fits = (n < 0x7fffffffu && per_node * n < 0.8 * (double)free_b) ? 1 : 0;
+    if (c->capacity && n > c->capacity) *fits = 0;
     return FG_OK;
   });
 }
@@ -596,8 +612,16 @@ int fg_set_seed(fg_ctx* c, uint32_t seed) {
   return FG_OK;
 }
 
+namespace {
+// The test hook's failure for call kind `kind`, when this is the call it names.
+bool injected(fg_ctx* c, int kind) {
+  return c->fail_kind == kind && ++c->fail_calls == c->fail_at;
+}
+}  // namespace
+
 int fg_enqueue(fg_ctx* c, uint32_t steps, const double* alphas, double decay) {
   return guarded(c, [&] {
+    if (injected(c, 1)) return fail(c, FG_ERR_HIP, "injected failure (D3FGPU_FAIL)");
     if (steps && !alphas) return fail(c, FG_ERR_INVALID, "alphas are null");
     if (!c->n) return FG_OK;
     const uint8_t* mask = c->pinned_any ? c->mask.as<uint8_t>() : nullptr;
@@ -652,6 +676,7 @@ int fg_busy(fg_ctx* c, int32_t* busy) {
 
 int fg_sync(fg_ctx* c) {
   return guarded(c, [&] {
+    if (injected(c, 2)) return fail(c, FG_ERR_HIP, "injected failure (D3FGPU_FAIL)");
     FG_HIP(hipStreamSynchronize(c->stream));
     retire_pending(c);
     return FG_OK;
@@ -688,6 +713,7 @@ int fg_readback_wait(fg_ctx* c, uint64_t ticket) {
   return guarded(c, [&] {
     if (!c->rb_inflight || ticket != c->rb_ticket) return fail(c, FG_ERR_STATE, "no such readback in flight");
     if (c->n) FG_HIP(hipEventSynchronize(c->rb_event));
+    if (injected(c, 2)) return fail(c, FG_ERR_HIP, "injected failure (D3FGPU_FAIL)");
     return FG_OK;
   });
 }
@@ -718,6 +744,15 @@ int fg_buffers(fg_ctx* c, fg_buffer_info* out) {
     out->count = c->n;
     out->stride = 16;
     out->epoch = c->epoch;
+    return FG_OK;
+  });
+}
+
+int fg_debug_copy_from_device(fg_ctx* c, const void* device_src, uint64_t bytes, void* host_dst) {
+  return guarded(c, [&] {
+    if (!device_src || !host_dst) return FG_ERR_INVALID;
+    FG_HIP(hipStreamSynchronize(c->stream));
+    FG_HIP(hipMemcpy(host_dst, device_src, (size_t)bytes, hipMemcpyDeviceToHost));
     return FG_OK;
   });
 }

@@ -1,256 +1,237 @@
-// The JS side of the engine: mirrors a node array in the HIP backend through the N-API
-// addon, with the reference engine's behaviour (src/gpu/engine.js):
+// The GPU engine of forceSimulationGPU: one HIP context (through the N-API addon) holding
+// the simulation's nodes, with the behaviour of the reference engine (src/gpu/engine.js):
 //
-//   * the GPU holds the truth for positions and velocities while the simulation runs;
-//   * nodes are written back asynchronously (readback), one readback at a time;
-//   * edits made on the JS side in the meantime -- x/y/z moved, fx/fy/fz pinned -- are found
-//     by comparing against what was last exchanged and uploaded before the next step
-//     (syncFromJS, engine.js:171-194); a node the user has touched is never overwritten by
-//     a readback that was already in flight (land, engine.js:198-216);
-//   * force descriptors are converted once per version: arrays cross to the backend only
-//     when a force's `version` changes (engine.js:357, passes/*.update).
+//   * while the simulation runs, the device holds the truth for positions and velocities;
+//   * the JS nodes follow by readbacks, one in flight at a time; a readback that an upload
+//     overtook is dropped (the C layer's epoch, fg_readback_fetch);
+//   * x/y/z or fx/fy/fz set on the JS side between submissions are sent up before the next
+//     one, and a readback never overwrites a node moved in the meantime.
+//
+// How: the nodes are packed into one Float64Array (addon.cc: x, y, z, vx, vy, vz, fx, fy,
+// fz per node) and the addon does the rest in native code -- it keeps the shadow of what
+// the device was told, uploads the runs of nodes that differ from it (push) and merges
+// readbacks into the packed state (land) -- so the JS side only packs and unpacks.
+// Force arrays cross to the device only when a force's `version` changes.
 import {native} from "./native.js";
 
-var TYPES = {center: 1, collide: 2, link: 3, manyBody: 4, position: 5, radial: 6};
+const STATE = 9;
+const TYPES = {center: 1, collide: 2, link: 3, manyBody: 4, position: 5, radial: 6};
 
-function num(v) {
-  return (v = +v) === v ? v : 0; // undefined / NaN -> 0
+function finite(v) {
+  v = +v;
+  return v === v ? v : 0; // undefined and NaN count as 0, as the reference's floats() does
 }
 
-function float64(values, n) {
-  var out = new Float64Array(n);
-  if (values) for (var i = 0; i < n; ++i) out[i] = +values[i];
+function column(values, count) {
+  const out = new Float64Array(count);
+  if (values) for (let i = 0; i < count; ++i) out[i] = +values[i];
   return out;
 }
 
-export function supported(desc) {
-  return !!(desc && TYPES[desc.type]);
+export function describable(force) {
+  if (typeof force.gpu !== "function") return false;
+  const d = force.gpu();
+  return !!(d && TYPES[d.type]);
 }
 
-export default function createEngine(device) {
-  var addon = native(),
-      handle = addon.create(device || 0),
-      nodes = [], n = 0, nDim = 2, alive = true, seed = 0,
-      enqueued = 0, landed = 0, epoch = 0, stale = false, inflight = null,
-      shadow = new Float64Array(0), pinned = new Uint8Array(0), pinnedAt = new Float64Array(0),
-      staging = null, forceIds = new Map(), nextId = 1, known = new Map();
+// The packed state of `nodes` (positions, velocities, pins) into `state`.
+function pack(nodes, nDim, state) {
+  for (let i = 0, n = nodes.length; i < n; ++i) {
+    const node = nodes[i], o = STATE * i;
+    state[o] = finite(node.x);
+    state[o + 1] = nDim > 1 ? finite(node.y) : 0;
+    state[o + 2] = nDim > 2 ? finite(node.z) : 0;
+    state[o + 3] = finite(node.vx);
+    state[o + 4] = nDim > 1 ? finite(node.vy) : 0;
+    state[o + 5] = nDim > 2 ? finite(node.vz) : 0;
+    state[o + 6] = node.fx == null ? NaN : +node.fx;
+    state[o + 7] = nDim > 1 && node.fy != null ? +node.fy : NaN;
+    state[o + 8] = nDim > 2 && node.fz != null ? +node.fz : NaN;
+  }
+}
 
-  function idOf(force) {
-    var id = forceIds.get(force);
-    if (!id) forceIds.set(force, id = nextId++);
+// Positions and velocities from the packed state back into `nodes`.
+function unpack(nodes, nDim, state) {
+  for (let i = 0, n = nodes.length; i < n; ++i) {
+    const node = nodes[i], o = STATE * i;
+    node.x = state[o];
+    node.vx = state[o + 3];
+    if (nDim > 1) node.y = state[o + 1], node.vy = state[o + 4];
+    if (nDim > 2) node.z = state[o + 2], node.vz = state[o + 5];
+  }
+}
+
+export default class GpuEngine {
+  constructor(device) {
+    this.addon = native();
+    this.handle = this.addon.create(device || 0);
+    this.nodes = [];
+    this.nDim = 2;
+    this.state = new Float64Array(0);
+    this.incoming = null;        // {pos, vel} buffers the readbacks land in
+    this.forces = null;
+    this.ids = new WeakMap();    // force object -> id for the backend's pass cache
+    this.nextId = 1;
+    this.sent = new Map();       // id -> version whose arrays the device has
+    this.submitted = 0;          // steps queued since the nodes were set
+    this.current = 0;            // steps the JS nodes reflect
+    this.inflight = null;
+    this.reload = false;         // the JS side moved every node: upload all before the next step
+    this.alive = true;
+  }
+
+  get count() {
+    return this.nodes.length;
+  }
+
+  fits(count) {
+    return this.addon.fits(this.handle, count, 3);
+  }
+
+  setNodes(nodes, nDim) {
+    this.nodes = nodes;
+    this.nDim = nDim;
+    this.state = new Float64Array(STATE * nodes.length);
+    this.incoming = null;
+    this.upload();
+    return this;
+  }
+
+  upload() {
+    pack(this.nodes, this.nDim, this.state);
+    this.addon.load(this.handle, this.state, this.nDim);
+    this.sent.clear(); // fresh node buffers: every force's arrays go up again
+    this.reload = false;
+    this.current = this.submitted;
+  }
+
+  // The next submission uploads every node first (the CPU moved them).
+  markStale() {
+    this.reload = true;
+  }
+
+  setSeed(random) {
+    this.addon.setSeed(this.handle, Math.floor(random() * 4294967296) >>> 0);
+  }
+
+  // The pass list, in application order; false when a force has no GPU description.
+  setForces(forces) {
+    if (!forces.every(describable)) return false;
+    this.forces = forces;
+    this.addon.setForces(this.handle, this.describe(forces));
+    return true;
+  }
+
+  idOf(force) {
+    let id = this.ids.get(force);
+    if (!id) this.ids.set(force, id = this.nextId++);
     return id;
   }
 
-  // --- JS -> GPU ---------------------------------------------------------------------
-
-  function uploadAll() {
-    var pos = new Float64Array(3 * n), vel = new Float64Array(3 * n),
-        fix = new Float64Array(3 * n), mask = new Uint8Array(n), i, j, node, m;
-    for (i = 0; i < n; ++i) {
-      node = nodes[i], j = 3 * i;
-      shadow[j] = pos[j] = num(node.x);
-      shadow[j + 1] = pos[j + 1] = nDim > 1 ? num(node.y) : 0;
-      shadow[j + 2] = pos[j + 2] = nDim > 2 ? num(node.z) : 0;
-      vel[j] = num(node.vx);
-      if (nDim > 1) vel[j + 1] = num(node.vy);
-      if (nDim > 2) vel[j + 2] = num(node.vz);
-      m = 0;
-      if (node.fx != null) m |= 1, pinnedAt[j] = fix[j] = +node.fx;
-      if (nDim > 1 && node.fy != null) m |= 2, pinnedAt[j + 1] = fix[j + 1] = +node.fy;
-      if (nDim > 2 && node.fz != null) m |= 4, pinnedAt[j + 2] = fix[j + 2] = +node.fz;
-      pinned[i] = mask[i] = m;
-    }
-    addon.setNodes(handle, n, nDim, pos, vel, fix, mask);
-    known = new Map(); // fresh node buffers: every force's arrays go up again
-    ++epoch;
-    stale = false;
-    landed = enqueued;
-  }
-
-  function uploadPositions(a, b) {
-    var out = new Float64Array(3 * (b - a)), i, j, k, node;
-    for (i = a; i < b; ++i) {
-      node = nodes[i], j = 3 * i, k = 3 * (i - a);
-      shadow[j] = out[k] = num(node.x);
-      shadow[j + 1] = out[k + 1] = nDim > 1 ? num(node.y) : 0;
-      shadow[j + 2] = out[k + 2] = nDim > 2 ? num(node.z) : 0;
-    }
-    addon.updatePositions(handle, a, out);
-  }
-
-  function uploadPinned(a, b) {
-    var fix = new Float64Array(3 * (b - a)), mask = new Uint8Array(b - a), i, j, k, node, m;
-    for (i = a; i < b; ++i) {
-      node = nodes[i], j = 3 * i, k = 3 * (i - a), m = 0;
-      if (node.fx != null) m |= 1, pinnedAt[j] = fix[k] = +node.fx;
-      if (nDim > 1 && node.fy != null) m |= 2, pinnedAt[j + 1] = fix[k + 1] = +node.fy;
-      if (nDim > 2 && node.fz != null) m |= 4, pinnedAt[j + 2] = fix[k + 2] = +node.fz;
-      pinned[i] = mask[i - a] = m;
-    }
-    addon.updatePins(handle, a, fix, mask);
-  }
-
-  function syncFromJS() {
-    var posRun = -1, pinRun = -1, i, j, node, x, y, z, m, posDirty, pinDirty;
-    for (i = 0; i <= n; ++i) {
-      posDirty = pinDirty = false;
-      if (i < n) {
-        node = nodes[i], j = 3 * i;
-        x = num(node.x), y = nDim > 1 ? num(node.y) : 0, z = nDim > 2 ? num(node.z) : 0;
-        posDirty = x !== shadow[j] || y !== shadow[j + 1] || z !== shadow[j + 2];
-        m = (node.fx != null ? 1 : 0) | (nDim > 1 && node.fy != null ? 2 : 0) | (nDim > 2 && node.fz != null ? 4 : 0);
-        pinDirty = m !== pinned[i]
-            || (m & 1) !== 0 && +node.fx !== pinnedAt[j]
-            || (m & 2) !== 0 && +node.fy !== pinnedAt[j + 1]
-            || (m & 4) !== 0 && +node.fz !== pinnedAt[j + 2];
-      }
-      if (posDirty) { if (posRun < 0) posRun = i; }
-      else if (posRun >= 0) uploadPositions(posRun, i), posRun = -1;
-      if (pinDirty) { if (pinRun < 0) pinRun = i; }
-      else if (pinRun >= 0) uploadPinned(pinRun, i), pinRun = -1;
-    }
-  }
-
-  // Descriptors in the backend's form; arrays only for a force whose version is new.
-  function describe(forces) {
-    return forces.map(function(force) {
-      var d = force.gpu(), id = idOf(force), fresh = known.get(id) !== d.version, out = {
-        type: TYPES[d.type], id: id, version: d.version || 0, iterations: d.iterations == null ? 1 : d.iterations,
-        x: num(d.x), y: num(d.y), z: num(d.z), strength: d.strength == null ? 1 : +d.strength,
+  describe(forces) {
+    const n = this.nodes.length;
+    return forces.map((force) => {
+      const d = force.gpu(), id = this.idOf(force), version = d.version || 0;
+      const out = {
+        type: TYPES[d.type], id: id, version: version, iterations: d.iterations == null ? 1 : d.iterations,
+        x: finite(d.x), y: finite(d.y), z: finite(d.z), strength: d.strength == null ? 1 : +d.strength,
         theta2: d.theta2 == null ? 0.81 : +d.theta2, distanceMin2: d.distanceMin2 == null ? 1 : +d.distanceMin2,
         distanceMax2: d.distanceMax2 == null ? Infinity : +d.distanceMax2, axis: d.axis | 0
       };
-      known.set(id, d.version);
-      if (!fresh) return out;
-      if (d.type === "manyBody") out.strengths = float64(d.strengths, n);
-      else if (d.type === "collide") out.radii = float64(d.radii, n);
-      else if (d.type === "radial") out.radii = float64(d.radiuses, n), out.strengths = float64(d.strengths, n);
-      else if (d.type === "position") out.targets = float64(d.targets, n), out.strengths = float64(d.strengths, n);
-      else if (d.type === "link") {
-        var m = d.links.length, links = new Uint32Array(2 * m);
-        for (var k = 0; k < m; ++k) links[2 * k] = d.links[k].source.index, links[2 * k + 1] = d.links[k].target.index;
-        out.links = links;
-        out.strengths = float64(d.strengths, m);
-        out.distances = float64(d.distances, m);
-        out.bias = float64(d.bias, m);
+      if (this.sent.get(id) === version) return out; // the device has this version's arrays
+      this.sent.set(id, version);
+      switch (d.type) {
+        case "manyBody": out.strengths = column(d.strengths, n); break;
+        case "collide": out.radii = column(d.radii, n); break;
+        case "radial": out.radii = column(d.radiuses, n), out.strengths = column(d.strengths, n); break;
+        case "position": out.targets = column(d.targets, n), out.strengths = column(d.strengths, n); break;
+        case "link": {
+          const m = d.links.length, ends = new Uint32Array(2 * m);
+          d.links.forEach((link, k) => { ends[2 * k] = link.source.index; ends[2 * k + 1] = link.target.index; });
+          out.links = ends;
+          out.strengths = column(d.strengths, m);
+          out.distances = column(d.distances, m);
+          out.bias = column(d.bias, m);
+          break;
+        }
       }
       return out;
     });
   }
 
-  // --- GPU -> JS ---------------------------------------------------------------------
-
-  function land(pos, vel) {
-    var i, j, node, x, y, z;
-    for (i = 0; i < n; ++i) {
-      node = nodes[i], j = 3 * i;
-      x = num(node.x), y = nDim > 1 ? num(node.y) : 0, z = nDim > 2 ? num(node.z) : 0;
-      if (x === shadow[j] && y === shadow[j + 1] && z === shadow[j + 2]) {
-        node.x = shadow[j] = pos[j];
-        if (nDim > 1) node.y = shadow[j + 1] = pos[j + 1];
-        if (nDim > 2) node.z = shadow[j + 2] = pos[j + 2];
-      }
-      node.vx = vel[j];
-      if (nDim > 1) node.vy = vel[j + 1];
-      if (nDim > 2) node.vz = vel[j + 2];
+  // Queue alphas.length steps; decay multiplies the velocities (1 - velocityDecay).
+  enqueue(alphas, decay) {
+    if (!this.alive || !this.nodes.length) return;
+    if (this.reload) {
+      this.upload();
+    } else {
+      pack(this.nodes, this.nDim, this.state);
+      this.addon.push(this.handle, this.state);
     }
+    if (this.forces) this.addon.setForces(this.handle, this.describe(this.forces));
+    this.addon.enqueue(this.handle, Float64Array.from(alphas), decay);
+    this.submitted += alphas.length;
   }
 
-  function readback() {
-    if (inflight) return inflight;
-    if (!alive || !n || landed === enqueued) return Promise.resolve();
-    var target = enqueued, era = epoch;
-    if (!staging || staging.n !== n) staging = {n: n, pos: new Float64Array(3 * n), vel: new Float64Array(3 * n)};
-    var s = staging;
-    return inflight = addon.readbackAsync(handle, s.pos, s.vel).then(function(ok) {
-      inflight = null;
-      if (ok && era === epoch && alive) land(s.pos, s.vel), landed = target;
-    }, function(error) {
-      inflight = null;
-      if (era === epoch && alive) throw error;
+  busy() {
+    return this.alive && this.addon.busy(this.handle);
+  }
+
+  // One readback: resolves once it has landed in the nodes (or was dropped as stale).
+  readback() {
+    if (this.inflight) return this.inflight;
+    if (!this.alive || !this.nodes.length || this.current === this.submitted) return Promise.resolve();
+    const n = this.nodes.length, target = this.submitted;
+    if (!this.incoming || this.incoming.pos.length !== 3 * n) {
+      this.incoming = {pos: new Float64Array(3 * n), vel: new Float64Array(3 * n)};
+    }
+    const buf = this.incoming, nodes = this.nodes;
+    this.inflight = this.addon.readbackAsync(this.handle, buf.pos, buf.vel).then((fresh) => {
+      this.inflight = null;
+      if (!fresh || !this.alive || nodes !== this.nodes) return;
+      pack(nodes, this.nDim, this.state);
+      this.addon.land(this.handle, this.state, buf.pos, buf.vel);
+      unpack(nodes, this.nDim, this.state);
+      this.current = target;
+    }, (error) => {
+      this.inflight = null;
+      if (this.alive && nodes === this.nodes) throw error;
     });
+    return this.inflight;
   }
 
-  var engine = {
-    get n() { return n; },
-    get nDim() { return nDim; },
+  // Resolves once the nodes reflect every queued step.
+  sync() {
+    const more = () => this.readback().then(() => {
+      if (this.alive && this.nodes.length && this.current !== this.submitted) return more();
+    });
+    return more();
+  }
 
-    setNodes: function(_nodes, _nDim) {
-      nodes = _nodes, n = nodes.length, nDim = _nDim;
-      enqueued = landed = 0;
-      shadow = new Float64Array(3 * n);
-      pinned = new Uint8Array(n);
-      pinnedAt = new Float64Array(3 * n);
-      uploadAll();
-      return engine;
-    },
+  // Resolves once the device has run every queued step (the nodes are left alone).
+  idle() {
+    return this.alive ? this.addon.idleAsync(this.handle).then(() => undefined) : Promise.resolve();
+  }
 
-    // Build the pass list from the forces (in application order). False when a force has
-    // no GPU description (the caller then reports it).
-    setForces: function(forces) {
-      var ok = forces.every(function(f) { return typeof f.gpu === "function" && supported(f.gpu()); });
-      if (!ok) return false;
-      this.forces = forces;
-      addon.setForces(handle, describe(forces));
-      return true;
-    },
+  buffers() {
+    return this.alive && this.nodes.length ? this.addon.buffers(this.handle) : null;
+  }
 
-    fits: function(count) {
-      return addon.fits(handle, count, 3);
-    },
+  stats() {
+    return this.addon.stats(this.handle);
+  }
 
-    setSeed: function(random) {
-      seed = Math.floor(random() * 4294967296) >>> 0;
-      addon.setSeed(handle, seed);
-    },
+  // A copy of `floats` floats of device memory at `pointer` (a gpuBuffers() pointer).
+  readDevice(pointer, floats) {
+    return this.addon.readDevice(this.handle, pointer, floats);
+  }
 
-    markStale: function() {
-      stale = true;
-    },
-
-    // Submit alphas.length steps; decay is the velocity multiplier (1 - velocityDecay).
-    enqueue: function(alphas, decay) {
-      if (!alive || !n) return;
-      if (stale) uploadAll();
-      else syncFromJS();
-      if (this.forces) addon.setForces(handle, describe(this.forces));
-      addon.enqueue(handle, Float64Array.from(alphas), decay);
-      enqueued += alphas.length;
-    },
-
-    busy: function() {
-      return alive && addon.busy(handle);
-    },
-
-    readback: readback,
-
-    idle: function() {
-      return alive ? addon.idleAsync(handle).then(function() {}) : Promise.resolve();
-    },
-
-    sync: function() {
-      function next() {
-        return readback().then(function() {
-          if (alive && n && landed !== enqueued) return next();
-        });
-      }
-      return next();
-    },
-
-    buffers: function() {
-      return alive && n ? addon.buffers(handle) : null;
-    },
-
-    stats: function() {
-      return addon.stats(handle);
-    },
-
-    destroy: function() {
-      if (!alive) return;
-      alive = false;
-      addon.destroy(handle);
-      nodes = [], shadow = pinnedAt = new Float64Array(0), pinned = new Uint8Array(0), staging = null;
-    }
-  };
-  return engine;
+  destroy() {
+    if (!this.alive) return;
+    this.alive = false;
+    this.addon.destroy(this.handle);
+    this.nodes = [];
+    this.state = new Float64Array(0);
+    this.incoming = null;
+  }
 }

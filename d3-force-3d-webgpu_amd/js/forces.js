@@ -1,17 +1,38 @@
-// The d3-force-3d forces, with the same constructors and accessors, for the GPU
-// simulation.  Each one computes its per-node (or per-link) values on initialize(), like
-// the originals, and describes itself with gpu() -> {type, version, ...} exactly as the
-// reference descriptors do (src/manyBody.js:142-144, src/link.js:127-129,
-// src/collide.js:134-136, src/center.js:49-51, src/radial.js:69-71, src/x.js:43-45).
-// `version` changes whenever those values are recomputed, which tells the backend to
-// upload them again.  The physics itself runs in the HIP backend: calling a force
-// directly on the CPU is not supported.
+// The d3-force-3d forces, with the same constructors and accessors.  Each one computes its
+// per-node (or per-link) values on initialize(), like the originals, and describes itself
+// with gpu() -> {type, version, ...} exactly as the reference descriptors do
+// (src/manyBody.js:142-144, src/link.js:127-129, src/collide.js:134-136,
+// src/center.js:49-51, src/radial.js:69-71, src/x.js:43-45); `version` changes whenever
+// those values are recomputed, which tells the GPU engine to upload them again.  Called as
+// d3 calls a force, force(alpha), each runs d3's CPU physics from that same description
+// (cpu.js), so they work in any d3 force simulation as well.
 import constant from "./constant.js";
+import {applyForce} from "./cpu.js";
 
-function gpuOnly(name) {
-  return function() {
-    throw new Error(name + " runs in forceSimulationGPU's HIP backend; it has no CPU implementation here");
+// d3-force-3d's initialize(nodes, ...args): the random source is the function among the
+// arguments and the dimensionality the 1, 2 or 3 (src/manyBody.js:117-121).
+function contextOf(args) {
+  return {
+    random: args.find(function(arg) { return typeof arg === "function"; }) || Math.random,
+    nDim: args.find(function(arg) { return [1, 2, 3].includes(arg); }) || 2
   };
+}
+
+// A force function that applies its own description (force.gpu()) to the nodes, random
+// source and dimensionality its initialize() received (kept aside, not on the function).
+const contexts = new WeakMap();
+
+function cpuForce() {
+  function force(alpha) {
+    var ctx = contexts.get(force);
+    if (ctx) applyForce(force.gpu(), ctx.nodes, ctx.nDim, ctx.random, alpha);
+  }
+  return force;
+}
+
+function attach(force, nodes, args) {
+  var c = contextOf(args);
+  contexts.set(force, {nodes: nodes || [], random: c.random, nDim: c.nDim});
 }
 
 function findNode(nodeById, nodeId) {
@@ -20,13 +41,9 @@ function findNode(nodeById, nodeId) {
   return node;
 }
 
-function dimsOf(args) {
-  return args.find(function(arg) { return [1, 2, 3].includes(arg); }) || 2;
-}
-
 export function forceManyBody() {
   var nodes, strength = constant(-30), strengths, distanceMin2 = 1, distanceMax2 = Infinity,
-      theta2 = 0.81, version = 0, force = gpuOnly("forceManyBody");
+      theta2 = 0.81, version = 0, force = cpuForce();
 
   function initialize() {
     if (!nodes) return;
@@ -36,7 +53,7 @@ export function forceManyBody() {
     ++version;
   }
 
-  force.initialize = function(_nodes) { nodes = _nodes; initialize(); };
+  force.initialize = function(_nodes, ...args) { nodes = _nodes; attach(force, _nodes, args); initialize(); };
   force.strength = function(_) {
     return arguments.length ? (strength = typeof _ === "function" ? _ : constant(+_), initialize(), force) : strength;
   };
@@ -59,7 +76,7 @@ export function forceManyBody() {
 export function forceLink(links) {
   var id = function(d) { return d.index; }, strength = defaultStrength, strengths,
       distance = constant(30), distances, nodes, count, bias, iterations = 1, version = 0,
-      force = gpuOnly("forceLink");
+      force = cpuForce();
 
   if (links == null) links = [];
 
@@ -97,7 +114,7 @@ export function forceLink(links) {
     ++version;
   }
 
-  force.initialize = function(_nodes) { nodes = _nodes; initialize(); };
+  force.initialize = function(_nodes, ...args) { nodes = _nodes; attach(force, _nodes, args); initialize(); };
   force.links = function(_) { return arguments.length ? (links = _, initialize(), force) : links; };
   force.id = function(_) { return arguments.length ? (id = _, force) : id; };
   force.iterations = function(_) { return arguments.length ? (iterations = +_, force) : iterations; };
@@ -115,7 +132,7 @@ export function forceLink(links) {
 }
 
 export function forceCollide(radius) {
-  var nodes, radii, strength = 1, iterations = 1, version = 0, force = gpuOnly("forceCollide");
+  var nodes, radii, strength = 1, iterations = 1, version = 0, force = cpuForce();
   if (typeof radius !== "function") radius = constant(radius == null ? 1 : +radius);
 
   function initialize() {
@@ -126,7 +143,7 @@ export function forceCollide(radius) {
     ++version;
   }
 
-  force.initialize = function(_nodes) { nodes = _nodes; initialize(); };
+  force.initialize = function(_nodes, ...args) { nodes = _nodes; attach(force, _nodes, args); initialize(); };
   force.iterations = function(_) { return arguments.length ? (iterations = +_, force) : iterations; };
   force.strength = function(_) { return arguments.length ? (strength = +_, force) : strength; };
   force.radius = function(_) {
@@ -139,11 +156,11 @@ export function forceCollide(radius) {
 }
 
 export function forceCenter(x, y, z) {
-  var strength = 1, force = gpuOnly("forceCenter");
+  var strength = 1, force = cpuForce();
   if (x == null) x = 0;
   if (y == null) y = 0;
   if (z == null) z = 0;
-  force.initialize = function() {};
+  force.initialize = function(_nodes, ...args) { attach(force, _nodes, args); };
   force.x = function(_) { return arguments.length ? (x = +_, force) : x; };
   force.y = function(_) { return arguments.length ? (y = +_, force) : y; };
   force.z = function(_) { return arguments.length ? (z = +_, force) : z; };
@@ -155,7 +172,7 @@ export function forceCenter(x, y, z) {
 }
 
 export function forceRadial(radius, x, y, z) {
-  var nodes, strength = constant(0.1), strengths, radiuses, version = 0, force = gpuOnly("forceRadial");
+  var nodes, strength = constant(0.1), strengths, radiuses, version = 0, force = cpuForce();
   if (typeof radius !== "function") radius = constant(+radius);
   if (x == null) x = 0;
   if (y == null) y = 0;
@@ -173,7 +190,7 @@ export function forceRadial(radius, x, y, z) {
     ++version;
   }
 
-  force.initialize = function(_nodes) { nodes = _nodes; initialize(); };
+  force.initialize = function(_nodes, ...args) { nodes = _nodes; attach(force, _nodes, args); initialize(); };
   force.strength = function(_) {
     return arguments.length ? (strength = typeof _ === "function" ? _ : constant(+_), initialize(), force) : strength;
   };
@@ -191,7 +208,7 @@ export function forceRadial(radius, x, y, z) {
 
 function positionForce(axis, name) {
   return function(target) {
-    var strength = constant(0.1), nodes, strengths, targets, version = 0, force = gpuOnly(name);
+    var strength = constant(0.1), nodes, strengths, targets, version = 0, force = cpuForce();
     if (typeof target !== "function") target = constant(target == null ? 0 : +target);
 
     function initialize() {
@@ -205,7 +222,7 @@ function positionForce(axis, name) {
       ++version;
     }
 
-    force.initialize = function(_nodes) { nodes = _nodes; initialize(); };
+    force.initialize = function(_nodes, ...args) { nodes = _nodes; attach(force, _nodes, args); initialize(); };
     force.strength = function(_) {
       return arguments.length ? (strength = typeof _ === "function" ? _ : constant(+_), initialize(), force) : strength;
     };

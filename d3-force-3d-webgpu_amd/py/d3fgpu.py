@@ -127,6 +127,7 @@ SIGNATURES = {
     "fg_readback_wait": (C.c_int, [C.c_void_p, C.c_uint64]),
     "fg_readback_fetch": (C.c_int, [C.c_void_p, C.c_uint64, _d, _d]),
     "fg_buffers": (C.c_int, [C.c_void_p, C.POINTER(BufferInfo)]),
+    "fg_debug_copy_from_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "fg_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "fg_timing_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "fg_get_timing": (C.c_int, [C.c_void_p, C.POINTER(Timing)]),
@@ -376,6 +377,13 @@ class Engine:
         b = BufferInfo()
         self._check(lib().fg_buffers(self.h, C.byref(b)))
         return b
+
+    def copy_from_device(self, pointer, count, dtype=np.float32):
+        """`count` elements of device memory at `pointer` (e.g. a buffers() pointer)."""
+        out = np.zeros(count, dtype)
+        self._check(lib().fg_debug_copy_from_device(self.h, C.c_void_p(pointer), out.nbytes,
+                                                    out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def stats(self):
         s = Stats()

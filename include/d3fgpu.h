@@ -112,7 +112,12 @@ int fg_create(fg_ctx** out, int device);
 int fg_destroy(fg_ctx* ctx);
 const char* fg_last_error(const fg_ctx* ctx);
 
-/* Whether `n` nodes in `ndim` dimensions fit this device (engine.fits, engine.js:334-338). */
+/* Whether `n` nodes in `ndim` dimensions fit this device (engine.fits, engine.js:334-338).
+ * Test hooks, read by fg_create from the environment: D3FGPU_CAPACITY=N makes fg_fits refuse
+ * more than N nodes (the reference fakes device limits for its oversize test,
+ * test/gpu-simulation-test.js:384-406); D3FGPU_FAIL=enqueue:K or wait:K makes the K-th
+ * fg_enqueue, or the K-th fg_sync / fg_readback_wait, return FG_ERR_HIP without touching
+ * the device, so that a host's failure handling can be exercised. */
 int fg_fits(fg_ctx* ctx, uint32_t n, int32_t ndim, int32_t* fits);
 
 /* ---- nodes (setNodes / uploadAll / uploadPositions / uploadPinned, engine.js:115-169, 267-310) */
@@ -176,6 +181,11 @@ int fg_readback_fetch(fg_ctx* ctx, uint64_t ticket, double* pos, double* vel);
 /* Device pointers of the state (gpuBuffers, engine.js:418-420).  Keeps an AoS float4
  * mirror up to date after every queued step from the first call on. */
 int fg_buffers(fg_ctx* ctx, fg_buffer_info* out);
+
+/* Test and debugging aid, with no counterpart in the reference: copies `bytes` from device
+ * memory of this context's GPU (for instance what fg_buffers points to) into host memory,
+ * after the queued work.  A renderer would read those buffers with its own HIP calls. */
+int fg_debug_copy_from_device(fg_ctx* ctx, const void* device_src, uint64_t bytes, void* host_dst);
 
 int fg_get_stats(fg_ctx* ctx, fg_stats* out);
 

@@ -2,6 +2,7 @@
 // and determinism (test/gpu-simulation-test.js is the model).
 import assert from "assert";
 import * as d3 from "../../d3-force-3d-webgpu_amd/js/index.js";
+import {native} from "../../d3-force-3d-webgpu_amd/js/native.js";
 import {it, run} from "./harness.mjs";
 
 function makeNodes(n, dims, size, seed) {
@@ -99,15 +100,30 @@ it("readback: false leaves the nodes alone until sync()", async function() {
   sim.destroy();
 });
 
-it("gpuBuffers() describes the device-resident state", async function() {
-  const sim = d3.forceSimulationGPU(makeNodes(33, 3), 3).stop().force("charge", d3.forceManyBody());
+it("gpuBuffers() points at the device-resident state, bit-equal to the nodes", async function() {
+  const nodes = makeNodes(33, 3);
+  const sim = d3.forceSimulationGPU(nodes, 3).stop().force("charge", d3.forceManyBody());
   assert.strictEqual(sim.gpuBuffers(), null, "null until the GPU is ready");
   await sim.gpuReady();
+  await sim.tickAsync(2);
   const b = sim.gpuBuffers();
   assert.strictEqual(b.count, 33);
   assert.strictEqual(b.stride, 16);
   assert.strictEqual(typeof b.positions, "bigint");
-  assert(b.positions !== 0n && b.velocities !== 0n);
+  // read the float4 per node through the pointer (test/gpu-simulation-test.js:421-443)
+  const addon = native(), h = addon.create(0);
+  try {
+    const p = addon.readDevice(h, b.positions, 4 * 33);
+    nodes.forEach(function(n, i) {
+      assert.strictEqual(p[4 * i], n.x);
+      assert.strictEqual(p[4 * i + 1], n.y);
+      assert.strictEqual(p[4 * i + 2], n.z);
+    });
+    const v = addon.readDevice(h, b.velocities, 4 * 33);
+    nodes.forEach(function(n, i) { assert.strictEqual(v[4 * i], n.vx); });
+  } finally {
+    addon.destroy(h);
+  }
   sim.destroy();
 });
 
@@ -161,10 +177,110 @@ it("takes forces added, changed and removed while it runs", async function() {
   sim.destroy();
 });
 
-it("rejects a force it cannot run", async function() {
-  const sim = d3.forceSimulationGPU(makeNodes(5, 2), 2).stop();
-  await sim.gpuReady();
-  assert.throws(() => sim.force("custom", function() {}), /cannot run on the GPU/);
+async function withWarnings(fn) {
+  const warn = console.warn, warnings = [];
+  console.warn = function() { warnings.push(Array.prototype.join.call(arguments, " ")); };
+  try { await fn(); } finally { console.warn = warn; }
+  return warnings;
+}
+
+function gravity() {
+  let nodes;
+  function force(alpha) { for (const n of nodes) n.vy -= 0.5 * alpha; }
+  force.initialize = function(_) { nodes = _; };
+  return force;
+}
+
+it("a force without gpu() moves the physics to the CPU (with a warning) and back", async function() {
+  const nodes = makeNodes(40, 3);
+  let sim;
+  const warnings = await withWarnings(async function() {
+    sim = d3.forceSimulationGPU(nodes, 3).stop().force("charge", d3.forceManyBody());
+    await sim.tickAsync(3);
+    assert.strictEqual(sim.isGPUEnabled(), true);
+    sim.force("gravity", gravity());
+    assert.strictEqual(sim.isGPUEnabled(), false);
+    const vy = nodes.map((n) => n.vy);
+    await sim.tickAsync(3);
+    assert(nodes.some((n, i) => n.vy !== vy[i]), "the CPU ran the forces, the custom one too");
+    sim.force("gravity", null);
+    assert.strictEqual(sim.isGPUEnabled(), true);
+    await sim.tickAsync(3);
+  });
+  assert.strictEqual(warnings.length, 1);
+  assert(/gravity/.test(warnings[0]), warnings[0]);
+  assert(nodes.every((n) => isFinite(n.x + n.y + n.z)));
+  assert(isF32(state(nodes)), "the last ticks ran on the GPU");
+  sim.destroy();
+});
+
+const state = (nodes) => nodes.map((n) => [n.x, n.y, n.z, n.vx, n.vy, n.vz]);
+const isF32 = (s) => s.every((d) => d.every((v) => v === undefined || Math.fround(v) === v));
+
+function record(sim, nodes, count) {
+  const states = [];
+  return new Promise(function(resolve) {
+    sim.on("tick", function() {
+      states.push(state(nodes));
+      if (states.length === count) sim.on("tick", null).stop(), resolve(states);
+    });
+  });
+}
+
+it("the timer's first cpuTicks ticks are the CPU simulation's, then the GPU's", async function() {
+  const links = () => Array.from({length: 59}, (_, i) => ({source: i + 1, target: i >> 1}));
+  const nodes = makeNodes(60, 3), cpuNodes = makeNodes(60, 3);
+  const sim = d3.forceSimulationGPU(nodes, 3, {cpuTicks: 10}).force("link", d3.forceLink(links())).force("charge", d3.forceManyBody());
+  const states = await record(sim, nodes, 11);
+  const cpu = d3.forceSimulationGPU(cpuNodes, 3, {cpuTicks: 0}).stop().force("link", d3.forceLink(links())).force("charge", d3.forceManyBody());
+  // the reference run on the CPU: the same forces called as d3 calls them
+  let alpha = 1;
+  const decay = 1 - Math.pow(0.001, 1 / 300), fs = [cpu.force("link"), cpu.force("charge")];
+  for (let k = 0; k < 10; ++k) {
+    alpha += (0 - alpha) * decay;
+    fs.forEach((f) => f(alpha));
+    for (const n of cpuNodes) n.x += n.vx *= 0.6, n.y += n.vy *= 0.6, n.z += n.vz *= 0.6;
+  }
+  assert.deepStrictEqual(states[9], state(cpuNodes), "ticks 1 to 10 ran on the CPU, to the last bit");
+  assert(!isF32(states[9]));
+  assert(isF32(states[10]), "tick 11 ran on the GPU");
+  assert.strictEqual(sim.isGPUEnabled(), true);
+  sim.destroy(), cpu.destroy();
+});
+
+it("a GPU failure while stepping continues on the CPU, with a warning", async function() {
+  const nodes = makeNodes(50, 2);
+  let sim;
+  process.env.D3FGPU_FAIL = "enqueue:3";  // the engine's third enqueue reports a failure
+  const warnings = await withWarnings(async function() {
+    sim = d3.forceSimulationGPU(nodes, 2).stop().force("charge", d3.forceManyBody());
+    assert.strictEqual(await sim.gpuReady(), true);
+    delete process.env.D3FGPU_FAIL;
+    await sim.tickAsync(1);
+    await sim.tickAsync(1);
+    await sim.tickAsync(2);  // refused: runs on the CPU
+    assert.strictEqual(sim.isGPUEnabled(), false);
+    await sim.tickAsync(2);
+  });
+  assert(warnings.some((w) => /failed/.test(w) && /injected/.test(w)), warnings.join("; "));
+  assert(nodes.every((n) => isFinite(n.x + n.y) && !isF32([[n.x]])), "the last ticks ran on the CPU");
+  sim.destroy();
+});
+
+it("nodes beyond the device's capacity run on the CPU, with a warning", async function() {
+  const nodes = makeNodes(30, 2);
+  let sim;
+  process.env.D3FGPU_CAPACITY = "10";
+  const warnings = await withWarnings(async function() {
+    sim = d3.forceSimulationGPU(nodes, 2).stop().force("charge", d3.forceManyBody());
+    assert.strictEqual(await sim.gpuReady(), false);
+    delete process.env.D3FGPU_CAPACITY;
+    sim.force("center", d3.forceCenter());  // must not bring the GPU back
+    await sim.tickAsync(2);
+  });
+  assert.strictEqual(sim.isGPUEnabled(), false);
+  assert(/exceed/.test(warnings.join()), warnings.join());
+  assert(nodes.every((n) => isFinite(n.x) && n.vx !== 0));
   sim.destroy();
 });
 
@@ -197,12 +313,13 @@ it("lays out graphs through a server and a batch, the same either way", async fu
   }
 });
 
-it("destroy() stops it for good", async function() {
+it("destroy() releases the GPU for good", async function() {
   const sim = d3.forceSimulationGPU(makeNodes(5, 2), 2).stop().force("charge", d3.forceManyBody());
   await sim.tickAsync(1);
+  assert.strictEqual(sim.isGPUEnabled(), true);
   sim.destroy();
   assert.strictEqual(sim.isGPUEnabled(), false);
-  assert.throws(() => sim.tick());
+  assert.strictEqual(sim.gpuBuffers(), null);
 });
 
 run();

@@ -664,3 +664,23 @@ def test_center_in_fresh_engines_over_reused_memory():
         mean = p32.astype(np.float64).mean(0)
         want = (p32 - mean.astype(np.float32)).astype(np.float64)
         assert np.array_equal(p, want), f"engine {k}: center did not move the nodes by the mean"
+
+
+def test_gpu_buffers_hold_the_state(engine_factory):
+    """fg_buffers: a float4 per node of the device state, kept current after every step
+    (engine.js:418-420, test/gpu-simulation-test.js:421-443), read through the pointers
+    bit-equal to the readback."""
+    n = 777
+    e = engine_factory()
+    pos = make_nodes_fast(n, 3, 300)
+    e.set_nodes(pos, ndim=3)
+    e.set_forces([d3fgpu.many_body(np.full(n, -30.0)), d3fgpu.center()])
+    b = e.buffers()
+    assert b.count == n and b.stride == 16
+    for ticks in (1, 4):
+        e.enqueue(np.full(ticks, 0.5), 0.6)
+        p, v = e.readback()
+        dp = e.copy_from_device(b.positions, 4 * n).reshape(n, 4)
+        dv = e.copy_from_device(b.velocities, 4 * n).reshape(n, 4)
+        np.testing.assert_array_equal(dp[:, :3].astype(np.float64), p)
+        np.testing.assert_array_equal(dv[:, :3].astype(np.float64), v)
