@@ -11,9 +11,11 @@ Workload (the 1M-node configuration of the benchmark set, synthetic):
   * default alpha schedule (alpha 1, alphaDecay 1 - 0.001^(1/300)), velocityDecay 0.4
 
 `value` is wall-clock ms per tick over `--steps` ticks (inputs resident in HBM, no
-readback inside the timed region).  A second, event-timed run gives the per-phase split
-and the walk kernel's time for the roofline.  The CPU baseline is the FP64 restatement of
-d3-force-3d's tick (the oracle, one thread) on a bounded sample of the same workload.
+readback inside the timed region).  A second, event-timed run gives the per-phase split,
+the walk kernel's time for its VALU roofline and each HBM-bound phase's GB/s.  The CPU
+baseline is forceSimulationGPU's CPU path -- d3-force-3d's physics in FP64, JavaScript on one
+thread -- timed in Node on this host (a bounded sample of the same workload, and the
+1,000-node benchmark graph in full); the C FP64 restatement of the same tick is timed beside it.
 """
 import argparse
 import json
@@ -29,10 +31,22 @@ sys.path.insert(0, os.path.join(ROOT, "d3-force-3d-webgpu_amd", "py"))
 METRIC = "ms/tick @1M nodes 3D; batched layouts/s + p50/p99 latency vs plain, 1 GPU"
 FLOP_PER_INTERACTION = 20          # n-body convention for one body-source evaluation
 FP32_PEAK_TFLOPS = 157.3           # MI355X FP32 (vector and dense matrix)
+HBM_PEAK_GBS = 8000.0              # MI355X HBM3E
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def collide_radii(n):
+    """The workload's collide radii: exp(ln 2 + 0.4 N(0, 1)), numpy PCG64 seed 7."""
+    return np.exp(np.log(2.0) + 0.4 * np.random.default_rng(7).standard_normal(n))
+
+
+def pinned(n):
+    """The workload's pins: every 100th node at a uniform position in [-500, 500]^3 (seed 11)."""
+    idx = np.arange(0, n, 100)
+    return idx, np.random.default_rng(11).uniform(-500, 500, (len(idx), 3))
 
 
 def make_workload(n, with_collide=True, with_radial=True, with_pins=True):
@@ -45,19 +59,17 @@ def make_workload(n, with_collide=True, with_radial=True, with_pins=True):
               d3fgpu.many_body(np.full(n, -30.0), theta=0.9),
               d3fgpu.center()]
     if with_collide:
-        rng = np.random.default_rng(7)
-        radii = np.exp(np.log(2.0) + 0.4 * rng.standard_normal(n))
-        forces.append(d3fgpu.collide(radii, strength=1.0, iterations=2))
+        forces.append(d3fgpu.collide(collide_radii(n), strength=1.0, iterations=2))
     if with_radial:
         forces.append(d3fgpu.radial(np.full(n, 200.0), np.full(n, 0.05)))
     fix = mask = None
     if with_pins:
-        rng = np.random.default_rng(11)
+        idx, at = pinned(n)
         mask = np.zeros(n, np.uint8)
-        mask[::100] = 7
+        mask[idx] = 7
         fix = np.zeros((n, 3))
-        fix[::100] = rng.uniform(-500, 500, (len(mask[::100]), 3))
-        pos[::100] = fix[::100]
+        fix[idx] = at
+        pos[idx] = at
     return pos, forces, fix, mask, len(src)
 
 
@@ -506,6 +518,76 @@ def cpu_baseline(n, ticks=1):
     return 1e3 * dt / ticks
 
 
+def node_cpu_baseline(n, light=False, ticks=None, min_ticks=3, budget=1.5):
+    """forceSimulationGPU's CPU path -- d3-force-3d's physics in FP64 on one thread, the
+    JavaScript the simulation runs without a GPU -- timed in Node on this host with the
+    reference benchmark page's method (examples/benchmark.html:105-117: one untimed tick,
+    then tick(1) calls, at least `min_ticks`, for `budget` seconds; or exactly `ticks`).
+    light: link + manyBody + center only; otherwise the whole 1M-node mix with its pins."""
+    import shutil
+    import subprocess
+    import tempfile
+    node = shutil.which("node")
+    if node is None:
+        return None
+    d = tempfile.mkdtemp(prefix="d3f-cpu-")
+    try:
+        src, dst = make_graph(n, 42)
+        np.ascontiguousarray(np.stack([src, dst], 1), np.uint32).tofile(os.path.join(d, "links.u32"))
+        case = {"n": n, "dims": 3, "forces": ["link", "manyBody", "center"], "min": min_ticks, "budget": budget}
+        if ticks:
+            case["ticks"] = ticks
+        if not light:
+            collide_radii(n).tofile(os.path.join(d, "radii.f64"))
+            idx, at = pinned(n)
+            np.ascontiguousarray(np.column_stack([idx, at]), np.float64).tofile(os.path.join(d, "pins.f64"))
+            case["forces"] += ["collide", "radial"]
+            case.update(collideIterations=2, radialRadius=200.0, radialStrength=0.05)
+        with open(os.path.join(d, "case.json"), "w") as f:
+            json.dump(case, f)
+        script = os.path.join(ROOT, "d3-force-3d-webgpu_amd", "js", "cpu-benchmark.mjs")
+        r = subprocess.run([node, script, d], capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            log("node CPU baseline failed: " + r.stderr[-2000:])
+            return None
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+# Algorithmic HBM bytes per node of each phase of a 1M-node tick (each input element read
+# once, each output written once), as this backend's kernels move them:
+PASS_BYTES = {
+    # bbox 12; keys 12 + 8 + 4 + 16; leaves + Karras nodes 16 + 16 + 4 + 8 + 16 + 4;
+    # reduction 16 + 16 + 64 record; d3 children 16 + 32
+    "tree_build": 12 + 40 + 64 + 96 + 48,
+    # histogram 8; four 8-bit passes over (u64 code, u32 node) read and written, 24 each;
+    # the run ordering 24 + 12
+    "tree_sort": 8 + 4 * 24 + 36,
+    # snapshot p + v -> float4 (24 + 16); CSR row 4, 2m/n = 3 entries of 20 B and the other
+    # end's float4 snapshot 16, velocities read and written 24
+    "link": 40 + 4 + 3 * 36 + 24,
+    # two iterations of keys, counting build, scan, scatter, run order and query (~180 B)
+    "collide": 2 * 180,
+    "center": 12 + 24,
+    "radial": 12 + 8 + 24,
+    "integrate": 48 + 4,
+}
+
+
+def pass_rooflines(n, phases):
+    """GB/s and fraction of 8 TB/s of each HBM-bound phase from its event-timed ms/tick."""
+    out = {}
+    for name, per_node in PASS_BYTES.items():
+        ms = phases.get(name)
+        if not ms:
+            continue
+        gbs = per_node * n / (ms * 1e-3) / 1e9
+        out[name] = {"bytes_per_node": per_node, "ms": ms, "gb_per_s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def copy_calibration():
     """A device copy of 256 MiB, three dispatches: what FETCH_SIZE and WRITE_SIZE read for a
     known number of bytes (the scale factor of the walk's counter readings)."""
@@ -547,9 +629,16 @@ def main():
     ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and exit")
+    ap.add_argument("--profile-ticks", type=int, default=0,
+                    help="only run this many ticks of the 1M-node workload (after 2 untimed) and exit, "
+                         "for profiler runs")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.profile_ticks:
+        r = run_gpu(args.nodes, args.profile_ticks, 2)
+        print(json.dumps({"profile_ticks": args.profile_ticks, "ms_per_tick": round(r["ms"], 4)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs one simulation per GPU on one GPU; "
@@ -569,11 +658,27 @@ def main():
     achieved = r["interactions"] * FLOP_PER_INTERACTION / (r["walk_ms"] * 1e-3) / 1e12 if r["walk_ms"] > 0 else 0.0
     cpu = None
     if not args.no_cpu:
-        log("CPU baseline: 1 tick of the same workload in the FP64 oracle ...")
-        cpu_ms = cpu_baseline(n, 1)
-        cpu = {"value": round(cpu_ms, 1), "unit": "ms/tick", "cores": 1, "kind": "port",
-               "sample": f"1 tick of the same {n}-node workload through the C FP64 restatement of "
-                         "d3-force-3d's tick (sequential link/collide, d3-octree Barnes-Hut), one thread"}
+        log("CPU baseline: forceSimulationGPU's CPU path in Node, a sample of the same workload ...")
+        js = node_cpu_baseline(n, min_ticks=2, budget=1.5)
+        log("CPU baseline: the 1,000-node benchmark graph, 300 ticks in Node ...")
+        js_a = node_cpu_baseline(1000, light=True, ticks=300)
+        log("CPU baseline: 1 tick of the same workload in the C FP64 restatement ...")
+        c_ms = cpu_baseline(n, 1)
+        c_port = {"value": round(c_ms, 1), "unit": "ms/tick", "cores": 1,
+                  "sample": f"1 tick of the same {n}-node workload, C FP64 restatement of d3-force-3d's tick "
+                            "(the parity oracle), one thread"}
+        if js:
+            cpu = {"value": round(js["ms_per_tick"], 1), "unit": "ms/tick", "cores": 1, "kind": "port",
+                   "sample": f"forceSimulationGPU's CPU path (d3-force-3d's physics, FP64 JavaScript, one thread) in "
+                             f"Node {js['node']}: one untimed tick then {js['ticks_timed']} timed ticks of the same "
+                             f"{n}-node workload (examples/benchmark.html:105-117)",
+                   "host_cpu": js["cpu_model"], "host_os_cpus": js["os_cpus"], "node": js["node"],
+                   "config_a_ms_per_tick": round(js_a["ms_per_tick"], 3) if js_a else None,
+                   "config_a_sample": "1,000-node benchmark graph, link + manyBody + center, 300 timed ticks after "
+                                      "one untimed (the reference publishes 3.5 ms/tick, Ryzen 9 9950X3D, README.md:86)",
+                   "c_port": c_port}
+        else:
+            cpu = dict(c_port, kind="port")
     out = {
         "metric": METRIC,
         "value": round(r["ms"], 4),
@@ -590,11 +695,14 @@ def main():
         "config": {"workload": "1M-node 3D layout: link + manyBody(theta 0.9) + center + collide(2 iterations, "
                                "lognormal radii) + radial(200, 0.05), 1% pinned",
                    "nodes": n, "links": r["links"], "dims": 3},
-        "roofline": {"bound": "mfma", "kernel": "tree_walk", "achieved": round(achieved, 3),
+        "roofline": {"bound": "valu", "kernel": "tree_walk", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": walk_traffic(n),
                      "walk_ms": round(r["walk_ms"], 4), "interactions": r["interactions"],
-                     "flop_per_interaction": FLOP_PER_INTERACTION},
+                     "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "bound_note": "no dense contraction on this path, so no MFMA: the walk is bound by VALU issue; "
+                                   "peak is MI355X's FP32 vector rate",
+                     "passes": pass_rooflines(n, r["phases"])},
         "cpu_baseline": cpu,
         "phases_ms_per_tick": r["phases"],
         "walk_errors": r["walk_errors"],

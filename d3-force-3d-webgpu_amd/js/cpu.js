@@ -82,8 +82,14 @@ class SpatialTree {
       size *= 2;
     }
     this.size = size;
+    this.depth = 0;  // the deepest cell's depth (a traversal stack needs (depth + 1) (fan - 1) + 1)
     this.root = n ? this.cell(0) : -1;
     for (let i = 1; i < n; ++i) this.add(P, i);
+  }
+
+  // Entries a depth-first traversal's stack can hold at most.
+  stackBound() {
+    return (this.depth + 2) * this.fan;
   }
 
   slot(P, i, corner, half) {
@@ -96,15 +102,17 @@ class SpatialTree {
 
   add(P, i) {
     const dims = this.dims, fan = this.fan, corner = this.origin.slice();
-    let half = this.size, parent = -1, at = -1, c = this.root;
+    let half = this.size, parent = -1, at = -1, c = this.root, level = 0;
     while (this.head[c] < 0) {
       half /= 2;
+      ++level;
       const k = this.slot(P, i, corner, half);
       parent = c, at = parent * fan + k;
       c = this.kids[at];
       if (c < 0) {
         const leaf = this.cell(i);  // (cell() may grow the arrays: take it before writing)
         this.kids[at] = leaf;
+        if (level > this.depth) this.depth = level;
         return;
       }
     }
@@ -121,6 +129,7 @@ class SpatialTree {
       if (parent < 0) this.root = s;
       else this.kids[at] = s;
       half /= 2;
+      ++level;
       let kj = 0;
       for (let a = 0; a < dims; ++a) if (P[a][j] >= corner[a] + half) kj |= 1 << a;
       const ki = this.slot(P, i, corner, half);
@@ -128,6 +137,7 @@ class SpatialTree {
         const leaf = this.cell(i);
         this.kids[s * fan + kj] = c;
         this.kids[s * fan + ki] = leaf;
+        if (level > this.depth) this.depth = level;
         return;
       }
       parent = s, at = s * fan + ki;
@@ -178,15 +188,20 @@ function manyBody(eng, d, alpha) {
     t.value[c] = v * damp;
     t.cx[c] = g0 / w, t.cy[c] = g1 / w, t.cz[c] = g2 / w;
   });
-  const cells = [], widths = [];
+  // pre-order over the cells with typed stacks; each term is added to the node's velocity
+  // in turn (a local copy of it, written back after the walk: the same roundings as d3's
+  // node.vx += term)
+  const head = t.head, kids = t.kids, next = t.next, val = t.value, cx = t.cx, cy = t.cy, cz = t.cz;
+  const stack = eng.stack(t.stackBound()), widths = eng.widths(t.stackBound());
   for (let i = 0; i < n; ++i) {
-    // terms go straight into the node's velocity, one by one, as d3 adds them
     const node = nodes[i], xi = node.x, yi = dims > 1 ? node.y : 0, zi = dims > 2 ? node.z : 0;
-    cells.push(t.root), widths.push(t.size);
-    while (cells.length) {
-      const c = cells.pop(), width = widths.pop(), value = t.value[c];
+    let vx = node.vx, vy = dims > 1 ? node.vy : 0, vz = dims > 2 ? node.vz : 0, sp = 0;
+    stack[0] = t.root, widths[0] = t.size, sp = 1;
+    while (sp > 0) {
+      --sp;
+      const c = stack[sp], width = widths[sp], value = val[c];
       if (!value) continue;
-      let x = t.cx[c] - xi, y = dims > 1 ? t.cy[c] - yi : 0, z = dims > 2 ? t.cz[c] - zi : 0;
+      let x = cx[c] - xi, y = dims > 1 ? cy[c] - yi : 0, z = dims > 2 ? cz[c] - zi : 0;
       let l = x * x + y * y + z * z;
       if (width * width / theta2 < l) {
         if (l < dmax2) {
@@ -194,35 +209,39 @@ function manyBody(eng, d, alpha) {
           if (dims > 1 && y === 0) y = jiggle(random), l += y * y;
           if (dims > 2 && z === 0) z = jiggle(random), l += z * z;
           if (l < dmin2) l = Math.sqrt(dmin2 * l);
-          node.vx += x * value * alpha / l;
-          if (dims > 1) node.vy += y * value * alpha / l;
-          if (dims > 2) node.vz += z * value * alpha / l;
+          vx += x * value * alpha / l;
+          if (dims > 1) vy += y * value * alpha / l;
+          if (dims > 2) vz += z * value * alpha / l;
         }
         continue;
       }
-      const h = t.head[c];
+      const h = head[c];
       if (h < 0) {
-        for (let k = fan - 1; k >= 0; --k) {
-          const q = t.kids[c * fan + k];
-          if (q >= 0) cells.push(q), widths.push(width / 2);
+        const half = width / 2;
+        for (let k = c * fan + fan - 1; k >= c * fan; --k) {
+          const q = kids[k];
+          if (q >= 0) stack[sp] = q, widths[sp] = half, ++sp;
         }
         continue;
       }
       if (l >= dmax2) continue;
-      if (h !== i || t.next[h] >= 0) {
+      if (h !== i || next[h] >= 0) {
         if (x === 0) x = jiggle(random), l += x * x;
         if (dims > 1 && y === 0) y = jiggle(random), l += y * y;
         if (dims > 2 && z === 0) z = jiggle(random), l += z * z;
         if (l < dmin2) l = Math.sqrt(dmin2 * l);
       }
-      for (let j = h; j >= 0; j = t.next[j]) {
+      for (let j = h; j >= 0; j = next[j]) {
         if (j === i) continue;
         const w = s[nodes[j].index] * alpha / l;
-        node.vx += x * w;
-        if (dims > 1) node.vy += y * w;
-        if (dims > 2) node.vz += z * w;
+        vx += x * w;
+        if (dims > 1) vy += y * w;
+        if (dims > 2) vz += z * w;
       }
     }
+    node.vx = vx;
+    if (dims > 1) node.vy = vy;
+    if (dims > 2) node.vz = vz;
   }
 }
 
@@ -250,7 +269,6 @@ function collide(eng, d) {
   const {nodes, nDim: dims, random} = eng, n = nodes.length, radii = d.radii;
   const strength = d.strength == null ? 1 : d.strength, iters = d.iterations == null ? 1 : d.iterations;
   const t = eng.treeFor(dims), px = eng.px, py = eng.py, pz = eng.pz, fan = t.fan;
-  const cells = [], corners = [], widths = [];
   for (let k = 0; k < iters; ++k) {
     for (let i = 0; i < n; ++i) {
       const q = nodes[i];
@@ -269,15 +287,19 @@ function collide(eng, d) {
       }
       t.radius[c] = r;
     });
+    const kidsNow = t.kids, headNow = t.head, radNow = t.radius;
+    const depth = t.stackBound(), stack = eng.stack(depth), widths = eng.widths(depth), corners = eng.corners(3 * depth);
     for (let i = 0; i < n; ++i) {
       const node = nodes[i], ri = radii[node.index], ri2 = ri * ri;
       const xi = node.x + node.vx, yi = dims > 1 ? node.y + node.vy : 0, zi = dims > 2 ? node.z + node.vz : 0;
-      cells.push(t.root), widths.push(t.size), corners.push(t.origin[0], t.origin[1], t.origin[2]);
-      while (cells.length) {
-        const c = cells.pop(), w = widths.pop();
-        const z0 = corners.pop(), y0 = corners.pop(), x0 = corners.pop();
-        let rj = t.radius[c], r = ri + rj;
-        const h = t.head[c];
+      let sp = 1;
+      stack[0] = t.root, widths[0] = t.size;
+      corners[0] = t.origin[0], corners[1] = t.origin[1], corners[2] = t.origin[2];
+      while (sp > 0) {
+        --sp;
+        const c = stack[sp], w = widths[sp], x0 = corners[3 * sp], y0 = corners[3 * sp + 1], z0 = corners[3 * sp + 2];
+        let rj = radNow[c], r = ri + rj;
+        const h = headNow[c];
         if (h >= 0) {
           const data = nodes[h];
           if (data.index > node.index) {
@@ -308,10 +330,13 @@ function collide(eng, d) {
         if (dims > 2 && (z0 > zi + r || z0 + w < zi - r)) continue;
         const half = w / 2;
         for (let q = fan - 1; q >= 0; --q) {
-          const kid = t.kids[c * fan + q];
+          const kid = kidsNow[c * fan + q];
           if (kid < 0) continue;
-          cells.push(kid), widths.push(half);
-          corners.push(q & 1 ? x0 + half : x0, q & 2 ? y0 + half : y0, q & 4 ? z0 + half : z0);
+          stack[sp] = kid, widths[sp] = half;
+          corners[3 * sp] = q & 1 ? x0 + half : x0;
+          corners[3 * sp + 1] = q & 2 ? y0 + half : y0;
+          corners[3 * sp + 2] = q & 4 ? z0 + half : z0;
+          ++sp;
         }
       }
     }
@@ -364,6 +389,20 @@ const scratch = {
   px: new Float64Array(0), py: new Float64Array(0), pz: new Float64Array(0),
   treeFor(dims) {
     return this.trees[dims] || (this.trees[dims] = new SpatialTree(dims));
+  },
+  // traversal stacks, grown to `size` entries when needed
+  cellStack: new Int32Array(0), widthStack: new Float64Array(0), cornerStack: new Float64Array(0),
+  stack(size) {
+    if (this.cellStack.length < size) this.cellStack = new Int32Array(size);
+    return this.cellStack;
+  },
+  widths(size) {
+    if (this.widthStack.length < size) this.widthStack = new Float64Array(size);
+    return this.widthStack;
+  },
+  corners(size) {
+    if (this.cornerStack.length < size) this.cornerStack = new Float64Array(size);
+    return this.cornerStack;
   }
 };
 
@@ -378,7 +417,8 @@ export function applyForce(desc, nodes, nDim, random, alpha) {
     scratch.pz = new Float64Array(n);
   }
   const ctx = {nodes: nodes, nDim: nDim, random: random || Math.random, px: scratch.px, py: scratch.py, pz: scratch.pz,
-               treeFor: (dims) => scratch.treeFor(dims)};
+               treeFor: (dims) => scratch.treeFor(dims), stack: (k) => scratch.stack(k),
+               widths: (k) => scratch.widths(k), corners: (k) => scratch.corners(k)};
   APPLY[desc.type](ctx, desc, alpha);
 }
 
