@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -387,6 +388,66 @@ void land(fg_ctx* c, double* pos, double* vel) {
     }
 }
 
+// ---- the sort that allocates nothing (fg_radix_sort_pairs_noalloc) -------------------
+//
+// Its work -- the ping-pong keys and values, digit histograms, look-back words, tile tickets,
+// epoch and error counter of a u32-key sort of at most NOALLOC_MAX_PAIRS pairs -- lives in
+// one file-scope device array, which each GPU holds from when the library's code is loaded
+// on it until the process ends.  Host code never uses the array's name as an address: the
+// device's own address of it is looked up once per device (at fg_create) and kept.
+constexpr uint32_t NOALLOC_MAX_PAIRS = 1u << 20;
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NA_KEYS = 0;
+constexpr size_t NA_VALS = NA_KEYS + align256(sizeof(uint32_t) * NOALLOC_MAX_PAIRS);
+constexpr size_t NA_HIST = NA_VALS + align256(sizeof(uint32_t) * NOALLOC_MAX_PAIRS);
+constexpr size_t NA_LOOK = NA_HIST + align256(sizeof(uint32_t) * 8 * 2048);
+constexpr uint64_t NA_LOOK_WORDS = sort_u32_lookback_words(NOALLOC_MAX_PAIRS);
+constexpr size_t NA_TICK = NA_LOOK + align256(sizeof(uint64_t) * NA_LOOK_WORDS);
+constexpr size_t NA_EPOCH = NA_TICK + align256(sizeof(uint32_t) * 16);
+constexpr size_t NA_ERR = NA_EPOCH + align256(sizeof(uint32_t) * 4);
+constexpr size_t NA_BYTES = NA_ERR + align256(sizeof(uint32_t) * 4);
+
+}  // namespace
+
+static __device__ __align__(256) uint64_t scratch[NA_BYTES / sizeof(uint64_t)];
+
+namespace {
+
+__global__ void clear_words(uint32_t* __restrict__ p, uint64_t words) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+constexpr int NA_DEVICES = 64;
+std::mutex na_mu;
+uint8_t* na_base[NA_DEVICES] = {};  // each device's address of `scratch`, once set up
+
+// Once per device, with that device current: look up the array's device address and zero
+// the look-back words and the epoch (later sorts tell stale words apart by the epoch, which
+// every sort advances).  Synchronous, so that a context on another stream of the same device
+// finds the array ready.
+hipError_t noalloc_setup(int device, hipStream_t stream) {
+  if (device < 0 || device >= NA_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(na_mu);
+  if (na_base[device]) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  uint8_t* base = (uint8_t*)p;
+  const uint64_t words = (NA_ERR - NA_LOOK) / sizeof(uint32_t);
+  hipLaunchKernelGGL(clear_words, dim3(1024), dim3(256), 0, stream, (uint32_t*)(base + NA_LOOK), words);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+  na_base[device] = base;
+  return hipSuccess;
+}
+
+uint8_t* noalloc_base(int device) {
+  std::lock_guard<std::mutex> lock(na_mu);
+  return device >= 0 && device < NA_DEVICES ? na_base[device] : nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -429,6 +490,12 @@ int fg_create(fg_ctx** out, int device) {
     return FG_ERR_HIP;
   }
   if (hipEventCreateWithFlags(&c->rb_event, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FG_ERR_HIP;
+  }
+  if (noalloc_setup(device, c->stream) != hipSuccess) {
+    (void)hipEventDestroy(c->rb_event);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return FG_ERR_HIP;
@@ -830,6 +897,47 @@ int fg_radix_sort_pairs(fg_ctx* c, uint32_t n, const uint32_t* keys, const uint3
     FG_HIP(hipStreamSynchronize(c->stream));
     uint32_t e = 0;
     FG_HIP(hipMemcpy(&e, w.errors, sizeof(e), hipMemcpyDeviceToHost));
+    if (e) return fail(c, FG_ERR_HIP, "sort: look-back wait bound reached");
+    return FG_OK;
+  });
+}
+
+int fg_radix_sort_pairs_noalloc(fg_ctx* c, uint32_t n, const uint32_t* keys, const uint32_t* values,
+                                uint32_t first_bit, uint32_t bits, uint32_t* keys_out, uint32_t* values_out) {
+  return guarded(c, [&] {
+    if (n && (!keys || !values || !keys_out || !values_out)) return FG_ERR_INVALID;
+    if (first_bit >= 32 || bits == 0 || first_bit + bits > 32) return fail(c, FG_ERR_INVALID, "bit range outside the key");
+    if (n > NOALLOC_MAX_PAIRS)
+      return fail(c, FG_ERR_INVALID, "fg_radix_sort_pairs_noalloc: " + std::to_string(n) + " pairs exceed the " +
+                                         std::to_string(NOALLOC_MAX_PAIRS) + " its scratch holds");
+    if (!n) return FG_OK;
+    uint8_t* base = noalloc_base(c->device);
+    if (!base) return fail(c, FG_ERR_STATE, "fg_radix_sort_pairs_noalloc: scratch not set up on this device");
+    const size_t bytes = sizeof(uint32_t) * n;
+    if (keys_out != keys) FG_HIP(hipMemcpyAsync(keys_out, keys, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (values_out != values) FG_HIP(hipMemcpyAsync(values_out, values, bytes, hipMemcpyDeviceToDevice, c->stream));
+    uint32_t* errors = (uint32_t*)(base + NA_ERR);
+    hipLaunchKernelGGL(clear_words, dim3(1), dim3(64), 0, c->stream, errors, (uint64_t)4);
+    FG_LAUNCH_CHECK();
+    SortWork w;
+    w.owned = false;
+    w.keys_alt = base + NA_KEYS;
+    w.vals_alt = (uint32_t*)(base + NA_VALS);
+    w.ghist = (uint32_t*)(base + NA_HIST);
+    w.lookback = (uint64_t*)(base + NA_LOOK);
+    w.tickets = (uint32_t*)(base + NA_TICK);
+    w.epoch = (uint32_t*)(base + NA_EPOCH);
+    w.errors = errors;
+    w.cap_n = NOALLOC_MAX_PAIRS;
+    w.cap_key_bytes = sizeof(uint32_t);
+    w.lookback_words = NA_LOOK_WORDS;
+    uint32_t *ok, *ov;
+    radix_sort_u32(c->stream, w, c->radix_bits, keys_out, values_out, n, first_bit, bits, &ok, &ov);
+    if (ok != keys_out) FG_HIP(hipMemcpyAsync(keys_out, ok, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (ov != values_out) FG_HIP(hipMemcpyAsync(values_out, ov, bytes, hipMemcpyDeviceToDevice, c->stream));
+    uint32_t e = 0;
+    FG_HIP(hipMemcpyAsync(&e, errors, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
     if (e) return fail(c, FG_ERR_HIP, "sort: look-back wait bound reached");
     return FG_OK;
   });

@@ -19,11 +19,21 @@ struct SortWork {
   uint32_t cap_n = 0;
   size_t cap_key_bytes = 0;
   uint64_t lookback_words = 0;
+  // false: the pointers are lent by the caller, sized for cap_n keys of cap_key_bytes and
+  // lookback_words words; reserve() then only checks a sort fits and release() frees nothing
+  bool owned = true;
 
   void reserve(uint32_t n, size_t key_bytes);
   void release();
   ~SortWork() { release(); }
 };
+
+// Look-back words that a sort of n u32 keys needs at the widest radix (11 bits): one per
+// (tile of SORT_U32_TILE pairs, digit).
+constexpr uint32_t SORT_U32_TILE = 3072;
+constexpr uint64_t sort_u32_lookback_words(uint32_t n) {
+  return (uint64_t)((n + SORT_U32_TILE - 1) / SORT_U32_TILE) * 2048u;
+}
 
 // Sorts bits [first_bit, first_bit + bits) of the keys, 8 or 11 bits per pass.  The sorted
 // pairs end up in (keys, vals) or in the work's alternate buffers: *out_* says which.

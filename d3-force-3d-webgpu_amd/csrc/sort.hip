@@ -244,6 +244,7 @@ void sort_launch(hipStream_t stream, SortWork& w, K* keys, uint32_t* vals, uint3
   // each wave's latency where the device is mostly idle, against a longer look-back)
   constexpr uint32_t SHORT_SORT = 131072;
   constexpr int ITEMS = sizeof(K) == 8 ? 8 : 12;
+  static_assert(sizeof(K) == 8 || THREADS * ITEMS == SORT_U32_TILE, "u32 tile out of step with sort.h");
   const bool short_sort = sizeof(K) == 8 && n <= SHORT_SORT;
   const uint32_t TILE = THREADS * (short_sort ? 4 : ITEMS);
   uint32_t passes = (bits + RB - 1) / RB;
@@ -373,6 +374,11 @@ void exclusive_scan_u32(hipStream_t stream, uint32_t* data, uint32_t n, uint32_t
 }
 
 void SortWork::reserve(uint32_t n, size_t key_bytes) {
+  if (!owned) {
+    // lent work: sort_launch checks the look-back words against the tiles it launches
+    if (n > cap_n || key_bytes > cap_key_bytes) throw ArgError(-1, "sort: lent work too small");
+    return;
+  }
   // tiles at the smallest tile (a short u64 sort: 256 * 4) and the widest radix (11 bits)
   uint64_t tiles = ceil_div(n > 0 ? n : 1, THREADS * 4);
   uint64_t need_lb = tiles * 2048;
@@ -395,13 +401,15 @@ void SortWork::reserve(uint32_t n, size_t key_bytes) {
 }
 
 void SortWork::release() {
-  if (keys_alt) (void)hipFree(keys_alt);
-  if (vals_alt) (void)hipFree(vals_alt);
-  if (ghist) (void)hipFree(ghist);
-  if (lookback) (void)hipFree(lookback);
-  if (tickets) (void)hipFree(tickets);
-  if (epoch) (void)hipFree(epoch);
-  if (errors) (void)hipFree(errors);
+  if (owned) {
+    if (keys_alt) (void)hipFree(keys_alt);
+    if (vals_alt) (void)hipFree(vals_alt);
+    if (ghist) (void)hipFree(ghist);
+    if (lookback) (void)hipFree(lookback);
+    if (tickets) (void)hipFree(tickets);
+    if (epoch) (void)hipFree(epoch);
+    if (errors) (void)hipFree(errors);
+  }
   keys_alt = nullptr;
   vals_alt = nullptr;
   ghist = tickets = epoch = errors = nullptr;

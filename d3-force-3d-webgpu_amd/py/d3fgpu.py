@@ -132,6 +132,8 @@ SIGNATURES = {
     "fg_timing_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "fg_get_timing": (C.c_int, [C.c_void_p, C.POINTER(Timing)]),
     "fg_radix_sort_pairs": (C.c_int, [C.c_void_p, C.c_uint32, _u, _u, C.c_uint32, C.c_uint32, _u, _u]),
+    "fg_radix_sort_pairs_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                              C.c_uint32, C.c_void_p, C.c_void_p]),
     "fg_debug_tree": (C.c_int, [C.c_void_p, _f, _i, _u, _u, _u, _u, _u, _u, _u, _d, _d]),
     "fg_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_uint64]),
     "fg_batch_destroy": (C.c_int, [C.c_void_p]),
@@ -406,6 +408,12 @@ class Engine:
         self._check(lib().fg_radix_sort_pairs(self.h, n, ptr(keys, C.c_uint32), ptr(values, C.c_uint32),
                                               first_bit, bits, ptr(ko, C.c_uint32), ptr(vo, C.c_uint32)))
         return ko, vo
+
+    def radix_sort_pairs_noalloc(self, n, keys, values, keys_out, values_out, first_bit=0, bits=32):
+        """The allocation-free sort on device buffers: keys, values, keys_out and values_out are
+        device addresses (ints) of n u32 each."""
+        self._check(lib().fg_radix_sort_pairs_noalloc(self.h, n, C.c_void_p(keys), C.c_void_p(values), first_bit,
+                                                      bits, C.c_void_p(keys_out), C.c_void_p(values_out)))
 
     def debug_tree(self):
         n = self.n

@@ -231,6 +231,21 @@ int fg_radix_sort_pairs(fg_ctx* ctx, uint32_t n, const uint32_t* keys, const uin
                         uint32_t* values_out);
 
 /*
+ * fg_radix_sort_pairs for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same sort and result, but keys, values, keys_out and values_out are DEVICE
+ * pointers of n u32 each (keys_out may equal keys and values_out values: in place), and the
+ * call allocates nothing: its work lives in a fixed array of about 14 MB that the library
+ * holds on each GPU it runs on until the process ends.  n is at most 1048576 (FG_ERR_INVALID
+ * beyond).  Queued on ctx's stream; returns once the sort is done.  Every call on one GPU,
+ * from any context, shares that array: calls that overlap on one GPU (other threads, other
+ * contexts' streams) race and must be serialised by the caller.
+ */
+int fg_radix_sort_pairs_noalloc(fg_ctx* ctx, uint32_t n, const uint32_t* keys,
+                                const uint32_t* values, uint32_t first_bit, uint32_t bits,
+                                uint32_t* keys_out, uint32_t* values_out);
+
+/*
  * The many-body tree of the last step that built one, for parity checks against the
  * reference algorithm (barnesHut.js:21-47).  Any pointer may be NULL.
  *   cube: origin x, y, z and size (4 floats); shift: LEVELS - log2(size)
