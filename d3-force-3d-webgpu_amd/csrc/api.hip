@@ -132,6 +132,7 @@ struct fg_ctx {
   uint32_t walk_packed_max = WALK_PACKED_BY_MODE;
   uint32_t walk_tile = 0;
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
+  uint32_t walk_wide_max = WALK_WIDE_BY_DIMS;
   uint32_t reduce_block = 0;
   const Pass* center_fused = nullptr;  // this tick's last pass, a center left to the integrator's launch
   // test hooks, read at creation, so that a host's fallbacks can be driven without a real
@@ -280,7 +281,7 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
       } else {
         if (!p.tree) p.tree.reset(new TreeBuffers());
         TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2, c->walk_packed_max, c->walk_tile,
-                      c->walk_solo_max, c->reduce_block};
+                      c->walk_solo_max, c->walk_wide_max, c->reduce_block};
         tree_manybody(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), vel[0],
                       vel[1], vel[2], tp, sp, c->radix_bits, c->T());
       }
@@ -475,6 +476,7 @@ int fg_create(fg_ctx** out, int device) {
   if (const char* v = getenv("D3FGPU_WALK_PACKED_MAX")) c->walk_packed_max = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_WALK_TILE")) c->walk_tile = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_WALK_SOLO")) c->walk_solo_max = (uint32_t)atoi(v);
+  if (const char* v = getenv("D3FGPU_WALK_WIDE")) c->walk_wide_max = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_REDUCE_BLOCK")) c->reduce_block = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_RADIX_BITS")) c->radix_bits = atoi(v) == 11 ? 11 : 8;
   if (const char* v = getenv("D3FGPU_CAPACITY")) c->capacity = strtoull(v, nullptr, 10);
@@ -840,6 +842,19 @@ int fg_get_stats(fg_ctx* c, fg_stats* out) {
         s.opened = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
         s.lane_visits = (uint64_t)h[6] | ((uint64_t)h[7] << 32);
         s.walk_errors = h[4];
+        if (const uint32_t nb = p->tree->slot_blocks) {  // the wide walk's counts, per workgroup
+          std::vector<uint32_t> sl((size_t)8 * nb);
+          FG_HIP(hipMemcpy(sl.data(), p->tree->slots, sizeof(uint32_t) * sl.size(), hipMemcpyDeviceToHost));
+          for (uint32_t b = 0; b < nb; ++b) {
+            const uint32_t* q = &sl[(size_t)8 * b];
+            const uint64_t op = (uint64_t)q[2] | ((uint64_t)q[3] << 32);
+            s.interactions += (uint64_t)q[0] | ((uint64_t)q[1] << 32);
+            s.opened += op;
+            s.lane_visits += op;  // one point per opening
+            s.walk_errors += q[4];
+            s.walk_max_stack = std::max(s.walk_max_stack, q[5]);
+          }
+        }
         uint32_t e[4] = {0, 0, 0, 0};
         if (p->tree->sort.errors) FG_HIP(hipMemcpy(e, p->tree->sort.errors, sizeof(e), hipMemcpyDeviceToHost));
         s.walk_errors += e[0];

@@ -12,6 +12,11 @@ namespace fg {
 constexpr uint32_t REF_POINT = 0x80000000u;  // a child reference that is a sorted point
 constexpr uint32_t TREE_COUNTERS = 16;       // words of TreeBuffers::counters
 constexpr uint32_t REF_NONE = 0xffffffffu;
+// Trees of at most this many points are walked one point per wave (the wide walk), or in
+// the tiled walk's exact mode where the wide walk is turned off; larger trees fill the
+// device with 64-point tiles and the tiled walk's cheaper accumulation.
+constexpr uint32_t WALK_EXACT_MAX = 163840;
+constexpr uint32_t WALK_SLOTS = 4096;         // workgroups of a walk that counts into slots, at most
 
 // What the walk reads of a binary node: its centre of charge (FP64), its strength at the
 // depth of its deepest cell, that depth, and its children in the d3 tree.
@@ -65,6 +70,11 @@ struct TreeBuffers {
   NodeRec* rec = nullptr;
   NodeSum* sums = nullptr;
   CubeInfo* cube = nullptr;
+  // the wide walk's counts: 8 words per workgroup (u64 interactions, u64 opened, errors,
+  // deepest stack), for the slot_blocks workgroups of the last walk (0: that walk counted
+  // into `counters`)
+  uint32_t* slots = nullptr;
+  uint32_t slot_blocks = 0;
   uint32_t* counters = nullptr; // TREE_COUNTERS words: u64 interactions [0,1], opened [2,3], walk errors [4], tile queue [5], lane visits [6,7], deepest stack [8], long runs [10]
   uint32_t* long_runs = nullptr;  // first positions of runs of equal high halves too long to order in place
   void* spill = nullptr;        // the walk's stack entries beyond LDS, per resident wave
@@ -79,6 +89,7 @@ struct TreeBuffers {
 
 constexpr uint32_t WALK_SOLO_BY_DIMS = 0xffffffffu;  // walk_solo_max: the measured default per dims
 constexpr uint32_t WALK_PACKED_BY_MODE = 0xffffffffu;  // walk_packed_max: the measured default per mode
+constexpr uint32_t WALK_WIDE_BY_DIMS = 0xffffffffu;  // walk_wide_max: the measured default per dims
 
 struct TreeParams {
   double theta2, dmin2, dmax2;
@@ -86,6 +97,8 @@ struct TreeParams {
   uint32_t walk_tile = 0;         // points per tile of a small (exact-mode) walk; 0: by size
   // trees of at most this many points walk one point per group of lanes (0: never)
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
+  // trees of at most this many points walk one point per wave (0: never)
+  uint32_t walk_wide_max = WALK_WIDE_BY_DIMS;
   uint32_t reduce_block = 0;  // points per workgroup of the reduction (512 or 1024; 0: by size)
 };
 

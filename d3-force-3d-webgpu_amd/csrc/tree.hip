@@ -1037,10 +1037,6 @@ __global__ void __launch_bounds__(B) tree_reduce(uint32_t n, const float4* __res
 // ---- walk ---------------------------------------------------------------------------
 
 constexpr int WALK_WAVES = 4;
-// Trees of at most this many points are walked in the exact mode: forces that do not
-// depend on how the points are tiled (so small trees can take small tiles, more waves);
-// larger trees fill the device with 64-point tiles and take the cheaper mode.
-constexpr uint32_t WALK_EXACT_MAX = 163840;
 constexpr int WALK_STACK = 48;   // stack entries per wave in LDS (a walk at 1M points peaks near 30)
 constexpr int WALK_DEPTH = 192;  // stack entries per wave in all; 7 * 21 + 8 = 155 in 3D for distinct codes
 
@@ -1770,6 +1766,235 @@ __global__ void __launch_bounds__(WAVES * WAVE) tree_walk_solo(const float4* __r
   walk_flush(C, lane, counters);
 }
 
+// ---- walk of one point per wave, several openings per step ------------------------
+//
+// Below a few hundred thousand points the walk is bound by the chain of openings each wave
+// goes through, one dependent record load after another, with too few waves on the device
+// to hide it.  Here a wave walks one point at a time and opens up to P = 64 / 2^D stack
+// entries per step: lane (e, c) takes child c of the e-th entry from the top, loads its
+// record straight into registers, decides it with d3's rules and adds its term to the
+// lane's own sums; the children it must open replace the opened entries on the stack.  The
+// chain is about P times shorter and every wave of the device has points to walk.  The
+// point's force is the sum of the lanes' sums, lane 0's fixed tree of adds (deterministic
+// for a given tree; not the order of the tiled walk: the two differ by rounding only).
+//
+// The stack: WIDE_DEPTH entries, the first WIDE_LDS in LDS and the rest in the wave's spill
+// area in global memory.  Entries open P at a time while the stack stays within WIDE_BROAD,
+// one at a time beyond (d3's depth-first order), so the stack exceeds WIDE_BROAD by at most
+// what a depth-first walk of one entry adds: 7 * 21 + 8 in 3D for distinct codes, as in the
+// other walks; deeper (long runs of coincident points) counts as a walk error.  Walks at
+// theta 0.9 peak near 20 entries.
+//
+// The grid is persistent (each wave takes points wave, wave + waves, ...), and each
+// workgroup writes its counts to a slot of its own (plain stores; the host adds the slots):
+// tens of thousands of workgroups adding to the same counter words serialise on them.
+constexpr uint32_t WIDE_DEPTH = 384;
+constexpr uint32_t WIDE_LDS = 64;
+constexpr uint32_t WIDE_BROAD = WIDE_DEPTH - (uint32_t)WALK_DEPTH;
+constexpr uint32_t WIDE_WAVES = 4;
+
+template <int D>
+struct WideLDS {
+  static constexpr uint32_t NK = 1u << D;
+  alignas(16) uint32_t kids[WIDE_LDS][NK];  // an entry's children go in as 16-byte stores
+  uint32_t depth[WIDE_LDS];
+};
+
+// Words of spill area per wave of the wide walk: the children and depth of each entry past
+// the LDS part.
+template <int D>
+constexpr uint32_t wide_spill_words() {
+  return (WIDE_DEPTH - WIDE_LDS) * ((1u << D) + 1u);
+}
+
+// Counts of a workgroup of a walk into its slot (every wave of the workgroup calls it).
+__device__ inline void walk_flush_slot(const WalkCount& C, uint32_t lane, uint32_t* slot) {
+  __shared__ unsigned long long s_sum[2];
+  __shared__ uint32_t s_err, s_max;
+  if (threadIdx.x == 0) s_sum[0] = s_sum[1] = 0, s_err = 0, s_max = 0;
+  __syncthreads();
+  uint32_t tot = C.inter;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, WAVE);
+  if (lane == 0) {
+    atomicAdd(&s_sum[0], (unsigned long long)tot);
+    atomicAdd(&s_sum[1], (unsigned long long)C.opened);
+    if (C.err) atomicAdd(&s_err, 1u);
+    atomicMax(&s_max, C.max_sp);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *(uint4*)slot = make_uint4((uint32_t)s_sum[0], (uint32_t)(s_sum[0] >> 32), (uint32_t)s_sum[1],
+                               (uint32_t)(s_sum[1] >> 32));
+    *(uint4*)(slot + 4) = make_uint4(s_err, s_max, 0u, 0u);
+  }
+}
+
+// The walk of sorted point s (tree-space index) of tree T by one wave of the wide walk.
+// inter (per lane), opened, max_sp and err accumulate the wave's counts; err set: the wave
+// stops.
+template <int D>
+__device__ __forceinline__ void wide_point(WideLDS<D>& L, uint32_t* sk, uint32_t* sd,
+                                           const float4* __restrict__ leaves,
+                                           const uint32_t* __restrict__ order,
+                                           const NodeRec* __restrict__ rec, const WalkTree& T, uint32_t s,
+                                           uint32_t lane, uint32_t& inter, uint64_t& opened, uint32_t& max_sp,
+                                           uint32_t& err) {
+  constexpr uint32_t NK = 1u << D, P = WAVE / NK;
+  const uint32_t e = lane / NK, c = lane % NK;
+  const uint32_t off = T.off, n = T.n;
+  const double size = T.size, theta2 = T.theta2, dmax2 = T.dmax2;
+  const float dmin2f = T.dmin2f, dmax2f = T.dmax2f, alpha = T.alpha;
+  const uint32_t step_no = T.step_no, seed = T.seed;
+  const uint32_t guard_max = 2u * n + 8u;
+  const float4 me = leaves[s];
+  const uint32_t self_node = order[s] - off;
+  uint32_t sp = 0;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  {  // the root: lane 0's term
+    const NodeRec& r = rec[off];
+    const NodeHdr h = *(const NodeHdr*)&r;
+    float rx = 0.f, ry = 0.f, rz = 0.f;
+    uint32_t ri = 0;
+    const bool done = cell_force<D, ACC_FUSED>(me.x, me.y, me.z, self_node, n, 0u, 0u, size * size, h, theta2,
+                                               dmax2, dmin2f, alpha, step_no, seed, rx, ry, rz, ri);
+    if (lane == 0) ax = rx, ay = ry, az = rz, inter += ri;
+    if (!done) {
+      if (lane == 0) L.depth[0] = r.depth;
+      if (lane < NK) L.kids[0][lane] = r.kids[lane];
+      sp = 1;
+    }
+  }
+  for (uint32_t guard = 0; sp > 0u; ++guard) {  // sp is the same on every lane
+    if (guard > guard_max) {
+      err = 1;
+      break;
+    }
+    // entries to open: P, or fewer if the stack has fewer or is past WIDE_BROAD
+    uint32_t p = sp < P ? sp : P;
+    const uint32_t room = sp < WIDE_BROAD ? (WIDE_BROAD - sp) / (NK - 1) : 0u;
+    if (p > room) p = room > 0u ? room : 1u;
+    const uint32_t base = sp - p;  // entries [base, sp) open now; their children go from base up
+    opened += p;
+    if (sp > WIDE_LDS) wait_all_memory();  // the wave's own spill stores, complete before these loads
+    uint32_t k = REF_NONE, top = 0;
+    if (e < p) {
+      const uint32_t ent = sp - 1u - e;
+      if (ent < WIDE_LDS) {
+        k = L.kids[ent][c];
+        top = L.depth[ent] + 1u;
+      } else {
+        k = load_coherent(&sk[(ent - WIDE_LDS) * NK + c]);
+        top = load_coherent(&sd[ent - WIDE_LDS]) + 1u;
+      }
+    }
+    bool open = false;
+    uint4 k0 = make_uint4(0u, 0u, 0u, 0u), k1 = k0;
+    uint32_t cdepth = 0;
+    if (k != REF_NONE) {
+      if (k & REF_POINT) {
+        const uint32_t j = k & ~REF_POINT;
+        const float4 q = leaves[j];
+        if (j != s)
+          point_force<D, ACC_FUSED>(me.x, me.y, me.z, self_node, q, order, j, off, dmax2f, dmin2f, alpha, step_no,
+                                    seed, ax, ay, az, inter);
+      } else {
+        const uint4* R = (const uint4*)(rec + k);
+        const uint4 q0 = R[0], q1 = R[1];
+        k0 = R[2];
+        k1 = R[3];
+        NodeHdr h;
+        __builtin_memcpy(&h, &q0, 16);
+        __builtin_memcpy((char*)&h + 16, &q1, 16);
+        const double wt = ldexp(size, -(int)top);
+        open = !cell_force<D, ACC_FUSED>(me.x, me.y, me.z, self_node, n, k - off, top, wt * wt, h, theta2, dmax2,
+                                         dmin2f, alpha, step_no, seed, ax, ay, az, inter);
+        cdepth = h.depth;
+      }
+    }
+    // the children to open, in lane order, over the entries just opened
+    const uint64_t bits = __builtin_amdgcn_ballot_w64(open);
+    const uint32_t nsp = base + (uint32_t)__popcll(bits);
+    if (nsp > WIDE_DEPTH) {  // no entry past the stack is written: stop here
+      err = 1;
+      break;
+    }
+    if (open) {
+      const uint32_t slot = base + mask_rank(bits);
+      const uint32_t kk[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+      if (slot < WIDE_LDS) {
+        L.depth[slot] = cdepth;
+        uint32_t* en = L.kids[slot];
+        if (NK == 8) {
+          *(uint4*)en = k0;
+          *(uint4*)(en + 4) = k1;
+        } else if (NK == 4) {
+          *(uint4*)en = k0;
+        } else {
+          *(uint2*)en = make_uint2(k0.x, k0.y);
+        }
+      } else {
+#pragma unroll
+        for (uint32_t q = 0; q < NK; ++q) store_coherent(&sk[(slot - WIDE_LDS) * NK + q], kk[q]);
+        store_coherent(&sd[slot - WIDE_LDS], cdepth);
+      }
+    }
+    sp = nsp;
+    max_sp = sp > max_sp ? sp : max_sp;
+  }
+  // the point's force: the lanes' sums added in lane 0's fixed tree
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    ax += __shfl_xor(ax, o, WAVE);
+    if (D > 1) ay += __shfl_xor(ay, o, WAVE);
+    if (D > 2) az += __shfl_xor(az, o, WAVE);
+  }
+  if (lane == 0 && !err) {
+    T.vx[self_node] += ax;
+    if (D > 1) T.vy[self_node] += ay;
+    if (D > 2) T.vz[self_node] += az;
+  }
+}
+
+template <int D, uint32_t WAVES>
+__global__ void __launch_bounds__(WAVES * WAVE) tree_walk_wide(const float4* __restrict__ leaves,
+                                                       const uint32_t* __restrict__ order,
+                                                       const NodeRec* __restrict__ rec,
+                                                       const CubeInfo* __restrict__ cube, float* __restrict__ vx,
+                                                       float* __restrict__ vy, float* __restrict__ vz,
+                                                       uint32_t* __restrict__ slots, uint32_t* __restrict__ spill,
+                                                       StepParams step, WalkArgs A) {
+  __shared__ WideLDS<D> lds[WAVES];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const uint32_t gw = blockIdx.x * WAVES + wave, waves = gridDim.x * WAVES;
+  uint32_t* sk = spill + (size_t)gw * wide_spill_words<D>();  // entry i >= WIDE_LDS: kids, then depths
+  uint32_t* sd = sk + (WIDE_DEPTH - WIDE_LDS) * (1u << D);
+  WalkTree T;
+  T.off = 0;
+  T.n = A.n;
+  T.size = (double)cube->size;
+  T.theta2 = A.theta2;
+  T.dmax2 = A.dmax2;
+  T.dmin2f = (float)A.dmin2;
+  T.dmax2f = (float)fmin(A.dmax2, 3.0e38);
+  T.alpha = step.alpha;
+  T.step_no = step.step;
+  T.seed = step.seed;
+  T.vx = vx;
+  T.vy = vy;
+  T.vz = vz;
+  uint32_t inter = 0, max_sp = 0, err = 0;
+  uint64_t opened = 0;
+  for (uint32_t s = gw; s < A.n && !err; s += waves)
+    wide_point<D>(lds[wave], sk, sd, leaves, order, rec, T, s, lane, inter, opened, max_sp, err);
+  WalkCount C;
+  C.opened = opened;
+  C.inter = inter;
+  C.err = err;
+  C.max_sp = max_sp;
+  walk_flush_slot(C, lane, slots + (size_t)blockIdx.x * 8);
+}
+
 // Resident workgroups of a persistent walk kernel on this device (all CUs).
 template <typename K>
 uint32_t resident_blocks(K kern, int threads, int cus) {
@@ -1937,12 +2162,53 @@ __global__ void __launch_bounds__(WAVES * WAVE)
     const uint32_t end = T.off + T.n;
     const uint32_t count = end - tile.y < (uint32_t)WAVE ? end - tile.y : (uint32_t)WAVE;
     __builtin_amdgcn_wave_barrier();
-    if (T.n <= WALK_EXACT_MAX)
-      walk_tile<D, STACK, true>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
-    else
-      walk_tile<D, STACK, false>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
+    walk_tile<D, STACK, false>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
   }
   walk_flush(C, lane, counters);
+}
+
+// The wide walk over the trees of a batch's jobs of at most WALK_EXACT_MAX points (the
+// larger ones take 64-point tiles in btree_walk): each wave takes tree-space points gw,
+// gw + waves, ... and walks those of active jobs as the single simulation's wide walk does,
+// the same terms in the same order.
+template <int D, uint32_t WAVES>
+__global__ void __launch_bounds__(WAVES * WAVE)
+    btree_walk_wide(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
+                    const NodeRec* __restrict__ rec, uint32_t total, const uint32_t* __restrict__ seg_of,
+                    const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
+                    uint32_t* __restrict__ slots, uint32_t* __restrict__ spill) {
+  __shared__ WideLDS<D> lds[WAVES];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const uint32_t gw = blockIdx.x * WAVES + wave, waves = gridDim.x * WAVES;
+  uint32_t* sk = spill + (size_t)gw * wide_spill_words<D>();
+  uint32_t* sd = sk + (WIDE_DEPTH - WIDE_LDS) * (1u << D);
+  uint32_t inter = 0, max_sp = 0, err = 0;
+  uint64_t opened = 0;
+  for (uint32_t s = gw; s < total && !err; s += waves) {
+    const BJob& B = jobs[seg_job[seg_of[s]]];
+    if (!B.active || B.n > WALK_EXACT_MAX) continue;
+    WalkTree T;
+    T.off = B.tree_off;
+    T.n = B.n;
+    T.size = (double)B.cube_size;
+    T.theta2 = B.theta2;
+    T.dmax2 = B.dmax2;
+    T.dmin2f = (float)B.dmin2;
+    T.dmax2f = (float)fmin(B.dmax2, 3.0e38);
+    T.alpha = B.alpha_f;
+    T.step_no = B.step;
+    T.seed = B.seed;
+    T.vx = B.v[B.vcur][0];
+    T.vy = B.v[B.vcur][1];
+    T.vz = B.v[B.vcur][2];
+    wide_point<D>(lds[wave], sk, sd, leaves, order, rec, T, s, lane, inter, opened, max_sp, err);
+  }
+  WalkCount C;
+  C.opened = opened;
+  C.inter = inter;
+  C.err = err;
+  C.max_sp = max_sp;
+  walk_flush_slot(C, lane, slots + (size_t)blockIdx.x * 8);
 }
 
 template <int D, int STACK, int WAVES>
@@ -1964,6 +2230,24 @@ void launch_bwalk(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, co
                      sp.walk_tiles, (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.counters,
                      (SpillEntry*)t.spill, tune.walk_packed_max);
   FG_LAUNCH_CHECK();
+}
+
+// The wide walk over the jobs of at most WALK_EXACT_MAX points (a wave per point).
+template <int D>
+void launch_bwalk_wide(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, int cus) {
+  t.slot_blocks = 0;
+  if (!sp.total) return;
+  auto kern = btree_walk_wide<D, WIDE_WAVES>;
+  uint32_t blocks = resident_blocks(kern, WIDE_WAVES * WAVE, cus);
+  if (blocks > WALK_SLOTS) blocks = WALK_SLOTS;
+  if (blocks > ceil_div(sp.total, WIDE_WAVES)) blocks = ceil_div(sp.total, WIDE_WAVES);
+  if (t.spill_bytes < sizeof(uint32_t) * (size_t)blocks * WIDE_WAVES * wide_spill_words<D>())
+    throw ArgError(-5, "walk: spill area too small");
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(WIDE_WAVES * WAVE), 0, st, (const float4*)t.leaves,
+                     (const uint32_t*)t.order, (const NodeRec*)t.rec, sp.total, (const uint32_t*)sp.seg_of,
+                     (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.slots, (uint32_t*)t.spill);
+  FG_LAUNCH_CHECK();
+  t.slot_blocks = blocks;
 }
 
 // The walk kernel for the tuned workgroup size and LDS stack depth (16, 32, 48, 64 or 96
@@ -2048,7 +2332,8 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
                      (const uint32_t*)sp.seg_job, (const BJob*)jobs, (const uint4*)t.links, (NodeRec*)t.rec);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(3);
-  launch_bwalk_tuned<D>(st, t, sp, jobs, tune, cus);
+  launch_bwalk_wide<D>(st, t, sp, jobs, cus);        // jobs of at most WALK_EXACT_MAX points
+  launch_bwalk_tuned<D>(st, t, sp, jobs, tune, cus);  // the larger jobs' 64-point tiles
   if (timer) timer->end(3);
 }
 
@@ -2074,6 +2359,26 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
   // (0.259 against 0.291)
   const uint32_t solo_max = p.walk_solo_max != WALK_SOLO_BY_DIMS ? p.walk_solo_max
                             : D == 3 ? 12288u : D == 2 ? 24576u : 32768u;
+  t.slot_blocks = 0;
+  // measured (link + manyBody + center ticks, 3D): a wave per point is the faster walk up to
+  // 200,000 points (0.661 against 0.669 ms/tick there, 0.398 against 0.499 at 100,000), the
+  // tiled one from 300,000 (0.772 against 0.910); up to the exact mode's limit
+  const uint32_t wide_max = p.walk_wide_max != WALK_WIDE_BY_DIMS ? p.walk_wide_max : WALK_EXACT_MAX;
+  if (n <= wide_max) {  // a wave per point
+    auto wk = tree_walk_wide<D, WIDE_WAVES>;
+    uint32_t blocks = resident_blocks(wk, WIDE_WAVES * WAVE, cus);
+    if (blocks > WALK_SLOTS) blocks = WALK_SLOTS;
+    if (blocks > ceil_div(n, WIDE_WAVES)) blocks = ceil_div(n, WIDE_WAVES);
+    const size_t sbytes = sizeof(uint32_t) * (size_t)blocks * WIDE_WAVES * wide_spill_words<D>();
+    if (t.spill_bytes < sbytes) t.reserve_spill(sbytes);
+    hipLaunchKernelGGL(wk, dim3(blocks), dim3(WIDE_WAVES * WAVE), 0, st, (const float4*)t.leaves,
+                       (const uint32_t*)t.order, (const NodeRec*)t.rec, (const CubeInfo*)t.cube, vx, vy, vz, t.slots,
+                       (uint32_t*)t.spill, step, A);
+    FG_LAUNCH_CHECK();
+    t.slot_blocks = blocks;
+    if (timer) timer->end(3);
+    return;
+  }
   if (exact && n <= solo_max) {  // a group of lanes per point
     constexpr uint32_t G = WAVE >> D;
     if (ceil_div(n, G) <= (uint32_t)cus)
@@ -2205,6 +2510,7 @@ void TreeBuffers::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&sums, sizeof(NodeSum) * m));
   FG_HIP(hipMalloc(&cube, sizeof(CubeInfo)));
   FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * TREE_COUNTERS));
+  FG_HIP(hipMalloc(&slots, sizeof(uint32_t) * 8 * WALK_SLOTS));
   FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   CubeInfo c0 = {};
   for (int a = 0; a < 3; ++a) c0.box[a] = 0xffffffffu, c0.box[3 + a] = 0u;
@@ -2227,13 +2533,14 @@ void TreeBuffers::release() {
   if (spill) (void)hipFree(spill);
   spill = nullptr;
   spill_bytes = 0;
-  void* ptrs[] = {keys, vals, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, cube, counters, long_runs};
+  void* ptrs[] = {keys, vals, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, cube, counters, slots, long_runs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = nullptr;
   vals = nullptr;
   leaves = pts = nullptr;
-  leaf_depth = leaf_parent = flags = counters = long_runs = nullptr;
+  leaf_depth = leaf_parent = flags = counters = slots = long_runs = nullptr;
+  slot_blocks = 0;
   links = nullptr;
   rec = nullptr;
   sums = nullptr;
@@ -2283,20 +2590,26 @@ void BatchTree::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   sort.reserve(m, sizeof(uint64_t));
   sort_seg.reserve(m, sizeof(uint32_t));
+  FG_HIP(hipMalloc(&slots, sizeof(uint32_t) * 8 * WALK_SLOTS));
   if (!spill) {
-    spill_bytes = spill_bytes_for(device_cus(), 16);
+    // the tiled walk's entries past a 16-entry LDS stack, or the wide walk's past its LDS
+    // part, for every wave a CU can hold (32)
+    const int cus = device_cus();
+    spill_bytes = std::max(spill_bytes_for(cus, 16),
+                           sizeof(uint32_t) * (size_t)32 * (size_t)(cus > 0 ? cus : 1) * wide_spill_words<3>());
     FG_HIP(hipMalloc(&spill, spill_bytes));
   }
   cap = n;
 }
 
 void BatchTree::release() {
-  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, counters, spill,
-                  long_runs};
+  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, counters, slots,
+                  spill, long_runs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = codes = skeys = nullptr;
-  vals = digit = order = leaf_depth = leaf_parent = flags = counters = long_runs = nullptr;
+  vals = digit = order = leaf_depth = leaf_parent = flags = counters = slots = long_runs = nullptr;
+  slot_blocks = 0;
   leaves = pts = nullptr;
   links = nullptr;
   rec = sums = spill = nullptr;

@@ -152,26 +152,27 @@ def test_manybody_tree_forces_do_not_depend_on_tiling(monkeypatch, dims, coincid
         members[n // 10: n // 10 + 40] = members[0] = True
     s = np.full(n, -30.0) * (1 + 0.5 * np.sin(np.arange(n)))
     out = []
+    monkeypatch.setenv("D3FGPU_WALK_WIDE", "0")
     for tile, packed, solo in [(64, 24, 0), (64, 0, 0), (64, 64, 0), (32, 0, 0), (16, 24, 0), (7, 64, 0),
                                (1, 24, 0), (0, 24, n)]:
         monkeypatch.setenv("D3FGPU_WALK_TILE", str(tile))
         monkeypatch.setenv("D3FGPU_WALK_PACKED_MAX", str(packed))
         monkeypatch.setenv("D3FGPU_WALK_SOLO", str(solo))
-        e = d3fgpu.Engine(0)
-        try:
-            _, dv = gpu_dv(e, pos, dims, [d3fgpu.many_body(s)])
-            assert e.stats().walk_errors == 0
-        finally:
-            e.close()
-        out.append(((tile, packed, solo), dv))
+        out.append(((tile, packed, solo), walk_dv(pos, dims, s)))
     for key, dv in out[1:]:
         assert np.array_equal(dv, out[0][1]), f"tile/packed/solo {key} differs from {out[0][0]}"
+    # the wave-per-point walk (the default at this size) sums the same terms in another order
+    for name in ("D3FGPU_WALK_WIDE", "D3FGPU_WALK_TILE", "D3FGPU_WALK_PACKED_MAX", "D3FGPU_WALK_SOLO"):
+        monkeypatch.delenv(name, raising=False)
+    wide = walk_dv(pos, dims, s)
+    assert rel_l2(wide, out[0][1]) < 1e-6
     ref, _ = oracle_manybody(pos, dims, s)
     keep = ~members
-    err = rel_l2(out[0][1][keep], ref[keep])
-    report(f"manyBody tree {dims}D n={n} clump{' + 40 coincident' if coincident else ''}, any tiling", err)
-    assert err < 1e-4
-    assert close_count(out[0][1][keep], ref[keep], rtol=1e-3) <= tree_outliers(n)
+    for label, dv in (("any tiling", out[0][1]), ("wave per point", wide)):
+        err = rel_l2(dv[keep], ref[keep])
+        report(f"manyBody tree {dims}D n={n} clump{' + 40 coincident' if coincident else ''}, {label}", err)
+        assert err < 1e-4
+        assert close_count(dv[keep], ref[keep], rtol=1e-3) <= tree_outliers(n)
 
 
 @pytest.mark.parametrize("dims", [1, 2, 3])
@@ -235,34 +236,64 @@ def test_gpu_jiggle_known_answer(theta, engine_factory):
             np.testing.assert_allclose(dv[1], -u * s[0] * 0.5, rtol=2e-6, atol=1e-12)
 
 
+def walk_dv(pos, dims, s):
+    """One many-body force on a fresh engine (walk knobs from the environment), with no
+    walk errors."""
+    e = d3fgpu.Engine(0)
+    try:
+        _, dv = gpu_dv(e, pos, dims, [d3fgpu.many_body(s)])
+        assert e.stats().walk_errors == 0
+    finally:
+        e.close()
+    return dv
+
+
 @pytest.mark.parametrize("dims,n", [(3, 12288), (3, 12289), (2, 24576), (2, 24577), (1, 20000), (1, 32768),
                                     (1, 32769)])
 def test_manybody_lane_group_walk_switch_points(monkeypatch, dims, n):
-    """Either side of the sizes up to which a tree is walked with a group of 2^D lanes per
-    point (12,288 points in 3D, 24,576 in 2D, 32,768 in 1D, with 1D's largest workgroups in
-    LDS at 20,000): the default walk equals the tiled exact-mode walk bit for bit, has no
-    walk errors, and matches the FP64 oracle."""
+    """With the wave-per-point walk off, either side of the sizes up to which a tree is
+    walked with a group of 2^D lanes per point (12,288 points in 3D, 24,576 in 2D, 32,768 in
+    1D, with 1D's largest workgroups in LDS at 20,000): that walk equals the tiled
+    exact-mode walk bit for bit, has no walk errors, and matches the FP64 oracle."""
     pos = make_nodes_fast(n, dims, 20 * n ** (1 / dims), seed=n)
     s = np.full(n, -30.0) * (1 + 0.25 * np.sin(np.arange(n)))
     out = []
+    monkeypatch.setenv("D3FGPU_WALK_WIDE", "0")
     for solo in (None, "0"):
         if solo is None:
             monkeypatch.delenv("D3FGPU_WALK_SOLO", raising=False)
         else:
             monkeypatch.setenv("D3FGPU_WALK_SOLO", solo)
-        e = d3fgpu.Engine(0)
-        try:
-            _, dv = gpu_dv(e, pos, dims, [d3fgpu.many_body(s)])
-            assert e.stats().walk_errors == 0
-        finally:
-            e.close()
-        out.append(dv)
+        out.append(walk_dv(pos, dims, s))
     assert np.array_equal(out[0], out[1])
     ref, _ = oracle_manybody(pos, dims, s)
     err = rel_l2(out[0], ref)
     report(f"manyBody tree {dims}D n={n} (lane-group walk switch)", err)
     assert err < 1e-4
     assert close_count(out[0], ref, rtol=1e-3) <= tree_outliers(n)
+
+
+@pytest.mark.parametrize("dims,n", [(3, 163840), (3, 163841), (2, 163840), (1, 163841)])
+def test_manybody_wave_per_point_walk_switch_point(monkeypatch, dims, n):
+    """Either side of 163,840 points, up to which a tree is walked one point per wave
+    (several openings per step): the default walk has no walk errors, is within rounding of
+    the tiled walk on the same tree (the wave-per-point walk sums the same terms in another
+    order), and matches the FP64 oracle."""
+    pos = make_nodes_fast(n, dims, 20 * n ** (1 / dims), seed=n + dims)
+    s = np.full(n, -30.0) * (1 + 0.25 * np.cos(np.arange(n)))
+    monkeypatch.delenv("D3FGPU_WALK_WIDE", raising=False)
+    dflt = walk_dv(pos, dims, s)
+    monkeypatch.setenv("D3FGPU_WALK_WIDE", "0")
+    tiled = walk_dv(pos, dims, s)
+    if n <= 163840:
+        assert 0 < rel_l2(dflt, tiled) < 1e-6
+    else:
+        assert np.array_equal(dflt, tiled)
+    ref, _ = oracle_manybody(pos, dims, s)
+    err = rel_l2(dflt, ref)
+    report(f"manyBody tree {dims}D n={n} (wave-per-point walk switch)", err)
+    assert err < 1e-4
+    assert close_count(dflt, ref, rtol=1e-3) <= tree_outliers(n)
 
 
 def test_manybody_tree_above_exact_mode(engine_factory):
