@@ -75,6 +75,7 @@ struct TreeBuffers {
   // into `counters`)
   uint32_t* slots = nullptr;
   uint32_t slot_blocks = 0;
+  uint8_t* cut_start = nullptr;  // per sorted point: 1 where a subtree-aligned reduction workgroup's subtree starts
   uint32_t* counters = nullptr; // TREE_COUNTERS words: u64 interactions [0,1], opened [2,3], walk errors [4], tile queue [5], lane visits [6,7], deepest stack [8], long runs [10]
   uint32_t* long_runs = nullptr;  // first positions of runs of equal high halves too long to order in place
   void* spill = nullptr;        // the walk's stack entries beyond LDS, per resident wave
@@ -99,7 +100,8 @@ struct TreeParams {
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
   // trees of at most this many points walk one point per wave (0: never)
   uint32_t walk_wide_max = WALK_WIDE_BY_DIMS;
-  uint32_t reduce_block = 0;  // points per workgroup of the reduction (512 or 1024; 0: by size)
+  uint32_t reduce_block = 0;
+  bool reduce_aligned = true;  // reduction workgroups aligned to subtrees  // points per workgroup of the reduction (512 or 1024; 0: by size)
 };
 
 // Bytes of walk spill area for `cus` CUs at an LDS stack of `stack` entries.

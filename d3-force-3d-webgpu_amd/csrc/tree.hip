@@ -548,16 +548,25 @@ constexpr uint32_t PARENT_LOCAL = 0x80000000u;  // in a parent reference: the pa
 template <int D>
 __device__ inline void karras_write(const uint64_t* __restrict__ skeys, int N, int i, uint32_t off,
                                     uint32_t block, uint4* __restrict__ links,
-                                    uint32_t* __restrict__ leaf_parent) {
+                                    uint32_t* __restrict__ leaf_parent, uint8_t* __restrict__ cut_start = nullptr) {
   int first, last, split, prefix;
   karras<D>(skeys, N, i, first, last, split, prefix);
+  // cut_start (subtree-aligned workgroups of the reduction): a node is local when it covers
+  // at most `block` points, and point split + 1 starts a maximal such subtree exactly when
+  // this node, which parts points split and split + 1, covers more (every gap between two
+  // sorted points is parted by one node, so every flag but point 0's is written once)
+  if (cut_start) {
+    cut_start[split + 1] = (uint8_t)(last - first + 1 > (int)block);
+    if (i == 0) cut_start[0] = 1;
+  }
   const uint32_t node = off + (uint32_t)i, a = off + (uint32_t)split, b = a + 1u;
   uint32_t left = a | (first == split ? REF_POINT : 0u);
   uint32_t right = b | (last == split + 1 ? REF_POINT : 0u);
   links[node].x = left;
   links[node].y = right;
   links[node].z = depth_of<D>(prefix);
-  const bool local = (off + (uint32_t)first) / block == (off + (uint32_t)last) / block;
+  const bool local = cut_start ? last - first + 1 <= (int)block
+                               : (off + (uint32_t)first) / block == (off + (uint32_t)last) / block;
   const uint32_t up = node | (local ? PARENT_LOCAL : 0u);
   if (left & REF_POINT) leaf_parent[a] = up;
   else links[a].w = up;
@@ -578,7 +587,8 @@ __global__ void __launch_bounds__(256) tree_leaves_build(const uint64_t* __restr
                                                          uint32_t* __restrict__ leaf_depth,
                                                          uint32_t* __restrict__ flags,
                                                          uint4* __restrict__ links,
-                                                         uint32_t* __restrict__ leaf_parent) {
+                                                         uint32_t* __restrict__ leaf_parent,
+                                                         uint8_t* __restrict__ cut_start) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
   leaves[s] = pts[order[s]];
@@ -590,7 +600,7 @@ __global__ void __launch_bounds__(256) tree_leaves_build(const uint64_t* __restr
   leaf_depth[s] = depth;
   if (s + 1 < n) {
     flags[s] = 0;
-    karras_write<D>(skeys, (int)n, (int)s, 0u, block, links, leaf_parent);
+    karras_write<D>(skeys, (int)n, (int)s, 0u, block, links, leaf_parent, cut_start);
   }
 }
 
@@ -1009,21 +1019,43 @@ __global__ void __launch_bounds__(REDUCE_ONE) tree_small(const float* __restrict
 // COLLAPSE (one tree): thread s also lists d3's children of binary node s once its climb
 // is over (the topology is complete before the kernel starts; the records' children are
 // not what the climb writes), which saves the collapse pass its launch.
+//
+// cut_start (subtree-aligned workgroups): workgroup b takes the maximal subtrees of at most
+// W = B / 2 points that start among points [b W, (b + 1) W) -- fewer than 2W points, from
+// the first such start to the next start at or after (b + 1) W -- so that only nodes of
+// more than W points are shared, and the chain of device-wide hand-offs is the depth of
+// those (8 levels against 16 at 5,000 points, 20 against 30 at 1M).  Otherwise workgroup b
+// takes points [b B, (b + 1) B).
 template <int D, uint32_t B, bool COLLAPSE>
 __global__ void __launch_bounds__(B) tree_reduce(uint32_t n, const float4* __restrict__ leaves,
                                                             const uint32_t* __restrict__ leaf_parent,
                                                             const uint4* __restrict__ links,
-                                                            uint32_t* flags, NodeRec* rec, NodeSum* sums) {
+                                                            uint32_t* flags, NodeRec* rec, NodeSum* sums,
+                                                            const uint8_t* __restrict__ cut_start = nullptr) {
   __shared__ ReduceLDS<B> S;
-  const uint32_t s = blockIdx.x * B + threadIdx.x;
-  const uint32_t lo = blockIdx.x * B;
+  __shared__ uint32_t s_span[2];
+  uint32_t lo = blockIdx.x * B, count = B;
+  if (cut_start) {
+    constexpr uint32_t W = B / 2;
+    const uint32_t w0 = blockIdx.x * W, i = w0 + threadIdx.x, half = threadIdx.x < W ? 0u : 1u;
+    if (threadIdx.x < 2) s_span[threadIdx.x] = 0xffffffffu;
+    __syncthreads();
+    // the first start in each half: one LDS atomic per wave (a wave lies in one half)
+    const uint64_t starts = __ballot(i < n && cut_start[i] != 0);
+    if (starts && (threadIdx.x & (WAVE - 1)) == 0) atomicMin(&s_span[half], w0 + (threadIdx.x & ~(WAVE - 1)) + (uint32_t)__ffsll((long long)starts) - 1u);
+    __syncthreads();
+    lo = s_span[0];
+    const uint32_t end = s_span[1] < n ? s_span[1] : n;
+    count = lo < end ? end - lo : 0u;  // none: this window only continues the previous one's subtree
+  }
+  const uint32_t s = lo + threadIdx.x;
   S.flag[threadIdx.x] = 0;
-  if (s < n) {
+  if (threadIdx.x < count && s < n) {
     S.leaf[threadIdx.x] = leaves[s];
     S.link[threadIdx.x] = links[s];  // index s is a node or a tree's last point (never read then)
   }
   __syncthreads();
-  if (s >= n || n < 2) return;
+  if (threadIdx.x >= count || s >= n || n < 2) return;
   reduce_climb<D, B>(s, n, lo, S, leaves, leaf_parent, links, flags, rec, sums);
   if (COLLAPSE && s + 1 < n) {
     uint32_t kids[8];
@@ -2468,18 +2500,22 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   // (1.884 -> 1.920)
   const uint32_t rb = p.reduce_block == 0 ? (n <= 32768u ? REDUCE_ONE : REDUCE_BLOCK)
                       : p.reduce_block == REDUCE_ONE ? REDUCE_ONE : REDUCE_BLOCK;
+  // subtree-aligned workgroups (windows of rb / 2 points) unless turned off
+  const bool aligned = p.reduce_aligned;
+  const uint32_t window = aligned ? rb / 2 : rb;
   hipLaunchKernelGGL((tree_leaves_build<D>), dim3(blocks), dim3(256), 0, st, (const uint64_t*)sk,
-                     (const uint32_t*)sv, (const float4*)t.pts, n, rb, t.leaves, t.leaf_depth, t.flags,
-                     t.links, t.leaf_parent);
+                     (const uint32_t*)sv, (const float4*)t.pts, n, window, t.leaves, t.leaf_depth, t.flags,
+                     t.links, t.leaf_parent, aligned ? t.cut_start : nullptr);
   FG_LAUNCH_CHECK();
+  const uint8_t* cuts = aligned ? (const uint8_t*)t.cut_start : nullptr;
   if (rb == REDUCE_ONE)
-    hipLaunchKernelGGL((tree_reduce<D, REDUCE_ONE, true>), dim3(ceil_div(n, REDUCE_ONE)), dim3(REDUCE_ONE), 0, st,
+    hipLaunchKernelGGL((tree_reduce<D, REDUCE_ONE, true>), dim3(ceil_div(n, window)), dim3(REDUCE_ONE), 0, st,
                        n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent, (const uint4*)t.links, t.flags,
-                       t.rec, t.sums);
+                       t.rec, t.sums, cuts);
   else
-    hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK, true>), dim3(ceil_div(n, REDUCE_BLOCK)), dim3(REDUCE_BLOCK), 0,
+    hipLaunchKernelGGL((tree_reduce<D, REDUCE_BLOCK, true>), dim3(ceil_div(n, window)), dim3(REDUCE_BLOCK), 0,
                        st, n, (const float4*)t.leaves, (const uint32_t*)t.leaf_parent,
-                       (const uint4*)t.links, t.flags, t.rec, t.sums);
+                       (const uint4*)t.links, t.flags, t.rec, t.sums, cuts);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1);
   walk<D>(st, t, n, vx, vy, vz, p, step, timer);
@@ -2511,6 +2547,7 @@ void TreeBuffers::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&cube, sizeof(CubeInfo)));
   FG_HIP(hipMalloc(&counters, sizeof(uint32_t) * TREE_COUNTERS));
   FG_HIP(hipMalloc(&slots, sizeof(uint32_t) * 8 * WALK_SLOTS));
+  FG_HIP(hipMalloc(&cut_start, m));
   FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   CubeInfo c0 = {};
   for (int a = 0; a < 3; ++a) c0.box[a] = 0xffffffffu, c0.box[3 + a] = 0u;
@@ -2533,13 +2570,15 @@ void TreeBuffers::release() {
   if (spill) (void)hipFree(spill);
   spill = nullptr;
   spill_bytes = 0;
-  void* ptrs[] = {keys, vals, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, cube, counters, slots, long_runs};
+  void* ptrs[] = {keys, vals, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, cube, counters, slots,
+                  cut_start, long_runs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = nullptr;
   vals = nullptr;
   leaves = pts = nullptr;
   leaf_depth = leaf_parent = flags = counters = slots = long_runs = nullptr;
+  cut_start = nullptr;
   slot_blocks = 0;
   links = nullptr;
   rec = nullptr;
