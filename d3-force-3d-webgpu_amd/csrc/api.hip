@@ -262,7 +262,9 @@ void update_pass(fg_ctx* c, Pass& p, const fg_force_desc& d) {
 // Enqueues one force pass.  defer_center: a center pass computes its shift only and
 // returns its work area, for the integrator to apply the shift (it must be the tick's last
 // pass); otherwise null.
-const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center = false) {
+// fuse: for a many-body pass, the center pass whose mean its tree's box kernel takes; for
+// that center pass itself, its own address (its shift is then ready).
+const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center = false, const Pass* fuse = nullptr) {
   const uint32_t n = c->n;
   const int nd = c->ndim;
   float* pos[3] = {c->P(0), c->P(1), c->P(2)};
@@ -283,8 +285,10 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
         if (!p.tree) p.tree.reset(new TreeBuffers());
         TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2, c->walk_packed_max, c->walk_tile,
                       c->walk_solo_max, c->walk_wide_max, c->reduce_block, c->reduce_aligned};
+        CenterFuse cf;
+        if (fuse) cf = CenterFuse{fuse->center.as<CenterWork>(), fuse->d.x, fuse->d.y, fuse->d.z, fuse->d.strength};
         tree_manybody(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), vel[0],
-                      vel[1], vel[2], tp, sp, c->radix_bits, c->T());
+                      vel[1], vel[2], tp, sp, c->radix_bits, c->T(), fuse ? &cf : nullptr);
       }
       break;
     }
@@ -320,12 +324,13 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
     }
     case FG_FORCE_CENTER: {
       PhaseScope ps(c->T(), FG_PHASE_CENTER);
-      if (defer_center && n <= CENTER_FUSE_MAX) {  // the integrator's launch does it all
+      const bool ready = fuse == &p;  // the many-body box kernel took the mean
+      if (defer_center && n <= CENTER_FUSE_MAX && !ready) {  // the integrator's launch does it all
         c->center_fused = &p;
         return nullptr;
       }
       launch_center(c->stream, n, nd, pos, p.center.as<CenterWork>(), p.d.x, p.d.y, p.d.z, p.d.strength,
-                    !defer_center);
+                    !defer_center, !ready);
       if (defer_center) return p.center.as<CenterWork>();
       break;
     }
@@ -702,8 +707,23 @@ int fg_enqueue(fg_ctx* c, uint32_t steps, const double* alphas, double decay) {
       // a center pass last in the list leaves its shift to the integrator (one launch fewer)
       const CenterWork* shift = nullptr;
       c->center_fused = nullptr;
+      // the first center pass, when a many-body pass before it builds a tree with the box
+      // kernel: that kernel reads the same positions (only center moves nodes during the
+      // forces) and takes the center's mean on its way, one launch fewer
+      int fuse_mb = -1, fuse_ct = -1;
+      for (size_t i = 0; i < c->passes.size(); ++i) {
+        const Pass& q = *c->passes[i];
+        if (q.type == FG_FORCE_CENTER) {
+          if (fuse_mb >= 0) fuse_ct = (int)i;
+          break;
+        }
+        if (q.type == FG_FORCE_MANYBODY && fuse_mb < 0 && q.d.theta2 > EXACT_THETA2 && tree_box_pass(c->n))
+          fuse_mb = (int)i;
+      }
+      const Pass* fused = fuse_ct >= 0 ? c->passes[fuse_ct].get() : nullptr;
       for (size_t i = 0; i < c->passes.size(); ++i)
-        shift = run_pass(c, *c->passes[i], sp, i + 1 == c->passes.size());
+        shift = run_pass(c, *c->passes[i], sp, i + 1 == c->passes.size(),
+                         (int)i == fuse_mb || (int)i == fuse_ct ? fused : nullptr);
       float* pos[3] = {c->P(0), c->P(1), c->P(2)};
       float* vel[3] = {c->V(0), c->V(1), c->V(2)};
       float* fix[3] = {c->fix[0].as<float>(), c->fix[1].as<float>(), c->fix[2].as<float>()};

@@ -25,6 +25,17 @@ struct CenterWork {
   uint32_t done;
 };
 
+// Workgroups of center_reduce, at most: the partial sums it folds in workgroup order.
+constexpr uint32_t CENTER_BLOCKS = 256;
+
+// A center force whose mean another kernel takes on its way (the many-body tree's box
+// kernel reads every position at the same point of the tick), with center_reduce's
+// arithmetic: the shift lands in w->shift, bit for bit what center_reduce would leave.
+struct CenterFuse {
+  CenterWork* w = nullptr;
+  double cx = 0, cy = 0, cz = 0, strength = 1;
+};
+
 // The collide grid's two classes of points (one when the radii are even): points up to
 // r0 live in cells 2 r0 wide, larger ones in cells 2 rmax wide (inv_* = 1 / cell width).
 struct GridClasses {
@@ -85,7 +96,7 @@ void launch_center_integrate(hipStream_t s, uint32_t n, int ndim, float* const* 
                              const uint8_t* mask, const float* const* fix, StepParams step, double cx, double cy,
                              double cz, double strength);
 void launch_center(hipStream_t s, uint32_t n, int ndim, float* const* pos, CenterWork* w, double cx,
-                   double cy, double cz, double strength, bool apply = true);
+                   double cy, double cz, double strength, bool apply = true, bool reduce = true);
 void launch_radial(hipStream_t s, uint32_t n, int ndim, float* const* pos, float* const* vel,
                    const float* radius, const float* weight, float cx, float cy, float cz,
                    StepParams step);

@@ -295,8 +295,6 @@ void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const f
 
 // ---- center -------------------------------------------------------------------------
 
-constexpr uint32_t CENTER_BLOCKS = 256;
-
 __global__ void __launch_bounds__(256) center_reduce(uint32_t n, int ndim, const float* __restrict__ x,
                                                      const float* __restrict__ y,
                                                      const float* __restrict__ z,
@@ -342,7 +340,7 @@ __global__ void __launch_bounds__(256) center_reduce(uint32_t n, int ndim, const
   __syncthreads();
   if (threadIdx.x != 0) return;
   for (int a = 0; a < 3; ++a) t[a] = 0;
-  for (int k = 0; k < 256; ++k)
+  for (uint32_t k = 0; k < gridDim.x && k < 256; ++k)  // the rest are +0: no change
     for (int a = 0; a < 3; ++a) t[a] += tsum[k][a];
   double c[3] = {cx, cy, cz};
   for (int a = 0; a < 3; ++a) w->shift[a] = a < ndim ? (float)((t[a] / n - c[a]) * strength) : 0.0f;
@@ -361,14 +359,16 @@ __global__ void __launch_bounds__(256) center_apply(uint32_t n, int ndim, float*
 }
 
 void launch_center(hipStream_t s, uint32_t n, int ndim, float* const* pos, CenterWork* w, double cx,
-                   double cy, double cz, double strength, bool apply) {
+                   double cy, double cz, double strength, bool apply, bool reduce) {
   if (!n) return;
   uint32_t blocks = ceil_div(n, 256);
   // few workgroups: each ends on one atomic on a shared counter, and those serialise
   uint32_t rb = blocks < CENTER_BLOCKS ? blocks : CENTER_BLOCKS;
-  hipLaunchKernelGGL(center_reduce, dim3(rb), dim3(256), 0, s, n, ndim, pos[0], pos[1], pos[2], w,
-                     cx, cy, cz, strength);
-  FG_LAUNCH_CHECK();
+  if (reduce) {  // else the shift is already in w (another kernel took the mean)
+    hipLaunchKernelGGL(center_reduce, dim3(rb), dim3(256), 0, s, n, ndim, pos[0], pos[1], pos[2], w,
+                       cx, cy, cz, strength);
+    FG_LAUNCH_CHECK();
+  }
   if (!apply) return;  // the integrator applies the shift
   hipLaunchKernelGGL(center_apply, dim3(blocks), dim3(256), 0, s, n, ndim, pos[0], pos[1], pos[2],
                      (const CenterWork*)w);

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "common.h"
+#include "forces.h"
 #include "sort.h"
 
 namespace fg {
@@ -111,9 +112,15 @@ size_t spill_bytes_for(int cus, int stack);
 int device_cus();
 
 // Build the tree over the current positions and add the many-body force to the velocities.
+// center (optional): a center force that runs after this one in the tick, with no force
+// between them that moves the nodes; its mean is taken by the box kernel (when there is one:
+// tree_box_pass) and its shift left in center->w.
 void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x,
                    const float* y, const float* z, const float* strength, float* vx, float* vy,
                    float* vz, const TreeParams& p, StepParams step, int radix_bits,
-                   PhaseTimer* timer = nullptr);
+                   PhaseTimer* timer = nullptr, const CenterFuse* center = nullptr);
+
+// Whether the tree of n points is built with the box kernel (larger than one workgroup's).
+bool tree_box_pass(uint32_t n);
 
 }  // namespace fg

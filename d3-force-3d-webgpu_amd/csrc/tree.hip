@@ -161,21 +161,32 @@ __device__ inline void cover_box(uint32_t* box, float origin[3], float& size, in
 // Also clears, in its first workgroup, the tree's counters and the high-half sort's
 // histograms (work the tick's later kernels need cleared, here without launches of their
 // own).
+//
+// center.w (optional): also the sums of a center force's mean, with center_reduce's
+// arithmetic (the same grid, the same points per thread in the same order, the same adds
+// over waves, workgroups and, in the last workgroup, the partials), so the shift it leaves
+// is bit for bit center_reduce's.
 template <int D>
 __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
                                                  const float* __restrict__ y,
                                                  const float* __restrict__ z, uint32_t n,
                                                  CubeInfo* __restrict__ cube, uint32_t* __restrict__ counters,
-                                                 SortClear sort_clear) {
+                                                 SortClear sort_clear, CenterFuse center) {
   if (blockIdx.x == 0) {
     if (threadIdx.x < TREE_COUNTERS) counters[threadIdx.x] = 0;
     sort_clear_block(sort_clear);
   }
+  CenterWork* const cw = center.w;
   float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  double sum[3] = {0, 0, 0};
   auto take = [&](uint32_t i) {
     const float p[3] = {x[i], D > 1 ? y[i] : 0.0f, D > 2 ? z[i] : 0.0f};
 #pragma unroll
     for (int a = 0; a < 3; ++a) lo[a] = fminf(lo[a], p[a]), hi[a] = fmaxf(hi[a], p[a]);
+    if (cw) {
+#pragma unroll
+      for (int a = 0; a < D; ++a) sum[a] += (double)p[a];
+    }
   };
   const uint32_t stride = gridDim.x * 256;
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -194,9 +205,24 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
     store_coherent(&cube->part[blockIdx.x][threadIdx.x], omin);
     store_coherent(&cube->part[blockIdx.x][3 + threadIdx.x], omax);
   }
+  __shared__ double s_sum[4][3];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  if (cw) {  // center_reduce's sums of this workgroup
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) sum[a] += __shfl_xor(sum[a], o, WAVE);
+    if (lane == 0)
+      for (int a = 0; a < 3; ++a) s_sum[wave][a] = sum[a];
+  }
   __shared__ uint32_t s_last;
   __syncthreads();
-  if (threadIdx.x == 0) s_last = publish_count(&cube->done) + 1 == gridDim.x;
+  if (threadIdx.x == 0) {
+    if (cw)
+      for (int a = 0; a < 3; ++a)
+        store_coherent(&cw->partial[blockIdx.x][a], s_sum[0][a] + s_sum[1][a] + s_sum[2][a] + s_sum[3][a]);
+    s_last = publish_count(&cube->done) + 1 == gridDim.x;
+  }
   __syncthreads();
   if (!s_last) return;
   uint32_t m[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
@@ -214,24 +240,38 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
       m[3 + a] = max(m[3 + a], (uint32_t)__shfl_xor((int)m[3 + a], o, WAVE));
     }
   __shared__ uint32_t s_m[4][6];
-  const uint32_t wave = threadIdx.x / WAVE;
   if ((threadIdx.x & (WAVE - 1)) == 0)
     for (int a = 0; a < 6; ++a) s_m[wave][a] = m[a];
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  // the last block turns the box into the cube
-  for (int a = 0; a < 3; ++a) {
-    cube->box[a] = min(min(s_m[0][a], s_m[1][a]), min(s_m[2][a], s_m[3][a]));
-    cube->box[3 + a] = max(max(s_m[0][3 + a], s_m[1][3 + a]), max(s_m[2][3 + a], s_m[3][3 + a]));
-  }
-  float origin[3], size;
-  int32_t shift;
-  cover_box<D>(cube->box, origin, size, shift);
+  if (threadIdx.x == 0) {  // the last block turns the box into the cube
+    for (int a = 0; a < 3; ++a) {
+      cube->box[a] = min(min(s_m[0][a], s_m[1][a]), min(s_m[2][a], s_m[3][a]));
+      cube->box[3 + a] = max(max(s_m[0][3 + a], s_m[1][3 + a]), max(s_m[2][3 + a], s_m[3][3 + a]));
+    }
+    float origin[3], size;
+    int32_t shift;
+    cover_box<D>(cube->box, origin, size, shift);
 #pragma unroll
-  for (int a = 0; a < 3; ++a) cube->origin[a] = origin[a];
-  cube->size = size;
-  cube->shift = shift;
-  cube->done = 0;
+    for (int a = 0; a < 3; ++a) cube->origin[a] = origin[a];
+    cube->size = size;
+    cube->shift = shift;
+    cube->done = 0;
+  }
+  if (cw) {  // then center_reduce's fold of the partials, in workgroup order
+    __shared__ double tsum[256][3];
+    double t[3] = {0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256)
+      for (int a = 0; a < 3; ++a) t[a] += load_coherent(&cw->partial[b][a]);
+    for (int a = 0; a < 3; ++a) tsum[threadIdx.x][a] = t[a];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int a = 0; a < 3; ++a) t[a] = 0;
+      for (uint32_t k = 0; k < gridDim.x && k < 256; ++k)  // the rest are +0: no change
+        for (int a = 0; a < 3; ++a) t[a] += tsum[k][a];
+      const double c[3] = {center.cx, center.cy, center.cz};
+      for (int a = 0; a < 3; ++a) cw->shift[a] = a < D ? (float)((t[a] / n - c[a]) * center.strength) : 0.0f;
+    }
+  }
 }
 
 // ---- keys ---------------------------------------------------------------------------
@@ -1896,16 +1936,16 @@ __device__ __forceinline__ void wide_point(WideLDS<D>& L, uint32_t* sk, uint32_t
       if (lane < NK) L.kids[0][lane] = r.kids[lane];
       sp = 1;
     }
+    sp = (uint32_t)__builtin_amdgcn_readfirstlane(sp);  // every lane decided the root alike
   }
-  for (uint32_t guard = 0; sp > 0u; ++guard) {  // sp is the same on every lane
+  for (uint32_t guard = 0; sp > 0u; ++guard) {  // sp is the same on every lane (kept in SGPRs)
     if (guard > guard_max) {
       err = 1;
       break;
     }
-    // entries to open: P, or fewer if the stack has fewer or is past WIDE_BROAD
-    uint32_t p = sp < P ? sp : P;
-    const uint32_t room = sp < WIDE_BROAD ? (WIDE_BROAD - sp) / (NK - 1) : 0u;
-    if (p > room) p = room > 0u ? room : 1u;
+    // entries to open: P (or all the stack has) while P entries' children fit within
+    // WIDE_BROAD, one beyond
+    const uint32_t p = sp + (NK - 1) * P <= WIDE_BROAD ? (sp < P ? sp : P) : 1u;
     const uint32_t base = sp - p;  // entries [base, sp) open now; their children go from base up
     opened += p;
     if (sp > WIDE_LDS) wait_all_memory();  // the wave's own spill stores, complete before these loads
@@ -1921,8 +1961,8 @@ __device__ __forceinline__ void wide_point(WideLDS<D>& L, uint32_t* sk, uint32_t
       }
     }
     bool open = false;
-    uint4 k0 = make_uint4(0u, 0u, 0u, 0u), k1 = k0;
-    uint32_t cdepth = 0;
+    uint4 k0, k1;     // set, with cdepth, wherever open is
+    uint32_t cdepth;
     if (k != REF_NONE) {
       if (k & REF_POINT) {
         const uint32_t j = k & ~REF_POINT;
@@ -1946,7 +1986,7 @@ __device__ __forceinline__ void wide_point(WideLDS<D>& L, uint32_t* sk, uint32_t
     }
     // the children to open, in lane order, over the entries just opened
     const uint64_t bits = __builtin_amdgcn_ballot_w64(open);
-    const uint32_t nsp = base + (uint32_t)__popcll(bits);
+    const uint32_t nsp = (uint32_t)__builtin_amdgcn_readfirstlane(base + (uint32_t)__popcll(bits));
     if (nsp > WIDE_DEPTH) {  // no entry past the stack is written: stop here
       err = 1;
       break;
@@ -2442,7 +2482,7 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
 template <int D>
 void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float* y, const float* z,
          const float* strength, float* vx, float* vy, float* vz, const TreeParams& p,
-         StepParams step, int radix_bits, PhaseTimer* timer) {
+         StepParams step, int radix_bits, PhaseTimer* timer, const CenterFuse* center) {
   if (timer) timer->begin(1);
   if (n <= REDUCE_ONE) {  // one workgroup builds and reduces the whole tree
     hipLaunchKernelGGL((tree_small<D>), dim3(1), dim3(REDUCE_ONE), 0, st, x, y, z, strength, n, t.cube,
@@ -2457,7 +2497,9 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   const uint32_t blocks = ceil_div(n, 256);
   const SortClear sc = sort_prepare(t.sort, radix_bits, n, Dims<D>::HALF);
   const uint32_t bb = blocks < BBOX_BLOCKS ? blocks : BBOX_BLOCKS;
-  hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube, t.counters, sc);
+  static_assert(BBOX_BLOCKS == CENTER_BLOCKS, "the box kernel takes center_reduce's grid");
+  hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube, t.counters, sc,
+                     center ? *center : CenterFuse{});
   FG_LAUNCH_CHECK();
   // the counting sort's ranks and counters live in leaf_parent and flags until the build
   const bool counting = n <= RANK_MAX;
@@ -2494,11 +2536,10 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
   if (timer) timer->end(2), timer->begin(1);
   t.skeys = sk;
   t.order = sv;
-  // points per reduction workgroup: 1,024 (one fewer device-coherent level on the climb,
-  // centres cached in LDS, one workgroup per CU) measured faster up to 20,000 points (3,000:
-  // 0.119 -> 0.114 ms/tick, 8,000 0.208 -> 0.205), even at 50,000-100,000, slower at 1M
-  // (1.884 -> 1.920)
-  const uint32_t rb = p.reduce_block == 0 ? (n <= 32768u ? REDUCE_ONE : REDUCE_BLOCK)
+  // points per reduction workgroup: 1,024 (windows of 512: fewer shared nodes, centres
+  // cached in LDS, one workgroup per CU) measured faster up to 100,000 points (0.383 -> 0.376
+  // ms/tick there, 50,000 even), 512 from 200,000 (0.663 against 0.669)
+  const uint32_t rb = p.reduce_block == 0 ? (n <= 131072u ? REDUCE_ONE : REDUCE_BLOCK)
                       : p.reduce_block == REDUCE_ONE ? REDUCE_ONE : REDUCE_BLOCK;
   // subtree-aligned workgroups (windows of rb / 2 points) unless turned off
   const bool aligned = p.reduce_aligned;
@@ -2593,13 +2634,16 @@ void TreeBuffers::release() {
 void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x,
                    const float* y, const float* z, const float* strength, float* vx, float* vy,
                    float* vz, const TreeParams& p, StepParams step, int radix_bits,
-                   PhaseTimer* timer) {
+                   PhaseTimer* timer, const CenterFuse* center) {
   if (n < 1) return;
+  if (center && !tree_box_pass(n)) throw std::logic_error("tree_manybody: no box kernel to take the center's mean");
   t.reserve(n);
-  if (ndim == 1) run<1>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer);
-  else if (ndim == 2) run<2>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer);
-  else run<3>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer);
+  if (ndim == 1) run<1>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer, center);
+  else if (ndim == 2) run<2>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer, center);
+  else run<3>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer, center);
 }
+
+bool tree_box_pass(uint32_t n) { return n > REDUCE_ONE; }
 
 }  // namespace fg
 
