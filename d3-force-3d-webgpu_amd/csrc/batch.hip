@@ -449,7 +449,7 @@ void rebuild_layout(fg_batch* b) {
       if (j->tree) {
         for (uint32_t f = 0; f < j->n; f += BTILE)
           ttile.push_back(BTile{(uint32_t)tj.size(), tt + f, std::min(BTILE, j->n - f), 0});
-        if (j->n > WALK_EXACT_MAX)  // smaller trees take the wide walk, a wave per point
+        if (j->n > WALK_GROUP_MAX)  // smaller trees take the wide or the group walk
           for (uint32_t f = 0; f < j->n; f += WAVE) walk.push_back(make_uint2((uint32_t)tj.size(), tt + f));
         ts.push_back(tt);
         tj.push_back(j->slot);

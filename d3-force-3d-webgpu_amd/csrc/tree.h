@@ -13,9 +13,14 @@ namespace fg {
 constexpr uint32_t REF_POINT = 0x80000000u;  // a child reference that is a sorted point
 constexpr uint32_t TREE_COUNTERS = 16;       // words of TreeBuffers::counters
 constexpr uint32_t REF_NONE = 0xffffffffu;
-// Trees of at most this many points are walked one point per wave (the wide walk), or in
-// the tiled walk's exact mode where the wide walk is turned off; larger trees fill the
-// device with 64-point tiles and the tiled walk's cheaper accumulation.
+// The walk by tree size (measured: link + manyBody + center ticks, 3D): trees of at most
+// WALK_WIDE_MAX points walk one point per wave (the wide walk), up to WALK_GROUP_MAX two
+// points per wave (the group walk), larger ones 64-point tiles with the tiled walk's
+// cheaper accumulation.  The tiled walk's exact mode (forces independent of the tiling)
+// covers trees of at most WALK_EXACT_MAX points where the other walks are turned off.
+constexpr uint32_t WALK_WIDE_MAX = 16384;
+constexpr uint32_t WALK_GROUP_MAX = 262144;
+constexpr uint32_t WALK_GROUP = 2;  // points per wave of the group walk
 constexpr uint32_t WALK_EXACT_MAX = 163840;
 constexpr uint32_t WALK_SLOTS = 4096;         // workgroups of a walk that counts into slots, at most
 
@@ -101,8 +106,11 @@ struct TreeParams {
   uint32_t walk_solo_max = WALK_SOLO_BY_DIMS;
   // trees of at most this many points walk one point per wave (0: never)
   uint32_t walk_wide_max = WALK_WIDE_BY_DIMS;
-  uint32_t reduce_block = 0;
-  bool reduce_aligned = true;  // reduction workgroups aligned to subtrees  // points per workgroup of the reduction (512 or 1024; 0: by size)
+  // points per wave of the group walk: 2..16 (any size), 1 (never), 0 (WALK_GROUP up to
+  // WALK_GROUP_MAX points)
+  uint32_t walk_group = 0;
+  uint32_t reduce_block = 0;    // points per workgroup of the reduction (512 or 1024; 0: by size)
+  bool reduce_aligned = true;  // reduction workgroups aligned to subtrees
 };
 
 // Bytes of walk spill area for `cus` CUs at an LDS stack of `stack` entries.

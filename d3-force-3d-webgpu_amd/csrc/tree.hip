@@ -2067,6 +2067,226 @@ __global__ void __launch_bounds__(WAVES * WAVE) tree_walk_wide(const float4* __r
   walk_flush_slot(C, lane, slots + (size_t)blockIdx.x * 8);
 }
 
+// The work items [first, end) and stride of wave `wave` of a persistent walk grid, XCD by
+// XCD: workgroups are dealt to the device's 8 XCDs in turn (blockIdx.x % 8), each with an
+// L2 of its own, so the grid's workgroups on one XCD take one contiguous eighth of the
+// items (points in Morton order: neighbouring points open the same nodes, whose records
+// then stay in that XCD's L2).  Any dealing of workgroups covers every item once.
+struct WalkRange {
+  uint32_t first, end, stride;
+};
+__device__ inline WalkRange xcd_range(uint32_t items, uint32_t waves_per_block, uint32_t wave) {
+  const uint32_t parts = gridDim.x >= 8u && gridDim.x % 8u == 0u ? 8u : 1u;
+  const uint32_t x = blockIdx.x % parts, lb = blockIdx.x / parts, nbx = gridDim.x / parts;
+  const uint32_t lo = (uint32_t)((uint64_t)items * x / parts), hi = (uint32_t)((uint64_t)items * (x + 1) / parts);
+  return WalkRange{lo + lb * waves_per_block + wave, hi, nbx * waves_per_block};
+}
+
+// ---- walk of a group of points per wave ----------------------------------------------
+//
+// As the wide walk, but a wave walks a group of T consecutive sorted points at once: a
+// stack entry is a node to open together with the mask of the group's points that must
+// open it, and lane (e, c) loads child c of the e-th entry once and decides it for each
+// point of that mask in turn (its terms into that point's sums on the lane).  The record
+// loads and the stack's pops and pushes are shared by the group, the lanes stay as full as
+// in the wide walk, and a point's force is its lanes' sums added in lane 0's fixed tree.
+// The root opens as the single child of an entry above it (depth ~0u: its chain starts at
+// depth 0).
+template <int D, uint32_t T>
+struct GroupLDS {
+  static constexpr uint32_t NK = 1u << D;
+  alignas(16) uint32_t kids[WIDE_LDS][NK];
+  uint32_t depth[WIDE_LDS];
+  uint32_t mask[WIDE_LDS];
+};
+
+// Words of spill area per wave of the group walk: children, depth and mask of each entry
+// past the LDS part.
+template <int D>
+constexpr uint32_t group_spill_words() {
+  return (WIDE_DEPTH - WIDE_LDS) * ((1u << D) + 2u);
+}
+
+template <int D, uint32_t T>
+__device__ __forceinline__ void group_walk(GroupLDS<D, T>& L, uint32_t* sk, const float4* __restrict__ leaves,
+                                           const uint32_t* __restrict__ order,
+                                           const NodeRec* __restrict__ rec, const WalkTree& T_, uint32_t s0,
+                                           uint32_t count, uint32_t lane, uint32_t& inter, uint64_t& opened,
+                                           uint64_t& visits, uint32_t& max_sp, uint32_t& err) {
+  constexpr uint32_t NK = 1u << D, P = WAVE / NK, SPW = WIDE_DEPTH - WIDE_LDS;
+  uint32_t* sd = sk + SPW * NK;  // spilled depths, then masks
+  uint32_t* sm = sd + SPW;
+  const uint32_t e = lane / NK, c = lane % NK;
+  const uint32_t off = T_.off, n = T_.n;
+  const double size = T_.size, theta2 = T_.theta2, dmax2 = T_.dmax2;
+  const float dmin2f = T_.dmin2f, dmax2f = T_.dmax2f, alpha = T_.alpha;
+  const uint32_t step_no = T_.step_no, seed = T_.seed;
+  const uint32_t guard_max = 2u * n + 8u;
+  float4 me[T];  // the group's points, the same on every lane (SGPRs)
+  uint32_t self[T];
+#pragma unroll
+  for (uint32_t b = 0; b < T; ++b) {
+    const float4 q = b < count ? leaves[s0 + b] : make_float4(0.f, 0.f, 0.f, 0.f);
+    me[b].x = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q.x)));
+    me[b].y = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q.y)));
+    me[b].z = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q.z)));
+    me[b].w = 0.f;
+    self[b] = (uint32_t)__builtin_amdgcn_readfirstlane(b < count ? order[s0 + b] - off : 0u);
+  }
+  float ax[T], ay[T], az[T];
+#pragma unroll
+  for (uint32_t b = 0; b < T; ++b) ax[b] = ay[b] = az[b] = 0.f;
+  // the entry above the root
+  if (lane < NK) L.kids[0][lane] = lane == 0 ? off : REF_NONE;
+  if (lane == 0) L.depth[0] = ~0u, L.mask[0] = count >= 32u ? ~0u : (1u << count) - 1u;
+  uint32_t sp = 1;
+  for (uint32_t guard = 0; sp > 0u; ++guard) {  // sp is the same on every lane (kept in SGPRs)
+    if (guard > guard_max) {
+      err = 1;
+      break;
+    }
+    const uint32_t p = sp + (NK - 1) * P <= WIDE_BROAD ? (sp < P ? sp : P) : 1u;
+    const uint32_t base = sp - p;
+    opened += p;
+    if (sp > WIDE_LDS) wait_all_memory();  // the wave's own spill stores, complete before these loads
+    uint32_t k = REF_NONE, top = 0, msk = 0;
+    if (e < p) {
+      const uint32_t ent = sp - 1u - e;
+      if (ent < WIDE_LDS) {
+        k = L.kids[ent][c];
+        top = L.depth[ent] + 1u;
+        msk = L.mask[ent];
+      } else {
+        k = load_coherent(&sk[(ent - WIDE_LDS) * NK + c]);
+        top = load_coherent(&sd[ent - WIDE_LDS]) + 1u;
+        msk = load_coherent(&sm[ent - WIDE_LDS]);
+      }
+    }
+    uint32_t omask = 0;  // the points that must open this lane's child
+    uint4 k0, k1;        // set, with cdepth, wherever omask is
+    uint32_t cdepth;
+    if (k != REF_NONE) {
+      if (c == 0) visits += (uint32_t)__popc(msk);
+      if (k & REF_POINT) {
+        const uint32_t j = k & ~REF_POINT;
+        const float4 q = leaves[j];
+#pragma unroll
+        for (uint32_t b = 0; b < T; ++b)
+          if (((msk >> b) & 1u) && j != s0 + b)
+            point_force<D, ACC_FUSED>(me[b].x, me[b].y, me[b].z, self[b], q, order, j, off, dmax2f, dmin2f, alpha,
+                                      step_no, seed, ax[b], ay[b], az[b], inter);
+      } else {
+        const uint4* R = (const uint4*)(rec + k);
+        const uint4 q0 = R[0], q1 = R[1];
+        k0 = R[2];
+        k1 = R[3];
+        NodeHdr h;
+        __builtin_memcpy(&h, &q0, 16);
+        __builtin_memcpy((char*)&h + 16, &q1, 16);
+        const double wt = ldexp(size, -(int)top);
+        const double w2 = wt * wt;
+#pragma unroll
+        for (uint32_t b = 0; b < T; ++b)
+          if ((msk >> b) & 1u)
+            if (!cell_force<D, ACC_FUSED>(me[b].x, me[b].y, me[b].z, self[b], n, k - off, top, w2, h, theta2, dmax2,
+                                          dmin2f, alpha, step_no, seed, ax[b], ay[b], az[b], inter))
+              omask |= 1u << b;
+        cdepth = h.depth;
+      }
+    }
+    const bool open = omask != 0u;
+    const uint64_t bits = __builtin_amdgcn_ballot_w64(open);
+    const uint32_t nsp = (uint32_t)__builtin_amdgcn_readfirstlane(base + (uint32_t)__popcll(bits));
+    if (nsp > WIDE_DEPTH) {  // no entry past the stack is written: stop here
+      err = 1;
+      break;
+    }
+    if (open) {
+      const uint32_t slot = base + mask_rank(bits);
+      const uint32_t kk[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+      if (slot < WIDE_LDS) {
+        L.depth[slot] = cdepth;
+        L.mask[slot] = omask;
+        uint32_t* en = L.kids[slot];
+        if (NK == 8) {
+          *(uint4*)en = k0;
+          *(uint4*)(en + 4) = k1;
+        } else if (NK == 4) {
+          *(uint4*)en = k0;
+        } else {
+          *(uint2*)en = make_uint2(k0.x, k0.y);
+        }
+      } else {
+#pragma unroll
+        for (uint32_t q = 0; q < NK; ++q) store_coherent(&sk[(slot - WIDE_LDS) * NK + q], kk[q]);
+        store_coherent(&sd[slot - WIDE_LDS], cdepth);
+        store_coherent(&sm[slot - WIDE_LDS], omask);
+      }
+    }
+    sp = nsp;
+    max_sp = sp > max_sp ? sp : max_sp;
+  }
+  // each point's force: its lanes' sums added in lane 0's fixed tree
+#pragma unroll
+  for (uint32_t b = 0; b < T; ++b) {
+    float sx = ax[b], sy = ay[b], sz = az[b];
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      sx += __shfl_xor(sx, o, WAVE);
+      if (D > 1) sy += __shfl_xor(sy, o, WAVE);
+      if (D > 2) sz += __shfl_xor(sz, o, WAVE);
+    }
+    if (lane == 0 && b < count && !err) {
+      T_.vx[self[b]] += sx;
+      if (D > 1) T_.vy[self[b]] += sy;
+      if (D > 2) T_.vz[self[b]] += sz;
+    }
+  }
+}
+
+template <int D, uint32_t T, uint32_t WAVES>
+__global__ void __launch_bounds__(WAVES * WAVE) tree_walk_group(const float4* __restrict__ leaves,
+                                                        const uint32_t* __restrict__ order,
+                                                        const NodeRec* __restrict__ rec,
+                                                        const CubeInfo* __restrict__ cube, float* __restrict__ vx,
+                                                        float* __restrict__ vy, float* __restrict__ vz,
+                                                        uint32_t* __restrict__ slots, uint32_t* __restrict__ spill,
+                                                        StepParams step, WalkArgs A) {
+  __shared__ GroupLDS<D, T> lds[WAVES];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const uint32_t gw = blockIdx.x * WAVES + wave;
+  uint32_t* sk = spill + (size_t)gw * group_spill_words<D>();
+  WalkTree W;
+  W.off = 0;
+  W.n = A.n;
+  W.size = (double)cube->size;
+  W.theta2 = A.theta2;
+  W.dmax2 = A.dmax2;
+  W.dmin2f = (float)A.dmin2;
+  W.dmax2f = (float)fmin(A.dmax2, 3.0e38);
+  W.alpha = step.alpha;
+  W.step_no = step.step;
+  W.seed = step.seed;
+  W.vx = vx;
+  W.vy = vy;
+  W.vz = vz;
+  uint32_t inter = 0, max_sp = 0, err = 0;
+  uint64_t opened = 0, visits = 0;
+  const uint32_t groups = (A.n + T - 1) / T;
+  const WalkRange wr = xcd_range(groups, WAVES, wave);
+  for (uint32_t g = wr.first; g < wr.end && !err; g += wr.stride) {
+    const uint32_t s0 = g * T, count = A.n - s0 < T ? A.n - s0 : T;
+    group_walk<D, T>(lds[wave], sk, leaves, order, rec, W, s0, count, lane, inter, opened, visits, max_sp, err);
+  }
+  WalkCount C;
+  C.opened = opened;
+  C.lane_visits = visits;
+  C.inter = inter;
+  C.err = err;
+  C.max_sp = max_sp;
+  walk_flush_slot(C, lane, slots + (size_t)blockIdx.x * 8);
+}
+
 // Resident workgroups of a persistent walk kernel on this device (all CUs).
 template <typename K>
 uint32_t resident_blocks(K kern, int threads, int cus) {
@@ -2239,8 +2459,8 @@ __global__ void __launch_bounds__(WAVES * WAVE)
   walk_flush(C, lane, counters);
 }
 
-// The wide walk over the trees of a batch's jobs of at most WALK_EXACT_MAX points (the
-// larger ones take 64-point tiles in btree_walk): each wave takes tree-space points gw,
+// The wide walk over the trees of a batch's jobs of at most WALK_WIDE_MAX points (the
+// larger ones take the group walk or 64-point tiles): each wave takes tree-space points gw,
 // gw + waves, ... and walks those of active jobs as the single simulation's wide walk does,
 // the same terms in the same order.
 template <int D, uint32_t WAVES>
@@ -2258,7 +2478,7 @@ __global__ void __launch_bounds__(WAVES * WAVE)
   uint64_t opened = 0;
   for (uint32_t s = gw; s < total && !err; s += waves) {
     const BJob& B = jobs[seg_job[seg_of[s]]];
-    if (!B.active || B.n > WALK_EXACT_MAX) continue;
+    if (!B.active || B.n > WALK_WIDE_MAX) continue;
     WalkTree T;
     T.off = B.tree_off;
     T.n = B.n;
@@ -2277,6 +2497,54 @@ __global__ void __launch_bounds__(WAVES * WAVE)
   }
   WalkCount C;
   C.opened = opened;
+  C.inter = inter;
+  C.err = err;
+  C.max_sp = max_sp;
+  walk_flush_slot(C, lane, slots + (size_t)blockIdx.x * 8);
+}
+
+// The group walk over the trees of a batch's jobs of more than WALK_WIDE_MAX and at most
+// WALK_GROUP_MAX points: groups of T consecutive sorted points from each job's first, as the
+// single simulation's group walk takes them (the same terms in the same order); the waves
+// take tree-space points XCD by XCD and walk those that start a group.
+template <int D, uint32_t T, uint32_t WAVES>
+__global__ void __launch_bounds__(WAVES * WAVE)
+    btree_walk_group(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
+                     const NodeRec* __restrict__ rec, uint32_t total, const uint32_t* __restrict__ seg_of,
+                     const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
+                     uint32_t* __restrict__ slots, uint32_t* __restrict__ spill) {
+  __shared__ GroupLDS<D, T> lds[WAVES];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const uint32_t gw = blockIdx.x * WAVES + wave;
+  uint32_t* sk = spill + (size_t)gw * group_spill_words<D>();
+  uint32_t inter = 0, max_sp = 0, err = 0;
+  uint64_t opened = 0, visits = 0;
+  const WalkRange wr = xcd_range(total, WAVES, wave);
+  for (uint32_t s = wr.first; s < wr.end && !err; s += wr.stride) {
+    const BJob& B = jobs[seg_job[seg_of[s]]];
+    if (!B.active || B.n <= WALK_WIDE_MAX || B.n > WALK_GROUP_MAX) continue;
+    const uint32_t local = s - B.tree_off;
+    if (local % T != 0u) continue;  // not the first point of a group
+    WalkTree W;
+    W.off = B.tree_off;
+    W.n = B.n;
+    W.size = (double)B.cube_size;
+    W.theta2 = B.theta2;
+    W.dmax2 = B.dmax2;
+    W.dmin2f = (float)B.dmin2;
+    W.dmax2f = (float)fmin(B.dmax2, 3.0e38);
+    W.alpha = B.alpha_f;
+    W.step_no = B.step;
+    W.seed = B.seed;
+    W.vx = B.v[B.vcur][0];
+    W.vy = B.v[B.vcur][1];
+    W.vz = B.v[B.vcur][2];
+    const uint32_t count = B.n - local < T ? B.n - local : T;
+    group_walk<D, T>(lds[wave], sk, leaves, order, rec, W, s, count, lane, inter, opened, visits, max_sp, err);
+  }
+  WalkCount C;
+  C.opened = opened;
+  C.lane_visits = visits;
   C.inter = inter;
   C.err = err;
   C.max_sp = max_sp;
@@ -2304,22 +2572,31 @@ void launch_bwalk(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, co
   FG_LAUNCH_CHECK();
 }
 
-// The wide walk over the jobs of at most WALK_EXACT_MAX points (a wave per point).
+// The wide walk (jobs of at most WALK_WIDE_MAX points) and the group walk (up to
+// WALK_GROUP_MAX), each into half of the counting slots; the batch's spill area covers
+// either kernel's resident waves.
 template <int D>
-void launch_bwalk_wide(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, int cus) {
+void launch_bwalk_small(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, int cus) {
   t.slot_blocks = 0;
   if (!sp.total) return;
-  auto kern = btree_walk_wide<D, WIDE_WAVES>;
-  uint32_t blocks = resident_blocks(kern, WIDE_WAVES * WAVE, cus);
-  if (blocks > WALK_SLOTS) blocks = WALK_SLOTS;
-  if (blocks > ceil_div(sp.total, WIDE_WAVES)) blocks = ceil_div(sp.total, WIDE_WAVES);
-  if (t.spill_bytes < sizeof(uint32_t) * (size_t)blocks * WIDE_WAVES * wide_spill_words<D>())
-    throw ArgError(-5, "walk: spill area too small");
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(WIDE_WAVES * WAVE), 0, st, (const float4*)t.leaves,
-                     (const uint32_t*)t.order, (const NodeRec*)t.rec, sp.total, (const uint32_t*)sp.seg_of,
-                     (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.slots, (uint32_t*)t.spill);
-  FG_LAUNCH_CHECK();
-  t.slot_blocks = blocks;
+  auto launch = [&](auto kern, uint32_t spill_words, uint32_t* slots) {
+    uint32_t blocks = resident_blocks(kern, WIDE_WAVES * WAVE, cus);
+    if (blocks > WALK_SLOTS / 2) blocks = WALK_SLOTS / 2;
+    if (blocks > ceil_div(sp.total, WIDE_WAVES)) blocks = ceil_div(sp.total, WIDE_WAVES);
+    if (blocks > 8) blocks -= blocks % 8;  // a whole number of workgroups per XCD
+    if (t.spill_bytes < sizeof(uint32_t) * (size_t)blocks * WIDE_WAVES * spill_words)
+      throw ArgError(-5, "walk: spill area too small");
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(WIDE_WAVES * WAVE), 0, st, (const float4*)t.leaves,
+                       (const uint32_t*)t.order, (const NodeRec*)t.rec, sp.total, (const uint32_t*)sp.seg_of,
+                       (const uint32_t*)sp.seg_job, (const BJob*)jobs, slots, (uint32_t*)t.spill);
+    FG_LAUNCH_CHECK();
+    return blocks;
+  };
+  const uint32_t wide = launch(btree_walk_wide<D, WIDE_WAVES>, wide_spill_words<D>(), t.slots);
+  // the slots of the second launch follow the first's
+  const uint32_t group = launch(btree_walk_group<D, WALK_GROUP, WIDE_WAVES>, group_spill_words<D>(),
+                                t.slots + (size_t)8 * wide);
+  t.slot_blocks = wide + group;
 }
 
 // The walk kernel for the tuned workgroup size and LDS stack depth (16, 32, 48, 64 or 96
@@ -2404,7 +2681,7 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
                      (const uint32_t*)sp.seg_job, (const BJob*)jobs, (const uint4*)t.links, (NodeRec*)t.rec);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(3);
-  launch_bwalk_wide<D>(st, t, sp, jobs, cus);        // jobs of at most WALK_EXACT_MAX points
+  launch_bwalk_small<D>(st, t, sp, jobs, cus);       // jobs of at most WALK_GROUP_MAX points
   launch_bwalk_tuned<D>(st, t, sp, jobs, tune, cus);  // the larger jobs' 64-point tiles
   if (timer) timer->end(3);
 }
@@ -2432,10 +2709,36 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
   const uint32_t solo_max = p.walk_solo_max != WALK_SOLO_BY_DIMS ? p.walk_solo_max
                             : D == 3 ? 12288u : D == 2 ? 24576u : 32768u;
   t.slot_blocks = 0;
-  // measured (link + manyBody + center ticks, 3D): a wave per point is the faster walk up to
-  // 200,000 points (0.661 against 0.669 ms/tick there, 0.398 against 0.499 at 100,000), the
-  // tiled one from 300,000 (0.772 against 0.910); up to the exact mode's limit
-  const uint32_t wide_max = p.walk_wide_max != WALK_WIDE_BY_DIMS ? p.walk_wide_max : WALK_EXACT_MAX;
+  const uint32_t wide_max = p.walk_wide_max != WALK_WIDE_BY_DIMS ? p.walk_wide_max : WALK_WIDE_MAX;
+  const bool forced = p.walk_group >= 2u && p.walk_group <= 16u && (p.walk_group & (p.walk_group - 1u)) == 0u;
+  const bool by_size = p.walk_group == 0u && n > wide_max && n <= WALK_GROUP_MAX;
+  if (forced || by_size) {  // a group of points per wave
+    const uint32_t T = forced ? p.walk_group : WALK_GROUP;
+    auto launch = [&](auto kern) {
+      uint32_t blocks = resident_blocks(kern, WIDE_WAVES * WAVE, cus);
+      if (blocks > WALK_SLOTS) blocks = WALK_SLOTS;
+      const uint32_t groups = ceil_div(n, T);
+      if (blocks > ceil_div(groups, WIDE_WAVES)) blocks = ceil_div(groups, WIDE_WAVES);
+      if (blocks > 8) blocks -= blocks % 8;  // a whole number of workgroups per XCD
+      const size_t sbytes = sizeof(uint32_t) * (size_t)blocks * WIDE_WAVES * group_spill_words<D>();
+      if (t.spill_bytes < sbytes) t.reserve_spill(sbytes);
+      hipLaunchKernelGGL(kern, dim3(blocks), dim3(WIDE_WAVES * WAVE), 0, st, (const float4*)t.leaves,
+                         (const uint32_t*)t.order, (const NodeRec*)t.rec, (const CubeInfo*)t.cube, vx, vy, vz,
+                         t.slots, (uint32_t*)t.spill, step, A);
+      FG_LAUNCH_CHECK();
+      t.slot_blocks = blocks;
+    };
+    if (T == 2) launch(tree_walk_group<D, 2, WIDE_WAVES>);
+    else if (T == 4) launch(tree_walk_group<D, 4, WIDE_WAVES>);
+    else if (T == 8) launch(tree_walk_group<D, 8, WIDE_WAVES>);
+    else launch(tree_walk_group<D, 16, WIDE_WAVES>);
+    if (timer) timer->end(3);
+    return;
+  }
+  // measured (link + manyBody + center ticks, 3D): a wave per point up to about 10,000
+  // points (5,000: 0.095 against 0.104 ms/tick for two per wave), two per wave from there
+  // (50,000 0.219 against 0.237, 100,000 0.321 against 0.373, 200,000 0.536 against the
+  // tiled walk's 0.656), the tiled walk from 300,000 (500,000: 1.160 against 1.193)
   if (n <= wide_max) {  // a wave per point
     auto wk = tree_walk_wide<D, WIDE_WAVES>;
     uint32_t blocks = resident_blocks(wk, WIDE_WAVES * WAVE, cus);
@@ -2675,11 +2978,11 @@ void BatchTree::reserve(uint32_t n) {
   sort_seg.reserve(m, sizeof(uint32_t));
   FG_HIP(hipMalloc(&slots, sizeof(uint32_t) * 8 * WALK_SLOTS));
   if (!spill) {
-    // the tiled walk's entries past a 16-entry LDS stack, or the wide walk's past its LDS
-    // part, for every wave a CU can hold (32)
+    // the tiled walk's entries past a 16-entry LDS stack, or the wide or group walk's past
+    // their LDS part, for every wave a CU can hold (32)
     const int cus = device_cus();
-    spill_bytes = std::max(spill_bytes_for(cus, 16),
-                           sizeof(uint32_t) * (size_t)32 * (size_t)(cus > 0 ? cus : 1) * wide_spill_words<3>());
+    spill_bytes = std::max(spill_bytes_for(cus, 16), sizeof(uint32_t) * (size_t)32 * (size_t)(cus > 0 ? cus : 1) *
+                                                         std::max(wide_spill_words<3>(), group_spill_words<3>()));
     FG_HIP(hipMalloc(&spill, spill_bytes));
   }
   cap = n;

@@ -153,6 +153,7 @@ def test_manybody_tree_forces_do_not_depend_on_tiling(monkeypatch, dims, coincid
     s = np.full(n, -30.0) * (1 + 0.5 * np.sin(np.arange(n)))
     out = []
     monkeypatch.setenv("D3FGPU_WALK_WIDE", "0")
+    monkeypatch.setenv("D3FGPU_WALK_GROUP", "1")
     for tile, packed, solo in [(64, 24, 0), (64, 0, 0), (64, 64, 0), (32, 0, 0), (16, 24, 0), (7, 64, 0),
                                (1, 24, 0), (0, 24, n)]:
         monkeypatch.setenv("D3FGPU_WALK_TILE", str(tile))
@@ -161,14 +162,21 @@ def test_manybody_tree_forces_do_not_depend_on_tiling(monkeypatch, dims, coincid
         out.append(((tile, packed, solo), walk_dv(pos, dims, s)))
     for key, dv in out[1:]:
         assert np.array_equal(dv, out[0][1]), f"tile/packed/solo {key} differs from {out[0][0]}"
-    # the wave-per-point walk (the default at this size) sums the same terms in another order
-    for name in ("D3FGPU_WALK_WIDE", "D3FGPU_WALK_TILE", "D3FGPU_WALK_PACKED_MAX", "D3FGPU_WALK_SOLO"):
+    # the wave-per-point walk (the default at this size) and the group walks (2, 4, 8 points
+    # per wave) sum the same terms in other orders
+    for name in ("D3FGPU_WALK_WIDE", "D3FGPU_WALK_TILE", "D3FGPU_WALK_PACKED_MAX", "D3FGPU_WALK_SOLO",
+                 "D3FGPU_WALK_GROUP"):
         monkeypatch.delenv(name, raising=False)
-    wide = walk_dv(pos, dims, s)
-    assert rel_l2(wide, out[0][1]) < 1e-6
+    others = [("wave per point", walk_dv(pos, dims, s))]
+    for g in (2, 4, 8):
+        monkeypatch.setenv("D3FGPU_WALK_GROUP", str(g))
+        others.append((f"{g} points per wave", walk_dv(pos, dims, s)))
+    monkeypatch.delenv("D3FGPU_WALK_GROUP")
+    for label, dv in others:
+        assert rel_l2(dv, out[0][1]) < 1e-6, label
     ref, _ = oracle_manybody(pos, dims, s)
     keep = ~members
-    for label, dv in (("any tiling", out[0][1]), ("wave per point", wide)):
+    for label, dv in [("any tiling", out[0][1])] + others:
         err = rel_l2(dv[keep], ref[keep])
         report(f"manyBody tree {dims}D n={n} clump{' + 40 coincident' if coincident else ''}, {label}", err)
         assert err < 1e-4
@@ -259,6 +267,7 @@ def test_manybody_lane_group_walk_switch_points(monkeypatch, dims, n):
     s = np.full(n, -30.0) * (1 + 0.25 * np.sin(np.arange(n)))
     out = []
     monkeypatch.setenv("D3FGPU_WALK_WIDE", "0")
+    monkeypatch.setenv("D3FGPU_WALK_GROUP", "1")
     for solo in (None, "0"):
         if solo is None:
             monkeypatch.delenv("D3FGPU_WALK_SOLO", raising=False)
@@ -273,32 +282,36 @@ def test_manybody_lane_group_walk_switch_points(monkeypatch, dims, n):
     assert close_count(out[0], ref, rtol=1e-3) <= tree_outliers(n)
 
 
-@pytest.mark.parametrize("dims,n", [(3, 163840), (3, 163841), (2, 163840), (1, 163841)])
-def test_manybody_wave_per_point_walk_switch_point(monkeypatch, dims, n):
-    """Either side of 163,840 points, up to which a tree is walked one point per wave
-    (several openings per step): the default walk has no walk errors, is within rounding of
-    the tiled walk on the same tree (the wave-per-point walk sums the same terms in another
-    order), and matches the FP64 oracle."""
+@pytest.mark.parametrize("dims,n", [(3, 16384), (3, 16385), (2, 16385), (1, 16384), (3, 262144), (3, 262145),
+                                    (2, 262145)])
+def test_manybody_walk_switch_points(monkeypatch, dims, n):
+    """Either side of the sizes where the walk changes: one point per wave up to 16,384
+    points, two per wave up to 262,144, 64-point tiles beyond.  The default walk has no
+    walk errors, is within rounding of the tiled walk on the same tree where it is another
+    walk (the same terms, summed in another order) and that walk itself beyond 262,144,
+    and matches the FP64 oracle."""
     pos = make_nodes_fast(n, dims, 20 * n ** (1 / dims), seed=n + dims)
     s = np.full(n, -30.0) * (1 + 0.25 * np.cos(np.arange(n)))
-    monkeypatch.delenv("D3FGPU_WALK_WIDE", raising=False)
+    for name in ("D3FGPU_WALK_WIDE", "D3FGPU_WALK_GROUP"):
+        monkeypatch.delenv(name, raising=False)
     dflt = walk_dv(pos, dims, s)
     monkeypatch.setenv("D3FGPU_WALK_WIDE", "0")
+    monkeypatch.setenv("D3FGPU_WALK_GROUP", "1")
     tiled = walk_dv(pos, dims, s)
-    if n <= 163840:
+    if n <= 262144:
         assert 0 < rel_l2(dflt, tiled) < 1e-6
     else:
         assert np.array_equal(dflt, tiled)
     ref, _ = oracle_manybody(pos, dims, s)
     err = rel_l2(dflt, ref)
-    report(f"manyBody tree {dims}D n={n} (wave-per-point walk switch)", err)
+    report(f"manyBody tree {dims}D n={n} (walk switch)", err)
     assert err < 1e-4
     assert close_count(dflt, ref, rtol=1e-3) <= tree_outliers(n)
 
 
 def test_manybody_tree_above_exact_mode(engine_factory):
-    """Past 163840 points the walk takes 64-point tiles and the cheaper accumulation."""
-    n = 170000
+    """Past 262,144 points the walk takes 64-point tiles and the cheaper accumulation."""
+    n = 300000
     pos = make_nodes_fast(n, 3, 20 * np.cbrt(n))
     s = np.full(n, -30.0)
     _, dv = gpu_dv(engine_factory(), pos, 3, [d3fgpu.many_body(s)])
