@@ -143,8 +143,6 @@ struct BatchTree {
   void* rec = nullptr;   // NodeRec[cap]
   void* sums = nullptr;  // NodeSum[cap]
   uint32_t* counters = nullptr;  // interactions [0,1], opened [2,3], errors [4], queue [5], lane visits [6,7], long runs [10]
-  uint32_t* slots = nullptr;     // the wide walk's counts per workgroup (8 words each), slot_blocks of them
-  uint32_t slot_blocks = 0;
   uint32_t* long_runs = nullptr; // first positions of runs of equal high halves too long to order in place
   void* spill = nullptr;         // the walk's stack entries beyond LDS, per resident wave
   size_t spill_bytes = 0;

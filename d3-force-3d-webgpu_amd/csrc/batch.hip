@@ -30,7 +30,6 @@
 #include "forces.h"
 #include "host.h"
 #include "sort.h"
-#include "tree.h"
 
 using namespace fg;
 
@@ -449,8 +448,7 @@ void rebuild_layout(fg_batch* b) {
       if (j->tree) {
         for (uint32_t f = 0; f < j->n; f += BTILE)
           ttile.push_back(BTile{(uint32_t)tj.size(), tt + f, std::min(BTILE, j->n - f), 0});
-        if (j->n > WALK_GROUP_MAX)  // smaller trees take the wide or the group walk
-          for (uint32_t f = 0; f < j->n; f += WAVE) walk.push_back(make_uint2((uint32_t)tj.size(), tt + f));
+        for (uint32_t f = 0; f < j->n; f += WAVE) walk.push_back(make_uint2((uint32_t)tj.size(), tt + f));
         ts.push_back(tt);
         tj.push_back(j->slot);
         tt += j->n;
@@ -573,16 +571,6 @@ uint32_t walk_errors(fg_batch* b, uint64_t* interactions, uint32_t* max_stack = 
     inter += (uint64_t)h[0] | ((uint64_t)h[1] << 32);
     e += h[4];
     deepest = std::max(deepest, h[8]);
-    if (const uint32_t nb = g.btree.slot_blocks) {  // the wide walk's counts, per workgroup
-      std::vector<uint32_t> sl((size_t)8 * nb);
-      FG_HIP(hipMemcpy(sl.data(), g.btree.slots, sizeof(uint32_t) * sl.size(), hipMemcpyDeviceToHost));
-      for (uint32_t k = 0; k < nb; ++k) {
-        const uint32_t* q = &sl[(size_t)8 * k];
-        inter += (uint64_t)q[0] | ((uint64_t)q[1] << 32);
-        e += q[4];
-        deepest = std::max(deepest, q[5]);
-      }
-    }
     uint32_t se[4] = {0, 0, 0, 0};
     if (g.btree.sort.errors) {
       FG_HIP(hipMemcpy(se, g.btree.sort.errors, sizeof(se), hipMemcpyDeviceToHost));

@@ -2454,101 +2454,12 @@ __global__ void __launch_bounds__(WAVES * WAVE)
     const uint32_t end = T.off + T.n;
     const uint32_t count = end - tile.y < (uint32_t)WAVE ? end - tile.y : (uint32_t)WAVE;
     __builtin_amdgcn_wave_barrier();
-    walk_tile<D, STACK, false>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
+    if (T.n <= WALK_EXACT_MAX)
+      walk_tile<D, STACK, true>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
+    else
+      walk_tile<D, STACK, false>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
   }
   walk_flush(C, lane, counters);
-}
-
-// The wide walk over the trees of a batch's jobs of at most WALK_WIDE_MAX points (the
-// larger ones take the group walk or 64-point tiles): each wave takes tree-space points gw,
-// gw + waves, ... and walks those of active jobs as the single simulation's wide walk does,
-// the same terms in the same order.
-template <int D, uint32_t WAVES>
-__global__ void __launch_bounds__(WAVES * WAVE)
-    btree_walk_wide(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
-                    const NodeRec* __restrict__ rec, uint32_t total, const uint32_t* __restrict__ seg_of,
-                    const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
-                    uint32_t* __restrict__ slots, uint32_t* __restrict__ spill) {
-  __shared__ WideLDS<D> lds[WAVES];
-  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-  const uint32_t gw = blockIdx.x * WAVES + wave, waves = gridDim.x * WAVES;
-  uint32_t* sk = spill + (size_t)gw * wide_spill_words<D>();
-  uint32_t* sd = sk + (WIDE_DEPTH - WIDE_LDS) * (1u << D);
-  uint32_t inter = 0, max_sp = 0, err = 0;
-  uint64_t opened = 0;
-  for (uint32_t s = gw; s < total && !err; s += waves) {
-    const BJob& B = jobs[seg_job[seg_of[s]]];
-    if (!B.active || B.n > WALK_WIDE_MAX) continue;
-    WalkTree T;
-    T.off = B.tree_off;
-    T.n = B.n;
-    T.size = (double)B.cube_size;
-    T.theta2 = B.theta2;
-    T.dmax2 = B.dmax2;
-    T.dmin2f = (float)B.dmin2;
-    T.dmax2f = (float)fmin(B.dmax2, 3.0e38);
-    T.alpha = B.alpha_f;
-    T.step_no = B.step;
-    T.seed = B.seed;
-    T.vx = B.v[B.vcur][0];
-    T.vy = B.v[B.vcur][1];
-    T.vz = B.v[B.vcur][2];
-    wide_point<D>(lds[wave], sk, sd, leaves, order, rec, T, s, lane, inter, opened, max_sp, err);
-  }
-  WalkCount C;
-  C.opened = opened;
-  C.inter = inter;
-  C.err = err;
-  C.max_sp = max_sp;
-  walk_flush_slot(C, lane, slots + (size_t)blockIdx.x * 8);
-}
-
-// The group walk over the trees of a batch's jobs of more than WALK_WIDE_MAX and at most
-// WALK_GROUP_MAX points: groups of T consecutive sorted points from each job's first, as the
-// single simulation's group walk takes them (the same terms in the same order); the waves
-// take tree-space points XCD by XCD and walk those that start a group.
-template <int D, uint32_t T, uint32_t WAVES>
-__global__ void __launch_bounds__(WAVES * WAVE)
-    btree_walk_group(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
-                     const NodeRec* __restrict__ rec, uint32_t total, const uint32_t* __restrict__ seg_of,
-                     const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
-                     uint32_t* __restrict__ slots, uint32_t* __restrict__ spill) {
-  __shared__ GroupLDS<D, T> lds[WAVES];
-  const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-  const uint32_t gw = blockIdx.x * WAVES + wave;
-  uint32_t* sk = spill + (size_t)gw * group_spill_words<D>();
-  uint32_t inter = 0, max_sp = 0, err = 0;
-  uint64_t opened = 0, visits = 0;
-  const WalkRange wr = xcd_range(total, WAVES, wave);
-  for (uint32_t s = wr.first; s < wr.end && !err; s += wr.stride) {
-    const BJob& B = jobs[seg_job[seg_of[s]]];
-    if (!B.active || B.n <= WALK_WIDE_MAX || B.n > WALK_GROUP_MAX) continue;
-    const uint32_t local = s - B.tree_off;
-    if (local % T != 0u) continue;  // not the first point of a group
-    WalkTree W;
-    W.off = B.tree_off;
-    W.n = B.n;
-    W.size = (double)B.cube_size;
-    W.theta2 = B.theta2;
-    W.dmax2 = B.dmax2;
-    W.dmin2f = (float)B.dmin2;
-    W.dmax2f = (float)fmin(B.dmax2, 3.0e38);
-    W.alpha = B.alpha_f;
-    W.step_no = B.step;
-    W.seed = B.seed;
-    W.vx = B.v[B.vcur][0];
-    W.vy = B.v[B.vcur][1];
-    W.vz = B.v[B.vcur][2];
-    const uint32_t count = B.n - local < T ? B.n - local : T;
-    group_walk<D, T>(lds[wave], sk, leaves, order, rec, W, s, count, lane, inter, opened, visits, max_sp, err);
-  }
-  WalkCount C;
-  C.opened = opened;
-  C.lane_visits = visits;
-  C.inter = inter;
-  C.err = err;
-  C.max_sp = max_sp;
-  walk_flush_slot(C, lane, slots + (size_t)blockIdx.x * 8);
 }
 
 template <int D, int STACK, int WAVES>
@@ -2570,33 +2481,6 @@ void launch_bwalk(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, co
                      sp.walk_tiles, (const uint32_t*)sp.seg_job, (const BJob*)jobs, t.counters,
                      (SpillEntry*)t.spill, tune.walk_packed_max);
   FG_LAUNCH_CHECK();
-}
-
-// The wide walk (jobs of at most WALK_WIDE_MAX points) and the group walk (up to
-// WALK_GROUP_MAX), each into half of the counting slots; the batch's spill area covers
-// either kernel's resident waves.
-template <int D>
-void launch_bwalk_small(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, int cus) {
-  t.slot_blocks = 0;
-  if (!sp.total) return;
-  auto launch = [&](auto kern, uint32_t spill_words, uint32_t* slots) {
-    uint32_t blocks = resident_blocks(kern, WIDE_WAVES * WAVE, cus);
-    if (blocks > WALK_SLOTS / 2) blocks = WALK_SLOTS / 2;
-    if (blocks > ceil_div(sp.total, WIDE_WAVES)) blocks = ceil_div(sp.total, WIDE_WAVES);
-    if (blocks > 8) blocks -= blocks % 8;  // a whole number of workgroups per XCD
-    if (t.spill_bytes < sizeof(uint32_t) * (size_t)blocks * WIDE_WAVES * spill_words)
-      throw ArgError(-5, "walk: spill area too small");
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(WIDE_WAVES * WAVE), 0, st, (const float4*)t.leaves,
-                       (const uint32_t*)t.order, (const NodeRec*)t.rec, sp.total, (const uint32_t*)sp.seg_of,
-                       (const uint32_t*)sp.seg_job, (const BJob*)jobs, slots, (uint32_t*)t.spill);
-    FG_LAUNCH_CHECK();
-    return blocks;
-  };
-  const uint32_t wide = launch(btree_walk_wide<D, WIDE_WAVES>, wide_spill_words<D>(), t.slots);
-  // the slots of the second launch follow the first's
-  const uint32_t group = launch(btree_walk_group<D, WALK_GROUP, WIDE_WAVES>, group_spill_words<D>(),
-                                t.slots + (size_t)8 * wide);
-  t.slot_blocks = wide + group;
 }
 
 // The walk kernel for the tuned workgroup size and LDS stack depth (16, 32, 48, 64 or 96
@@ -2681,8 +2565,7 @@ void run_batch(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const
                      (const uint32_t*)sp.seg_job, (const BJob*)jobs, (const uint4*)t.links, (NodeRec*)t.rec);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1), timer->begin(3);
-  launch_bwalk_small<D>(st, t, sp, jobs, cus);       // jobs of at most WALK_GROUP_MAX points
-  launch_bwalk_tuned<D>(st, t, sp, jobs, tune, cus);  // the larger jobs' 64-point tiles
+  launch_bwalk_tuned<D>(st, t, sp, jobs, tune, cus);
   if (timer) timer->end(3);
 }
 
@@ -2976,26 +2859,20 @@ void BatchTree::reserve(uint32_t n) {
   FG_HIP(hipMalloc(&long_runs, sizeof(uint32_t) * (m / (ORDER_RUN + 1) + 1)));
   sort.reserve(m, sizeof(uint64_t));
   sort_seg.reserve(m, sizeof(uint32_t));
-  FG_HIP(hipMalloc(&slots, sizeof(uint32_t) * 8 * WALK_SLOTS));
   if (!spill) {
-    // the tiled walk's entries past a 16-entry LDS stack, or the wide or group walk's past
-    // their LDS part, for every wave a CU can hold (32)
-    const int cus = device_cus();
-    spill_bytes = std::max(spill_bytes_for(cus, 16), sizeof(uint32_t) * (size_t)32 * (size_t)(cus > 0 ? cus : 1) *
-                                                         std::max(wide_spill_words<3>(), group_spill_words<3>()));
+    spill_bytes = spill_bytes_for(device_cus(), 16);
     FG_HIP(hipMalloc(&spill, spill_bytes));
   }
   cap = n;
 }
 
 void BatchTree::release() {
-  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, counters, slots,
-                  spill, long_runs};
+  void* ptrs[] = {keys, codes, skeys, vals, digit, leaves, pts, leaf_depth, leaf_parent, links, flags, rec, sums, counters, spill,
+                  long_runs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   keys = codes = skeys = nullptr;
-  vals = digit = order = leaf_depth = leaf_parent = flags = counters = slots = long_runs = nullptr;
-  slot_blocks = 0;
+  vals = digit = order = leaf_depth = leaf_parent = flags = counters = long_runs = nullptr;
   leaves = pts = nullptr;
   links = nullptr;
   rec = sums = spill = nullptr;
