@@ -555,6 +555,34 @@ def node_cpu_baseline(n, light=False, ticks=None, min_ticks=3, budget=1.5):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def node_tick_async(sizes=(1000, 5000, 20000, 50000, 100000, 200000, 1000000)):
+    """The reference's own GPU timing, through this backend's Node host: forceSimulationGPU
+    with link (distance 30) + manyBody + center on the benchmark page's graph, three warm-up
+    ticks, then await tickAsync(k) for k = 10, 20, ... until one call takes 0.75 s or k
+    reaches 640 (examples/benchmark.html:119-132); each call holds its ticks, the readback
+    and the update of the nodes in JavaScript."""
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        return None
+    script = os.path.join(ROOT, "d3-force-3d-webgpu_amd", "js", "gpu-benchmark.mjs")
+    r = subprocess.run([node, script] + [str(n) for n in sizes], capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        log("node tickAsync leg failed: " + r.stderr[-2000:])
+        return None
+    out = {}
+    for line in r.stdout.strip().splitlines():
+        row = json.loads(line)
+        ms = row["ms_per_tick"]
+        out[str(row["n"])] = {"ms_per_tick": round(ms, 4), "ticks": row["ticks"], "links": row["links"],
+                              "reference_ms": REF_SIZES_MS.get(row["n"]),
+                              "vs_reference": round(REF_SIZES_MS[row["n"]] / ms, 3) if row["n"] in REF_SIZES_MS else None}
+        log(f"node tickAsync {row['n']}: {ms:.4f} ms/tick over {row['ticks']} ticks")
+    out["node"] = json.loads(r.stdout.strip().splitlines()[-1])["node"] if r.stdout.strip() else None
+    return out
+
+
 # Algorithmic HBM bytes per node of each phase of a 1M-node tick (each input element read
 # once, each output written once), as this backend's kernels move them:
 PASS_BYTES = {
@@ -626,6 +654,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs per serving run (0: skip)")
     ap.add_argument("--no-config-b", action="store_true", help="skip the 200k-node theta 0.9 / theta 0 leg")
     ap.add_argument("--no-ref-sizes", action="store_true", help="skip the reference's size table (1k..200k)")
+    ap.add_argument("--no-node-gpu", action="store_true",
+                    help="skip the reference's tickAsync timing through the Node host")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and exit")
@@ -716,6 +746,8 @@ def main():
         out["config_b"] = bench_config_b()
     if not args.no_ref_sizes:
         out["reference_sizes"] = bench_ref_sizes()
+    if not args.no_node_gpu:
+        out["node_tick_async"] = node_tick_async()
     if args.batch_jobs > 0:
         out["batched"] = bench_batch(args.batch_jobs, not args.no_plain_jobs)
     if args.serve_jobs > 0:
