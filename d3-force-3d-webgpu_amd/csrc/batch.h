@@ -108,6 +108,7 @@ struct BSpace {
   uint32_t segs = 0;        // jobs in the space
   uint32_t tiles = 0;       // reduction tiles
   uint32_t walk_tiles = 0;  // 64-point walk tiles (tree space)
+  uint32_t max_n = 0;       // points of the largest job in the space
   uint32_t* seg_of = nullptr;   // [total] segment of each index
   uint32_t* seg_job = nullptr;  // [segs] job slot of each segment
   BTile* tile = nullptr;        // [tiles]

@@ -424,7 +424,7 @@ void rebuild_layout(fg_batch* b) {
     std::vector<uint32_t> ns, nj, ts, tj, gs, gj;
     std::vector<BTile> ntile, ttile;
     std::vector<uint2> walk;
-    uint32_t nt = 0, tt = 0, gt = 0;
+    uint32_t nt = 0, tt = 0, gt = 0, tree_max = 0;
     g.max_link = g.max_coll = 0;
     g.center = g.radial = false;
     g.pos[0] = g.pos[1] = g.pos[2] = false;
@@ -449,6 +449,7 @@ void rebuild_layout(fg_batch* b) {
         for (uint32_t f = 0; f < j->n; f += BTILE)
           ttile.push_back(BTile{(uint32_t)tj.size(), tt + f, std::min(BTILE, j->n - f), 0});
         for (uint32_t f = 0; f < j->n; f += WAVE) walk.push_back(make_uint2((uint32_t)tj.size(), tt + f));
+        tree_max = std::max(tree_max, j->n);
         ts.push_back(tt);
         tj.push_back(j->slot);
         tt += j->n;
@@ -479,6 +480,7 @@ void rebuild_layout(fg_batch* b) {
     g.tree.tiles = (uint32_t)ttile.size();
     g.tree.tile = g.tree_tile.as<BTile>();
     g.tree.walk_tiles = (uint32_t)walk.size();
+    g.tree.max_n = tree_max;
     g.tree.walk = g.tree_walk.as<uint2>();
     // scratch sized before any capture
     if (g.max_link && g.snap.bytes < sizeof(float4) * (size_t)nt) g.snap.alloc(sizeof(float4) * (size_t)nt);

@@ -97,6 +97,7 @@ struct TreeBuffers {
 constexpr uint32_t WALK_SOLO_BY_DIMS = 0xffffffffu;  // walk_solo_max: the measured default per dims
 constexpr uint32_t WALK_PACKED_BY_MODE = 0xffffffffu;  // walk_packed_max: the measured default per mode
 constexpr uint32_t WALK_WIDE_BY_DIMS = 0xffffffffu;  // walk_wide_max: the measured default per dims
+constexpr uint32_t WALK_POOL_BY_MODE = 0xffffffffu;  // walk_pool_k: the measured default
 
 struct TreeParams {
   double theta2, dmin2, dmax2;
@@ -111,6 +112,9 @@ struct TreeParams {
   uint32_t walk_group = 0;
   uint32_t reduce_block = 0;    // points per workgroup of the reduction (512 or 1024; 0: by size)
   bool reduce_aligned = true;  // reduction workgroups aligned to subtrees
+  // the tiled walk's cheaper mode: openings by at most this many points go to the pool of
+  // single-point openings (0: no pool)
+  uint32_t walk_pool_k = WALK_POOL_BY_MODE;
 };
 
 // Bytes of walk spill area for `cus` CUs at an LDS stack of `stack` entries.

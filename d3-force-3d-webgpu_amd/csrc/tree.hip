@@ -1119,7 +1119,21 @@ constexpr int WALK_DEPTH = 192;  // stack entries per wave in all; 7 * 21 + 8 = 
 // node being opened (points: their float4); the wave's points and their force sums from
 // the packed mode.  Entries from STACK to WALK_DEPTH live in the wave's spill area in
 // global memory, which deep walks (many coincident points) reach.
-template <int STACK>
+//
+// The cheaper mode adds a pool of single-point openings: a node that at most POOL_K of the
+// tile's points must open goes into the pool as one entry per point (node, point, the depth
+// its children's chains start at) instead of onto the stack, and a pool step opens up to
+// 64 / 2^D entries at once, lane (entry, child) deciding one child for one point -- deep in
+// the tree, where a wave's opening served a few points with a few lanes each, every lane
+// now has a child to decide.  Entries that find the pool full go onto the stack as
+// one-point entries.
+constexpr uint32_t POOL_K = 12;     // openings by at most this many points go to the pool
+constexpr uint32_t POOL_SIZE = 64;  // pool entries per wave
+// Waves per SIMD the pooled walk is compiled for: 6 (80 registers, a few spilled) measured
+// faster than the 5 its registers allow (1M points, 3D: 1.19 -> 1.08 ms) and than 7 (1.10).
+constexpr int POOL_WAVES = 6;
+
+template <int STACK, bool POOLED = false>
 struct WalkLDS {
   uint32_t depth[STACK + 1];
   uint64_t mask[STACK + 1];
@@ -1130,6 +1144,7 @@ struct WalkLDS {
   uint32_t list[WAVE];    // the points opening the current node, in lane order
   uint32_t cur[8];        // the children of the node being opened
   uint64_t ballots[8];
+  uint2 pool[POOLED ? POOL_SIZE : 1];  // node, point << 8 | chain start depth
 };
 
 // The jiggled displacement of two coincident bodies: out of line, it is rare.
@@ -1201,20 +1216,28 @@ __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self
   const double dx = h.cx - mx, dy = D > 1 ? h.cy - my : 0.0, dz = D > 2 ? h.cz - mz : 0.0;
   const double l = ACC == ACC_FUSED ? dx * dx + dy * dy + dz * dz : sq_len<D>(dx, dy, dz);
   const double reach = theta2 * l;
-  double w2 = w2top;
-  uint32_t Dd = top;
-  while (Dd <= h.depth && !(w2 < reach)) {
-    w2 *= 0.25;
-    ++Dd;
+  // The widths of the chain shrink by 4 (exactly) per level, so some level accepts the cell
+  // exactly when its deepest one does: one test decides opening.  The level d3 stops at is
+  // the first that accepts, found only for a chain of more than one level.
+  const uint32_t chain = h.depth - top;
+  if (!((chain == 0u ? w2top : ldexp(w2top, -2 * (int)chain)) < reach)) return false;
+  float vd = h.value;
+  if (chain != 0u) {
+    double w2 = w2top;
+    uint32_t Dd = top;
+    while (!(w2 < reach)) {
+      w2 *= 0.25;
+      ++Dd;
+    }
+    vd = h.value * dampf<D>(h.depth - Dd);
   }
-  if (Dd > h.depth) return false;
   if (l >= dmax2) return true;
   float ex = (float)dx, ey = (float)dy, ez = (float)dz, fl = (float)l;
   if (l == 0.0) {
     const float4 j = coincident_far(self_node, n + cell, D, step_no, seed);
     ex = j.x, ey = j.y, ez = j.z, fl = j.w;
   }
-  const float f = h.value * dampf<D>(h.depth - Dd) * alpha * inv_dist2(fl, dmin2f);
+  const float f = vd * alpha * inv_dist2(fl, dmin2f);
   accumulate<ACC>(fx, ex, f);
   if (D > 1) accumulate<ACC>(fy, ey, f);
   if (D > 2) accumulate<ACC>(fz, ez, f);
@@ -1280,13 +1303,16 @@ struct WalkCount {
 };
 
 // One wave walks the union of the traversals of sorted points [base, base + count) of one
-// tree (count <= 64) and adds their forces to the velocities.
-template <int D, int STACK, bool EXACT>
-__device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __restrict__ leaves,
+// tree (count <= 64) and adds their forces to the velocities.  POOLED: the cheaper mode
+// sends openings by few points to the pool of single-point openings.
+template <int D, int STACK, bool EXACT, bool POOLED>
+__device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float4* __restrict__ leaves,
                                           const uint32_t* __restrict__ order,
                                           const NodeRec* __restrict__ rec, const WalkTree& T,
                                           uint32_t base, uint32_t count, uint32_t packed_max,
-                                          uint32_t lane, SpillEntry* spill, WalkCount& C) {
+                                          uint32_t lane, SpillEntry* spill, WalkCount& C,
+                                          uint32_t pool_k = POOL_K) {
+  constexpr bool POOL = POOLED && !EXACT;
   constexpr int NK = 1 << D;
   const uint32_t off = T.off, n = T.n;
   const uint32_t s = base + lane;
@@ -1338,8 +1364,110 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
   const uint32_t guard_max = 2u * n + 8u;
   // lane t stages dwords 2(t & 7), +1 of child t >> 3
   const uint32_t my_c = lane >> 3, my_w = lane & 7;
-  for (uint32_t guard = 0; sp > 0; ++guard) {
-    if (guard > guard_max) {
+
+  // The pool of single-point openings (POOL only): pn entries, the last pushed opened first.
+  // A step takes up to TAKE entries; lane (e, c) decides child c of entry e's node for that
+  // entry's point, the terms of an entry summed over its NK lanes and added to the point's
+  // sum in LDS; the children to open go back into the pool, those that find it full onto the
+  // stack as one-point entries.
+  constexpr uint32_t TAKE = (uint32_t)WAVE / NK;
+  uint32_t pn = 0;
+  // every entry is a (point, node) pair met once, so a tile takes at most 64 n entries
+  const uint64_t pool_guard_max = 64ull * n + 8ull;
+  uint64_t pool_guard = 0;
+  auto pool_step = [&]() {
+    if constexpr (POOL) {
+      if (++pool_guard > pool_guard_max) {
+        err = 1;
+        pn = 0;
+        return;
+      }
+      const uint32_t take = pn < TAKE ? pn : TAKE;
+      const uint32_t pe = lane / NK, pc = lane % NK;
+      const bool act = pe < take;
+      uint2 ent = make_uint2(0u, 0u);
+      if (act) ent = L.pool[pn - 1u - pe];
+      pn -= take;
+      ++opened;
+      lane_visits += take * NK;
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t p = ent.y >> 8, ptop = ent.y & 255u;
+      uint32_t kid = REF_NONE;
+      if (act) kid = rec[ent.x].kids[pc];
+      float fx = 0.f, fy = 0.f, fz = 0.f;
+      bool open = false;
+      uint32_t cdepth = 0;
+      if (kid != REF_NONE) {
+        const float4 bd = L.body[p];
+        const uint32_t bnode = __float_as_uint(bd.w);
+        if (kid & REF_POINT) {
+          const uint32_t j = kid & ~REF_POINT;
+          if (j != base + p)
+            point_force<D, ACC_FUSED>(bd.x, bd.y, bd.z, bnode, leaves[j], order, j, off, dmax2f, dmin2f, alpha,
+                                      step_no, seed, fx, fy, fz, inter);
+        } else {
+          const NodeHdr h = *(const NodeHdr*)&rec[kid];
+          const double wt = ldexp(size, -(int)ptop);
+          open = !cell_force<D, ACC_FUSED>(bd.x, bd.y, bd.z, bnode, n, kid - off, ptop, wt * wt, h, theta2, dmax2,
+                                           dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
+          cdepth = h.depth;
+        }
+      }
+      fx = group_sum<NK>(fx);
+      if (D > 1) fy = group_sum<NK>(fy);
+      if (D > 2) fz = group_sum<NK>(fz);
+      if (act && pc == 0u) {
+        __hip_atomic_fetch_add(&L.acc[0][p], fx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (D > 1) __hip_atomic_fetch_add(&L.acc[1][p], fy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (D > 2) __hip_atomic_fetch_add(&L.acc[2][p], fz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      const uint64_t ob = __builtin_amdgcn_ballot_w64(open);
+      if (ob) {
+        const uint32_t room = POOL_SIZE - pn, rank = mask_rank(ob);
+        if (open && rank < room) L.pool[pn + rank] = make_uint2(kid, p << 8 | (cdepth + 1u));
+        const uint32_t cnt = (uint32_t)__popcll(ob);
+        uint64_t over = __builtin_amdgcn_ballot_w64(open && rank >= room);
+        pn += cnt < room ? cnt : room;
+        while (over) {  // the pool is full: one-point stack entries
+          const uint32_t src = (uint32_t)__builtin_ctzll(over);
+          over &= over - 1ull;
+          if (sp >= (uint32_t)WALK_DEPTH) {
+            err = 1;
+            continue;
+          }
+          const uint32_t okid = (uint32_t)__builtin_amdgcn_readlane((int)kid, (int)src);
+          const uint32_t op = (uint32_t)__builtin_amdgcn_readlane((int)p, (int)src);
+          const uint32_t od = (uint32_t)__builtin_amdgcn_readlane((int)cdepth, (int)src);
+          const uint32_t k8 = lane < 8u ? rec[okid].kids[lane] : 0u;
+          if (sp < (uint32_t)STACK) {
+            if (lane == 0) {
+              L.depth[sp] = od;
+              L.mask[sp] = (uint64_t)1 << op;
+            }
+            if (lane < 8u) L.kids[sp][lane] = k8;
+          } else {
+            SpillEntry* E = spill + (sp - STACK);
+            if (lane == 0) {
+              store_coherent(&E->mask, (uint64_t)1 << op);
+              store_coherent(&E->depth, od);
+            }
+            if (lane < 8u) store_coherent(&E->kids[lane], k8);
+          }
+          ++sp;
+          max_sp = sp > max_sp ? sp : max_sp;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  };
+
+  for (uint32_t guard = 0; sp > 0 || (POOL && pn > 0);) {
+    // a pool step as soon as the pool fills one, and to drain it once the stack is done
+    if (POOL && (sp == 0 || pn >= TAKE)) {
+      pool_step();
+      continue;
+    }
+    if (++guard > guard_max) {
       err = 1;
       break;
     }
@@ -1432,10 +1560,11 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
         open_bits = (uint32_t)(bal >> ((i * NK) % WAVE)) & ((1u << NK) - 1u);
       }
     } else if (!EXACT) {
-      // one lane per point, children in turn
+      // one lane per point, children in turn; the child, its depth and its strength are the
+      // same on every lane, so the tests on them are scalar branches
 #pragma unroll 1
       for (uint32_t c = 0; c < nk; ++c) {
-        const uint32_t k = L.cur[c];
+        const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)my_kid, (int)c);
         if (!in) continue;
         if (k & REF_POINT) {
           const uint32_t j = k & ~REF_POINT;
@@ -1444,7 +1573,9 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
                                   off, dmax2f, dmin2f, alpha, step_no, seed, ax, ay, az, inter);
           continue;
         }
-        const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
+        NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
+        h.depth = (uint32_t)__builtin_amdgcn_readfirstlane((int)h.depth);
+        h.value = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(h.value)));
         if (!cell_force<D, ACC_FUSED>(me4.x, me4.y, me4.z, self_node, n, k - off, top, w2top, h, theta2,
                                   dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter))
           open_bits |= 1u << c;
@@ -1541,11 +1672,20 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
       for (uint32_t c = 0; c < nk; ++c) {
         const uint64_t open = __builtin_amdgcn_ballot_w64(((open_bits >> c) & 1u) != 0u);
         if (!open) continue;
+        const uint32_t cdepth = ((const NodeHdr*)&L.stage[c][0])->depth;
+        if constexpr (POOL) {
+          const uint32_t cnt = (uint32_t)__popcll(open);
+          if (cnt <= pool_k && pn + cnt <= POOL_SIZE) {  // one pool entry per point
+            const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)my_kid, (int)c);
+            if ((open >> lane) & 1ull) L.pool[pn + mask_rank(open)] = make_uint2(k, lane << 8 | (cdepth + 1u));
+            pn += cnt;
+            continue;
+          }
+        }
         if (sp >= (uint32_t)WALK_DEPTH) {
           err = 1;
           continue;
         }
-        const uint32_t cdepth = ((const NodeHdr*)&L.stage[c][0])->depth;
         if (sp < (uint32_t)STACK) {
           if (lane == 0) {
             L.depth[sp] = cdepth;
@@ -1562,6 +1702,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK>& L, const float4* __res
         }
         ++sp;
       }
+      if constexpr (POOL) __builtin_amdgcn_wave_barrier();
     }
     max_sp = sp > max_sp ? sp : max_sp;
   }
@@ -1618,6 +1759,7 @@ struct WalkArgs {
   uint32_t n;
   uint32_t packed_max;  // openings by at most this many points use the packed mode
   uint32_t tile;        // points per tile (1..64)
+  uint32_t pool_k;      // openings by at most this many points go to the pool (cheaper mode)
 };
 
 // Points per tile of an exact walk over n points, on a device of `simds` SIMDs where the
@@ -1642,13 +1784,13 @@ inline uint32_t walk_tile_for(uint32_t n, uint32_t simds, uint32_t waves, uint32
 // queue (counters[5]) until none is left; `spill` holds WALK_DEPTH - STACK entries per
 // resident wave.  Tiles other than 64 points need EXACT.
 template <int D, int STACK, int WAVES, bool EXACT>
-__global__ void __launch_bounds__(WAVES * WAVE)
+__global__ void __launch_bounds__(WAVES * WAVE, EXACT ? 1 : POOL_WAVES)
     tree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
               const NodeRec* __restrict__ rec, const CubeInfo* __restrict__ cube,
               float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
               uint32_t* __restrict__ counters, SpillEntry* __restrict__ spill, StepParams step,
               WalkArgs A) {
-  __shared__ WalkLDS<STACK> lds[WAVES];
+  __shared__ WalkLDS<STACK, !EXACT> lds[WAVES];
   const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
   SpillEntry* my_spill = spill + (size_t)(blockIdx.x * WAVES + wave) * (WALK_DEPTH - STACK);
   WalkTree T;
@@ -1684,8 +1826,8 @@ __global__ void __launch_bounds__(WAVES * WAVE)
     const uint32_t base = t * A.tile;
     const uint32_t count = A.n - base < A.tile ? A.n - base : A.tile;
     __builtin_amdgcn_wave_barrier();
-    walk_tile<D, STACK, EXACT>(lds[wave], leaves, order, rec, T, base, count, A.packed_max, lane, my_spill,
-                               C);
+    walk_tile<D, STACK, EXACT, !EXACT>(lds[wave], leaves, order, rec, T, base, count, A.packed_max, lane, my_spill,
+                               C, A.pool_k);
   }
   walk_flush(C, lane, counters);
 }
@@ -2419,13 +2561,13 @@ __global__ void __launch_bounds__(256) btree_build(uint32_t total, const uint64_
 // Persistent walk: every wave takes 64-point tiles of any job from one queue until it is
 // empty; the per-tile work is the single-simulation walk's, in the same mode for the
 // job's size (so a job's forces are the single simulation's whatever its tiles).
-template <int D, int STACK, int WAVES>
-__global__ void __launch_bounds__(WAVES * WAVE)
+template <int D, int STACK, int WAVES, bool POOLED>
+__global__ void __launch_bounds__(WAVES * WAVE, POOLED && STACK <= 64 ? POOL_WAVES : 1)
     btree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
                const NodeRec* __restrict__ rec, const uint2* __restrict__ tiles, uint32_t ntiles,
                const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
                uint32_t* __restrict__ counters, SpillEntry* __restrict__ spill, uint32_t packed_max) {
-  __shared__ WalkLDS<STACK> lds[WAVES];
+  __shared__ WalkLDS<STACK, POOLED> lds[WAVES];
   const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
   SpillEntry* my_spill = spill + (size_t)(blockIdx.x * WAVES + wave) * (WALK_DEPTH - STACK);
   WalkCount C;
@@ -2455,9 +2597,9 @@ __global__ void __launch_bounds__(WAVES * WAVE)
     const uint32_t count = end - tile.y < (uint32_t)WAVE ? end - tile.y : (uint32_t)WAVE;
     __builtin_amdgcn_wave_barrier();
     if (T.n <= WALK_EXACT_MAX)
-      walk_tile<D, STACK, true>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
+      walk_tile<D, STACK, true, POOLED>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
     else
-      walk_tile<D, STACK, false>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
+      walk_tile<D, STACK, false, POOLED>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
   }
   walk_flush(C, lane, counters);
 }
@@ -2465,7 +2607,9 @@ __global__ void __launch_bounds__(WAVES * WAVE)
 template <int D, int STACK, int WAVES>
 void launch_bwalk(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const BatchTune& tune,
                   int cus) {
-  auto kern = btree_walk<D, STACK, WAVES>;
+  // the pool of single-point openings (and its registers) only where a job walks the
+  // cheaper mode
+  auto kern = sp.max_n > WALK_EXACT_MAX ? btree_walk<D, STACK, WAVES, true> : btree_walk<D, STACK, WAVES, false>;
   int per_cu = 0;
   FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WAVES * WAVE, 0));
   if (per_cu < 1) per_cu = 1;
@@ -2585,6 +2729,7 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
   // -> 0.495 ms/tick against 24, 20,000-50,000 flat); 24 for the cheaper mode (1M: 32 within
   // the noise, 40 and 16 slower)
   A.packed_max = p.walk_packed_max != WALK_PACKED_BY_MODE ? p.walk_packed_max : exact ? 32u : 24u;
+  A.pool_k = p.walk_pool_k != WALK_POOL_BY_MODE ? p.walk_pool_k : POOL_K;
   // measured on 256 CUs (link + manyBody + center ticks): a group of lanes per point is the
   // faster walk up to 12,000 points in 3D (0.254 against 0.258 ms/tick) but not at 16,000
   // (0.296 against 0.264), and still at 24,000 in 2D (0.234 against 0.237) and 32,000 in 1D
