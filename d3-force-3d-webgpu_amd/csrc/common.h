@@ -47,6 +47,18 @@ inline void upload_device(void* dst, const void* src, size_t bytes) {
 }
 
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+// Workgroups are dealt to the device's 8 XCDs in turn (workgroup b runs on XCD b % 8), each
+// XCD with an L2 of its own.  A grid of xcd_blocks(blocks) workgroups numbers its logical
+// blocks with xcd_block(): XCD x takes the contiguous logical blocks [x per, (x + 1) per),
+// so neighbouring blocks -- neighbouring data -- share one L2.  Logical blocks past `blocks`
+// (fewer than 8) have no work; grids under 64 blocks keep their order.
+constexpr uint32_t XCD_COUNT = 8;
+inline uint32_t xcd_blocks(uint32_t blocks) { return blocks < 64u ? blocks : ceil_div(blocks, XCD_COUNT) * XCD_COUNT; }
+__device__ inline uint32_t xcd_block() {
+  if (gridDim.x < 64u) return blockIdx.x;
+  const uint32_t per = gridDim.x / XCD_COUNT;
+  return (blockIdx.x % XCD_COUNT) * per + blockIdx.x / XCD_COUNT;
+}
 
 // ---- device helpers ------------------------------------------------------------------
 

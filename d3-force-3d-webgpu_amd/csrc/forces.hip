@@ -725,14 +725,33 @@ __device__ inline uint32_t spread3_3(uint32_t v) {  // 2 bits -> every third bit
   v &= 3u;
   return (v & 1u) | ((v & 2u) << 2);
 }
+__device__ inline uint32_t spread3_10(uint32_t v) {  // 10 bits -> every third bit
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  return (v | (v << 2)) & 0x09249249u;
+}
+__device__ inline uint32_t spread2_16(uint32_t v) {  // 16 bits -> every other bit
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  return (v | (v << 1)) & 0x55555555u;
+}
 __device__ inline uint32_t spread2_3(uint32_t v) {  // 3 bits -> every other bit
   v &= 7u;
   return (v & 1u) | ((v & 2u) << 1) | ((v & 4u) << 2);
 }
 
 // Slot of a cell: blocks of 4^3 cells (8^2 in 2D, 64 in 1D) take 64 consecutive slots, the
-// cells of a block in Morton order, from an offset hashed from the block's coordinates
-// (and, in a batch, the job's segment; 0 for a single simulation).
+// cells of a block in Morton order, and the blocks are placed by the Morton code of their
+// coordinates' low bits plus an offset hashed from the tag (the class and, in a batch, the
+// job's segment), all modulo the table: blocks near each other in space get slots near
+// each other, so a point's neighbour cells, and the neighbourhoods of the points one XCD
+// queries, share cache lines (1M points, 3D: two collide iterations 0.526 -> 0.479 ms
+// against blocks placed by a hash of their coordinates).  Distant blocks that share
+// slots only add candidates the pair test rejects.
 template <int D>
 __device__ inline uint32_t grid_slot(int cx, int cy, int cz, uint32_t seg, uint32_t mask) {
   uint32_t ux = (uint32_t)cx, uy = (uint32_t)cy, uz = (uint32_t)cz;
@@ -747,8 +766,12 @@ __device__ inline uint32_t grid_slot(int cx, int cy, int cz, uint32_t seg, uint3
     bx = ux >> 6, by = 0, bz = 0;
     m = ux & 63u;
   }
-  uint32_t h = pcg_hash((bx * 73856093u) ^ (by * 19349663u) ^ (bz * 83492791u) ^ (seg * 2654435761u));
-  return ((h << 6) + m) & mask;
+  uint32_t pos;
+  if (D == 3) pos = spread3_10(bx) | spread3_10(by) << 1 | spread3_10(bz) << 2;
+  else if (D == 2) pos = spread2_16(bx) | spread2_16(by) << 1;
+  else pos = bx;
+  pos += pcg_hash(seg * 2654435761u);
+  return ((pos << 6) + m) & mask;
 }
 
 template <int D>
@@ -1021,7 +1044,8 @@ __global__ void __launch_bounds__(256) grid_query(uint32_t n, const uint32_t* __
                                                   float* __restrict__ vz, GridClasses gc,
                                                   uint32_t mask, float strength,
                                                   StepParams step) {
-  uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  // each XCD queries one contiguous eighth of the sorted points (0.479 -> 0.468 ms)
+  const uint32_t s = xcd_block() * 256 + threadIdx.x;
   if (s >= n) return;
   const float4 me = sorted[s];
   const uint32_t i = svals[s];
@@ -1079,7 +1103,7 @@ void grid_run(hipStream_t s, GridWork& g, uint32_t n, float* const* pos, float* 
                      (const uint32_t*)g.arrived, (const float4*)g.snap, g.svals, g.sorted,
                      (const uint32_t*)nullptr, (uint32_t*)nullptr);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(grid_query<D>, gr, b, 0, s, n, (const uint32_t*)g.svals, (const uint32_t*)g.start,
+  hipLaunchKernelGGL(grid_query<D>, dim3(xcd_blocks(gr.x)), b, 0, s, n, (const uint32_t*)g.svals, (const uint32_t*)g.start,
                      (const float4*)g.sorted, vel[0], vel[1], vel[2], gc, mask, strength, step);
   FG_LAUNCH_CHECK();
 }
@@ -1222,7 +1246,7 @@ __global__ void __launch_bounds__(256) bgrid_query(uint32_t total, const uint32_
                                                    const uint32_t* __restrict__ seg_job,
                                                    const BJob* __restrict__ jobs, uint32_t mask,
                                                    uint32_t iteration) {
-  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t s = xcd_block() * 256 + threadIdx.x;
   if (s >= total) return;
   const uint32_t seg = sseg[s];
   const BJob& B = jobs[seg_job[seg]];
@@ -1437,7 +1461,7 @@ void batch_collide(hipStream_t s, int ndim, BatchGrid& g, const BSpace& sp, BJob
                      (const uint32_t*)g.arrived, (const float4*)g.snap, g.svals, g.sorted,
                      (const uint32_t*)sp.seg_of, g.sseg);
   FG_LAUNCH_CHECK();
-  FG_BY_DIM(ndim, bgrid_query, gr, b, 0, s, n, (const uint32_t*)g.svals, (const uint32_t*)g.start,
+  FG_BY_DIM(ndim, bgrid_query, dim3(xcd_blocks(gr.x)), b, 0, s, n, (const uint32_t*)g.svals, (const uint32_t*)g.start,
             (const float4*)g.sorted, (const uint32_t*)g.sseg, (const uint32_t*)sp.seg_job, (const BJob*)jobs, mask,
             iteration);
 }

@@ -2724,7 +2724,7 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
   A.dmax2 = p.dmax2;
   A.n = n;
   const int cus = device_cus();
-  const bool exact = n <= WALK_EXACT_MAX;
+  const bool exact = n <= p.walk_exact_max;
   // measured (link + manyBody + center ticks): 32 for the exact mode (100,000 points 0.511
   // -> 0.495 ms/tick against 24, 20,000-50,000 flat); 24 for the cheaper mode (1M: 32 within
   // the noise, 40 and 16 slower)
