@@ -1127,10 +1127,9 @@ constexpr int WALK_DEPTH = 192;  // stack entries per wave in all; 7 * 21 + 8 = 
 // the tree, where a wave's opening served a few points with a few lanes each, every lane
 // now has a child to decide.  Entries that find the pool full go onto the stack as
 // one-point entries.
-constexpr uint32_t POOL_K = 12;     // openings by at most this many points go to the pool
+constexpr uint32_t POOL_K = 24;     // openings by at most this many points go to the pool
 constexpr uint32_t POOL_SIZE = 64;  // pool entries per wave
-// Waves per SIMD the pooled walk is compiled for: 6 (80 registers, a few spilled) measured
-// faster than the 5 its registers allow (1M points, 3D: 1.19 -> 1.08 ms) and than 7 (1.10).
+// Waves per SIMD the pooled walk is compiled for: 6 (at most 80 registers).
 constexpr int POOL_WAVES = 6;
 
 template <int STACK, bool POOLED = false>
@@ -1385,34 +1384,64 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
       const uint32_t take = pn < TAKE ? pn : TAKE;
       const uint32_t pe = lane / NK, pc = lane % NK;
       const bool act = pe < take;
-      uint2 ent = make_uint2(0u, 0u);
-      if (act) ent = L.pool[pn - 1u - pe];
+      const uint2 ent = L.pool[act ? pn - 1u - pe : 0u];
       pn -= take;
       ++opened;
       lane_visits += take * NK;
       __builtin_amdgcn_wave_barrier();
-      const uint32_t p = ent.y >> 8, ptop = ent.y & 255u;
-      uint32_t kid = REF_NONE;
-      if (act) kid = rec[ent.x].kids[pc];
-      float fx = 0.f, fy = 0.f, fz = 0.f;
-      bool open = false;
-      uint32_t cdepth = 0;
-      if (kid != REF_NONE) {
-        const float4 bd = L.body[p];
-        const uint32_t bnode = __float_as_uint(bd.w);
-        if (kid & REF_POINT) {
-          const uint32_t j = kid & ~REF_POINT;
-          if (j != base + p)
-            point_force<D, ACC_FUSED>(bd.x, bd.y, bd.z, bnode, leaves[j], order, j, off, dmax2f, dmin2f, alpha,
-                                      step_no, seed, fx, fy, fz, inter);
-        } else {
-          const NodeHdr h = *(const NodeHdr*)&rec[kid];
-          const double wt = ldexp(size, -(int)ptop);
-          open = !cell_force<D, ACC_FUSED>(bd.x, bd.y, bd.z, bnode, n, kid - off, ptop, wt * wt, h, theta2, dmax2,
-                                           dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
-          cdepth = h.depth;
+      // Straight-line code: every lane loads a node header and a point (the root and the
+      // tile's first point where it has none, so every address is valid) and d3's rules
+      // (barnesHut.js:705-746, manyBody.js:73-113) become selects; the two rare cases --
+      // a chain of more than one level that is accepted, a source at the point's own
+      // position -- run only when some lane meets them.
+      const uint32_t p = act ? ent.y >> 8 : 0u, ptop = ent.y & 255u;
+      const uint32_t kid = act ? rec[act ? ent.x : off].kids[pc] : REF_NONE;
+      // (bitwise & and | on the decisions: no short-circuit branches)
+      const bool is_pt = (kid != REF_NONE) & ((kid & REF_POINT) != 0u);
+      const bool is_node = (kid != REF_NONE) & ((kid & REF_POINT) == 0u);
+      const uint32_t j = is_pt ? kid & ~REF_POINT : base;
+      const NodeHdr h = *(const NodeHdr*)&rec[is_node ? kid : off];
+      const float4 q = leaves[j];
+      const float4 bd = L.body[p];
+      const double dx = (is_node ? h.cx : (double)q.x) - (double)bd.x;
+      const double dy = D > 1 ? (is_node ? h.cy : (double)q.y) - (double)bd.y : 0.0;
+      const double dz = D > 2 ? (is_node ? h.cz : (double)q.z) - (double)bd.z : 0.0;
+      const double l = dx * dx + dy * dy + dz * dz;
+      const double wt = ldexp(size, -(int)ptop);
+      const int chain = (int)h.depth - (int)ptop;
+      // a chain's widths shrink by exactly 4 per level: its deepest level decides; a node
+      // above its parent's depth (inside a run of equal codes) always opens
+      const bool accept = (chain >= 0) & (ldexp(wt * wt, -2 * (chain > 0 ? chain : 0)) < theta2 * l);
+      const bool strong = h.value != 0.0f;
+      const bool open = is_node & strong & !accept;
+      const bool use = ((is_node & strong & accept) | (is_pt & (j != base + p))) & (l < dmax2);
+      float vd = is_node ? h.value : q.w;
+      const bool longer = use & is_node & (chain > 0);
+      if (__builtin_amdgcn_ballot_w64(longer)) {
+        if (longer) {  // the first level of the chain that accepts
+          const double reach = theta2 * l;
+          double w2 = wt * wt;
+          uint32_t Dd = ptop;
+          while (!(w2 < reach)) {
+            w2 *= 0.25;
+            ++Dd;
+          }
+          vd = h.value * dampf<D>(h.depth - Dd);
         }
       }
+      float ex = (float)dx, ey = (float)dy, ez = (float)dz, fl = (float)l;
+      const bool same = use & (l == 0.0);
+      if (__builtin_amdgcn_ballot_w64(same)) {
+        if (same) {  // the hash jiggle of a coincident pair
+          const float4 jj = coincident_far(__float_as_uint(bd.w), is_node ? n + (kid - off) : order[j] - off, D,
+                                           step_no, seed);
+          ex = jj.x, ey = jj.y, ez = jj.z, fl = jj.w;
+        }
+      }
+      const float f = use ? vd * alpha * inv_dist2(fl, dmin2f) : 0.f;
+      float fx = ex * f, fy = D > 1 ? ey * f : 0.f, fz = D > 2 ? ez * f : 0.f;
+      inter += use ? 1u : 0u;
+      const uint32_t cdepth = h.depth;
       fx = group_sum<NK>(fx);
       if (D > 1) fy = group_sum<NK>(fy);
       if (D > 2) fz = group_sum<NK>(fz);
