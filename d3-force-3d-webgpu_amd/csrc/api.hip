@@ -138,6 +138,7 @@ struct fg_ctx {
   uint32_t walk_group = 0;
   uint32_t walk_pool_k = WALK_POOL_BY_MODE;
   uint32_t walk_exact_max = WALK_EXACT_MAX;
+  uint32_t walk_pool_cap = 0xffffffffu;
   bool reduce_aligned = true;
   uint32_t reduce_block = 0;
   const Pass* center_fused = nullptr;  // this tick's last pass, a center left to the integrator's launch
@@ -290,7 +291,7 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
         if (!p.tree) p.tree.reset(new TreeBuffers());
         TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2, c->walk_packed_max, c->walk_tile,
                       c->walk_solo_max, c->walk_wide_max, c->walk_group, c->reduce_block, c->reduce_aligned,
-                      c->walk_pool_k, c->walk_exact_max};
+                      c->walk_pool_k, c->walk_pool_cap, c->walk_exact_max};
         CenterFuse cf;
         if (fuse) cf = CenterFuse{fuse->center.as<CenterWork>(), fuse->d.x, fuse->d.y, fuse->d.z, fuse->d.strength};
         tree_manybody(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), vel[0],
@@ -521,6 +522,7 @@ int fg_create(fg_ctx** out, int device) {
   if (const char* v = getenv("D3FGPU_WALK_GROUP")) c->walk_group = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_WALK_POOL_K")) c->walk_pool_k = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_WALK_EXACT")) c->walk_exact_max = (uint32_t)atoi(v);
+  if (const char* v = getenv("D3FGPU_WALK_POOL_CAP")) c->walk_pool_cap = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_REDUCE_ALIGNED")) c->reduce_aligned = atoi(v) != 0;
   if (const char* v = getenv("D3FGPU_REDUCE_BLOCK")) c->reduce_block = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_RADIX_BITS")) c->radix_bits = atoi(v) == 11 ? 11 : 8;

@@ -115,6 +115,7 @@ struct TreeParams {
   // the tiled walk's cheaper mode: openings by at most this many points go to the pool of
   // single-point openings (0: no pool)
   uint32_t walk_pool_k = WALK_POOL_BY_MODE;
+  uint32_t walk_pool_cap = 0xffffffffu;  // pool entries per wave, at most the compiled size
   // the tiled walk takes its exact mode for trees of at most this many points
   uint32_t walk_exact_max = WALK_EXACT_MAX;
 };

@@ -1310,7 +1310,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
                                           const NodeRec* __restrict__ rec, const WalkTree& T,
                                           uint32_t base, uint32_t count, uint32_t packed_max,
                                           uint32_t lane, SpillEntry* spill, WalkCount& C,
-                                          uint32_t pool_k = POOL_K) {
+                                          uint32_t pool_k = POOL_K, uint32_t pool_cap = POOL_SIZE) {
   constexpr bool POOL = POOLED && !EXACT;
   constexpr int NK = 1 << D;
   const uint32_t off = T.off, n = T.n;
@@ -1452,7 +1452,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
       }
       const uint64_t ob = __builtin_amdgcn_ballot_w64(open);
       if (ob) {
-        const uint32_t room = POOL_SIZE - pn, rank = mask_rank(ob);
+        const uint32_t room = pool_cap > pn ? pool_cap - pn : 0u, rank = mask_rank(ob);
         if (open && rank < room) L.pool[pn + rank] = make_uint2(kid, p << 8 | (cdepth + 1u));
         const uint32_t cnt = (uint32_t)__popcll(ob);
         uint64_t over = __builtin_amdgcn_ballot_w64(open && rank >= room);
@@ -1704,7 +1704,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
         const uint32_t cdepth = ((const NodeHdr*)&L.stage[c][0])->depth;
         if constexpr (POOL) {
           const uint32_t cnt = (uint32_t)__popcll(open);
-          if (cnt <= pool_k && pn + cnt <= POOL_SIZE) {  // one pool entry per point
+          if (cnt <= pool_k && pn + cnt <= pool_cap) {  // one pool entry per point
             const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)my_kid, (int)c);
             if ((open >> lane) & 1ull) L.pool[pn + mask_rank(open)] = make_uint2(k, lane << 8 | (cdepth + 1u));
             pn += cnt;
@@ -1789,6 +1789,7 @@ struct WalkArgs {
   uint32_t packed_max;  // openings by at most this many points use the packed mode
   uint32_t tile;        // points per tile (1..64)
   uint32_t pool_k;      // openings by at most this many points go to the pool (cheaper mode)
+  uint32_t pool_cap;    // pool entries per wave (at most POOL_SIZE)
 };
 
 // Points per tile of an exact walk over n points, on a device of `simds` SIMDs where the
@@ -1856,7 +1857,7 @@ __global__ void __launch_bounds__(WAVES * WAVE, EXACT ? 1 : POOL_WAVES)
     const uint32_t count = A.n - base < A.tile ? A.n - base : A.tile;
     __builtin_amdgcn_wave_barrier();
     walk_tile<D, STACK, EXACT, !EXACT>(lds[wave], leaves, order, rec, T, base, count, A.packed_max, lane, my_spill,
-                               C, A.pool_k);
+                               C, A.pool_k, A.pool_cap);
   }
   walk_flush(C, lane, counters);
 }
@@ -2759,6 +2760,7 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
   // the noise, 40 and 16 slower)
   A.packed_max = p.walk_packed_max != WALK_PACKED_BY_MODE ? p.walk_packed_max : exact ? 32u : 24u;
   A.pool_k = p.walk_pool_k != WALK_POOL_BY_MODE ? p.walk_pool_k : POOL_K;
+  A.pool_cap = p.walk_pool_cap < POOL_SIZE ? p.walk_pool_cap : POOL_SIZE;
   // measured on 256 CUs (link + manyBody + center ticks): a group of lanes per point is the
   // faster walk up to 12,000 points in 3D (0.254 against 0.258 ms/tick) but not at 16,000
   // (0.296 against 0.264), and still at 24,000 in 2D (0.234 against 0.237) and 32,000 in 1D

@@ -309,6 +309,35 @@ def test_manybody_walk_switch_points(monkeypatch, dims, n):
     assert close_count(dflt, ref, rtol=1e-3) <= tree_outliers(n)
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_manybody_pool_of_single_point_openings(monkeypatch, dims):
+    """The tiled walk's cheaper mode (past 262,144 points) sends openings needed by few of a
+    tile's points to a pool of single-point openings.  No pool, the default, every opening
+    below the whole tile pooled, and a pool of 8 entries (always full: its entries go to the
+    stack as one-point openings) sum the same terms in other orders: within rounding of each
+    other, no walk errors, and within the FP64 oracle's bar."""
+    n = 300000
+    pos = make_nodes_fast(n, dims, 20 * n ** (1 / dims), seed=7 + dims)
+    s = np.full(n, -30.0) * (1 + 0.25 * np.sin(np.arange(n)))
+    settings = [{"D3FGPU_WALK_POOL_K": "0"}, {}, {"D3FGPU_WALK_POOL_K": "63"},
+                {"D3FGPU_WALK_POOL_K": "63", "D3FGPU_WALK_POOL_CAP": "8"}]
+    out = []
+    for env in settings:
+        for name in ("D3FGPU_WALK_POOL_K", "D3FGPU_WALK_POOL_CAP"):
+            monkeypatch.delenv(name, raising=False)
+        for name, value in env.items():
+            monkeypatch.setenv(name, value)
+        out.append(walk_dv(pos, dims, s))
+    for dv in out[1:]:
+        assert rel_l2(dv, out[0]) < 1e-6
+    ref, _ = oracle_manybody(pos, dims, s)
+    for env, dv in zip(settings, out):
+        err = rel_l2(dv, ref)
+        report(f"manyBody tree {dims}D n={n} pool {env or 'default'}", err)
+        assert err < 1e-4
+        assert close_count(dv, ref, rtol=1e-3) <= tree_outliers(n)
+
+
 def test_manybody_tree_above_exact_mode(engine_factory):
     """Past 262,144 points the walk takes 64-point tiles and the cheaper accumulation."""
     n = 300000
