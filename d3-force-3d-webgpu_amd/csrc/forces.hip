@@ -1128,17 +1128,21 @@ GridClasses grid_classes(const float* radii, uint32_t n) {
   gc.inv_b = gc.inv_s = 1.0f / (2.0f * rmax);
   gc.r0 = rmax;
   gc.rmax = rmax;
-  static const bool one_class = getenv("D3FGPU_GRID_ONE_CLASS") != nullptr;  // experiment override
+  // experiment overrides: one class; the quantile of the fine class and the ratio above which
+  // a coarse class is split off
+  static const bool one_class = getenv("D3FGPU_GRID_ONE_CLASS") != nullptr;
+  static const double quant = getenv("D3FGPU_GRID_QUANTILE") ? atof(getenv("D3FGPU_GRID_QUANTILE")) : 0.95;
+  static const double split = getenv("D3FGPU_GRID_SPLIT") ? atof(getenv("D3FGPU_GRID_SPLIT")) : 4.0;
   if (n >= 1024 && !one_class) {
     std::vector<float> r(n);
     for (uint32_t i = 0; i < n; ++i) r[i] = std::fabs(radii[i]);
-    const size_t k = (size_t)(0.95 * (double)(n - 1));
+    const size_t k = (size_t)(quant * (double)(n - 1));
     std::nth_element(r.begin(), r.begin() + k, r.end());
-    const float r95 = r[k];
-    if (r95 > 0 && rmax > 4.0f * r95) {
+    const float rq = r[k];
+    if (rq > 0 && rmax > (float)split * rq) {
       gc.two = 1;
-      gc.r0 = r95;
-      gc.inv_s = 1.0f / (2.0f * r95);
+      gc.r0 = rq;
+      gc.inv_s = 1.0f / (2.0f * rq);
     }
   }
   return gc;

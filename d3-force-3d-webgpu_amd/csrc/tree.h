@@ -126,6 +126,16 @@ size_t spill_bytes_for(int cus, int stack);
 // Compute units of the current device.
 int device_cus();
 
+// Build the tree over the current positions (and strengths): what the walk reads.  center:
+// as in tree_manybody.
+void tree_build(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x, const float* y,
+                const float* z, const float* strength, const TreeParams& p, int radix_bits,
+                PhaseTimer* timer = nullptr, const CenterFuse* center = nullptr);
+
+// Add the many-body force of the tree tree_build left to the velocities.
+void tree_forces(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, float* vx, float* vy, float* vz,
+                 const TreeParams& p, StepParams step, PhaseTimer* timer = nullptr);
+
 // Build the tree over the current positions and add the many-body force to the velocities.
 // center (optional): a center force that runs after this one in the tick, with no force
 // between them that moves the nodes; its mean is taken by the box kernel (when there is one:

@@ -2842,9 +2842,9 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
 }
 
 template <int D>
-void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float* y, const float* z,
-         const float* strength, float* vx, float* vy, float* vz, const TreeParams& p,
-         StepParams step, int radix_bits, PhaseTimer* timer, const CenterFuse* center) {
+void build(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float* y, const float* z,
+           const float* strength, const TreeParams& p, int radix_bits, PhaseTimer* timer,
+           const CenterFuse* center) {
   if (timer) timer->begin(1);
   if (n <= REDUCE_ONE) {  // one workgroup builds and reduces the whole tree
     hipLaunchKernelGGL((tree_small<D>), dim3(1), dim3(REDUCE_ONE), 0, st, x, y, z, strength, n, t.cube,
@@ -2853,7 +2853,6 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
     t.skeys = t.keys;
     t.order = t.vals;
     if (timer) timer->end(1);
-    if (n >= 2) walk<D>(st, t, n, vx, vy, vz, p, step, timer);
     return;
   }
   const uint32_t blocks = ceil_div(n, 256);
@@ -2921,7 +2920,6 @@ void run(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float
                        (const uint4*)t.links, t.flags, t.rec, t.sums, cuts);
   FG_LAUNCH_CHECK();
   if (timer) timer->end(1);
-  walk<D>(st, t, n, vx, vy, vz, p, step, timer);
 }
 
 }  // namespace
@@ -2993,16 +2991,31 @@ void TreeBuffers::release() {
   cap = 0;
 }
 
+void tree_build(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x, const float* y,
+                const float* z, const float* strength, const TreeParams& p, int radix_bits, PhaseTimer* timer,
+                const CenterFuse* center) {
+  if (n < 1) return;
+  if (center && !tree_box_pass(n)) throw std::logic_error("tree_build: no box kernel to take the center's mean");
+  t.reserve(n);
+  if (ndim == 1) build<1>(s, t, n, x, y, z, strength, p, radix_bits, timer, center);
+  else if (ndim == 2) build<2>(s, t, n, x, y, z, strength, p, radix_bits, timer, center);
+  else build<3>(s, t, n, x, y, z, strength, p, radix_bits, timer, center);
+}
+
+void tree_forces(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, float* vx, float* vy, float* vz,
+                 const TreeParams& p, StepParams step, PhaseTimer* timer) {
+  if (n < 2) return;
+  if (ndim == 1) walk<1>(s, t, n, vx, vy, vz, p, step, timer);
+  else if (ndim == 2) walk<2>(s, t, n, vx, vy, vz, p, step, timer);
+  else walk<3>(s, t, n, vx, vy, vz, p, step, timer);
+}
+
 void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x,
                    const float* y, const float* z, const float* strength, float* vx, float* vy,
                    float* vz, const TreeParams& p, StepParams step, int radix_bits,
                    PhaseTimer* timer, const CenterFuse* center) {
-  if (n < 1) return;
-  if (center && !tree_box_pass(n)) throw std::logic_error("tree_manybody: no box kernel to take the center's mean");
-  t.reserve(n);
-  if (ndim == 1) run<1>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer, center);
-  else if (ndim == 2) run<2>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer, center);
-  else run<3>(s, t, n, x, y, z, strength, vx, vy, vz, p, step, radix_bits, timer, center);
+  tree_build(s, t, n, ndim, x, y, z, strength, p, radix_bits, timer, center);
+  tree_forces(s, t, n, ndim, vx, vy, vz, p, step, timer);
 }
 
 bool tree_box_pass(uint32_t n) { return n > REDUCE_ONE; }

@@ -355,9 +355,17 @@ class Engine:
     def sync(self):
         self._check(lib().fg_sync(self.h))
 
-    def readback(self):
-        pos = np.zeros((self.n, 3))
-        vel = np.zeros((self.n, 3))
+    def readback(self, out=None):
+        """Positions and velocities as (n, 3) float64 arrays; out: a (pos, vel) pair of such
+        arrays (C-contiguous) to fill instead of new ones."""
+        if out is None:
+            pos = np.zeros((self.n, 3))
+            vel = np.zeros((self.n, 3))
+        else:
+            pos, vel = out
+            for a in (pos, vel):
+                if a.shape != (self.n, 3) or a.dtype != np.float64 or not a.flags.c_contiguous:
+                    raise ValueError("readback: out arrays must be C-contiguous float64 of shape (n, 3)")
         self._check(lib().fg_readback(self.h, ptr(pos), ptr(vel)))
         return pos, vel
 
