@@ -139,6 +139,7 @@ struct fg_ctx {
   uint32_t walk_pool_k = WALK_POOL_BY_MODE;
   uint32_t walk_exact_max = WALK_EXACT_MAX;
   uint32_t walk_pool_cap = 0xffffffffu;
+  bool fuse_link = true;  // a link pass in a one-workgroup tree build's launch (D3FGPU_FUSE_LINK=0: not)
   bool reduce_aligned = true;
   uint32_t reduce_block = 0;
   const Pass* center_fused = nullptr;  // this tick's last pass, a center left to the integrator's launch
@@ -270,7 +271,10 @@ void update_pass(fg_ctx* c, Pass& p, const fg_force_desc& d) {
 // pass); otherwise null.
 // fuse: for a many-body pass, the center pass whose mean its tree's box kernel takes; for
 // that center pass itself, its own address (its shift is then ready).
-const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center = false, const Pass* fuse = nullptr) {
+// link: for a many-body pass whose tree one workgroup builds, the link pass right before it,
+// which the build's launch runs (its buffer swap included).
+const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center = false, const Pass* fuse = nullptr,
+                           const Pass* link = nullptr) {
   const uint32_t n = c->n;
   const int nd = c->ndim;
   float* pos[3] = {c->P(0), c->P(1), c->P(2)};
@@ -292,6 +296,15 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
         TreeParams tp{p.d.theta2, p.d.distance_min2, p.d.distance_max2, c->walk_packed_max, c->walk_tile,
                       c->walk_solo_max, c->walk_wide_max, c->walk_group, c->reduce_block, c->reduce_aligned,
                       c->walk_pool_k, c->walk_pool_cap, c->walk_exact_max};
+        if (link) {  // the link pass, from the current velocities into the other set, in the build's launch
+          const LinkFuse lf{c->V(0), c->V(1), c->V(2), c->W(0), c->W(1), c->W(2), link->row.as<uint32_t>(),
+                            link->ent.as<LinkEntry>(), sp.alpha, sp.step, sp.seed};
+          tree_build(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), tp, c->radix_bits, c->T(),
+                     nullptr, &lf);
+          c->cur ^= 1;
+          tree_forces(c->stream, *p.tree, n, nd, c->V(0), c->V(1), c->V(2), tp, sp, c->T());
+          break;
+        }
         CenterFuse cf;
         if (fuse) cf = CenterFuse{fuse->center.as<CenterWork>(), fuse->d.x, fuse->d.y, fuse->d.z, fuse->d.strength};
         tree_manybody(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), vel[0],
@@ -523,6 +536,7 @@ int fg_create(fg_ctx** out, int device) {
   if (const char* v = getenv("D3FGPU_WALK_POOL_K")) c->walk_pool_k = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_WALK_EXACT")) c->walk_exact_max = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_WALK_POOL_CAP")) c->walk_pool_cap = (uint32_t)atoi(v);
+  if (const char* v = getenv("D3FGPU_FUSE_LINK")) c->fuse_link = atoi(v) != 0;
   if (const char* v = getenv("D3FGPU_REDUCE_ALIGNED")) c->reduce_aligned = atoi(v) != 0;
   if (const char* v = getenv("D3FGPU_REDUCE_BLOCK")) c->reduce_block = (uint32_t)atoi(v);
   if (const char* v = getenv("D3FGPU_RADIX_BITS")) c->radix_bits = atoi(v) == 11 ? 11 : 8;
@@ -761,9 +775,25 @@ int fg_enqueue(fg_ctx* c, uint32_t steps, const double* alphas, double decay) {
           fuse_mb = (int)i;
       }
       const Pass* fused = fuse_ct >= 0 ? c->passes[fuse_ct].get() : nullptr;
-      for (size_t i = 0; i < c->passes.size(); ++i)
+      // a link pass of one iteration right before a many-body pass whose tree one workgroup
+      // builds: the build's launch runs it on the CUs the build leaves idle (link only changes
+      // velocities, the build reads positions), one launch fewer
+      int link_mb = -1;
+      for (size_t i = 0; i + 1 < c->passes.size() && c->fuse_link; ++i) {
+        const Pass& a = *c->passes[i];
+        const Pass& b = *c->passes[i + 1];
+        if (a.type == FG_FORCE_LINK && a.d.iterations == 1 && a.m && b.type == FG_FORCE_MANYBODY &&
+            b.d.theta2 > EXACT_THETA2 && tree_link_pass(c->n)) {
+          link_mb = (int)i + 1;
+          break;
+        }
+      }
+      for (size_t i = 0; i < c->passes.size(); ++i) {
+        if ((int)i + 1 == link_mb) continue;  // run by the next pass's build
         shift = run_pass(c, *c->passes[i], sp, i + 1 == c->passes.size(),
-                         (int)i == fuse_mb || (int)i == fuse_ct ? fused : nullptr);
+                         (int)i == fuse_mb || (int)i == fuse_ct ? fused : nullptr,
+                         (int)i == link_mb ? c->passes[i - 1].get() : nullptr);
+      }
       float* pos[3] = {c->P(0), c->P(1), c->P(2)};
       float* vel[3] = {c->V(0), c->V(1), c->V(2)};
       float* fix[3] = {c->fix[0].as<float>(), c->fix[1].as<float>(), c->fix[2].as<float>()};

@@ -18,6 +18,68 @@ struct LinkEntry {
   float bias;
 };
 
+// The snapshot p + v of node j: read from the snapshot array, or summed where it is read.
+struct SnapArray {
+  const float4* __restrict__ snap;
+  __device__ float4 operator()(uint32_t j) const { return snap[j]; }
+};
+template <int D>
+struct SnapDirect {
+  const float *__restrict__ x, *__restrict__ y, *__restrict__ z;
+  const float *__restrict__ vx, *__restrict__ vy, *__restrict__ vz;
+  __device__ float4 operator()(uint32_t j) const {
+    return make_float4(x[j] + vx[j], D > 1 ? y[j] + vy[j] : 0.f, D > 2 ? z[j] + vz[j] : 0.f, 0.f);
+  }
+};
+
+// One node's incident links: d3's spring along each link between the snapshots of its
+// ends, and the node's share of it (bias for the target, 1 - bias for the source).
+template <int D, class Snap>
+__device__ inline void link_node(uint32_t i, Snap snap,
+                                 const float* __restrict__ vx, const float* __restrict__ vy,
+                                 const float* __restrict__ vz, float* __restrict__ ox,
+                                 float* __restrict__ oy, float* __restrict__ oz,
+                                 const uint32_t* __restrict__ row, const LinkEntry* __restrict__ ent,
+                                 float alpha, uint32_t sn, uint32_t seed) {
+  const float4 me = snap(i);
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  const uint32_t e0 = row[i], e1 = row[i + 1];
+  for (uint32_t e = e0; e < e1; ++e) {
+    const LinkEntry L = ent[e];
+    const float4 t = snap(L.other);
+    const bool target = L.code & 1u;
+    const uint32_t id = L.code >> 1;
+    // d = snap[target] - snap[source], the same vector at both ends
+    float dx = target ? me.x - t.x : t.x - me.x;
+    float dy = target ? me.y - t.y : t.y - me.y;
+    float dz = target ? me.z - t.z : t.z - me.z;
+    if (dx == 0.f) dx = jiggle_hash(id, 0u, 0u, sn, seed);
+    if (D > 1 && dy == 0.f) dy = jiggle_hash(id, 0u, 1u, sn, seed);
+    if (D > 2 && dz == 0.f) dz = jiggle_hash(id, 0u, 2u, sn, seed);
+    const float l = sqrtf(dx * dx + (D > 1 ? dy * dy : 0.f) + (D > 2 ? dz * dz : 0.f));
+    const float f = (l - L.distance) / l * alpha * L.strength;
+    const float w = target ? -L.bias : 1.0f - L.bias;
+    ax += dx * f * w;
+    if (D > 1) ay += dy * f * w;
+    if (D > 2) az += dz * f * w;
+  }
+  ox[i] = vx[i] + ax;
+  if (D > 1) oy[i] = vy[i] + ay;
+  if (D > 2) oz[i] = vz[i] + az;
+}
+
+// A link pass run by extra workgroups of another kernel (the one-workgroup tree build):
+// every node's incident links from the snapshots summed where they are read, velocities
+// from (vx, vy, vz) into (ox, oy, oz).
+struct LinkFuse {
+  const float *vx, *vy, *vz;
+  float *ox, *oy, *oz;
+  const uint32_t* row;
+  const LinkEntry* ent;
+  float alpha;
+  uint32_t step, seed;
+};
+
 struct CenterWork {
   static constexpr uint32_t MAX_BLOCKS = 1024;
   double partial[MAX_BLOCKS][3];

@@ -127,10 +127,14 @@ size_t spill_bytes_for(int cus, int stack);
 int device_cus();
 
 // Build the tree over the current positions (and strengths): what the walk reads.  center:
-// as in tree_manybody.
+// as in tree_manybody.  link (optional, only where tree_link_pass): a link pass that comes
+// right before this force, run by the build's launch on the CUs the build leaves idle.
 void tree_build(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x, const float* y,
                 const float* z, const float* strength, const TreeParams& p, int radix_bits,
-                PhaseTimer* timer = nullptr, const CenterFuse* center = nullptr);
+                PhaseTimer* timer = nullptr, const CenterFuse* center = nullptr, const LinkFuse* link = nullptr);
+
+// Whether the tree of n points is built by one workgroup, whose launch can take a link pass.
+bool tree_link_pass(uint32_t n);
 
 // Add the many-body force of the tree tree_build left to the velocities.
 void tree_forces(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, float* vx, float* vy, float* vz,

@@ -930,8 +930,14 @@ __global__ void __launch_bounds__(REDUCE_ONE) tree_small(const float* __restrict
                                                          uint64_t* __restrict__ skeys, uint32_t* __restrict__ order,
                                                          float4* __restrict__ leaves, uint32_t* __restrict__ leaf_depth,
                                                          uint4* __restrict__ links, uint32_t* flags, NodeRec* rec,
-                                                         NodeSum* sums) {
+                                                         NodeSum* sums, LinkFuse link) {
   constexpr uint32_t B = REDUCE_ONE, W = B / WAVE;
+  if (blockIdx.x > 0) {  // the workgroups past the first run the link pass before this force
+    for (uint32_t v = (blockIdx.x - 1) * B + threadIdx.x; v < n; v += (gridDim.x - 1) * B)
+      link_node<D>(v, SnapDirect<D>{x, y, z, link.vx, link.vy, link.vz}, link.vx, link.vy, link.vz, link.ox, link.oy,
+                   link.oz, link.row, link.ent, link.alpha, link.step, link.seed);
+    return;
+  }
   __shared__ ReduceLDS<B> S;
   __shared__ uint64_t s_key[2][B];
   __shared__ uint32_t s_idx[2][B];
@@ -2844,11 +2850,13 @@ void walk(hipStream_t st, TreeBuffers& t, uint32_t n, float* vx, float* vy, floa
 template <int D>
 void build(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const float* y, const float* z,
            const float* strength, const TreeParams& p, int radix_bits, PhaseTimer* timer,
-           const CenterFuse* center) {
+           const CenterFuse* center, const LinkFuse* link) {
   if (timer) timer->begin(1);
-  if (n <= REDUCE_ONE) {  // one workgroup builds and reduces the whole tree
-    hipLaunchKernelGGL((tree_small<D>), dim3(1), dim3(REDUCE_ONE), 0, st, x, y, z, strength, n, t.cube,
-                       t.counters, t.keys, t.vals, t.leaves, t.leaf_depth, t.links, t.flags, t.rec, t.sums);
+  if (n <= REDUCE_ONE) {  // one workgroup builds and reduces the whole tree (more run a link pass)
+    const uint32_t blocks = link ? 1 + ceil_div(n, REDUCE_ONE) : 1;
+    hipLaunchKernelGGL((tree_small<D>), dim3(blocks), dim3(REDUCE_ONE), 0, st, x, y, z, strength, n, t.cube,
+                       t.counters, t.keys, t.vals, t.leaves, t.leaf_depth, t.links, t.flags, t.rec, t.sums,
+                       link ? *link : LinkFuse{});
     FG_LAUNCH_CHECK();
     t.skeys = t.keys;
     t.order = t.vals;
@@ -2993,13 +3001,14 @@ void TreeBuffers::release() {
 
 void tree_build(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float* x, const float* y,
                 const float* z, const float* strength, const TreeParams& p, int radix_bits, PhaseTimer* timer,
-                const CenterFuse* center) {
+                const CenterFuse* center, const LinkFuse* link) {
   if (n < 1) return;
   if (center && !tree_box_pass(n)) throw std::logic_error("tree_build: no box kernel to take the center's mean");
+  if (link && !tree_link_pass(n)) throw std::logic_error("tree_build: no one-workgroup build to take a link pass");
   t.reserve(n);
-  if (ndim == 1) build<1>(s, t, n, x, y, z, strength, p, radix_bits, timer, center);
-  else if (ndim == 2) build<2>(s, t, n, x, y, z, strength, p, radix_bits, timer, center);
-  else build<3>(s, t, n, x, y, z, strength, p, radix_bits, timer, center);
+  if (ndim == 1) build<1>(s, t, n, x, y, z, strength, p, radix_bits, timer, center, link);
+  else if (ndim == 2) build<2>(s, t, n, x, y, z, strength, p, radix_bits, timer, center, link);
+  else build<3>(s, t, n, x, y, z, strength, p, radix_bits, timer, center, link);
 }
 
 void tree_forces(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, float* vx, float* vy, float* vz,
@@ -3019,6 +3028,8 @@ void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const fl
 }
 
 bool tree_box_pass(uint32_t n) { return n > REDUCE_ONE; }
+
+bool tree_link_pass(uint32_t n) { return n >= 1 && n <= REDUCE_ONE; }
 
 }  // namespace fg
 
