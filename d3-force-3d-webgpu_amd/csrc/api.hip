@@ -271,8 +271,8 @@ void update_pass(fg_ctx* c, Pass& p, const fg_force_desc& d) {
 // pass); otherwise null.
 // fuse: for a many-body pass, the center pass whose mean its tree's box kernel takes; for
 // that center pass itself, its own address (its shift is then ready).
-// link: for a many-body pass whose tree one workgroup builds, the link pass right before it,
-// which the build's launch runs (its buffer swap included).
+// link: for a many-body pass whose tree build can take it (tree_link_pass), the link pass
+// right before it, which the build's first launch runs (its buffer swap included).
 const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center = false, const Pass* fuse = nullptr,
                            const Pass* link = nullptr) {
   const uint32_t n = c->n;
@@ -299,8 +299,10 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
         if (link) {  // the link pass, from the current velocities into the other set, in the build's launch
           const LinkFuse lf{c->V(0), c->V(1), c->V(2), c->W(0), c->W(1), c->W(2), link->row.as<uint32_t>(),
                             link->ent.as<LinkEntry>(), sp.alpha, sp.step, sp.seed};
+          CenterFuse cf;
+          if (fuse) cf = CenterFuse{fuse->center.as<CenterWork>(), fuse->d.x, fuse->d.y, fuse->d.z, fuse->d.strength};
           tree_build(c->stream, *p.tree, n, nd, pos[0], pos[1], pos[2], p.a.as<float>(), tp, c->radix_bits, c->T(),
-                     nullptr, &lf);
+                     fuse ? &cf : nullptr, &lf);
           c->cur ^= 1;
           tree_forces(c->stream, *p.tree, n, nd, c->V(0), c->V(1), c->V(2), tp, sp, c->T());
           break;
@@ -775,9 +777,10 @@ int fg_enqueue(fg_ctx* c, uint32_t steps, const double* alphas, double decay) {
           fuse_mb = (int)i;
       }
       const Pass* fused = fuse_ct >= 0 ? c->passes[fuse_ct].get() : nullptr;
-      // a link pass of one iteration right before a many-body pass whose tree one workgroup
-      // builds: the build's launch runs it on the CUs the build leaves idle (link only changes
-      // velocities, the build reads positions), one launch fewer
+      // a link pass of one iteration right before a many-body pass, on a graph small enough
+      // for the one-launch link pass: the tree build's first launch (the one-workgroup build,
+      // or the box kernel) runs it in workgroups of its own on CUs it leaves idle (link only
+      // changes velocities, the build reads positions), one launch fewer
       int link_mb = -1;
       for (size_t i = 0; i + 1 < c->passes.size() && c->fuse_link; ++i) {
         const Pass& a = *c->passes[i];

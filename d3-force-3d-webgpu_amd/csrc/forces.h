@@ -68,6 +68,10 @@ __device__ inline void link_node(uint32_t i, Snap snap,
   if (D > 2) oz[i] = vz[i] + az;
 }
 
+// Graphs of at most this many nodes take the one-launch link pass (the snapshots summed where
+// they are read).
+constexpr uint32_t LINK_DIRECT_MAX = 32768;
+
 // A link pass run by extra workgroups of another kernel (the one-workgroup tree build):
 // every node's incident links from the snapshots summed where they are read, velocities
 // from (vx, vy, vz) into (ox, oy, oz).

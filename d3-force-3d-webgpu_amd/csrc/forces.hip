@@ -209,8 +209,6 @@ __global__ void __launch_bounds__(256) link_direct_kernel(uint32_t n, const floa
                step.step, step.seed);
 }
 
-// Graphs of at most this many nodes take the one-launch link pass.
-constexpr uint32_t LINK_DIRECT_MAX = 32768;
 
 void launch_link(hipStream_t s, uint32_t n, int ndim, float* const* pos, const float* const* vin,
                  float* const* vout, const uint32_t* row, const LinkEntry* ent, float4* snap,

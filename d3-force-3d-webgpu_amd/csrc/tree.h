@@ -133,7 +133,8 @@ void tree_build(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const float
                 const float* z, const float* strength, const TreeParams& p, int radix_bits,
                 PhaseTimer* timer = nullptr, const CenterFuse* center = nullptr, const LinkFuse* link = nullptr);
 
-// Whether the tree of n points is built by one workgroup, whose launch can take a link pass.
+// Whether the build of a tree of n points can take a link pass in its first launch (the
+// one-workgroup build or the box kernel: the graphs that take the one-launch link pass).
 bool tree_link_pass(uint32_t n);
 
 // Add the many-body force of the tree tree_build left to the velocities.

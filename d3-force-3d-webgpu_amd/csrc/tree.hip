@@ -171,7 +171,14 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
                                                  const float* __restrict__ y,
                                                  const float* __restrict__ z, uint32_t n,
                                                  CubeInfo* __restrict__ cube, uint32_t* __restrict__ counters,
-                                                 SortClear sort_clear, CenterFuse center) {
+                                                 SortClear sort_clear, CenterFuse center, uint32_t boxes,
+                                                 LinkFuse link) {
+  if (blockIdx.x >= boxes) {  // the workgroups past the box's run the link pass before this force
+    for (uint32_t v = (blockIdx.x - boxes) * 256 + threadIdx.x; v < n; v += (gridDim.x - boxes) * 256)
+      link_node<D>(v, SnapDirect<D>{x, y, z, link.vx, link.vy, link.vz}, link.vx, link.vy, link.vz, link.ox, link.oy,
+                   link.oz, link.row, link.ent, link.alpha, link.step, link.seed);
+    return;
+  }
   if (blockIdx.x == 0) {
     if (threadIdx.x < TREE_COUNTERS) counters[threadIdx.x] = 0;
     sort_clear_block(sort_clear);
@@ -188,7 +195,7 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
       for (int a = 0; a < D; ++a) sum[a] += (double)p[a];
     }
   };
-  const uint32_t stride = gridDim.x * 256;
+  const uint32_t stride = boxes * 256;
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   for (; i + 3 * stride < n; i += 4 * stride) {  // four independent loads in flight
     take(i);
@@ -221,12 +228,12 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
     if (cw)
       for (int a = 0; a < 3; ++a)
         store_coherent(&cw->partial[blockIdx.x][a], s_sum[0][a] + s_sum[1][a] + s_sum[2][a] + s_sum[3][a]);
-    s_last = publish_count(&cube->done) + 1 == gridDim.x;
+    s_last = publish_count(&cube->done) + 1 == boxes;
   }
   __syncthreads();
   if (!s_last) return;
   uint32_t m[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256)
+  for (uint32_t b = threadIdx.x; b < boxes; b += 256)
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       m[a] = min(m[a], load_coherent(&cube->part[b][a]));
@@ -260,13 +267,13 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
   if (cw) {  // then center_reduce's fold of the partials, in workgroup order
     __shared__ double tsum[256][3];
     double t[3] = {0, 0, 0};
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256)
+    for (uint32_t b = threadIdx.x; b < boxes; b += 256)
       for (int a = 0; a < 3; ++a) t[a] += load_coherent(&cw->partial[b][a]);
     for (int a = 0; a < 3; ++a) tsum[threadIdx.x][a] = t[a];
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int a = 0; a < 3; ++a) t[a] = 0;
-      for (uint32_t k = 0; k < gridDim.x && k < 256; ++k)  // the rest are +0: no change
+      for (uint32_t k = 0; k < boxes && k < 256; ++k)  // the rest are +0: no change
         for (int a = 0; a < 3; ++a) t[a] += tsum[k][a];
       const double c[3] = {center.cx, center.cy, center.cz};
       for (int a = 0; a < 3; ++a) cw->shift[a] = a < D ? (float)((t[a] / n - c[a]) * center.strength) : 0.0f;
@@ -2867,8 +2874,9 @@ void build(hipStream_t st, TreeBuffers& t, uint32_t n, const float* x, const flo
   const SortClear sc = sort_prepare(t.sort, radix_bits, n, Dims<D>::HALF);
   const uint32_t bb = blocks < BBOX_BLOCKS ? blocks : BBOX_BLOCKS;
   static_assert(BBOX_BLOCKS == CENTER_BLOCKS, "the box kernel takes center_reduce's grid");
-  hipLaunchKernelGGL((tree_bbox<D>), dim3(bb), dim3(256), 0, st, x, y, z, n, t.cube, t.counters, sc,
-                     center ? *center : CenterFuse{});
+  const uint32_t link_blocks = link ? blocks : 0u;  // the link pass: one node per thread
+  hipLaunchKernelGGL((tree_bbox<D>), dim3(bb + link_blocks), dim3(256), 0, st, x, y, z, n, t.cube, t.counters, sc,
+                     center ? *center : CenterFuse{}, bb, link ? *link : LinkFuse{});
   FG_LAUNCH_CHECK();
   // the counting sort's ranks and counters live in leaf_parent and flags until the build
   const bool counting = n <= RANK_MAX;
@@ -3029,7 +3037,7 @@ void tree_manybody(hipStream_t s, TreeBuffers& t, uint32_t n, int ndim, const fl
 
 bool tree_box_pass(uint32_t n) { return n > REDUCE_ONE; }
 
-bool tree_link_pass(uint32_t n) { return n >= 1 && n <= REDUCE_ONE; }
+bool tree_link_pass(uint32_t n) { return n >= 1 && n <= LINK_DIRECT_MAX; }
 
 }  // namespace fg
 

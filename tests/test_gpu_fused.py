@@ -1,6 +1,7 @@
 """Passes that share a launch leave the state they leave apart, bit for bit: a link pass of one
-iteration right before a many-body pass whose tree one workgroup builds runs in that build's
-launch (up to 1,024 nodes), in 1D, 2D and 3D, over ticks of link + manyBody + center."""
+iteration right before a many-body pass runs in the tree build's first launch -- the
+one-workgroup build up to 1,024 nodes, the box kernel (with the center mean) up to 32,768 --
+in 1D, 2D and 3D, over ticks of link + manyBody + center."""
 import numpy as np
 import pytest
 
@@ -27,7 +28,7 @@ def state_after(monkeypatch, fuse, pos, dims, forces, ticks):
 
 
 @pytest.mark.parametrize("dims", [1, 2, 3])
-@pytest.mark.parametrize("n", [2, 300, 1024, 1025])
+@pytest.mark.parametrize("n", [2, 300, 1024, 1025, 5000, 32768, 32769])
 def test_link_in_the_tree_build_launch_changes_nothing(monkeypatch, dims, n):
     pos = make_nodes(n, dims)
     src, dst = make_graph(n, 42)
