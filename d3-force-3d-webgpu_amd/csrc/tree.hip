@@ -1209,38 +1209,39 @@ __device__ inline T sq_len(T dx, T dy, T dz) {
 // 1 / l' for d3's squared distance l, where below distanceMin l' = sqrt(dmin2 l)
 // (manyBody.js:95): rsq(l * max(l, dmin2)) either way, one reciprocal square root.
 __device__ inline float inv_dist2(float l, float dmin2) {
-  return __builtin_amdgcn_rsqf(l * fmaxf(l, dmin2));
+  return __builtin_amdgcn_rsqf(l * (l < dmin2 ? dmin2 : l));  // (a select: fmaxf quiets NaNs first)
 }
 
-// The force of one node (its cell chain from depth `top`, whose first width squared is
-// w2top) on one body, with d3's rules (barnesHut.js:719-746, manyBody.js:73-99).
+// d3's opening test w * w / theta2 < l (manyBody.js:85) for a cell at depth d: its width is
+// size / 2^d (every split halves it exactly), so w * w / theta2 is exactly thr0 / 4^d with
+// thr0 = size * size / theta2, the root's.
+__device__ inline double open_below(double thr0, uint32_t d) {
+  return ldexp(thr0, -2 * (int)d);
+}
+
+// The force of one node (its cell chain from depth `top`; thr0 = size * size / theta2 of
+// the tree) on one body, with d3's rules (barnesHut.js:719-746, manyBody.js:73-99).
 // `cell` is the node's index within its tree (the jiggle's stand-in index is n + cell).
 // Returns whether the body is done with the node (false: it must open it).  ACC: how the
 // force reaches fx, fy, fz (accumulate).
 template <int D, int ACC>
 __device__ inline bool cell_force(double mx, double my, double mz, uint32_t self_node, uint32_t n,
-                                  uint32_t cell, uint32_t top, double w2top, const NodeHdr& h,
-                                  double theta2, double dmax2, float dmin2f, float alpha,
+                                  uint32_t cell, uint32_t top, double thr0, const NodeHdr& h,
+                                  double dmax2, float dmin2f, float alpha,
                                   uint32_t step_no, uint32_t seed, float& fx, float& fy, float& fz,
                                   uint32_t& inter) {
   if (top > h.depth) return false;  // points sharing the smallest cell: open
   if (h.value == 0.0f) return true;  // d3 skips cells with no strength
   const double dx = h.cx - mx, dy = D > 1 ? h.cy - my : 0.0, dz = D > 2 ? h.cz - mz : 0.0;
   const double l = ACC == ACC_FUSED ? dx * dx + dy * dy + dz * dz : sq_len<D>(dx, dy, dz);
-  const double reach = theta2 * l;
-  // The widths of the chain shrink by 4 (exactly) per level, so some level accepts the cell
-  // exactly when its deepest one does: one test decides opening.  The level d3 stops at is
-  // the first that accepts, found only for a chain of more than one level.
-  const uint32_t chain = h.depth - top;
-  if (!((chain == 0u ? w2top : ldexp(w2top, -2 * (int)chain)) < reach)) return false;
+  // The thresholds of the chain shrink by 4 (exactly) per level, so some level accepts the
+  // cell exactly when its deepest one does: one test decides opening.  The level d3 stops at
+  // is the first that accepts, found only for a chain of more than one level.
+  if (!(open_below(thr0, h.depth) < l)) return false;
   float vd = h.value;
-  if (chain != 0u) {
-    double w2 = w2top;
+  if (h.depth != top) {
     uint32_t Dd = top;
-    while (!(w2 < reach)) {
-      w2 *= 0.25;
-      ++Dd;
-    }
+    while (!(open_below(thr0, Dd) < l)) ++Dd;
     vd = h.value * dampf<D>(h.depth - Dd);
   }
   if (l >= dmax2) return true;
@@ -1303,7 +1304,7 @@ struct alignas(16) SpillEntry {
 // points and nodes are [off, off + n) of the shared arrays.
 struct WalkTree {
   uint32_t off, n;
-  double size, theta2, dmax2;
+  double size, thr0, dmax2;  // thr0 = size * size / theta2
   float dmin2f, dmax2f, alpha;
   uint32_t step_no, seed;
   float *vx, *vy, *vz;  // the velocities, indexed by the node index within the tree
@@ -1331,8 +1332,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
   const bool live = lane < count;
   const float alpha = T.alpha;
   const uint32_t step_no = T.step_no, seed = T.seed;
-  const double size = T.size;
-  const double theta2 = T.theta2, dmax2 = T.dmax2;
+  const double thr0 = T.thr0, dmax2 = T.dmax2;
   const float dmin2f = T.dmin2f, dmax2f = T.dmax2f;
   const float4 me4 = live ? leaves[s] : make_float4(0.f, 0.f, 0.f, 0.f);
   const uint32_t self_node = live ? order[s] - off : 0u;
@@ -1350,8 +1350,8 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
   {
     const NodeRec& r = rec[off];
     const NodeHdr h = *(const NodeHdr*)&r;
-    bool done = live ? cell_force<D, EXACT ? ACC_SET : ACC_FUSED>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, size * size, h,
-                                     theta2, dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter)
+    bool done = live ? cell_force<D, EXACT ? ACC_SET : ACC_FUSED>(me4.x, me4.y, me4.z, self_node, n, 0u, 0u, thr0, h,
+                                     dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter)
                      : true;
     if (EXACT) {
       L.acc[0][lane] = ax;
@@ -1420,11 +1420,10 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
       const double dy = D > 1 ? (is_node ? h.cy : (double)q.y) - (double)bd.y : 0.0;
       const double dz = D > 2 ? (is_node ? h.cz : (double)q.z) - (double)bd.z : 0.0;
       const double l = dx * dx + dy * dy + dz * dz;
-      const double wt = ldexp(size, -(int)ptop);
       const int chain = (int)h.depth - (int)ptop;
-      // a chain's widths shrink by exactly 4 per level: its deepest level decides; a node
+      // a chain's thresholds shrink by exactly 4 per level: its deepest level decides; a node
       // above its parent's depth (inside a run of equal codes) always opens
-      const bool accept = (chain >= 0) & (ldexp(wt * wt, -2 * (chain > 0 ? chain : 0)) < theta2 * l);
+      const bool accept = (chain >= 0) & (open_below(thr0, h.depth) < l);
       const bool strong = h.value != 0.0f;
       const bool open = is_node & strong & !accept;
       const bool use = ((is_node & strong & accept) | (is_pt & (j != base + p))) & (l < dmax2);
@@ -1432,13 +1431,8 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
       const bool longer = use & is_node & (chain > 0);
       if (__builtin_amdgcn_ballot_w64(longer)) {
         if (longer) {  // the first level of the chain that accepts
-          const double reach = theta2 * l;
-          double w2 = wt * wt;
           uint32_t Dd = ptop;
-          while (!(w2 < reach)) {
-            w2 *= 0.25;
-            ++Dd;
-          }
+          while (!(open_below(thr0, Dd) < l)) ++Dd;
           vd = h.value * dampf<D>(h.depth - Dd);
         }
       }
@@ -1553,8 +1547,6 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
     const uint32_t nk = (uint32_t)__popcll(__ballot(lane < (uint32_t)NK && my_kid != REF_NONE));
     // the children's cell chains start one level below the popped node's cell
     const uint32_t top = (uint32_t)__builtin_amdgcn_readfirstlane(L.depth[e]) + 1;
-    const double wt = ldexp(size, -(int)top);
-    const double w2top = wt * wt;
 
     uint32_t open_bits = 0;  // bit c: this lane's point must open child c
     if (k_active <= packed_max) {
@@ -1579,7 +1571,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
                              dmax2f, dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           } else {
             const NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
-            open_item = !cell_force<D, EXACT ? ACC_SET : ACC_FUSED>(bd.x, bd.y, bd.z, bnode, n, k - off, top, w2top, h, theta2,
+            open_item = !cell_force<D, EXACT ? ACC_SET : ACC_FUSED>(bd.x, bd.y, bd.z, bnode, n, k - off, top, thr0, h,
                                        dmax2, dmin2f, alpha, step_no, seed, fx, fy, fz, inter);
           }
         }
@@ -1618,7 +1610,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
         NodeHdr h = *(const NodeHdr*)&L.stage[c][0];
         h.depth = (uint32_t)__builtin_amdgcn_readfirstlane((int)h.depth);
         h.value = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(h.value)));
-        if (!cell_force<D, ACC_FUSED>(me4.x, me4.y, me4.z, self_node, n, k - off, top, w2top, h, theta2,
+        if (!cell_force<D, ACC_FUSED>(me4.x, me4.y, me4.z, self_node, n, k - off, top, thr0, h,
                                   dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter))
           open_bits |= 1u << c;
       }
@@ -1646,7 +1638,7 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
                                   off, dmax2f, dmin2f, alpha, step_no, seed, px, py, pz, inter);
           } else {
             const NodeHdr hd = *(const NodeHdr*)&L.stage[c][0];
-            if (!cell_force<D, ACC>(me4.x, me4.y, me4.z, self_node, n, k - off, top, w2top, hd, theta2,
+            if (!cell_force<D, ACC>(me4.x, me4.y, me4.z, self_node, n, k - off, top, thr0, hd,
                                     dmax2, dmin2f, alpha, step_no, seed, px, py, pz, inter))
               open_bits |= 1u << c;
           }
@@ -1840,7 +1832,7 @@ __global__ void __launch_bounds__(WAVES * WAVE, EXACT ? 1 : POOL_WAVES)
   T.off = 0;
   T.n = A.n;
   T.size = (double)cube->size;
-  T.theta2 = A.theta2;
+  T.thr0 = T.size * T.size / A.theta2;
   T.dmax2 = A.dmax2;
   T.dmin2f = (float)A.dmin2;
   T.dmax2f = (float)fmin(A.dmax2, 3.0e38);
@@ -1911,7 +1903,7 @@ __global__ void __launch_bounds__(WAVES * WAVE) tree_walk_solo(const float4* __r
   const uint32_t n = A.n, s = (blockIdx.x * WAVES + wave) * G + g;
   const bool live = s < n;
   const double size = (double)cube->size;
-  const double theta2 = A.theta2, dmax2 = A.dmax2;
+  const double thr0 = size * size / A.theta2, dmax2 = A.dmax2;
   const float dmin2f = (float)A.dmin2, dmax2f = (float)fmin(A.dmax2, 3.0e38);
   const float alpha = step.alpha;
   const uint32_t step_no = step.step, seed = step.seed;
@@ -1922,8 +1914,8 @@ __global__ void __launch_bounds__(WAVES * WAVE) tree_walk_solo(const float4* __r
   {  // the root: one term (every lane of the group the same; lane 0 counts it)
     const NodeRec& r = rec[0];
     const NodeHdr h = *(const NodeHdr*)&r;
-    const bool done = live ? cell_force<D, ACC_SET>(me.x, me.y, me.z, self_node, n, 0u, 0u, size * size, h,
-                                                      theta2, dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter)
+    const bool done = live ? cell_force<D, ACC_SET>(me.x, me.y, me.z, self_node, n, 0u, 0u, thr0, h,
+                                                      dmax2, dmin2f, alpha, step_no, seed, ax, ay, az, inter)
                            : true;
     if (c != 0) inter = 0;
     if (!done) {
@@ -1945,8 +1937,6 @@ __global__ void __launch_bounds__(WAVES * WAVE) tree_walk_solo(const float4* __r
     const uint32_t k = L.kids[g][sp][c];
     const uint32_t top = L.depth[g][sp] + 1;
     opened += c == 0 ? 1u : 0u;
-    const double wt = ldexp(size, -(int)top);
-    const double w2top = wt * wt;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     bool open = false;
     uint4 k0 = make_uint4(0u, 0u, 0u, 0u), k1 = k0;
@@ -1966,7 +1956,7 @@ __global__ void __launch_bounds__(WAVES * WAVE) tree_walk_solo(const float4* __r
         NodeHdr h;
         __builtin_memcpy(&h, &q0, 16);
         __builtin_memcpy((char*)&h + 16, &q1, 16);
-        open = !cell_force<D, ACC_SET>(me.x, me.y, me.z, self_node, n, k, top, w2top, h, theta2, dmax2, dmin2f,
+        open = !cell_force<D, ACC_SET>(me.x, me.y, me.z, self_node, n, k, top, thr0, h, dmax2, dmin2f,
                                        alpha, step_no, seed, fx, fy, fz, inter);
         cdepth = h.depth;
       }
@@ -2100,7 +2090,7 @@ __device__ __forceinline__ void wide_point(WideLDS<D>& L, uint32_t* sk, uint32_t
   constexpr uint32_t NK = 1u << D, P = WAVE / NK;
   const uint32_t e = lane / NK, c = lane % NK;
   const uint32_t off = T.off, n = T.n;
-  const double size = T.size, theta2 = T.theta2, dmax2 = T.dmax2;
+  const double thr0 = T.thr0, dmax2 = T.dmax2;
   const float dmin2f = T.dmin2f, dmax2f = T.dmax2f, alpha = T.alpha;
   const uint32_t step_no = T.step_no, seed = T.seed;
   const uint32_t guard_max = 2u * n + 8u;
@@ -2113,7 +2103,7 @@ __device__ __forceinline__ void wide_point(WideLDS<D>& L, uint32_t* sk, uint32_t
     const NodeHdr h = *(const NodeHdr*)&r;
     float rx = 0.f, ry = 0.f, rz = 0.f;
     uint32_t ri = 0;
-    const bool done = cell_force<D, ACC_FUSED>(me.x, me.y, me.z, self_node, n, 0u, 0u, size * size, h, theta2,
+    const bool done = cell_force<D, ACC_FUSED>(me.x, me.y, me.z, self_node, n, 0u, 0u, thr0, h,
                                                dmax2, dmin2f, alpha, step_no, seed, rx, ry, rz, ri);
     if (lane == 0) ax = rx, ay = ry, az = rz, inter += ri;
     if (!done) {
@@ -2163,8 +2153,7 @@ __device__ __forceinline__ void wide_point(WideLDS<D>& L, uint32_t* sk, uint32_t
         NodeHdr h;
         __builtin_memcpy(&h, &q0, 16);
         __builtin_memcpy((char*)&h + 16, &q1, 16);
-        const double wt = ldexp(size, -(int)top);
-        open = !cell_force<D, ACC_FUSED>(me.x, me.y, me.z, self_node, n, k - off, top, wt * wt, h, theta2, dmax2,
+        open = !cell_force<D, ACC_FUSED>(me.x, me.y, me.z, self_node, n, k - off, top, thr0, h, dmax2,
                                          dmin2f, alpha, step_no, seed, ax, ay, az, inter);
         cdepth = h.depth;
       }
@@ -2230,7 +2219,7 @@ __global__ void __launch_bounds__(WAVES * WAVE) tree_walk_wide(const float4* __r
   T.off = 0;
   T.n = A.n;
   T.size = (double)cube->size;
-  T.theta2 = A.theta2;
+  T.thr0 = T.size * T.size / A.theta2;
   T.dmax2 = A.dmax2;
   T.dmin2f = (float)A.dmin2;
   T.dmax2f = (float)fmin(A.dmax2, 3.0e38);
@@ -2303,7 +2292,7 @@ __device__ __forceinline__ void group_walk(GroupLDS<D, T>& L, uint32_t* sk, cons
   uint32_t* sm = sd + SPW;
   const uint32_t e = lane / NK, c = lane % NK;
   const uint32_t off = T_.off, n = T_.n;
-  const double size = T_.size, theta2 = T_.theta2, dmax2 = T_.dmax2;
+  const double thr0 = T_.thr0, dmax2 = T_.dmax2;
   const float dmin2f = T_.dmin2f, dmax2f = T_.dmax2f, alpha = T_.alpha;
   const uint32_t step_no = T_.step_no, seed = T_.seed;
   const uint32_t guard_max = 2u * n + 8u;
@@ -2368,12 +2357,10 @@ __device__ __forceinline__ void group_walk(GroupLDS<D, T>& L, uint32_t* sk, cons
         NodeHdr h;
         __builtin_memcpy(&h, &q0, 16);
         __builtin_memcpy((char*)&h + 16, &q1, 16);
-        const double wt = ldexp(size, -(int)top);
-        const double w2 = wt * wt;
 #pragma unroll
         for (uint32_t b = 0; b < T; ++b)
           if ((msk >> b) & 1u)
-            if (!cell_force<D, ACC_FUSED>(me[b].x, me[b].y, me[b].z, self[b], n, k - off, top, w2, h, theta2, dmax2,
+            if (!cell_force<D, ACC_FUSED>(me[b].x, me[b].y, me[b].z, self[b], n, k - off, top, thr0, h, dmax2,
                                           dmin2f, alpha, step_no, seed, ax[b], ay[b], az[b], inter))
               omask |= 1u << b;
         cdepth = h.depth;
@@ -2445,7 +2432,7 @@ __global__ void __launch_bounds__(WAVES * WAVE) tree_walk_group(const float4* __
   W.off = 0;
   W.n = A.n;
   W.size = (double)cube->size;
-  W.theta2 = A.theta2;
+  W.thr0 = W.size * W.size / A.theta2;
   W.dmax2 = A.dmax2;
   W.dmin2f = (float)A.dmin2;
   W.dmax2f = (float)fmin(A.dmax2, 3.0e38);
@@ -2626,7 +2613,7 @@ __global__ void __launch_bounds__(WAVES * WAVE, POOLED && STACK <= 64 ? POOL_WAV
     T.off = B.tree_off;
     T.n = B.n;
     T.size = (double)B.cube_size;
-    T.theta2 = B.theta2;
+    T.thr0 = T.size * T.size / B.theta2;
     T.dmax2 = B.dmax2;
     T.dmin2f = (float)B.dmin2;
     T.dmax2f = (float)fmin(B.dmax2, 3.0e38);
