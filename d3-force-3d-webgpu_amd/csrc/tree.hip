@@ -1402,18 +1402,24 @@ __device__ __forceinline__ void walk_tile(WalkLDS<STACK, POOLED>& L, const float
       ++opened;
       lane_visits += take * NK;
       __builtin_amdgcn_wave_barrier();
-      // Straight-line code: every lane loads a node header and a point (the root and the
-      // tile's first point where it has none, so every address is valid) and d3's rules
+      // Straight-line code: every lane loads a node header and a point (the arrays' first
+      // node and the tile's first point where it has none, so every address is valid; the
+      // same for every tree of a batch) and d3's rules
       // (barnesHut.js:705-746, manyBody.js:73-113) become selects; the two rare cases --
       // a chain of more than one level that is accepted, a source at the point's own
       // position -- run only when some lane meets them.
       const uint32_t p = act ? ent.y >> 8 : 0u, ptop = ent.y & 255u;
-      const uint32_t kid = act ? rec[act ? ent.x : off].kids[pc] : REF_NONE;
+      uint32_t kid = REF_NONE;
+      // (the child slot passed through an empty asm: hoisted, the address of slot pc of the
+      // first record is a 64-bit loop invariant that the register budget spills)
+      uint32_t slot = pc;
+      asm volatile("" : "+v"(slot));
+      if (act) kid = rec[ent.x].kids[slot];
       // (bitwise & and | on the decisions: no short-circuit branches)
       const bool is_pt = (kid != REF_NONE) & ((kid & REF_POINT) != 0u);
       const bool is_node = (kid != REF_NONE) & ((kid & REF_POINT) == 0u);
       const uint32_t j = is_pt ? kid & ~REF_POINT : base;
-      const NodeHdr h = *(const NodeHdr*)&rec[is_node ? kid : off];
+      const NodeHdr h = *(const NodeHdr*)&rec[is_node ? kid : 0u];
       const float4 q = leaves[j];
       const float4 bd = L.body[p];
       const double dx = (is_node ? h.cx : (double)q.x) - (double)bd.x;
