@@ -1282,9 +1282,13 @@ __device__ inline void point_force(float mx, float my, float mz, uint32_t self_n
 // The sum over each aligned group of NK (2, 4 or 8) lanes, in registers (DPP): the two
 // halves of a group of 8 mirrored onto each other, then the pairs of a quad, then
 // neighbours; every lane of a group ends with the group's sum.  All lanes must be active.
+// (bound_ctrl set: with every row and bank enabled and these patterns, no lane is left out,
+// and the compiler folds the move into the add; the add is kept from fusing with the
+// product that made x, which would need the moved value in a register of its own)
 template <int CTRL>
 __device__ inline float dpp_plus(float x) {
-  return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+#pragma clang fp contract(off)
+  return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, true));
 }
 template <int NK>
 __device__ inline float group_sum(float x) {
