@@ -162,7 +162,7 @@ struct BatchGrid {  // as GridWork, over a batch's grid space
   uint32_t* arrived = nullptr;
   uint32_t* svals = nullptr;
   float4* sorted = nullptr;
-  float4* snap = nullptr;
+  float4* snap = nullptr;       // per node, then the query's increments in sorted order
   uint32_t* sseg = nullptr;     // segment of each sorted point
   void reserve(uint32_t n);
   void release();

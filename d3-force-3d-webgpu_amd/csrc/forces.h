@@ -127,13 +127,13 @@ struct GridRun {
 struct GridWork {
   uint32_t cap = 0, bits = 0;
   uint32_t* keys = nullptr;     // slot of each node
-  uint32_t* rank = nullptr;     // arrival rank of each node in its slot
+  uint32_t* rank = nullptr;     // arrival rank of each node in its slot, then its sorted position
   uint32_t* start = nullptr;    // [2^bits + 1]: counts per slot, then first sorted position
   uint32_t* partial = nullptr;  // scan scratch
   uint32_t* arrived = nullptr;  // nodes by slot in arrival order
   uint32_t* svals = nullptr;    // nodes by slot, ascending node index within a slot
   float4* sorted = nullptr;     // snapshots (pos + vel, radius) in sorted order
-  float4* snap = nullptr;       // the same per node
+  float4* snap = nullptr;       // the same per node, then the query's increments in sorted order
   void reserve(uint32_t n);
   void release();
   ~GridWork() { release(); }
