@@ -334,7 +334,7 @@ const CenterWork* run_pass(fg_ctx* c, Pass& p, StepParams sp, bool defer_center 
           if (!p.grid) p.grid.reset(new GridWork());
           float* vel[3] = {c->V(0), c->V(1), c->V(2)};
           launch_collide_grid(c->stream, *p.grid, n, nd, pos, vel, p.a.as<float>(), p.grid_classes,
-                              (float)p.d.strength, c->radix_bits, sp);
+                              (float)p.d.strength, c->radix_bits, sp, k, p.d.iterations);
         } else {
           const float* vin[3] = {c->V(0), c->V(1), c->V(2)};
           float* vout[3] = {c->W(0), c->W(1), c->W(2)};

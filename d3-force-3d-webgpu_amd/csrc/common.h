@@ -60,6 +60,15 @@ __device__ inline uint32_t xcd_block() {
   return (blockIdx.x % XCD_COUNT) * per + blockIdx.x / XCD_COUNT;
 }
 
+// A launch that a word on the device can cancel: skip() when the word (a non-negative
+// float's bits) is at most `lim` -- every workgroup reads the same word, so all of a launch
+// returns at once, before any barrier.  No word: never skips.
+struct SkipIf {
+  const uint32_t* word = nullptr;
+  float lim = 0.f;
+  __device__ bool operator()() const { return word && __uint_as_float(*word) <= lim; }
+};
+
 // ---- device helpers ------------------------------------------------------------------
 
 // The reference jiggle hash (wgsl.js:28-39): pcg, then a value in [-5e-7, 5e-7).

@@ -133,7 +133,10 @@ struct GridWork {
   uint32_t* arrived = nullptr;  // nodes by slot in arrival order
   uint32_t* svals = nullptr;    // nodes by slot, ascending node index within a slot
   float4* sorted = nullptr;     // snapshots (pos + vel, radius) in sorted order
-  float4* snap = nullptr;       // the same per node, then the query's increments in sorted order
+  float4* snap = nullptr;       // the same per node (the positions the points were filed at)
+  float4* dv = nullptr;         // the query's increments in sorted order
+  float4* cur = nullptr;        // the positions of the next iteration on this grid, in sorted order
+  uint32_t* margin2 = nullptr;  // [2]: the largest squared move from snap, by iteration parity
   void reserve(uint32_t n);
   void release();
   ~GridWork() { release(); }
@@ -184,8 +187,10 @@ void launch_manybody_pairs(hipStream_t s, uint32_t n, int ndim, float* const* po
 void launch_collide_pairs(hipStream_t s, uint32_t n, int ndim, float* const* pos,
                           const float* const* vin, float* const* vout, const float* radius,
                           float strength, StepParams step);
+// Iteration k of `iterations` of a grid collide: the first builds the grid, the later ones
+// query the grid the first built (with each point's move since then as margin).
 void launch_collide_grid(hipStream_t s, GridWork& g, uint32_t n, int ndim, float* const* pos,
                          float* const* vel, const float* radius, const GridClasses& gc, float strength,
-                         int radix_bits, StepParams step);
+                         int radix_bits, StepParams step, int k, int iterations);
 
 }  // namespace fg

@@ -66,6 +66,7 @@ __device__ inline void sort_clear_block(const SortClear& c) {
 // In-place exclusive prefix sum of n u32 values; `partial` holds scan_partials_size(n)
 // words of scratch.
 uint32_t scan_partials_size(uint32_t n);
-void exclusive_scan_u32(hipStream_t stream, uint32_t* data, uint32_t n, uint32_t* partial);
+// (skip: a launch-time condition on the device under which the scan does nothing)
+void exclusive_scan_u32(hipStream_t stream, uint32_t* data, uint32_t n, uint32_t* partial, SkipIf skip = {});
 
 }  // namespace fg

@@ -297,7 +297,8 @@ constexpr uint32_t SCAN_ITEMS = 8, SCAN_TILE = THREADS * SCAN_ITEMS;
 
 // partial[b] = the sum of tile b.
 __global__ void __launch_bounds__(THREADS) scan_tile_sums(const uint32_t* __restrict__ data, uint32_t n,
-                                                          uint32_t* __restrict__ partial) {
+                                                          uint32_t* __restrict__ partial, SkipIf skip) {
+  if (skip()) return;
   __shared__ uint32_t s_tmp[WAVES];
   const uint32_t base = blockIdx.x * SCAN_TILE;
   uint32_t sum = 0;
@@ -312,7 +313,9 @@ __global__ void __launch_bounds__(THREADS) scan_tile_sums(const uint32_t* __rest
 }
 
 // One block: the tile sums become each tile's exclusive offset.
-__global__ void __launch_bounds__(THREADS) scan_offsets(uint32_t* __restrict__ partial, uint32_t tiles) {
+__global__ void __launch_bounds__(THREADS) scan_offsets(uint32_t* __restrict__ partial, uint32_t tiles,
+                                                        SkipIf skip) {
+  if (skip()) return;
   __shared__ uint32_t s_tmp[WAVES];
   uint32_t carry = 0;
   for (uint32_t b = 0; b < tiles; b += THREADS) {
@@ -327,7 +330,8 @@ __global__ void __launch_bounds__(THREADS) scan_offsets(uint32_t* __restrict__ p
 
 // Tile b scanned in LDS (each thread takes SCAN_ITEMS consecutive values) plus its offset.
 __global__ void __launch_bounds__(THREADS) scan_apply(uint32_t* __restrict__ data, uint32_t n,
-                                                      const uint32_t* __restrict__ partial) {
+                                                      const uint32_t* __restrict__ partial, SkipIf skip) {
+  if (skip()) return;
   __shared__ uint32_t s[SCAN_TILE];
   __shared__ uint32_t s_tmp[WAVES];
   const uint32_t base = blockIdx.x * SCAN_TILE;
@@ -362,14 +366,14 @@ __global__ void __launch_bounds__(THREADS) scan_apply(uint32_t* __restrict__ dat
 
 uint32_t scan_partials_size(uint32_t n) { return ceil_div(n > 0 ? n : 1, SCAN_TILE); }
 
-void exclusive_scan_u32(hipStream_t stream, uint32_t* data, uint32_t n, uint32_t* partial) {
+void exclusive_scan_u32(hipStream_t stream, uint32_t* data, uint32_t n, uint32_t* partial, SkipIf skip) {
   if (!n) return;
   const uint32_t tiles = scan_partials_size(n);
-  hipLaunchKernelGGL(scan_tile_sums, dim3(tiles), dim3(THREADS), 0, stream, (const uint32_t*)data, n, partial);
+  hipLaunchKernelGGL(scan_tile_sums, dim3(tiles), dim3(THREADS), 0, stream, (const uint32_t*)data, n, partial, skip);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(scan_offsets, dim3(1), dim3(THREADS), 0, stream, partial, tiles);
+  hipLaunchKernelGGL(scan_offsets, dim3(1), dim3(THREADS), 0, stream, partial, tiles, skip);
   FG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(scan_apply, dim3(tiles), dim3(THREADS), 0, stream, data, n, (const uint32_t*)partial);
+  hipLaunchKernelGGL(scan_apply, dim3(tiles), dim3(THREADS), 0, stream, data, n, (const uint32_t*)partial, skip);
   FG_LAUNCH_CHECK();
 }
 
