@@ -543,6 +543,49 @@ def test_collide_grid_two_iterations_lognormal(engine_factory):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_collide_grid_later_iterations_reuse_the_grid(engine_factory, dims):
+    """Four iterations where the points move a little: the later ones query the first
+    one's grid (points filed where they were then, pairs taken at their positions now,
+    cells within the largest move visited too).  Against four collide forces of one
+    iteration each (every one builds its own grid; the same Jacobi steps), and in 2D/3D
+    against the oracle's four iterations.  (On a line, chains of overlaps let points pass
+    each other and nearly meet, where f32 and the FP64 oracle can push a pair apart in
+    opposite directions: the oracle is left out there.)"""
+    n = 20000
+    rng = np.random.default_rng(11)
+    pos = make_nodes_fast(n, dims, 10 * n ** (1 / dims), seed=3)
+    radii = np.exp(np.log(2) + 0.4 * rng.standard_normal(n))
+    _, dv = gpu_dv(engine_factory(), pos, dims, [d3fgpu.collide(radii, strength=0.7, iterations=4)])
+    _, dv1 = gpu_dv(engine_factory(), pos, dims, [d3fgpu.collide(radii, strength=0.7, iterations=1) for _ in range(4)])
+    err1 = rel_l2(dv, dv1)
+    report(f"collide grid 4 iterations {dims}D against 4 grids", err1)
+    assert err1 < 1e-5
+    if dims > 1:
+        ref = oracle_collide(pos, np.zeros_like(pos), dims, radii, 0.7, 4)
+        err = rel_l2(dv, ref)
+        report(f"collide grid 4 iterations {dims}D", err, touched=int(np.sum(np.any(ref != 0, 1))))
+        assert err < 1e-4
+        assert close_count(dv, ref, rtol=1e-3) == 0
+
+
+def test_collide_grid_rebuilds_after_large_moves(engine_factory):
+    """Points crowded deep inside each other's radii (about 50 overlaps each) move several
+    units -- more than a quarter of a 6-unit cell -- in one iteration, so the next ones build
+    grids of their own.  Against the oracle's three iterations."""
+    n, dims = 12000, 3
+    pos = make_nodes_fast(n, dims, 60.0, seed=17)
+    radii = np.full(n, 3.0)
+    f = d3fgpu.collide(radii, strength=1.0, iterations=3)
+    _, dv = gpu_dv(engine_factory(), pos, dims, [f])
+    ref = oracle_collide(pos, np.zeros_like(pos), dims, radii, 1.0, 3)
+    moved = float(np.max(np.linalg.norm(ref, axis=1)))
+    err = rel_l2(dv, ref)
+    report("collide grid rebuilt after large moves", err, largest_move=round(moved, 2))
+    assert moved > 1.5  # the first iteration's moves exceed a quarter cell
+    assert err < 1e-4
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_collide_grid_heavy_tail(engine_factory, dims):
     """1% of the nodes ten times larger than the rest: the two-class grid, where a large node
