@@ -731,19 +731,14 @@ __device__ inline int3 grid_cell(float4 q, float inv) {
 // The grid's class of a radius: 0 (fine grid) up to |r| = r0, 1 (coarse grid) above.
 __device__ inline uint32_t grid_class(float r, const GridClasses& gc) { return fabsf(r) > gc.r0 ? 1u : 0u; }
 
-// Collide sorted point s (snapshot `me` with its radius, node index `node`) with every
-// point closer than the sum of their radii (passes/collide.js:34-80 on candidates from
-// passes/collideGrid.js).  Class-0 cells are 2 r0 wide, class-1 cells 2 rmax; a partner
-// of class c lies within me.w + (c ? rmax : r0), so the cells to visit in class c are
-// those within ceil((me.w + r_c) / cell_c) of the point's cell -- one ring except for a
-// coarse point among fine cells.  Classes, cells and, within a cell, sorted points are
-// visited in a fixed order; a point found under a cell's slot counts only if it is of that
-// class and lies in that cell, so slots shared by cells -- or, in a batch (SEG), by other
-// jobs -- change the work but never the sum.  runs[slot] is the slot's [first, end) of the
-// sorted points ((0, 0) when empty); svals[t] - node_off is sorted point t's node.
-// The pairs of sorted point s with the points of one cell (class c, coordinates cx, cy, cz)
-// whose run is `run`.
-template <int D, bool SEG>
+// The pair of sorted point s with candidate t found under the slot of cell (cx, cy, cz) of
+// class c: q is where t was filed (its cell is checked when CELL), qc where it is now.
+// Within the ring of 27 cells the check can go: the cells of one class there never share a
+// slot (two of them differ by at most one block per axis, so the Morton codes of their
+// blocks differ in one of the lowest 3 bits, never by a multiple of the table's blocks),
+// and a point filed in a cell elsewhere that shares the slot lies farther away than any
+// cell within reach, so the pair test rejects it.
+template <int D, bool SEG, bool CELL = true>
 __device__ inline void grid_pair(uint32_t t, float4 q, float4 qc, uint32_t qseg, int cx, int cy, int cz, uint32_t c,
                                  float inv, uint32_t s, float4 me, float ri2, uint32_t node,
                                  const uint32_t* __restrict__ svals, uint32_t seg, uint32_t node_off,
@@ -756,8 +751,10 @@ __device__ inline void grid_pair(uint32_t t, float4 q, float4 qc, uint32_t qseg,
   if (t == s) return;
   if (SEG && qseg != seg) return;
   if (grid_class(q.w, gc) != c) return;
-  const int3 qcell = grid_cell<D>(q, inv);
-  if (qcell.x != cx || qcell.y != cy || qcell.z != cz) return;
+  if (CELL) {
+    const int3 qcell = grid_cell<D>(q, inv);
+    if (qcell.x != cx || qcell.y != cy || qcell.z != cz) return;
+  }
   const float rj = q.w, r = me.w + rj;
   float ex = me.x - qc.x, ey = me.y - qc.y, ez = me.z - qc.z;
   float l = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
@@ -831,15 +828,15 @@ __device__ inline void grid_slab_pairs(const GridRun (&rr)[SLAB], int3 h, int cz
       qc[u] = q[u];
       qs[u] = 0u;
       if (jj < total) {
-        q[u] = sorted[tt[u]];
-        qc[u] = REUSE ? cur[tt[u]] : q[u];
+        q[u] = REUSE ? cur[tt[u]] : sorted[tt[u]];  // (no cell check here: only the radius)
+        qc[u] = q[u];
         if (SEG) qs[u] = sseg[tt[u]];
       }
     }
 #pragma unroll
     for (uint32_t u = 0; u < GRID_BATCH; ++u) {
       if (j + u >= total) break;
-      grid_pair<D, SEG>(tt[u], q[u], qc[u], qs[u], h.x + ox[u], h.y + oy[u], cz, c, inv, s, me, ri2, node, svals, seg,
+      grid_pair<D, SEG, false>(tt[u], q[u], qc[u], qs[u], h.x + ox[u], h.y + oy[u], cz, c, inv, s, me, ri2, node, svals, seg,
                         node_off, gc, strength, sn, seed, ax, ay, az);
     }
   }
@@ -868,13 +865,13 @@ __device__ inline bool grid_cell_near(float ux, float uy, float uz, int cx, int 
 // coarse point among fine cells -- and of those only the cells whose box comes that close
 // to the point are looked up at all.  Classes, cells and, within a cell, sorted points are
 // visited in a fixed order; a point found under a cell's slot counts only if it is of that
-// class and lies in that cell, so slots shared by cells -- or, in a batch (SEG), by other
-// jobs -- change the work but never the sum.  The slot's run of sorted points is
+// class, of the job (a batch, SEG) and -- beyond the ring of 27 -- filed in that cell, so
+// slots shared by cells or jobs change the work but never the sum.  The slot's run of sorted points is
 // [start[slot], start[slot + 1]); svals[t] - node_off is sorted point t's node.  In one
 // ring, the runs of a z-slab of cells are read together so that their latencies overlap.
 //
 // REUSE: a later iteration on the grid an earlier one built.  Points stay filed under the
-// cells of their positions then (sorted, which the cell test reads); the pair test takes
+// cells of their positions then (sorted, which the cell check reads); the pair test takes
 // their positions now (cur, and `me`), none more than `margin` from where it was filed, so
 // the cells to visit are those within |me.w| + r_c + margin of `me` -- the ring of 27
 // unless the point lies within that excess of a face of its cell.
