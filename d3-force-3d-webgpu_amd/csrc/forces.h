@@ -18,10 +18,21 @@ struct LinkEntry {
   float bias;
 };
 
+// A pointer as lane src of the wave holds it.
+template <class T>
+__device__ inline T* lane_ptr(T* p, int src) {
+  const uint64_t u = (uint64_t)p;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, src);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), src);
+  return (T*)(((uint64_t)hi << 32) | lo);
+}
+
 // The snapshot p + v of node j: read from the snapshot array, or summed where it is read.
+// at(src): the same as lane src of the wave has it.
 struct SnapArray {
   const float4* __restrict__ snap;
   __device__ float4 operator()(uint32_t j) const { return snap[j]; }
+  __device__ SnapArray at(int src) const { return SnapArray{lane_ptr(snap, src)}; }
 };
 template <int D>
 struct SnapDirect {
@@ -30,42 +41,97 @@ struct SnapDirect {
   __device__ float4 operator()(uint32_t j) const {
     return make_float4(x[j] + vx[j], D > 1 ? y[j] + vy[j] : 0.f, D > 2 ? z[j] + vz[j] : 0.f, 0.f);
   }
+  __device__ SnapDirect at(int src) const {
+    return SnapDirect{lane_ptr(x, src),  lane_ptr(y, src),  lane_ptr(z, src),
+                      lane_ptr(vx, src), lane_ptr(vy, src), lane_ptr(vz, src)};
+  }
 };
 
-// One node's incident links: d3's spring along each link between the snapshots of its
-// ends, and the node's share of it (bias for the target, 1 - bias for the source).
+// One link's term at one of its ends (snapshot me, the other end's t): d3's spring along the
+// link between the two snapshots, and this end's share of it (bias for the target, 1 - bias
+// for the source), added to (ax, ay, az).
+template <int D>
+__device__ inline void link_term(float4 me, float4 t, const LinkEntry& L, float alpha, uint32_t sn, uint32_t seed,
+                                 float& ax, float& ay, float& az) {
+  const bool target = L.code & 1u;
+  const uint32_t id = L.code >> 1;
+  // d = snap[target] - snap[source], the same vector at both ends
+  float dx = target ? me.x - t.x : t.x - me.x;
+  float dy = target ? me.y - t.y : t.y - me.y;
+  float dz = target ? me.z - t.z : t.z - me.z;
+  if (dx == 0.f) dx = jiggle_hash(id, 0u, 0u, sn, seed);
+  if (D > 1 && dy == 0.f) dy = jiggle_hash(id, 0u, 1u, sn, seed);
+  if (D > 2 && dz == 0.f) dz = jiggle_hash(id, 0u, 2u, sn, seed);
+  const float l = sqrtf(dx * dx + (D > 1 ? dy * dy : 0.f) + (D > 2 ? dz * dz : 0.f));
+  const float f = (l - L.distance) / l * alpha * L.strength;
+  const float w = target ? -L.bias : 1.0f - L.bias;
+  ax += dx * f * w;
+  if (D > 1) ay += dy * f * w;
+  if (D > 2) az += dz * f * w;
+}
+
+// A node with more incident links than this has them summed by the whole wave (below): one
+// lane per node would leave the wave waiting on the longest row (a hub of a scale-free graph
+// has hundreds).
+constexpr uint32_t LINK_LANE_MAX = 32;
+
+// The link pass for the nodes of one wave (every lane calls it; `live`: lane i has node i):
+// each node's incident links, ox = vx + the sum of their terms.  A node with at most
+// LINK_LANE_MAX links is summed by its own lane in CSR order; a longer row by all 64 lanes,
+// lane k taking its entries k, k + 64, ... in order and the lanes' sums added in a fixed
+// tree -- the same sum wherever the node's wave is, so a job still advances alike alone and
+// in a batch.  The pointers may differ by lane (a batch's jobs): a long row uses its lane's.
 template <int D, class Snap>
-__device__ inline void link_node(uint32_t i, Snap snap,
+__device__ inline void link_wave(uint32_t i, bool live, Snap snap,
                                  const float* __restrict__ vx, const float* __restrict__ vy,
                                  const float* __restrict__ vz, float* __restrict__ ox,
                                  float* __restrict__ oy, float* __restrict__ oz,
                                  const uint32_t* __restrict__ row, const LinkEntry* __restrict__ ent,
                                  float alpha, uint32_t sn, uint32_t seed) {
-  const float4 me = snap(i);
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  const uint32_t e0 = row[i], e1 = row[i + 1];
-  for (uint32_t e = e0; e < e1; ++e) {
-    const LinkEntry L = ent[e];
-    const float4 t = snap(L.other);
-    const bool target = L.code & 1u;
-    const uint32_t id = L.code >> 1;
-    // d = snap[target] - snap[source], the same vector at both ends
-    float dx = target ? me.x - t.x : t.x - me.x;
-    float dy = target ? me.y - t.y : t.y - me.y;
-    float dz = target ? me.z - t.z : t.z - me.z;
-    if (dx == 0.f) dx = jiggle_hash(id, 0u, 0u, sn, seed);
-    if (D > 1 && dy == 0.f) dy = jiggle_hash(id, 0u, 1u, sn, seed);
-    if (D > 2 && dz == 0.f) dz = jiggle_hash(id, 0u, 2u, sn, seed);
-    const float l = sqrtf(dx * dx + (D > 1 ? dy * dy : 0.f) + (D > 2 ? dz * dz : 0.f));
-    const float f = (l - L.distance) / l * alpha * L.strength;
-    const float w = target ? -L.bias : 1.0f - L.bias;
-    ax += dx * f * w;
-    if (D > 1) ay += dy * f * w;
-    if (D > 2) az += dz * f * w;
+  const uint32_t e0 = live ? row[i] : 0u, e1 = live ? row[i + 1] : 0u;
+  const bool lone = live && e1 - e0 <= LINK_LANE_MAX;
+  if (lone) {
+    const float4 me = snap(i);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (uint32_t e = e0; e < e1; ++e) {
+      const LinkEntry L = ent[e];
+      link_term<D>(me, snap(L.other), L, alpha, sn, seed, ax, ay, az);
+    }
+    ox[i] = vx[i] + ax;
+    if (D > 1) oy[i] = vy[i] + ay;
+    if (D > 2) oz[i] = vz[i] + az;
   }
-  ox[i] = vx[i] + ax;
-  if (D > 1) oy[i] = vy[i] + ay;
-  if (D > 2) oz[i] = vz[i] + az;
+  uint64_t many = __builtin_amdgcn_ballot_w64(live && !lone);
+  const uint32_t lane = __lane_id();
+  while (many) {
+    const int src = (int)__builtin_ctzll(many);
+    many &= many - 1ull;
+    const uint32_t j = (uint32_t)__builtin_amdgcn_readlane((int)i, src);
+    const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)e0, src);
+    const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)e1, src);
+    const Snap sj = snap.at(src);
+    const LinkEntry* ej = lane_ptr(ent, src);
+    const float aj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(alpha), src));
+    const uint32_t snj = (uint32_t)__builtin_amdgcn_readlane((int)sn, src);
+    const uint32_t seedj = (uint32_t)__builtin_amdgcn_readlane((int)seed, src);
+    const float4 me = sj(j);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (uint32_t e = b0 + lane; e < b1; e += WAVE) {
+      const LinkEntry L = ej[e];
+      link_term<D>(me, sj(L.other), L, aj, snj, seedj, ax, ay, az);
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+      ax += __shfl_xor(ax, o, WAVE);
+      if (D > 1) ay += __shfl_xor(ay, o, WAVE);
+      if (D > 2) az += __shfl_xor(az, o, WAVE);
+    }
+    if ((int)lane == src) {
+      ox[j] = vx[j] + ax;
+      if (D > 1) oy[j] = vy[j] + ay;
+      if (D > 2) oz[j] = vz[j] + az;
+    }
+  }
 }
 
 // Graphs of at most this many nodes take the one-launch link pass (the snapshots summed where

@@ -184,9 +184,8 @@ __global__ void __launch_bounds__(256) link_kernel(uint32_t n, const float4* __r
                                                    const uint32_t* __restrict__ row,
                                                    const LinkEntry* __restrict__ ent,
                                                    StepParams step) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  link_node<D>(i, SnapArray{snap}, vx, vy, vz, ox, oy, oz, row, ent, step.alpha, step.step, step.seed);
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;  // (every lane stays: long rows take the wave)
+  link_wave<D>(i, i < n, SnapArray{snap}, vx, vy, vz, ox, oy, oz, row, ent, step.alpha, step.step, step.seed);
 }
 
 // The same with the snapshots summed where they are read: one launch instead of two, for
@@ -203,9 +202,8 @@ __global__ void __launch_bounds__(256) link_direct_kernel(uint32_t n, const floa
                                                           const uint32_t* __restrict__ row,
                                                           const LinkEntry* __restrict__ ent,
                                                           StepParams step) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  link_node<D>(i, SnapDirect<D>{x, y, z, vx, vy, vz}, vx, vy, vz, ox, oy, oz, row, ent, step.alpha,
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  link_wave<D>(i, i < n, SnapDirect<D>{x, y, z, vx, vy, vz}, vx, vy, vz, ox, oy, oz, row, ent, step.alpha,
                step.step, step.seed);
 }
 
@@ -1227,15 +1225,24 @@ namespace fg {
 
 namespace {
 
-struct NodeRef {
-  const BJob* B;
-  uint32_t v;  // the node's index within its job
-};
+// f(B, seg) with the record B of segment seg's job.  When every active lane of the wave is
+// in one segment -- nearly always: a segment is a whole job of thousands of nodes -- the
+// record is read through a pointer the wave shares (scalar loads, once for the wave), else
+// lane by lane.
+template <class F>
+__device__ inline void with_job(uint32_t seg, const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
+                                F&& f) {
+  const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)seg);
+  if (__builtin_amdgcn_ballot_w64(seg != s0) == 0) f(jobs[seg_job[s0]], s0);
+  else f(jobs[seg_job[seg]], seg);
+}
 
-__device__ inline NodeRef node_ref(uint32_t i, const uint32_t* seg_of, const uint32_t* seg_job,
-                                   const BJob* jobs, uint32_t BJob::*off) {
-  const BJob* B = jobs + seg_job[seg_of[i]];
-  return NodeRef{B, i - B->*off};
+// Node i of a batch: f(B, v) with its job's record B and its index v within the job.
+template <class F>
+__device__ inline void with_node(uint32_t i, const uint32_t* __restrict__ seg_of,
+                                 const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
+                                 uint32_t BJob::*off, F&& f) {
+  with_job(seg_of[i], seg_job, jobs, [&](const BJob& B, uint32_t) { f(B, i - B.*off); });
 }
 
 // d3's alpha step (simulation.js:54-73) for every job, in FP64 without contraction so the
@@ -1267,12 +1274,12 @@ __global__ void __launch_bounds__(256) bsnapshot(uint32_t total, const uint32_t*
                                                  float4* __restrict__ snap) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
-  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
-  const BJob& B = *r.B;
-  if (!B.active || !(B.forces & BF_LINK) || iteration >= B.link_iters) return;
-  const uint32_t c = iteration & 1u, v = r.v;
-  snap[i] = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
-                        D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, 0.f);
+  with_node(i, seg_of, seg_job, jobs, &BJob::node_off, [&](const BJob& B, uint32_t v) {
+    if (!B.active || !(B.forces & BF_LINK) || iteration >= B.link_iters) return;
+    const uint32_t c = iteration & 1u;
+    snap[i] = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
+                          D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, 0.f);
+  });
 }
 
 template <int D>
@@ -1280,14 +1287,13 @@ __global__ void __launch_bounds__(256) blink(uint32_t total, const uint32_t* __r
                                              const uint32_t* __restrict__ seg_job,
                                              const BJob* __restrict__ jobs, uint32_t iteration,
                                              const float4* __restrict__ snap) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= total) return;
-  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
-  const BJob& B = *r.B;
-  if (!B.active || !(B.forces & BF_LINK) || iteration >= B.link_iters) return;
+  // every lane stays (long rows take the wave); a lane past the end reads the last node's job
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x, k = i < total ? i : total - 1;
+  const BJob& B = jobs[seg_job[seg_of[k]]];
+  const bool live = i < total && B.active && (B.forces & BF_LINK) && iteration < B.link_iters;
   const uint32_t c = iteration & 1u;
-  link_node<D>(r.v, SnapArray{snap + B.node_off}, B.v[c][0], B.v[c][1], B.v[c][2], B.v[c ^ 1][0], B.v[c ^ 1][1],
-               B.v[c ^ 1][2], B.link_row, B.link_ent, B.alpha_f, B.step, B.seed);
+  link_wave<D>(k - B.node_off, live, SnapArray{snap + B.node_off}, B.v[c][0], B.v[c][1], B.v[c][2], B.v[c ^ 1][0],
+               B.v[c ^ 1][1], B.v[c ^ 1][2], B.link_row, B.link_ent, B.alpha_f, B.step, B.seed);
 }
 
 template <int D>
@@ -1298,15 +1304,15 @@ __global__ void __launch_bounds__(256) bgrid_keys(uint32_t total, const uint32_t
                                                   float4* __restrict__ snap, uint32_t* __restrict__ count) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
-  const uint32_t seg = seg_of[i];
-  const BJob& B = jobs[seg_job[seg]];
-  const uint32_t v = i - B.grid_off, c = B.vcur;
-  const float4 q = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
-                               D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, B.radius[v]);
-  const uint32_t key = grid_key<D>(q, seg, B.coll, mask);
-  keys[i] = key;
-  snap[i] = q;
-  rank[i] = atomicAdd(&count[key], 1u);
+  with_job(seg_of[i], seg_job, jobs, [&](const BJob& B, uint32_t seg) {
+    const uint32_t v = i - B.grid_off, c = B.vcur;
+    const float4 q = make_float4(B.p[0][v] + B.v[c][0][v], D > 1 ? B.p[1][v] + B.v[c][1][v] : 0.f,
+                                 D > 2 ? B.p[2][v] + B.v[c][2][v] : 0.f, B.radius[v]);
+    const uint32_t key = grid_key<D>(q, seg, B.coll, mask);
+    keys[i] = key;
+    snap[i] = q;
+    rank[i] = atomicAdd(&count[key], 1u);
+  });
 }
 
 template <int D>
@@ -1340,14 +1346,14 @@ __global__ void __launch_bounds__(256) bgrid_apply(uint32_t total, const uint32_
                                                    const float4* __restrict__ dv) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
-  const BJob& B = jobs[seg_job[seg_of[i]]];
-  if (!B.active || iteration >= B.coll_iters) return;
-  const uint32_t v = i - B.grid_off;
-  const float4 d = dv[place[i]];
-  float* const* vel = B.v[B.vcur];
-  vel[0][v] += d.x;
-  if (D > 1) vel[1][v] += d.y;
-  if (D > 2) vel[2][v] += d.z;
+  with_node(i, seg_of, seg_job, jobs, &BJob::grid_off, [&](const BJob& B, uint32_t v) {
+    if (!B.active || iteration >= B.coll_iters) return;
+    const float4 d = dv[place[i]];
+    float* const* vel = B.v[B.vcur];
+    vel[0][v] += d.x;
+    if (D > 1) vel[1][v] += d.y;
+    if (D > 2) vel[2][v] += d.z;
+  });
 }
 
 // Centre: FP64 sums per tile (job-relative tiles, fixed order inside a tile); the last tile
@@ -1396,11 +1402,11 @@ __global__ void __launch_bounds__(256) bcenter_apply(uint32_t total, const uint3
                                                      const BJob* __restrict__ jobs) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
-  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
-  const BJob& B = *r.B;
-  if (!B.active || !(B.forces & BF_CENTER)) return;
-  for (uint32_t a = 0; a < B.ndim; ++a)
-    if (B.center_shift[a] != 0.0f) B.p[a][r.v] -= B.center_shift[a];
+  with_node(i, seg_of, seg_job, jobs, &BJob::node_off, [&](const BJob& B, uint32_t v) {
+    if (!B.active || !(B.forces & BF_CENTER)) return;
+    for (uint32_t a = 0; a < B.ndim; ++a)
+      if (B.center_shift[a] != 0.0f) B.p[a][v] -= B.center_shift[a];
+  });
 }
 
 template <int D>
@@ -1409,10 +1415,8 @@ __global__ void __launch_bounds__(256) bradial(uint32_t total, const uint32_t* _
                                                const BJob* __restrict__ jobs) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
-  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
-  const BJob& B = *r.B;
+  with_node(i, seg_of, seg_job, jobs, &BJob::node_off, [&](const BJob& B, uint32_t v) {
   if (!B.active || !(B.forces & BF_RADIAL)) return;
-  const uint32_t v = r.v;
   float dx = B.p[0][v] - B.radial_c[0];
   float dy = D > 1 ? B.p[1][v] - B.radial_c[1] : 0.f;
   float dz = D > 2 ? B.p[2][v] - B.radial_c[2] : 0.f;
@@ -1425,6 +1429,7 @@ __global__ void __launch_bounds__(256) bradial(uint32_t total, const uint32_t* _
   vel[0][v] += dx * k;
   if (D > 1) vel[1][v] += dy * k;
   if (D > 2) vel[2][v] += dz * k;
+  });
 }
 
 __global__ void __launch_bounds__(256) bposition(uint32_t total, const uint32_t* __restrict__ seg_of,
@@ -1432,11 +1437,10 @@ __global__ void __launch_bounds__(256) bposition(uint32_t total, const uint32_t*
                                                  const BJob* __restrict__ jobs, int axis) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
-  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
-  const BJob& B = *r.B;
-  if (!B.active || !(B.forces & (BF_POS_X << axis)) || axis >= (int)B.ndim) return;
-  const uint32_t v = r.v;
-  B.v[B.vcur][axis][v] += (B.pos_goal[axis][v] - B.p[axis][v]) * B.pos_weight[axis][v] * B.alpha_f;
+  with_node(i, seg_of, seg_job, jobs, &BJob::node_off, [&](const BJob& B, uint32_t v) {
+    if (!B.active || !(B.forces & (BF_POS_X << axis)) || axis >= (int)B.ndim) return;
+    B.v[B.vcur][axis][v] += (B.pos_goal[axis][v] - B.p[axis][v]) * B.pos_weight[axis][v] * B.alpha_f;
+  });
 }
 
 // v *= decay; p += v; pinned axes p = fix, v = 0 (wgsl.js:67-84).  Reads the job's current
@@ -1447,19 +1451,19 @@ __global__ void __launch_bounds__(256) bintegrate(uint32_t total, const uint32_t
                                                   const BJob* __restrict__ jobs) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
-  const NodeRef r = node_ref(i, seg_of, seg_job, jobs, &BJob::node_off);
-  const BJob& B = *r.B;
-  if (!B.active) return;
-  const uint32_t v = r.v, c = B.vcur;
-  const uint32_t m = B.mask ? B.mask[v] : 0u;
-  const float decay = B.decay;
+  with_node(i, seg_of, seg_job, jobs, &BJob::node_off, [&](const BJob& B, uint32_t v) {
+    if (!B.active) return;
+    const uint32_t c = B.vcur;
+    const uint32_t m = B.mask ? B.mask[v] : 0u;
+    const float decay = B.decay;
 #pragma unroll
-  for (int a = 0; a < D; ++a) {
-    float vv = B.v[c][a][v] * decay, p = B.p[a][v] + vv;
-    if ((m >> a) & 1u) p = B.fix[a][v], vv = 0.0f;
-    B.p[a][v] = p;
-    B.v[0][a][v] = vv;
-  }
+    for (int a = 0; a < D; ++a) {
+      float vv = B.v[c][a][v] * decay, p = B.p[a][v] + vv;
+      if ((m >> a) & 1u) p = B.fix[a][v], vv = 0.0f;
+      B.p[a][v] = p;
+      B.v[0][a][v] = vv;
+    }
+  });
 }
 
 }  // namespace

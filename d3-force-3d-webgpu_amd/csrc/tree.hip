@@ -174,9 +174,11 @@ __global__ void __launch_bounds__(256) tree_bbox(const float* __restrict__ x,
                                                  SortClear sort_clear, CenterFuse center, uint32_t boxes,
                                                  LinkFuse link) {
   if (blockIdx.x >= boxes) {  // the workgroups past the box's run the link pass before this force
-    for (uint32_t v = (blockIdx.x - boxes) * 256 + threadIdx.x; v < n; v += (gridDim.x - boxes) * 256)
-      link_node<D>(v, SnapDirect<D>{x, y, z, link.vx, link.vy, link.vz}, link.vx, link.vy, link.vz, link.ox, link.oy,
-                   link.oz, link.row, link.ent, link.alpha, link.step, link.seed);
+    // (a whole wave per step: long rows take the wave)
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    for (uint32_t w = (blockIdx.x - boxes) * 256 + (threadIdx.x & ~(WAVE - 1)); w < n; w += (gridDim.x - boxes) * 256)
+      link_wave<D>(w + lane, w + lane < n, SnapDirect<D>{x, y, z, link.vx, link.vy, link.vz}, link.vx, link.vy,
+                   link.vz, link.ox, link.oy, link.oz, link.row, link.ent, link.alpha, link.step, link.seed);
     return;
   }
   if (blockIdx.x == 0) {
@@ -944,9 +946,11 @@ __global__ void __launch_bounds__(REDUCE_ONE) tree_small(const float* __restrict
                                                          NodeSum* sums, LinkFuse link) {
   constexpr uint32_t B = REDUCE_ONE, W = B / WAVE;
   if (blockIdx.x > 0) {  // the workgroups past the first run the link pass before this force
-    for (uint32_t v = (blockIdx.x - 1) * B + threadIdx.x; v < n; v += (gridDim.x - 1) * B)
-      link_node<D>(v, SnapDirect<D>{x, y, z, link.vx, link.vy, link.vz}, link.vx, link.vy, link.vz, link.ox, link.oy,
-                   link.oz, link.row, link.ent, link.alpha, link.step, link.seed);
+    // (a whole wave per step: long rows take the wave)
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    for (uint32_t w = (blockIdx.x - 1) * B + (threadIdx.x & ~(WAVE - 1)); w < n; w += (gridDim.x - 1) * B)
+      link_wave<D>(w + lane, w + lane < n, SnapDirect<D>{x, y, z, link.vx, link.vy, link.vz}, link.vx, link.vy,
+                   link.vz, link.ox, link.oy, link.oz, link.row, link.ent, link.alpha, link.step, link.seed);
     return;
   }
   __shared__ ReduceLDS<B> S;
