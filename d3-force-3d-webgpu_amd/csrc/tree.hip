@@ -2607,15 +2607,18 @@ __global__ void __launch_bounds__(256) btree_build(uint32_t total, const uint64_
 }
 
 // Persistent walk: every wave takes 64-point tiles of any job from one queue until it is
-// empty; the per-tile work is the single-simulation walk's, in the same mode for the
-// job's size (so a job's forces are the single simulation's whatever its tiles).
-template <int D, int STACK, int WAVES, bool POOLED>
-__global__ void __launch_bounds__(WAVES * WAVE, POOLED && STACK <= 64 ? POOL_WAVES : 1)
+// empty; the per-tile work is the single-simulation tiled walk's in its cheaper mode with
+// the pool of single-point openings, for jobs of every size.  A job's tiles are its own 64
+// points at a time from its first, so its forces are the same alone and in any mix.
+// (Config D's 256 jobs of 500-50,000 nodes: 131-142 -> 166 layouts/s against the exact
+// mode below 163,840 points.)
+template <int D, int STACK, int WAVES>
+__global__ void __launch_bounds__(WAVES * WAVE, STACK <= 64 ? POOL_WAVES : 1)
     btree_walk(const float4* __restrict__ leaves, const uint32_t* __restrict__ order,
                const NodeRec* __restrict__ rec, const uint2* __restrict__ tiles, uint32_t ntiles,
                const uint32_t* __restrict__ seg_job, const BJob* __restrict__ jobs,
                uint32_t* __restrict__ counters, SpillEntry* __restrict__ spill, uint32_t packed_max) {
-  __shared__ WalkLDS<STACK, POOLED> lds[WAVES];
+  __shared__ WalkLDS<STACK, true> lds[WAVES];
   const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
   SpillEntry* my_spill = spill + (size_t)(blockIdx.x * WAVES + wave) * (WALK_DEPTH - STACK);
   WalkCount C;
@@ -2644,10 +2647,7 @@ __global__ void __launch_bounds__(WAVES * WAVE, POOLED && STACK <= 64 ? POOL_WAV
     const uint32_t end = T.off + T.n;
     const uint32_t count = end - tile.y < (uint32_t)WAVE ? end - tile.y : (uint32_t)WAVE;
     __builtin_amdgcn_wave_barrier();
-    if (T.n <= WALK_EXACT_MAX)
-      walk_tile<D, STACK, true, POOLED>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
-    else
-      walk_tile<D, STACK, false, POOLED>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
+    walk_tile<D, STACK, false, true>(lds[wave], leaves, order, rec, T, tile.y, count, packed_max, lane, my_spill, C);
   }
   walk_flush(C, lane, counters);
 }
@@ -2655,9 +2655,7 @@ __global__ void __launch_bounds__(WAVES * WAVE, POOLED && STACK <= 64 ? POOL_WAV
 template <int D, int STACK, int WAVES>
 void launch_bwalk(hipStream_t st, BatchTree& t, const BSpace& sp, BJob* jobs, const BatchTune& tune,
                   int cus) {
-  // the pool of single-point openings (and its registers) only where a job walks the
-  // cheaper mode
-  auto kern = sp.max_n > WALK_EXACT_MAX ? btree_walk<D, STACK, WAVES, true> : btree_walk<D, STACK, WAVES, false>;
+  auto kern = btree_walk<D, STACK, WAVES>;
   int per_cu = 0;
   FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WAVES * WAVE, 0));
   if (per_cu < 1) per_cu = 1;

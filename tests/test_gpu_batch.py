@@ -221,19 +221,19 @@ def plain_run(n, nd, seed, ticks, **kw):
 
 @pytest.mark.parametrize("n,nd,ticks", [(3000, 3, 30), (2000, 2, 30), (170000, 3, 3)])
 def test_batch_equals_single_simulation_link_manybody(monkeypatch, n, nd, ticks):
-    """The batch walks every job with 64-point tiles: bit for bit the single simulation with
-    that walk (its exact mode up to 163,840 points, whose forces do not depend on the
-    tiling; 170,000 nodes take the cheaper accumulation), and within rounding of the single
-    simulation's default walk for such sizes (one or two points per wave, the same terms
-    summed in another order) after one tick: Barnes-Hut's opening decisions are not
-    continuous, so over many ticks such rounding differences grow to the approximation's
-    own size."""
+    """The batch walks every job with 64-point tiles in the cheaper mode (the pool of
+    single-point openings): bit for bit the single simulation with that walk (tiles of 64,
+    no exact mode), and within rounding of the single simulation's default walk for such
+    sizes (one or two points per wave, the same terms summed in another order) after one
+    tick: Barnes-Hut's opening decisions are not continuous, so over many ticks such
+    rounding differences grow to the approximation's own size."""
     p2, v2 = run_alone(make_job(n, nd, 21, collide=False, center=False), ticks)
     _, v0 = plain_run(n, nd, 21, 1, collide=False, center=False)
     _, v2_1 = run_alone(make_job(n, nd, 21, collide=False, center=False), 1)
     assert rel_l2(v0, v2_1) < 1e-6
     monkeypatch.setenv("D3FGPU_WALK_WIDE", "0")
     monkeypatch.setenv("D3FGPU_WALK_GROUP", "1")
+    monkeypatch.setenv("D3FGPU_WALK_EXACT", "0")
     p1, v1 = plain_run(n, nd, 21, ticks, collide=False, center=False)
     assert np.array_equal(p1, p2)
     assert np.array_equal(v1, v2)
@@ -243,7 +243,7 @@ def test_batch_equals_single_simulation_link_manybody(monkeypatch, n, nd, ticks)
                                                    (12000, 3, None, 10, 1e-4)])
 def test_batch_close_to_single_simulation_full_mix(monkeypatch, n, nd, radius, ticks, tol):
     """center sums and collide visits in another order: agreement to rounding (the single
-    simulation with the batch's walk, 64-point tiles).
+    simulation with the batch's walk, 64-point tiles in the cheaper mode).
 
     Below 8192 nodes the single simulation collides all pairs and the batch through its
     grid, so the sums round differently from the first tick (rel-L2 ~5e-9 after one tick
@@ -252,6 +252,7 @@ def test_batch_close_to_single_simulation_full_mix(monkeypatch, n, nd, radius, t
     rounding), so that case is held to 1e-5 after 5 ticks."""
     monkeypatch.setenv("D3FGPU_WALK_WIDE", "0")
     monkeypatch.setenv("D3FGPU_WALK_GROUP", "1")
+    monkeypatch.setenv("D3FGPU_WALK_EXACT", "0")
     p1, _ = plain_run(n, nd, 31, ticks, radius=radius)
     p2, _ = run_alone(make_job(n, nd, 31, radius=radius), ticks)
     err = rel_l2(p2 - phyllotaxis(n, nd), p1 - phyllotaxis(n, nd))
