@@ -240,13 +240,16 @@ void sort_launch(hipStream_t stream, SortWork& w, K* keys, uint32_t* vals, uint3
                  uint32_t first_bit, uint32_t bits, K** out_keys, uint32_t** out_vals,
                  bool prepared = false) {
   constexpr uint32_t R = 1u << RB;
-  // u64 keys: 8 per thread, or 4 for a short sort (twice the tiles: more waves to hide
-  // each wave's latency where the device is mostly idle, against a longer look-back)
-  constexpr uint32_t SHORT_SORT = 131072;
+  // u64 keys: 8 per thread, 4 for a short sort (twice the tiles: more waves to hide each
+  // wave's latency where the device is mostly idle, against a longer look-back), 16 for a
+  // long one (half the tiles, a shorter look-back: 1M keys 0.139 -> 0.129 ms for the tree's
+  // four passes; 200k slower with 16, 1M slower with 24)
+  constexpr uint32_t SHORT_SORT = 131072, LONG_SORT = 524288;
   constexpr int ITEMS = sizeof(K) == 8 ? 8 : 12;
   static_assert(sizeof(K) == 8 || THREADS * ITEMS == SORT_U32_TILE, "u32 tile out of step with sort.h");
   const bool short_sort = sizeof(K) == 8 && n <= SHORT_SORT;
-  const uint32_t TILE = THREADS * (short_sort ? 4 : ITEMS);
+  const bool long_sort = sizeof(K) == 8 && n > LONG_SORT;
+  const uint32_t TILE = THREADS * (short_sort ? 4 : long_sort ? 16 : ITEMS);
   uint32_t passes = (bits + RB - 1) / RB;
   K* a = keys;
   uint32_t* av = vals;
@@ -274,7 +277,13 @@ void sort_launch(hipStream_t stream, SortWork& w, K* keys, uint32_t* vals, uint3
                          (const K*)a, (const uint32_t*)av, b, bv, n, first_bit + p * RB, pb,
                          (const uint32_t*)(w.ghist + p * R), w.lookback, w.tickets + p,
                          (const uint32_t*)w.epoch, p, w.errors);
-    else
+    else if (long_sort) {
+      if constexpr (sizeof(K) == 8)
+        hipLaunchKernelGGL((sort_pass<K, RB, 16>), dim3(tiles), dim3(THREADS), 0, stream,
+                           (const K*)a, (const uint32_t*)av, b, bv, n, first_bit + p * RB, pb,
+                           (const uint32_t*)(w.ghist + p * R), w.lookback, w.tickets + p,
+                           (const uint32_t*)w.epoch, p, w.errors);
+    } else
       hipLaunchKernelGGL((sort_pass<K, RB, ITEMS>), dim3(tiles), dim3(THREADS), 0, stream,
                          (const K*)a, (const uint32_t*)av, b, bv, n, first_bit + p * RB, pb,
                          (const uint32_t*)(w.ghist + p * R), w.lookback, w.tickets + p,
