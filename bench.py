@@ -260,10 +260,25 @@ def run_batched(specs, cache=None):
                 layouts=[out[j] for j in ids])
 
 
-def run_plain_jobs(specs):
-    """The single-simulation engine, job after job, each to alpha < alphaMin."""
+# The single simulation's walk overrides that leave every job on the tiled walk in its cheaper
+# mode (the walk limits are read when an engine is created).
+TILED_WALK_ENV = {"D3FGPU_WALK_WIDE": "0", "D3FGPU_WALK_GROUP": "1", "D3FGPU_WALK_EXACT": "0"}
+
+
+def run_plain_jobs(specs, env=None):
+    """The single-simulation engine, job after job, each to alpha < alphaMin (`env`: variables
+    set while the engine is created)."""
     import d3fgpu
-    e = d3fgpu.Engine(0)
+    saved = {k: os.environ.get(k) for k in (env or {})}
+    os.environ.update(env or {})
+    try:
+        e = d3fgpu.Engine(0)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     outs = []
     t0 = time.perf_counter()
     for n, d, sd in specs:
@@ -298,14 +313,24 @@ def bench_batch(jobs, with_plain=True):
     if with_plain:
         pl = run_plain_jobs(specs)
         log(f"plain job by job: {pl['wall']:.2f}s")
-        errs = []
-        for (n, d, sd), a, c in zip(specs, bt["layouts"], pl["layouts"]):
-            p0 = phyllotaxis(n, d)
-            den = np.linalg.norm(c - p0)
-            errs.append(float(np.linalg.norm(a - c) / (den if den > 0 else 1.0)))
+        # control: the same engine again on the first jobs with only its walk changed (the sum
+        # of each point's forces taken in another order); over the ticks to alpha < alphaMin
+        # the layouts move apart by as much as the batch's do from the plain ones
+        few = min(64, len(specs))
+        other = run_plain_jobs(specs[:few], TILED_WALK_ENV)
+        log(f"plain job by job, tiled walk, first {few}: {other['wall']:.2f}s")
+
+        def spread(xs, ys, sp):
+            errs = []
+            for (n, d, sd), a, c in zip(sp, xs, ys):
+                den = np.linalg.norm(c - phyllotaxis(n, d))
+                errs.append(float(np.linalg.norm(a - c) / (den if den > 0 else 1.0)))
+            return {"max": max(errs), "median": float(np.median(errs))}
         out.update({"plain_layouts_per_s": round(jobs / pl["wall"], 3), "plain_wall_s": round(pl["wall"], 3),
                     "speedup_vs_plain": round(pl["wall"] / bt["wall"], 2),
-                    "layout_rel_l2_vs_plain": {"max": max(errs), "median": float(np.median(errs))}})
+                    "layout_rel_l2_vs_plain": spread(bt["layouts"], pl["layouts"], specs),
+                    "layout_rel_l2_plain_tiled_walk_vs_plain": dict(
+                        jobs=few, **spread(other["layouts"], pl["layouts"][:few], specs[:few]))})
     return out
 
 
