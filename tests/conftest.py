@@ -30,7 +30,8 @@ def engine_factory():
 
 
 def pytest_sessionfinish(session, exitstatus):
-    """Write the rel-L2 figures the GPU parity tests measured to gpurun_out/parity.json."""
+    """Write the rel-L2 figures the GPU parity tests measured to parity.json in the run's
+    output folder."""
     import json
     try:
         from helpers import PARITY
